@@ -1,0 +1,822 @@
+/*
+ * capi.cpp — the extern "C" entry points of include/blz/blz_ral.h.
+ *
+ * Each call: plan (host) -> compile to one device program blob -> upload blob and
+ * zero a small workspace -> one kernel launch -> read back counts. Errors never
+ * escape as exceptions: they become a status code plus message, the C form of the
+ * reference's std::runtime_error convention (Utils.cuh:25-55).
+ */
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <mutex>
+#include <new>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/blz/blz_ral.h"
+#include "kernels.h"
+#include "planner.h"
+#include "program_compiler.h"
+
+using namespace blz;
+
+namespace {
+
+struct BlzError : std::runtime_error {
+  int code;
+  BlzError(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+void hip_check(hipError_t e, const char *what) {
+  if (e == hipSuccess) return;
+  if (e == hipErrorOutOfMemory) throw BlzError(BLZ_ERR_OOM, std::string(what) + ": out of device memory");
+  throw BlzError(BLZ_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+#define HIPCHECK(x) hip_check((x), #x)
+
+void invalid(const std::string &m) { throw BlzError(BLZ_ERR_INVALID, m); }
+
+int write_err(char *err, size_t len, const std::string &m) {
+  if (err && len > 0) {
+    size_t n = std::min(len - 1, m.size());
+    std::memcpy(err, m.data(), n);
+    err[n] = 0;
+  }
+  return 0;
+}
+
+template <class F>
+int guarded(char *err, size_t len, F &&f) {
+  try {
+    f();
+    write_err(err, len, "");
+    return BLZ_OK;
+  } catch (const BlzError &e) {
+    write_err(err, len, e.what());
+    return e.code;
+  } catch (const std::bad_alloc &) {
+    write_err(err, len, "host out of memory");
+    return BLZ_ERR_OOM;
+  } catch (const std::exception &e) {
+    write_err(err, len, e.what());
+    const std::string m = e.what();
+    if (m.find("not supported") != std::string::npos || m.find("unsupported") != std::string::npos)
+      return BLZ_ERR_UNSUPPORTED;
+    return BLZ_ERR_INVALID;
+  }
+}
+
+hipStream_t S(blz_stream_t s) { return (hipStream_t)s; }
+
+int device_cus() {
+  static int cus = 0;
+  static std::once_flag once;
+  std::call_once(once, [] {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+      cus = 256;
+  });
+  return cus;
+}
+
+int64_t rows_per_tile(int r) { return (int64_t)block_threads() * r; }
+
+int interp_grid(const Compiled &c, int64_t n) {
+  const int64_t tiles = (n + rows_per_tile(c.R) - 1) / rows_per_tile(c.R);
+  const int64_t resident = (int64_t)device_cus() * 8;  // persistent workgroups
+  return (int)std::max<int64_t>(1, std::min<int64_t>(tiles, resident));
+}
+
+void check_column(const gdf_column *c, const char *what, int64_t n) {
+  if (!c) invalid(std::string(what) + " is NULL");
+  if (c->size < 0) invalid(std::string(what) + " has a negative size");
+  if (n >= 0 && c->size != n) invalid(std::string(what) + " size differs from the input row count");
+  const int w = blz_dtype_width(c->dtype);
+  if (c->size > 0 && w > 0 && c->data == nullptr) invalid(std::string(what) + " has no data buffer");
+  if (w > 0 && ((uintptr_t)c->data % w) != 0) invalid(std::string(what) + " data is not aligned to its width");
+  if (c->valid && ((uintptr_t)c->valid % 8) != 0) invalid(std::string(what) + " validity is not 8-byte aligned");
+}
+
+// Device scratch of one launch: blob | null counts | counters | tile states.
+struct LaunchBuffers {
+  uint8_t *base = nullptr;
+  size_t bytes = 0;
+  size_t zero_off = 0, zero_bytes = 0;
+  Workspace ws{};
+  uint8_t *blob = nullptr;
+};
+
+size_t round_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
+
+void layout(LaunchBuffers &lb, size_t blob_bytes, int n_out, int64_t n_tiles) {
+  const size_t blob_sz = round_up(blob_bytes, 256);
+  const size_t nc = round_up(sizeof(unsigned long long) * std::max(n_out, 1), 64);
+  const size_t cnt = 64;  // tile counter + n_sel
+  const size_t ts = round_up(sizeof(unsigned long long) * std::max<int64_t>(n_tiles, 1), 64);
+  lb.bytes = blob_sz + nc + cnt + ts;
+  lb.zero_off = blob_sz;
+  lb.zero_bytes = nc + cnt + ts;
+}
+void bind(LaunchBuffers &lb, int n_out) {
+  uint8_t *p = lb.base;
+  lb.blob = p;
+  uint8_t *z = p + lb.zero_off;
+  lb.ws.null_counts = (unsigned long long *)z;
+  const size_t nc = round_up(sizeof(unsigned long long) * std::max(n_out, 1), 64);
+  lb.ws.tile_counter = (unsigned int *)(z + nc);
+  lb.ws.n_sel = (unsigned long long *)(z + nc + 8);
+  lb.ws.tile_state = (unsigned long long *)(z + nc + 64);
+}
+
+struct RunResult {
+  int64_t n_sel = 0;
+  std::vector<int32_t> nulls;
+};
+
+// One-shot execution of a compiled program (stream-ordered allocations, blocking).
+RunResult run_once(const Compiled &c, bool compact, int64_t n, int n_out, hipStream_t st) {
+  RunResult r;
+  r.nulls.assign(n_out, 0);
+  if (n == 0) return r;
+  const int64_t tiles = (n + rows_per_tile(c.R) - 1) / rows_per_tile(c.R);
+  LaunchBuffers lb;
+  layout(lb, c.blob.size(), n_out, compact ? tiles : 0);
+  HIPCHECK(hipMallocAsync((void **)&lb.base, lb.bytes, st));
+  bind(lb, n_out);
+  std::vector<unsigned long long> host(n_out + 1, 0);
+  hipError_t e = hipMemcpyAsync(lb.blob, c.blob.data(), c.blob.size(), hipMemcpyHostToDevice, st);
+  if (e == hipSuccess) e = launch_fill_zero(lb.base + lb.zero_off, (int64_t)lb.zero_bytes,
+                                            (int)std::min<size_t>(1024, lb.zero_bytes / 2048 + 1), st);
+  if (e == hipSuccess) e = launch_interp(c.S, c.R, compact, interp_grid(c, n), st, lb.blob, lb.ws);
+  if (e == hipSuccess && n_out > 0)
+    e = hipMemcpyAsync(host.data(), lb.ws.null_counts, sizeof(unsigned long long) * n_out,
+                       hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess && compact)
+    e = hipMemcpyAsync(host.data() + n_out, lb.ws.n_sel, sizeof(unsigned long long), hipMemcpyDeviceToHost, st);
+  hipError_t f = hipFreeAsync(lb.base, st);
+  hipError_t s = hipStreamSynchronize(st);
+  HIPCHECK(e);
+  HIPCHECK(f);
+  HIPCHECK(s);
+  for (int o = 0; o < n_out; o++) r.nulls[o] = (int32_t)host[o];
+  r.n_sel = compact ? (int64_t)host[n_out] : n;
+  return r;
+}
+
+std::vector<int> frame_dtypes(gdf_column *const *frame, int n) {
+  std::vector<int> d(n);
+  for (int i = 0; i < n; i++) d[i] = frame[i]->dtype;
+  return d;
+}
+
+DevInputInfo dev_info(const gdf_column *c) {
+  return DevInputInfo{c->data, c->valid, c->dtype, c->null_count};
+}
+
+// Maps frame columns to device inputs (in first-use order).
+struct InputMap {
+  std::vector<int> frame_to_dev;
+  std::vector<DevInputInfo> inputs;
+  explicit InputMap(int n_frame) : frame_to_dev(n_frame, -1) {}
+  int use(gdf_column *const *frame, int f) {
+    if (frame_to_dev[f] < 0) {
+      frame_to_dev[f] = (int)inputs.size();
+      inputs.push_back(dev_info(frame[f]));
+    }
+    return frame_to_dev[f];
+  }
+  void remap(gdf_column *const *frame, PlanSpec &p) {
+    for (auto &c : p.in_dev) c = use(frame, c);
+  }
+};
+
+void check_frame(gdf_column *const *frame, int n_frame) {
+  if (n_frame <= 0 || !frame) invalid("empty frame");
+  const int64_t n = frame[0] ? frame[0]->size : 0;
+  for (int i = 0; i < n_frame; i++) check_column(frame[i], "frame column", n);
+}
+
+void *dalloc(size_t bytes) {
+  if (bytes == 0) return nullptr;
+  void *p = nullptr;
+  HIPCHECK(hipMalloc(&p, bytes));
+  return p;
+}
+
+// Allocates a result column of capacity n (validity zeroed: compaction ORs words).
+gdf_column alloc_column(int dtype, int64_t n, bool with_valid, hipStream_t st) {
+  gdf_column c{};
+  c.dtype = (gdf_dtype)dtype;
+  const int w = blz_dtype_width((gdf_dtype)dtype);
+  if (w == 0) throw BlzError(BLZ_ERR_UNSUPPORTED, "dtype without a fixed width is not supported");
+  c.data = dalloc(round_up((size_t)n * w, 64));
+  if (with_valid && n > 0) {
+    const size_t vb = gdf_valid_allocation_size((gdf_size_type)n);
+    c.valid = (gdf_valid_type *)dalloc(vb);
+    HIPCHECK(launch_fill_zero(c.valid, (int64_t)vb, (int)std::min<size_t>(1024, vb / 2048 + 1), st));
+  }
+  c.size = (gdf_size_type)n;
+  return c;
+}
+
+std::string condition_of(const char *qp) {
+  std::string s = qp ? qp : "";
+  if (s.find("condition=[") != std::string::npos) return planner::get_named_expression(s, "condition");
+  return s;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *blz_version(void) { return "blz-mi355x 0.1 (gfx950)"; }
+
+int blz_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n;
+}
+
+int blz_perform_operation(gdf_column *const *output_columns, int n_outputs,
+                          gdf_column *const *input_columns, int n_inputs,
+                          const column_index_type *left_inputs, const column_index_type *right_inputs,
+                          const column_index_type *outputs, int n_operations,
+                          const column_index_type *final_output_positions,
+                          const gdf_binary_operator *operators, const gdf_unary_operator *unary_operators,
+                          const gdf_scalar *left_scalars, const gdf_scalar *right_scalars, int scalar_mode,
+                          int32_t *out_null_counts, blz_stream_t stream, char *err, size_t err_len) {
+  return guarded(err, err_len, [&] {
+    if (n_inputs <= 0) invalid("perform_operation needs at least one input column");
+    if (n_inputs > kMaxSlots) invalid("more than 64 input columns");
+    if (n_outputs < 0 || n_operations < 0) invalid("negative count");
+    const int64_t n = input_columns[0]->size;
+    for (int i = 0; i < n_inputs; i++) check_column(input_columns[i], "input column", n);
+    for (int i = 0; i < n_outputs; i++) check_column(output_columns[i], "output column", n);
+    PlanSpec p;
+    for (int c = 0; c < n_inputs; c++) p.in_dev.push_back(c);
+    p.left.assign(left_inputs, left_inputs + n_operations);
+    p.right.assign(right_inputs, right_inputs + n_operations);
+    p.outs.assign(outputs, outputs + n_operations);
+    p.final_pos.assign(final_output_positions, final_output_positions + n_outputs);
+    for (int k = 0; k < n_operations; k++) {
+      p.binops.push_back(operators[k]);
+      p.unops.push_back(unary_operators[k]);
+      p.lsc.push_back(left_scalars[k]);
+      p.rsc.push_back(right_scalars[k]);
+    }
+    CompileRequest req;
+    req.n_rows = n;
+    for (int c = 0; c < n_inputs; c++) req.inputs.push_back(dev_info(input_columns[c]));
+    req.proj = &p;
+    req.scalar_mode = scalar_mode;
+    for (int o = 0; o < n_outputs; o++)
+      req.outputs.push_back({output_columns[o]->data, output_columns[o]->valid, output_columns[o]->dtype, 0, o});
+    Compiled c = compile_program(req);
+    RunResult r = run_once(c, false, n, n_outputs, S(stream));
+    if (out_null_counts)
+      for (int o = 0; o < n_outputs; o++) out_null_counts[o] = output_columns[o]->valid ? r.nulls[o] : 0;
+  });
+}
+
+int blz_evaluate_expression(gdf_column *const *frame, int n_frame, const char *expression,
+                            gdf_column *output, int scalar_mode, int32_t *out_null_count,
+                            blz_stream_t stream, char *err, size_t err_len) {
+  return guarded(err, err_len, [&] {
+    check_frame(frame, n_frame);
+    if (!expression) invalid("expression is NULL");
+    const int64_t n = frame[0]->size;
+    check_column(output, "output column", n);
+    planner::ExprPlan ep = planner::plan_expression(frame_dtypes(frame, n_frame), expression);
+    if (ep.bare) {
+      const gdf_column *src = frame[ep.bare_col];
+      if (src->dtype != output->dtype) invalid("bare-column expression: output dtype differs from the column");
+      const size_t bytes = (size_t)n * blz_dtype_width(src->dtype);
+      if (bytes) HIPCHECK(hipMemcpyAsync(output->data, src->data, bytes, hipMemcpyDeviceToDevice, S(stream)));
+      if (output->valid && src->valid && n > 0)
+        HIPCHECK(hipMemcpyAsync(output->valid, src->valid, ((size_t)n + 7) / 8, hipMemcpyDeviceToDevice, S(stream)));
+      HIPCHECK(hipStreamSynchronize(S(stream)));
+      if (out_null_count) *out_null_count = src->valid ? src->null_count : 0;
+      return;
+    }
+    InputMap im(n_frame);
+    im.remap(frame, ep.spec);
+    CompileRequest req;
+    req.n_rows = n;
+    req.inputs = im.inputs;
+    req.proj = &ep.spec;
+    req.scalar_mode = scalar_mode;
+    req.outputs.push_back({output->data, output->valid, output->dtype, 0, 0});
+    Compiled c = compile_program(req);
+    RunResult r = run_once(c, false, n, 1, S(stream));
+    if (out_null_count) *out_null_count = output->valid ? r.nulls[0] : 0;
+  });
+}
+
+namespace {
+
+// Fused filter (+ optional projection). `proj_qp` NULL keeps every frame column.
+void filter_project_impl(gdf_column *const *frame, int n_frame, const char *filter_qp, const char *proj_qp,
+                         gdf_column *out_columns, int max_out, int *n_out, int32_t *is_alias,
+                         int64_t *n_kept, int scalar_mode, hipStream_t st) {
+  check_frame(frame, n_frame);
+  const int64_t n = frame[0]->size;
+  const auto dt = frame_dtypes(frame, n_frame);
+  InputMap im(n_frame);
+  planner::ExprPlan fp;
+  const bool has_filter = filter_qp != nullptr;
+  if (has_filter) {
+    fp = planner::plan_expression(dt, condition_of(filter_qp));
+    if (fp.bare) im.use(frame, fp.bare_col);
+    else im.remap(frame, fp.spec);
+  }
+  planner::ProjectPlan pp;
+  std::vector<planner::ProjOut> outs;
+  if (proj_qp) {
+    pp = planner::parse_project(dt, proj_qp);
+    if (pp.has_eval) im.remap(frame, pp.spec);
+    outs = pp.outputs;
+  } else {
+    for (int i = 0; i < n_frame; i++) outs.push_back({1, i, dt[i], ""});
+  }
+  if ((int)outs.size() > max_out) invalid("more outputs than out_columns can hold");
+  // Allocate outputs: evaluated outputs always carry validity (gdf_column_cpp
+  // allocates it, GDFColumn.cu:302-338); gathered ones iff their input has it.
+  std::vector<gdf_column> res(outs.size());
+  CompileRequest req;
+  req.n_rows = n;
+  req.scalar_mode = scalar_mode;
+  req.compact = has_filter;
+  int64_t kept = n;
+  try {
+    for (size_t i = 0; i < outs.size(); i++) {
+      const auto &o = outs[i];
+      const bool with_valid = o.kind == 0 ? true : frame[o.index]->valid != nullptr;
+      res[i] = alloc_column(o.dtype, n, with_valid, st);
+      if (o.kind == 0) req.outputs.push_back({res[i].data, res[i].valid, o.dtype, 0, o.index});
+      else req.outputs.push_back({res[i].data, res[i].valid, o.dtype, 1, im.use(frame, o.index)});
+    }
+    req.inputs = im.inputs;
+    if (has_filter) {
+      if (fp.bare) req.bare_stencil_input = im.frame_to_dev[fp.bare_col];
+      else req.pred = &fp.spec;
+    }
+    if (proj_qp && pp.has_eval) req.proj = &pp.spec;
+    std::vector<int32_t> nulls(outs.size(), 0);
+    if (n > 0 && !req.inputs.empty()) {
+      Compiled c = compile_program(req);
+      RunResult r = run_once(c, has_filter, n, (int)outs.size(), st);
+      kept = r.n_sel;
+      nulls = r.nulls;
+    } else {
+      HIPCHECK(hipStreamSynchronize(st));
+      if (n > 0 && req.inputs.empty()) invalid("nothing to evaluate");
+    }
+    for (size_t i = 0; i < outs.size(); i++) {
+      res[i].size = (gdf_size_type)kept;
+      res[i].null_count = (res[i].valid && kept > 0) ? nulls[i] : 0;
+      out_columns[i] = res[i];
+      if (is_alias) is_alias[i] = 0;
+    }
+  } catch (...) {
+    for (auto &c : res) { (void)hipFree(c.data); (void)hipFree(c.valid); }
+    throw;
+  }
+  if (n_out) *n_out = (int)outs.size();
+  if (n_kept) *n_kept = kept;
+}
+
+}  // namespace
+
+int blz_process_filter(gdf_column *const *frame, int n_frame, const char *query_part,
+                       gdf_column *out_columns, int64_t *n_kept, int scalar_mode, blz_stream_t stream,
+                       char *err, size_t err_len) {
+  return guarded(err, err_len, [&] {
+    if (!query_part) invalid("query_part is NULL");
+    int n_out = 0;
+    filter_project_impl(frame, n_frame, query_part, nullptr, out_columns, n_frame, &n_out, nullptr, n_kept,
+                        scalar_mode, S(stream));
+  });
+}
+
+int blz_filter_project(gdf_column *const *frame, int n_frame, const char *filter_query_part,
+                       const char *project_query_part, gdf_column *out_columns, int max_out, int *n_out,
+                       int64_t *n_kept, int scalar_mode, blz_stream_t stream, char *err, size_t err_len) {
+  return guarded(err, err_len, [&] {
+    if (!filter_query_part) invalid("filter_query_part is NULL");
+    filter_project_impl(frame, n_frame, filter_query_part, project_query_part, out_columns, max_out, n_out,
+                        nullptr, n_kept, scalar_mode, S(stream));
+  });
+}
+
+int blz_execute_project(gdf_column *const *frame, int n_frame, const char *query_part,
+                        gdf_column *out_columns, int32_t *is_alias, int max_out, int *n_out,
+                        int scalar_mode, blz_stream_t stream, char *err, size_t err_len) {
+  return guarded(err, err_len, [&] {
+    check_frame(frame, n_frame);
+    if (!query_part) invalid("query_part is NULL");
+    const int64_t n = frame[0]->size;
+    planner::ProjectPlan pp = planner::parse_project(frame_dtypes(frame, n_frame), query_part);
+    if ((int)pp.outputs.size() > max_out) invalid("more outputs than out_columns can hold");
+    std::vector<gdf_column> res(pp.outputs.size());
+    std::vector<int> eval_pos;
+    CompileRequest req;
+    req.n_rows = n;
+    req.scalar_mode = scalar_mode;
+    InputMap im(n_frame);
+    try {
+      for (size_t i = 0; i < pp.outputs.size(); i++) {
+        const auto &o = pp.outputs[i];
+        if (o.kind == 0) {
+          res[i] = alloc_column(o.dtype, n, true, S(stream));
+          req.outputs.push_back({res[i].data, res[i].valid, o.dtype, 0, o.index});
+          eval_pos.push_back((int)i);
+        } else {
+          res[i] = *frame[o.index];
+        }
+      }
+      std::vector<int32_t> nulls(req.outputs.size(), 0);
+      if (pp.has_eval && n > 0) {
+        im.remap(frame, pp.spec);
+        req.inputs = im.inputs;
+        req.proj = &pp.spec;
+        Compiled c = compile_program(req);
+        RunResult r = run_once(c, false, n, (int)req.outputs.size(), S(stream));
+        nulls = r.nulls;
+      }
+      // update_null_count on every output (CalciteInterpreter.cpp:332-335).
+      for (size_t k = 0; k < eval_pos.size(); k++) res[eval_pos[k]].null_count = n > 0 ? nulls[k] : 0;
+      for (size_t i = 0; i < pp.outputs.size(); i++) {
+        if (pp.outputs[i].kind == 1) {
+          gdf_column &c = res[i];
+          if (c.valid && c.size > 0) {
+            unsigned long long *d = nullptr, h = 0;
+            HIPCHECK(hipMallocAsync((void **)&d, 8, S(stream)));
+            HIPCHECK(launch_fill_zero(d, 8, 1, S(stream)));
+            HIPCHECK(launch_null_count(c.valid, c.size, d, 256, S(stream)));
+            HIPCHECK(hipMemcpyAsync(&h, d, 8, hipMemcpyDeviceToHost, S(stream)));
+            HIPCHECK(hipFreeAsync(d, S(stream)));
+            HIPCHECK(hipStreamSynchronize(S(stream)));
+            c.null_count = (gdf_size_type)h;
+          } else {
+            c.null_count = 0;
+          }
+        }
+        out_columns[i] = res[i];
+        if (is_alias) is_alias[i] = pp.outputs[i].kind == 1;
+      }
+    } catch (...) {
+      for (int p : eval_pos) { (void)hipFree(res[p].data); (void)hipFree(res[p].valid); }
+      throw;
+    }
+    if (n_out) *n_out = (int)pp.outputs.size();
+  });
+}
+
+int blz_materialize_column(const gdf_column *input, gdf_column *output, const gdf_column *row_indices,
+                           blz_stream_t stream, char *err, size_t err_len) {
+  return guarded(err, err_len, [&] {
+    check_column(input, "input column", -1);
+    check_column(row_indices, "row_indices", -1);
+    check_column(output, "output column", row_indices->size);
+    if (row_indices->dtype != GDF_INT32) throw BlzError(BLZ_ERR_UNSUPPORTED, "row_indices must be INT32");
+    const int w = blz_dtype_width(input->dtype);
+    if (w == 0 || blz_dtype_width(output->dtype) != w) invalid("input and output widths differ");
+    const int64_t m = row_indices->size;
+    unsigned long long *d = nullptr, h = 0;
+    HIPCHECK(hipMallocAsync((void **)&d, 8, S(stream)));
+    hipError_t e = launch_fill_zero(d, 8, 1, S(stream));
+    if (e == hipSuccess && m > 0) {
+      const int64_t words = (m + 63) / 64;
+      const int grid = (int)std::max<int64_t>(1, std::min<int64_t>((words + 3) / 4, (int64_t)device_cus() * 16));
+      e = launch_gather(w, input->data, input->valid, input->size, (const int32_t *)row_indices->data, m, output->data,
+                        output->valid, d, grid, S(stream));
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(&h, d, 8, hipMemcpyDeviceToHost, S(stream));
+    (void)hipFreeAsync(d, S(stream));
+    hipError_t s = hipStreamSynchronize(S(stream));
+    HIPCHECK(e);
+    HIPCHECK(s);
+    output->null_count = (m == 0 || output->valid == nullptr) ? 0 : (gdf_size_type)h;
+  });
+}
+
+int blz_gdf_sequence(int32_t *data, size_t size, int32_t init_val, int32_t step, blz_stream_t stream,
+                     char *err, size_t err_len) {
+  return guarded(err, err_len, [&] {
+    if (size == 0) return;
+    if (!data) invalid("data is NULL");
+    const int grid = (int)std::min<size_t>((size + 255) / 256, (size_t)device_cus() * 8);
+    HIPCHECK(launch_sequence(data, (int64_t)size, init_val, step, grid, S(stream)));
+    HIPCHECK(hipStreamSynchronize(S(stream)));
+  });
+}
+
+int blz_apply_boolean_mask(const gdf_column *input, const gdf_column *stencil, gdf_column *out,
+                           blz_stream_t stream, char *err, size_t err_len) {
+  return guarded(err, err_len, [&] {
+    check_column(input, "input column", -1);
+    check_column(stencil, "stencil", input->size);
+    const int64_t n = input->size;
+    CompileRequest req;
+    req.n_rows = n;
+    req.compact = true;
+    req.inputs = {dev_info(input), dev_info(stencil)};
+    req.bare_stencil_input = 1;
+    gdf_column res = alloc_column(input->dtype, n, input->valid != nullptr, S(stream));
+    try {
+      req.outputs.push_back({res.data, res.valid, input->dtype, 1, 0});
+      int64_t kept = 0;
+      int32_t nulls = 0;
+      if (n > 0) {
+        Compiled c = compile_program(req);
+        RunResult r = run_once(c, true, n, 1, S(stream));
+        kept = r.n_sel;
+        nulls = r.nulls[0];
+      }
+      res.size = (gdf_size_type)kept;
+      res.null_count = (res.valid && kept > 0) ? nulls : 0;
+      *out = res;
+    } catch (...) {
+      (void)hipFree(res.data);
+      (void)hipFree(res.valid);
+      throw;
+    }
+  });
+}
+
+int blz_update_null_count(gdf_column *column, blz_stream_t stream, char *err, size_t err_len) {
+  return guarded(err, err_len, [&] {
+    check_column(column, "column", -1);
+    if (column->size == 0 || column->valid == nullptr) { column->null_count = 0; return; }
+    unsigned long long *d = nullptr, h = 0;
+    HIPCHECK(hipMallocAsync((void **)&d, 8, S(stream)));
+    hipError_t e = launch_fill_zero(d, 8, 1, S(stream));
+    if (e == hipSuccess) e = launch_null_count(column->valid, column->size, d, 256, S(stream));
+    if (e == hipSuccess) e = hipMemcpyAsync(&h, d, 8, hipMemcpyDeviceToHost, S(stream));
+    (void)hipFreeAsync(d, S(stream));
+    hipError_t s = hipStreamSynchronize(S(stream));
+    HIPCHECK(e);
+    HIPCHECK(s);
+    column->null_count = (gdf_size_type)h;
+  });
+}
+
+// ---- prepared programs ----------------------------------------------------------
+
+struct blz_program_s {
+  Compiled c;
+  bool compact = false;
+  int64_t n = 0;
+  int n_out = 0;
+  int grid = 1;
+  LaunchBuffers lb;
+  std::vector<gdf_column> outs;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
+
+int blz_program_create(blz_program *prog, gdf_column *const *frame, int n_frame, const char *filter_qp,
+                       const char *proj_qp, gdf_column *out_columns, int n_out_cap, int *n_out,
+                       int scalar_mode, int s, int r, char *err, size_t err_len) {
+  return guarded(err, err_len, [&] {
+    if (!prog) invalid("prog is NULL");
+    *prog = nullptr;
+    check_frame(frame, n_frame);
+    const int64_t n = frame[0]->size;
+    const auto dt = frame_dtypes(frame, n_frame);
+    InputMap im(n_frame);
+    planner::ExprPlan fp;
+    const bool has_filter = filter_qp != nullptr;
+    if (has_filter) {
+      fp = planner::plan_expression(dt, condition_of(filter_qp));
+      if (fp.bare) im.use(frame, fp.bare_col);
+      else im.remap(frame, fp.spec);
+    }
+    planner::ProjectPlan pp;
+    std::vector<planner::ProjOut> outs;
+    if (proj_qp) {
+      pp = planner::parse_project(dt, proj_qp);
+      if (pp.has_eval) im.remap(frame, pp.spec);
+      outs = pp.outputs;
+    } else {
+      for (int i = 0; i < n_frame; i++) outs.push_back({1, i, dt[i], ""});
+    }
+    if ((int)outs.size() > n_out_cap) invalid("more outputs than out_columns can hold");
+    auto *p = new blz_program_s();
+    try {
+      CompileRequest req;
+      req.n_rows = n;
+      req.scalar_mode = scalar_mode;
+      req.compact = has_filter;
+      req.force_s = s;
+      req.force_r = r;
+      for (size_t i = 0; i < outs.size(); i++) {
+        const auto &o = outs[i];
+        const bool with_valid = o.kind == 0 ? true : frame[o.index]->valid != nullptr;
+        gdf_column c = alloc_column(o.dtype, n, with_valid, nullptr);
+        p->outs.push_back(c);
+        if (o.kind == 0) req.outputs.push_back({c.data, c.valid, o.dtype, 0, o.index});
+        else req.outputs.push_back({c.data, c.valid, o.dtype, 1, im.use(frame, o.index)});
+      }
+      req.inputs = im.inputs;
+      if (has_filter) {
+        if (fp.bare) req.bare_stencil_input = im.frame_to_dev[fp.bare_col];
+        else req.pred = &fp.spec;
+      }
+      if (proj_qp && pp.has_eval) req.proj = &pp.spec;
+      if (req.inputs.empty()) invalid("nothing to evaluate");
+      p->c = compile_program(req);
+      p->compact = has_filter;
+      p->n = n;
+      p->n_out = (int)outs.size();
+      p->grid = interp_grid(p->c, std::max<int64_t>(n, 1));
+      const int64_t tiles = (n + rows_per_tile(p->c.R) - 1) / rows_per_tile(p->c.R);
+      layout(p->lb, p->c.blob.size(), p->n_out, has_filter ? tiles : 0);
+      HIPCHECK(hipMalloc((void **)&p->lb.base, p->lb.bytes));
+      bind(p->lb, p->n_out);
+      HIPCHECK(hipMemcpy(p->lb.blob, p->c.blob.data(), p->c.blob.size(), hipMemcpyHostToDevice));
+      HIPCHECK(hipEventCreate(&p->ev0));
+      HIPCHECK(hipEventCreate(&p->ev1));
+      HIPCHECK(hipDeviceSynchronize());
+      for (size_t i = 0; i < p->outs.size(); i++) out_columns[i] = p->outs[i];
+      if (n_out) *n_out = (int)p->outs.size();
+    } catch (...) {
+      blz_program_destroy(p);
+      throw;
+    }
+    *prog = p;
+  });
+}
+
+namespace {
+hipError_t program_enqueue(blz_program p, hipStream_t st) {
+  if (p->n == 0) return hipSuccess;
+  hipError_t e = launch_fill_zero(p->lb.base + p->lb.zero_off, (int64_t)p->lb.zero_bytes,
+                                  (int)std::min<size_t>(1024, p->lb.zero_bytes / 2048 + 1), st);
+  // compaction ORs boundary validity words: clear the outputs' validity first
+  for (size_t i = 0; e == hipSuccess && p->compact && i < p->outs.size(); i++)
+    if (p->outs[i].valid) {
+      const size_t vb = gdf_valid_allocation_size((gdf_size_type)p->n);
+      e = launch_fill_zero(p->outs[i].valid, (int64_t)vb, (int)std::min<size_t>(1024, vb / 2048 + 1), st);
+    }
+  if (e == hipSuccess) e = launch_interp(p->c.S, p->c.R, p->compact, p->grid, st, p->lb.blob, p->lb.ws);
+  return e;
+}
+}  // namespace
+
+int blz_program_launch(blz_program prog, blz_stream_t stream, char *err, size_t err_len) {
+  return guarded(err, err_len, [&] {
+    if (!prog) invalid("prog is NULL");
+    HIPCHECK(program_enqueue(prog, S(stream)));
+  });
+}
+
+int blz_program_result(blz_program prog, blz_stream_t stream, int64_t *n_kept, int32_t *null_counts,
+                       char *err, size_t err_len) {
+  return guarded(err, err_len, [&] {
+    if (!prog) invalid("prog is NULL");
+    std::vector<unsigned long long> h(prog->n_out + 1, 0);
+    if (prog->n > 0) {
+      if (prog->n_out > 0)
+        HIPCHECK(hipMemcpyAsync(h.data(), prog->lb.ws.null_counts, 8 * prog->n_out, hipMemcpyDeviceToHost,
+                                S(stream)));
+      if (prog->compact)
+        HIPCHECK(hipMemcpyAsync(h.data() + prog->n_out, prog->lb.ws.n_sel, 8, hipMemcpyDeviceToHost, S(stream)));
+    }
+    HIPCHECK(hipStreamSynchronize(S(stream)));
+    const int64_t kept = prog->compact ? (int64_t)h[prog->n_out] : prog->n;
+    if (n_kept) *n_kept = kept;
+    if (null_counts)
+      for (int o = 0; o < prog->n_out; o++)
+        null_counts[o] = (prog->outs[o].valid && kept > 0) ? (int32_t)h[o] : 0;
+  });
+}
+
+int blz_program_time(blz_program prog, blz_stream_t stream, int iters, double *mean_ms, char *err,
+                     size_t err_len) {
+  return guarded(err, err_len, [&] {
+    if (!prog || iters <= 0) invalid("bad arguments");
+    double total = 0;
+    for (int i = 0; i < iters; i++) {
+      // the zero-fill of counters is part of a launch but not of the kernel time
+      HIPCHECK(launch_fill_zero(prog->lb.base + prog->lb.zero_off, (int64_t)prog->lb.zero_bytes,
+                                (int)std::min<size_t>(1024, prog->lb.zero_bytes / 2048 + 1), S(stream)));
+      for (size_t o = 0; prog->compact && o < prog->outs.size(); o++)
+        if (prog->outs[o].valid) {
+          const size_t vb = gdf_valid_allocation_size((gdf_size_type)prog->n);
+          HIPCHECK(launch_fill_zero(prog->outs[o].valid, (int64_t)vb, (int)std::min<size_t>(1024, vb / 2048 + 1),
+                                    S(stream)));
+        }
+      HIPCHECK(hipEventRecord(prog->ev0, S(stream)));
+      HIPCHECK(launch_interp(prog->c.S, prog->c.R, prog->compact, prog->grid, S(stream), prog->lb.blob,
+                             prog->lb.ws));
+      HIPCHECK(hipEventRecord(prog->ev1, S(stream)));
+      HIPCHECK(hipEventSynchronize(prog->ev1));
+      float ms = 0;
+      HIPCHECK(hipEventElapsedTime(&ms, prog->ev0, prog->ev1));
+      total += ms;
+    }
+    *mean_ms = total / iters;
+  });
+}
+
+int blz_program_info(blz_program prog, int *s, int *r, int *n_ops, int *n_vops, int *n_slots, int *grid) {
+  if (!prog) return BLZ_ERR_INVALID;
+  if (s) *s = prog->c.S;
+  if (r) *r = prog->c.R;
+  if (n_ops) *n_ops = prog->c.n_ops;
+  if (n_vops) *n_vops = prog->c.n_vops;
+  if (n_slots) *n_slots = prog->c.n_slots;
+  if (grid) *grid = prog->grid;
+  return BLZ_OK;
+}
+
+int blz_program_destroy(blz_program prog) {
+  if (!prog) return BLZ_OK;
+  (void)hipDeviceSynchronize();
+  for (auto &c : prog->outs) { (void)hipFree(c.data); (void)hipFree(c.valid); }
+  if (prog->lb.base) (void)hipFree(prog->lb.base);
+  if (prog->ev0) (void)hipEventDestroy(prog->ev0);
+  if (prog->ev1) (void)hipEventDestroy(prog->ev1);
+  delete prog;
+  return BLZ_OK;
+}
+
+// ---- memory / streams --------------------------------------------------------------
+
+int blz_device_malloc(void **ptr, size_t bytes) {
+  return hipMalloc(ptr, bytes) == hipSuccess ? BLZ_OK : BLZ_ERR_OOM;
+}
+int blz_device_free(void *ptr) { return hipFree(ptr) == hipSuccess ? BLZ_OK : BLZ_ERR_HIP; }
+int blz_memcpy_htod(void *dst, const void *src, size_t bytes, blz_stream_t stream) {
+  if (bytes == 0) return BLZ_OK;
+  if (hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, S(stream)) != hipSuccess) return BLZ_ERR_HIP;
+  return hipStreamSynchronize(S(stream)) == hipSuccess ? BLZ_OK : BLZ_ERR_HIP;
+}
+int blz_memcpy_dtoh(void *dst, const void *src, size_t bytes, blz_stream_t stream) {
+  if (bytes == 0) return BLZ_OK;
+  if (hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, S(stream)) != hipSuccess) return BLZ_ERR_HIP;
+  return hipStreamSynchronize(S(stream)) == hipSuccess ? BLZ_OK : BLZ_ERR_HIP;
+}
+int blz_memset_device(void *dst, int value, size_t bytes, blz_stream_t stream) {
+  if (bytes == 0) return BLZ_OK;
+  if (hipMemsetAsync(dst, value, bytes, S(stream)) != hipSuccess) return BLZ_ERR_HIP;
+  return hipStreamSynchronize(S(stream)) == hipSuccess ? BLZ_OK : BLZ_ERR_HIP;
+}
+int blz_stream_create(blz_stream_t *stream) {
+  hipStream_t s;
+  if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) return BLZ_ERR_HIP;
+  *stream = (blz_stream_t)s;
+  return BLZ_OK;
+}
+int blz_stream_synchronize(blz_stream_t stream) {
+  return hipStreamSynchronize(S(stream)) == hipSuccess ? BLZ_OK : BLZ_ERR_HIP;
+}
+int blz_stream_destroy(blz_stream_t stream) {
+  return hipStreamDestroy(S(stream)) == hipSuccess ? BLZ_OK : BLZ_ERR_HIP;
+}
+int blz_free_column(gdf_column *column) {
+  if (!column) return BLZ_OK;
+  hipError_t a = column->data ? hipFree(column->data) : hipSuccess;
+  hipError_t b = column->valid ? hipFree(column->valid) : hipSuccess;
+  std::memset(column, 0, sizeof(*column));
+  return (a == hipSuccess && b == hipSuccess) ? BLZ_OK : BLZ_ERR_HIP;
+}
+
+int blz_plan_expression(const int32_t *frame_dtypes_in, int n_frame, const char *expression, int cap,
+                        column_index_type *left, column_index_type *right, column_index_type *outputs,
+                        int32_t *binops, int32_t *unops, gdf_scalar *lscalars, gdf_scalar *rscalars,
+                        int *n_ops, int32_t *in_cols, int *n_in, column_index_type *final_pos, char *err,
+                        size_t err_len) {
+  return guarded(err, err_len, [&] {
+    if (!expression) invalid("expression is NULL");
+    std::vector<int> dt(frame_dtypes_in, frame_dtypes_in + n_frame);
+    planner::ExprPlan ep = planner::plan_expression(dt, expression);
+    if (ep.bare) {
+      *n_ops = 0;
+      *n_in = 1;
+      in_cols[0] = ep.bare_col;
+      *final_pos = -1;
+      return;
+    }
+    const int k = (int)ep.spec.left.size();
+    if (k > cap || (int)ep.spec.in_dev.size() > cap) invalid("plan larger than cap");
+    for (int i = 0; i < k; i++) {
+      left[i] = ep.spec.left[i];
+      right[i] = ep.spec.right[i];
+      outputs[i] = ep.spec.outs[i];
+      binops[i] = ep.spec.binops[i];
+      unops[i] = ep.spec.unops[i];
+      lscalars[i] = ep.spec.lsc[i];
+      rscalars[i] = ep.spec.rsc[i];
+    }
+    *n_ops = k;
+    for (size_t i = 0; i < ep.spec.in_dev.size(); i++) in_cols[i] = ep.spec.in_dev[i];
+    *n_in = (int)ep.spec.in_dev.size();
+    *final_pos = ep.spec.final_pos[0];
+  });
+}
+
+}  // extern "C"

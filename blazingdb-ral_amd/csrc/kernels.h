@@ -1,0 +1,37 @@
+/*
+ * kernels.h — host-callable launchers of the gfx950 kernels (kernels.hip).
+ */
+#ifndef BLZ_KERNELS_H_
+#define BLZ_KERNELS_H_
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace blz {
+
+// Per-launch device scratch. Every counter must be zero at launch.
+struct Workspace {
+  unsigned long long *null_counts;  // [n_out] nulls per output (outputs with validity)
+  unsigned int *tile_counter;       // compaction: dynamic tile ids
+  unsigned long long *tile_state;   // compaction: [n_tiles] look-back state words
+  unsigned long long *n_sel;        // compaction: rows kept
+};
+
+// Register-file interpreter (project or fused filter+project/gather).
+// (s, r) in {(8,1), (8,2), (16,1), (32,1), (64,1)}: s slots, r rows per lane.
+hipError_t launch_interp(int s, int r, bool compact, int grid, hipStream_t st, const uint8_t *blob,
+                         Workspace ws);
+// materialize_column: width in {1,2,4,8}.
+hipError_t launch_gather(int width, const void *in, const uint8_t *in_valid, int64_t n_in, const int32_t *idx,
+                         int64_t n_idx, void *out, uint8_t *out_valid, unsigned long long *null_count,
+                         int grid, hipStream_t st);
+hipError_t launch_sequence(int32_t *out, int64_t n, int32_t init, int32_t step, int grid, hipStream_t st);
+hipError_t launch_fill_zero(void *p, int64_t bytes, int grid, hipStream_t st);
+hipError_t launch_null_count(const uint8_t *valid, int64_t n, unsigned long long *out, int grid,
+                             hipStream_t st);
+
+int blob_capacity();   // bytes of LDS the program blob may occupy
+int block_threads();   // threads per workgroup of every kernel
+
+}  // namespace blz
+#endif

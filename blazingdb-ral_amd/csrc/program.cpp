@@ -1,0 +1,672 @@
+/*
+ * program.cpp — compile reference-format plans into one device program blob.
+ *
+ * Host work that the reference does per call inside the InterpreterFunctor
+ * constructor (interpreter_ops.cuh:959-1266) and per row inside the kernel
+ * (interpreter_ops.cuh:473-505, 1301-1374), done once here:
+ *  1. per-op type resolution (plan dtypes -> int64 / binary64 kernel types), with
+ *     the same rules as interpreter_ops.cuh:1095-1178 and
+ *     CalciteExpressionParsing.cpp:177-270;
+ *  2. lowering each op to a fully typed device op with explicit operand and
+ *     result conversions (C++ usual arithmetic conversions made explicit);
+ *     scalar operands become immediates (scalar_mode picks the reference's shipped
+ *     float-literal truncation, interpreter_ops.cuh:167-168, or the intended
+ *     value-preserving widening);
+ *  3. SSA over the plan's register slots, Sethi-Ullman ordering of each output's
+ *     expression tree and linear-scan allocation of device slots, so a C1-style
+ *     predicate needs 2 temporaries instead of the planner's 5;
+ *  4. the validity program (AND / OR / copy of 64-row words, process_valids
+ *     :507-571) constant-folded: forced-valid inputs (the Q3 rule of :1312-1327)
+ *     and scalars fold away, so null-free plans run no validity ops at all.
+ */
+#include "program_compiler.h"
+
+#include <algorithm>
+#include <cstring>
+#include <map>
+#include <stdexcept>
+
+#include "kernels.h"
+
+namespace blz {
+namespace {
+
+bool is_int_kernel(int t) {  // interpreter_ops.cuh:180-190
+  return t == GDF_INT32 || t == GDF_INT64 || t == GDF_INT16 || t == GDF_INT8 ||
+         t == GDF_DATE32 || t == GDF_DATE64 || t == GDF_TIMESTAMP || t == GDF_STRING_CATEGORY;
+}
+bool is_float(int t) { return t == GDF_FLOAT32 || t == GDF_FLOAT64; }
+bool is_signed_t(int t) {  // CalciteExpressionParsing.cpp:21-31
+  return t == GDF_INT8 || t == GDF_INT16 || t == GDF_INT32 || t == GDF_INT64 ||
+         t == GDF_FLOAT32 || t == GDF_FLOAT64 || t == GDF_DATE32 || t == GDF_DATE64 ||
+         t == GDF_TIMESTAMP;
+}
+bool is_date(int t) { return t == GDF_DATE32 || t == GDF_DATE64 || t == GDF_TIMESTAMP; }
+int wid(int t) { return blz_dtype_width((gdf_dtype)t); }
+
+}  // namespace
+
+int unary_output_dtype(int l, int u) {  // CalciteExpressionParsing.cpp:177-191
+  if (u == BLZ_CAST_FLOAT64) return GDF_FLOAT64;
+  if (is_date(l)) return GDF_INT16;
+  if (u == BLZ_SIN || u == BLZ_COS || u == BLZ_ASIN || u == BLZ_ACOS || u == BLZ_TAN ||
+      u == BLZ_COTAN || u == BLZ_ATAN || u == BLZ_LOG || u == BLZ_LN)
+    return is_float(l) ? l : GDF_FLOAT64;
+  if (u == BLZ_IS_NULL || u == BLZ_IS_NOT_NULL) return GDF_INT8;
+  return l;
+}
+
+int binary_output_dtype(int l, int r, int op) {  // CalciteExpressionParsing.cpp:194-270
+  if (!is_signed_t(l) || !is_signed_t(r)) return GDF_invalid;
+  switch (op) {
+    case GDF_ADD: case GDF_SUB: case GDF_MUL: case GDF_DIV: case GDF_MOD:
+      if (is_float(l) || is_float(r)) {
+        if (is_float(l) && is_float(r)) return wid(l) >= wid(r) ? l : r;
+        return is_float(l) ? l : r;
+      }
+      return wid(l) >= wid(r) ? l : r;
+    case GDF_EQUAL: case GDF_NOT_EQUAL: case GDF_GREATER: case GDF_GREATER_EQUAL:
+    case GDF_LESS: case GDF_LESS_EQUAL:
+      return GDF_INT8;
+    case GDF_POW:
+      return (is_float(l) || is_float(r)) ? GDF_FLOAT64 : GDF_INT64;
+    case GDF_COALESCE:
+      return l;
+    default:
+      return GDF_invalid;
+  }
+}
+
+// interpreter_ops.cuh:156-175 (scale_to_64_bit_return_bytes), see scalar_mode.
+uint64_t scalar_bits(const gdf_scalar &s, int mode) {
+  switch (s.dtype) {
+    case GDF_INT8: return (uint64_t)(int64_t)s.data.si08;
+    case GDF_INT16: return (uint64_t)(int64_t)s.data.si16;
+    case GDF_INT32: case GDF_STRING_CATEGORY: return (uint64_t)(int64_t)s.data.si32;
+    case GDF_INT64: return (uint64_t)s.data.si64;
+    case GDF_DATE32: return (uint64_t)(int64_t)s.data.dt32;
+    case GDF_DATE64: return (uint64_t)s.data.dt64;
+    case GDF_TIMESTAMP: return (uint64_t)s.data.tmst;
+    case GDF_FLOAT32: {
+      if (mode == BLZ_SCALAR_SHIPPED) return (uint64_t)(int64_t)s.data.fp32;
+      double d = (double)s.data.fp32;
+      uint64_t b; std::memcpy(&b, &d, 8); return b;
+    }
+    case GDF_FLOAT64: {
+      if (mode == BLZ_SCALAR_SHIPPED) return (uint64_t)(int64_t)s.data.fp64;
+      uint64_t b; std::memcpy(&b, &s.data.fp64, 8); return b;
+    }
+    default: return 0;
+  }
+}
+
+namespace {
+
+int64_t host_f2i(double d) {
+  if (d != d) return 0;
+  if (d >= 9223372036854775807.0) return INT64_MAX;
+  if (d <= -9223372036854775808.0) return INT64_MIN;
+  return (int64_t)d;
+}
+uint64_t host_conv(uint64_t x, int cv) {
+  if (cv == CV_I2F) { double d = (double)(int64_t)x; uint64_t b; std::memcpy(&b, &d, 8); return b; }
+  if (cv == CV_F2I) { double d; std::memcpy(&d, &x, 8); return (uint64_t)host_f2i(d); }
+  return x;
+}
+
+int load_kind(int dtype) {
+  switch (dtype) {
+    case GDF_INT8: return LD_I8;
+    case GDF_INT16: return LD_I16;
+    case GDF_INT32: case GDF_DATE32: case GDF_STRING_CATEGORY: return LD_I32;
+    case GDF_INT64: case GDF_DATE64: case GDF_TIMESTAMP: return LD_I64;
+    case GDF_FLOAT32: return LD_F32;
+    case GDF_FLOAT64: return LD_F64;
+    case GDF_BOOL8: return LD_U8;
+    default: return LD_NONE;
+  }
+}
+
+// write_data (interpreter_ops.cuh:317-381): BOOL8 and others are not written.
+int store_kind_program(int dtype) {
+  switch (dtype) {
+    case GDF_INT8: return ST_I8;
+    case GDF_INT16: return ST_I16;
+    case GDF_INT32: case GDF_DATE32: case GDF_STRING_CATEGORY: return ST_I32;
+    case GDF_INT64: case GDF_DATE64: case GDF_TIMESTAMP: return ST_I64;
+    case GDF_FLOAT32: return ST_F32;
+    case GDF_FLOAT64: return ST_F64;
+    default: return ST_NONE;
+  }
+}
+// A gathered column keeps its bytes (materialize_column copies by width).
+int store_kind_alias(int dtype) {
+  if (dtype == GDF_BOOL8) return ST_I8;
+  return store_kind_program(dtype);
+}
+
+// ---- SSA graph ----
+struct Operand {
+  bool imm = false;
+  uint64_t bits = 0;
+  int node = -1;  // value node when !imm
+};
+struct Node {
+  int kind = 0;  // 0 input column (dev input = aux), 1 op, 2 constant zero
+  int aux = 0;
+  DevOp op{};
+  Operand a, b;
+  bool is_bool = false;
+  int program = 0;  // 0 predicate, 1 projection
+  // scheduling / allocation
+  int need = 0;
+  int order = -1;
+  int last_use = -1;
+  int slot = -1;
+};
+
+// Validity values: -1 all valid, -2 all null, >= 0 dynamic vnode id.
+constexpr int V_ALL = -1, V_NONE = -2;
+struct VNode {
+  int kind = 0;  // 0 raw bitmap of dev input aux, 1 op
+  int aux = 0;
+  int code = VOP_AND;
+  int a = 0, b = 0;
+  int program = 0;
+  int vslot = -1;
+  bool needed = false;
+  int last_use = -1;
+  int order = -1;
+};
+
+struct Builder {
+  std::vector<Node> nodes;
+  std::vector<VNode> vnodes;
+  std::vector<int> input_node;   // dev input -> node id
+  std::vector<int> input_vnode;  // dev input -> vnode id (raw bitmap)
+  int zero_node = -1;
+
+  int in_node(int c) { return input_node[c]; }
+  int zero() {
+    if (zero_node < 0) {
+      Node z; z.kind = 2; z.op.code = OP_ZERO;
+      nodes.push_back(z);
+      zero_node = (int)nodes.size() - 1;
+    }
+    return zero_node;
+  }
+  int vraw(int c) { return input_vnode[c]; }
+  int vcombine(int code, int a, int b, int program) {
+    if (code == VOP_AND) {
+      if (a == V_NONE || b == V_NONE) return V_NONE;
+      if (a == V_ALL) return b;
+      if (b == V_ALL) return a;
+      if (a == b) return a;
+    } else {
+      if (a == V_ALL || b == V_ALL) return V_ALL;
+      if (a == V_NONE) return b;
+      if (b == V_NONE) return a;
+      if (a == b) return a;
+    }
+    VNode v; v.kind = 1; v.code = code; v.a = a; v.b = b; v.program = program;
+    vnodes.push_back(v);
+    return (int)vnodes.size() - 1;
+  }
+};
+
+struct PlanResult {
+  std::vector<int> finals;   // value node per final output
+  std::vector<int> vfinals;  // validity value per final output
+};
+
+void fail(const std::string &m) { throw std::runtime_error(m); }
+
+// Builds the SSA graph of one reference-format plan.
+PlanResult build_plan(Builder &B, const PlanSpec &p, const std::vector<DevInputInfo> &dev_in,
+                      int program, int scalar_mode) {
+  const int n_ops = (int)p.left.size();
+  const int n_in = (int)p.in_dev.size();
+  if ((int)p.right.size() != n_ops || (int)p.outs.size() != n_ops || (int)p.binops.size() != n_ops ||
+      (int)p.unops.size() != n_ops || (int)p.lsc.size() != n_ops || (int)p.rsc.size() != n_ops)
+    fail("plan vectors have different lengths");
+  if (n_ops > kMaxOps) fail("plan has more than 256 operations");
+  auto slot_ok = [](int s) { return s >= 0 && s < kMaxSlots; };
+  for (int k = 0; k < n_ops; k++) {
+    if (!slot_ok(p.outs[k])) fail("output register outside [0, 64)");
+    if (p.left[k] >= kMaxSlots || p.right[k] >= kMaxSlots) fail("operand register outside [0, 64)");
+  }
+  for (int f : p.final_pos)
+    if (!slot_ok(f)) fail("final output register outside [0, 64)");
+
+  // Slot -> current value; plan input slot c holds dev input in_dev[c].
+  std::vector<int> cur(kMaxSlots, -1), vcur(kMaxSlots, V_NONE);
+  for (int c = 0; c < n_in; c++) {
+    const DevInputInfo &d = dev_in[p.in_dev[c]];
+    cur[c] = B.in_node(p.in_dev[c]);
+    // process_valids input rule (interpreter_ops.cuh:1312-1327): forced all-valid
+    // when there is no bitmap, null_count == 0, or the op with the column's index
+    // is IS_NULL / IS_NOT_NULL.
+    bool forced = d.valid == nullptr || d.null_count == 0 ||
+                  (c < n_ops && (p.unops[c] == BLZ_IS_NULL || p.unops[c] == BLZ_IS_NOT_NULL));
+    vcur[c] = forced ? V_ALL : B.vraw(p.in_dev[c]);
+  }
+  // Host type resolution (interpreter_ops.cuh:1095-1178).
+  std::map<int, int> out_map;
+  for (int k = 0; k < n_ops; k++) {
+    const int L = p.left[k], R = p.right[k];
+    int lt = GDF_invalid, rt = GDF_invalid;
+    uint64_t lbits = 0, rbits = 0;
+    if (L >= 0 && L < n_in) lt = dev_in[p.in_dev[L]].dtype;
+    else if (L == SCALAR_NULL_INDEX) lt = p.lsc[k].dtype;
+    else if (L == SCALAR_INDEX) {
+      lt = is_int_kernel(p.lsc[k].dtype) ? GDF_INT64 : GDF_FLOAT64;
+      lbits = scalar_bits(p.lsc[k], scalar_mode);
+    } else if (L >= n_in) lt = out_map.count(L) ? out_map[L] : GDF_invalid;
+    if (R >= 0 && R < n_in) rt = dev_in[p.in_dev[R]].dtype;
+    else if (R == SCALAR_NULL_INDEX) rt = p.rsc[k].dtype;
+    else if (R == SCALAR_INDEX) {
+      rt = is_int_kernel(p.rsc[k].dtype) ? GDF_INT64 : GDF_FLOAT64;
+      rbits = scalar_bits(p.rsc[k], scalar_mode);
+    } else if (R >= n_in) rt = out_map.count(R) ? out_map[R] : GDF_invalid;
+    const bool unary = R == BLZ_UNARY_INDEX;
+    const int t = unary ? unary_output_dtype(lt, p.unops[k]) : binary_output_dtype(lt, rt, p.binops[k]);
+    int ot = GDF_invalid;
+    if (is_signed_t(t) && !is_float(t)) ot = GDF_INT64;
+    else if (is_float(t)) ot = GDF_FLOAT64;
+    out_map[p.outs[k]] = ot;
+    const bool lk = is_float(lt), rk = is_float(rt), ok = is_float(ot);
+
+    // Operands: slot value, scalar immediate, or 0 (null scalar / undefined).
+    auto operand = [&](int idx, uint64_t bits) {
+      Operand o;
+      if (idx == SCALAR_INDEX) { o.imm = true; o.bits = bits; }
+      else if (idx >= 0) {
+        int v = cur[idx];
+        if (v < 0) { o.imm = true; o.bits = 0; }  // never-written slot reads 0
+        else o.node = v;
+      } else { o.imm = true; o.bits = 0; }
+      return o;
+    };
+    Node nd;
+    nd.kind = 1;
+    nd.program = program;
+    nd.a = operand(L, lbits);
+    if (!unary) nd.b = operand(R, rbits); else { nd.b.imm = true; nd.b.bits = 0; }
+    int ca = CV_NONE, cb = CV_NONE, natural_f = 0, code = OP_IDENT;
+    int vcol = 0xFF;
+    auto in_col_of = [&](int idx) { return (idx >= 0 && idx < n_in) ? p.in_dev[idx] : 0xFF; };
+    bool is_bool = false;
+    if (!unary) {
+      const bool fp = lk || rk;
+      auto arith = [&](int ci, int cf) {
+        code = fp ? cf : ci;
+        if (fp) { ca = lk ? CV_NONE : CV_I2F; cb = rk ? CV_NONE : CV_I2F; }
+        natural_f = fp;
+      };
+      auto cmp = [&](int ci, int cf) {
+        code = fp ? cf : ci;
+        if (fp) { ca = lk ? CV_NONE : CV_I2F; cb = rk ? CV_NONE : CV_I2F; }
+        natural_f = 0;
+        is_bool = true;
+      };
+      switch (p.binops[k]) {
+        case GDF_ADD: arith(OP_ADD_I, OP_ADD_F); break;
+        case GDF_SUB: arith(OP_SUB_I, OP_SUB_F); break;
+        case GDF_MUL: arith(OP_MUL_I, OP_MUL_F); break;
+        case GDF_DIV: case GDF_FLOOR_DIV: arith(OP_DIV_I, OP_DIV_F); break;
+        case GDF_MOD:
+          code = OP_MOD_I; ca = lk ? CV_F2I : CV_NONE; cb = rk ? CV_F2I : CV_NONE; natural_f = 0;
+          break;
+        case GDF_POW:
+          if (is_float(lt) || is_float(rt)) {
+            code = OP_POW_F; ca = lk ? CV_NONE : CV_I2F; cb = rk ? CV_NONE : CV_I2F; natural_f = 1;
+          } else {
+            code = OP_POW_I; natural_f = 0;
+          }
+          break;
+        case GDF_EQUAL: cmp(OP_EQ_I, OP_EQ_F); break;
+        case GDF_NOT_EQUAL: cmp(OP_NE_I, OP_NE_F); break;
+        case GDF_LESS: cmp(OP_LT_I, OP_LT_F); break;
+        case GDF_GREATER: cmp(OP_GT_I, OP_GT_F); break;
+        case GDF_LESS_EQUAL: cmp(OP_LE_I, OP_LE_F); break;
+        case GDF_GREATER_EQUAL: cmp(OP_GE_I, OP_GE_F); break;
+        case GDF_COALESCE:
+          code = OP_COALESCE;
+          ca = (lk && !ok) ? CV_F2I : (!lk && ok) ? CV_I2F : CV_NONE;
+          cb = (rk && !ok) ? CV_F2I : (!rk && ok) ? CV_I2F : CV_NONE;
+          natural_f = ok;
+          vcol = in_col_of(L);
+          break;
+        default:
+          fail("binary operator without a kernel branch (op " + std::to_string(p.binops[k]) + ")");
+      }
+    } else {
+      const int u = p.unops[k];
+      auto math = [&](int c) { code = c; ca = lk ? CV_NONE : CV_I2F; natural_f = 1; };
+      auto datepart = [&](int c32, int c64) {
+        code = lt == GDF_DATE32 ? c32 : c64; ca = lk ? CV_F2I : CV_NONE; natural_f = 0;
+      };
+      switch (u) {
+        case BLZ_FLOOR: math(OP_FLOOR); break;
+        case BLZ_CEIL: math(OP_CEIL); break;
+        case BLZ_SIN: math(OP_SIN); break;
+        case BLZ_COS: math(OP_COS); break;
+        case BLZ_ASIN: math(OP_ASIN); break;
+        case BLZ_ACOS: math(OP_ACOS); break;
+        case BLZ_TAN: math(OP_TAN); break;
+        case BLZ_COTAN: math(OP_COTAN); break;
+        case BLZ_ATAN: math(OP_ATAN); break;
+        case BLZ_ABS: math(OP_ABS); break;
+        case BLZ_LN: math(OP_LN); break;
+        case BLZ_LOG: math(OP_LOG10); break;
+        case BLZ_CAST_FLOAT64: math(OP_IDENT); break;
+        case BLZ_NOT: code = lk ? OP_NOT_F : OP_NOT_I; natural_f = 0; is_bool = true; break;
+        case BLZ_YEAR: datepart(OP_YEAR32, OP_YEAR64); break;
+        case BLZ_MONTH: datepart(OP_MONTH32, OP_MONTH64); break;
+        case BLZ_DAY: datepart(OP_DAY32, OP_DAY64); break;
+        case BLZ_HOUR: datepart(OP_ZERO, OP_HOUR64); break;
+        case BLZ_MINUTE: datepart(OP_ZERO, OP_MINUTE64); break;
+        case BLZ_SECOND: datepart(OP_ZERO, OP_SECOND64); break;
+        case BLZ_IS_NULL: code = OP_IS_NULL; vcol = in_col_of(L); natural_f = 0; is_bool = true; break;
+        case BLZ_IS_NOT_NULL: code = OP_IS_NOT_NULL; vcol = in_col_of(L); natural_f = 0; is_bool = true; break;
+        default: code = OP_IDENT; natural_f = lk; break;  // `computed = left_value`
+      }
+    }
+    int co = CV_NONE;
+    if (natural_f && !ok) co = CV_F2I;
+    else if (!natural_f && ok) co = CV_I2F;
+    if (co == CV_I2F) is_bool = false;
+    // Immediates are converted here once.
+    if (nd.a.imm && ca) { nd.a.bits = host_conv(nd.a.bits, ca); ca = CV_NONE; }
+    if (nd.b.imm && cb) { nd.b.bits = host_conv(nd.b.bits, cb); cb = CV_NONE; }
+    // AND of two 0/1 values (GDF_MUL from the `AND` token) is a bitwise and.
+    if (code == OP_MUL_I && ca == CV_NONE && cb == CV_NONE && co == CV_NONE && !nd.a.imm &&
+        !nd.b.imm && B.nodes[nd.a.node].is_bool && B.nodes[nd.b.node].is_bool) {
+      code = OP_AND_B;
+      is_bool = true;
+    }
+    nd.op.code = (uint8_t)code;
+    nd.op.conv = (uint8_t)(ca | (cb << 2) | (co << 4));
+    nd.op.vcol = (uint8_t)vcol;
+    nd.is_bool = is_bool;
+    B.nodes.push_back(nd);
+    cur[p.outs[k]] = (int)B.nodes.size() - 1;
+
+    // Validity (process_valids, interpreter_ops.cuh:507-571).
+    auto vop = [&](int idx) {
+      if (idx == SCALAR_INDEX) return V_ALL;
+      if (idx >= 0) return vcur[idx];
+      return V_NONE;  // null scalar (-3) and undefined operands
+    };
+    if (!unary) {
+      const int vc = p.binops[k] == GDF_COALESCE ? VOP_OR : VOP_AND;
+      vcur[p.outs[k]] = B.vcombine(vc, vop(L), vop(R), program);
+    } else {
+      vcur[p.outs[k]] = vop(L);
+    }
+  }
+  PlanResult r;
+  for (int f : p.final_pos) {
+    r.finals.push_back(cur[f] >= 0 ? cur[f] : B.zero());
+    r.vfinals.push_back(vcur[f]);
+  }
+  return r;
+}
+
+// Sethi-Ullman "need" of a value (inputs are resident: 0).
+int compute_need(Builder &B, int id) {
+  Node &n = B.nodes[id];
+  if (n.kind != 1) return 0;
+  if (n.need > 0) return n.need;
+  int na = n.a.imm ? 0 : compute_need(B, n.a.node);
+  int nb = n.b.imm ? 0 : compute_need(B, n.b.node);
+  n.need = std::max(1, na == nb ? na + (na > 0 ? 1 : 0) : std::max(na, nb));
+  return n.need;
+}
+
+void schedule(Builder &B, int id, std::vector<int> &order) {
+  Node &n = B.nodes[id];
+  if (n.order >= 0 || n.kind == 0) return;
+  if (n.kind == 2) {  // constant zero gets a real op so it owns a slot
+    n.order = (int)order.size();
+    order.push_back(id);
+    return;
+  }
+  int ca = n.a.imm ? -1 : n.a.node, cb = n.b.imm ? -1 : n.b.node;
+  int na = ca >= 0 ? compute_need(B, ca) : 0, nb = cb >= 0 ? compute_need(B, cb) : 0;
+  if (nb > na) std::swap(ca, cb);
+  if (ca >= 0) schedule(B, ca, order);
+  if (cb >= 0) schedule(B, cb, order);
+  n.order = (int)order.size();
+  order.push_back(id);
+}
+
+void mark_vneeded(Builder &B, int v) {
+  if (v < 0) return;
+  VNode &n = B.vnodes[v];
+  if (n.needed) return;
+  n.needed = true;
+  if (n.kind == 1) { mark_vneeded(B, n.a); mark_vneeded(B, n.b); }
+}
+
+template <typename T>
+void put(std::vector<uint8_t> &blob, size_t off, const T &v) {
+  std::memcpy(blob.data() + off, &v, sizeof(T));
+}
+
+}  // namespace
+
+Compiled compile_program(const CompileRequest &req) {
+  Compiled out;
+  const int n_dev = (int)req.inputs.size();
+  if (n_dev > kMaxSlots) fail("more than 64 input columns");
+  Builder B;
+  for (int c = 0; c < n_dev; c++) {
+    Node n; n.kind = 0; n.aux = c;
+    B.nodes.push_back(n);
+    B.input_node.push_back(c);
+    VNode v; v.kind = 0; v.aux = c;
+    B.vnodes.push_back(v);
+    B.input_vnode.push_back(c);
+  }
+  PlanResult pr, qr;
+  if (req.compact && req.pred) pr = build_plan(B, *req.pred, req.inputs, 0, req.scalar_mode);
+  if (req.proj) qr = build_plan(B, *req.proj, req.inputs, 1, req.scalar_mode);
+
+  // Roots.
+  int stencil_node = -1, stencil_v = V_ALL;
+  if (req.compact) {
+    if (req.pred) {
+      if (pr.finals.size() != 1) fail("predicate plan must have exactly one output");
+      stencil_node = pr.finals[0];
+      stencil_v = pr.vfinals[0];
+    } else {
+      if (req.bare_stencil_input < 0 || req.bare_stencil_input >= n_dev) fail("bad stencil column");
+      stencil_node = B.in_node(req.bare_stencil_input);
+      stencil_v = req.inputs[req.bare_stencil_input].valid ? B.vraw(req.bare_stencil_input) : V_ALL;
+    }
+  }
+  std::vector<int> out_node(req.outputs.size(), -1), out_v(req.outputs.size(), V_ALL);
+  for (size_t o = 0; o < req.outputs.size(); o++) {
+    const OutSpec &s = req.outputs[o];
+    if (s.kind == 0) {
+      if (!req.proj || s.index < 0 || s.index >= (int)qr.finals.size()) fail("bad program output");
+      out_node[o] = qr.finals[s.index];
+      out_v[o] = qr.vfinals[s.index];
+    } else {
+      if (s.index < 0 || s.index >= n_dev) fail("bad gathered column");
+      out_node[o] = B.in_node(s.index);
+      out_v[o] = req.inputs[s.index].valid ? B.vraw(s.index) : V_ALL;
+    }
+  }
+
+  // Schedule: predicate first, then the projection outputs in order.
+  std::vector<int> order_p, order_q;
+  if (stencil_node >= 0) schedule(B, stencil_node, order_p);
+  for (size_t o = 0; o < req.outputs.size(); o++)
+    if (req.outputs[o].kind == 0) schedule(B, out_node[o], order_q);
+
+  // Liveness: position in the global op sequence; roots live to their program's end.
+  std::vector<int> seq = order_p;
+  seq.insert(seq.end(), order_q.begin(), order_q.end());
+  const int end_p = (int)order_p.size() - 1, end_q = (int)seq.size() - 1;
+  for (int i = 0; i < (int)seq.size(); i++) {
+    Node &n = B.nodes[seq[i]];
+    n.order = i;
+    if (!n.a.imm && n.a.node >= 0) B.nodes[n.a.node].last_use = std::max(B.nodes[n.a.node].last_use, i);
+    if (!n.b.imm && n.b.node >= 0) B.nodes[n.b.node].last_use = std::max(B.nodes[n.b.node].last_use, i);
+  }
+  if (stencil_node >= 0 && B.nodes[stencil_node].kind != 0)
+    B.nodes[stencil_node].last_use = std::max(B.nodes[stencil_node].last_use, end_p + 1);
+  for (size_t o = 0; o < req.outputs.size(); o++)
+    if (req.outputs[o].kind == 0 && B.nodes[out_node[o]].kind != 0)
+      B.nodes[out_node[o]].last_use = std::max(B.nodes[out_node[o]].last_use, end_q + 1);
+
+  // Linear-scan slot allocation; inputs own slots [0, n_dev).
+  for (int c = 0; c < n_dev; c++) B.nodes[c].slot = c;
+  std::vector<int> slot_free_at(kMaxSlots * 4, -1);  // slot -> index after which it is free
+  int n_slots = n_dev;
+  std::vector<std::vector<int>> dying(seq.size() + 2);
+  for (int i = 0; i < (int)seq.size(); i++) {
+    Node &n = B.nodes[seq[i]];
+    // free operands whose last use is this op
+    for (const Operand *o : {&n.a, &n.b}) {
+      if (o->imm || o->node < 0) continue;
+      Node &c = B.nodes[o->node];
+      if (c.kind != 0 && c.last_use == i && c.slot >= 0) slot_free_at[c.slot] = -1;
+    }
+    int s = -1;
+    for (int k = n_dev; k < (int)slot_free_at.size(); k++)
+      if (slot_free_at[k] < 0) { s = k; break; }
+    if (s < 0) fail("out of device registers");
+    slot_free_at[s] = 1;
+    n.slot = s;
+    n_slots = std::max(n_slots, s + 1);
+    if (n.last_use < 0) slot_free_at[s] = -1;  // dead value (should not happen)
+    // a value with its last use beyond this op keeps the slot; values whose life
+    // ends at a program boundary are released there
+    if (i == end_p) {
+      for (int j = 0; j <= end_p; j++) {
+        Node &m = B.nodes[seq[j]];
+        if (m.last_use <= end_p + 1 && m.slot >= 0) slot_free_at[m.slot] = -1;
+      }
+    }
+  }
+  if (n_slots > kMaxSlots) fail("plan needs more than 64 device registers");
+
+  // Validity program: needed vnodes, in creation order (already topological).
+  mark_vneeded(B, stencil_v);
+  for (int v : out_v) mark_vneeded(B, v);
+  int vs_next = n_dev, n_vslots = n_dev;
+  std::vector<VOp> vops_p, vops_q;
+  for (int c = 0; c < n_dev; c++)
+    if (B.vnodes[c].needed) B.vnodes[c].vslot = c;
+  for (size_t v = n_dev; v < B.vnodes.size(); v++) {
+    VNode &n = B.vnodes[v];
+    if (!n.needed) continue;
+    n.vslot = vs_next++;
+    VOp op{(uint8_t)n.code, (uint8_t)n.vslot, (uint8_t)B.vnodes[n.a].vslot, (uint8_t)B.vnodes[n.b].vslot};
+    (n.program == 0 ? vops_p : vops_q).push_back(op);
+  }
+  n_vslots = vs_next;
+  if (n_vslots > kMaxVSlots) fail("plan needs more than 64 validity registers");
+  auto vsrc = [&](int v) { return v == V_ALL ? -1 : v == V_NONE ? -2 : B.vnodes[v].vslot; };
+
+  // Register-file shape.
+  const int need = std::max(n_slots, n_vslots);
+  int S = need <= 8 ? 8 : need <= 16 ? 16 : need <= 32 ? 32 : 64;
+  int R = S == 8 ? 2 : 1;
+  if (req.force_s > 0) { S = req.force_s; R = req.force_r > 0 ? req.force_r : 1; }
+  if (S < need) fail("requested register file too small for the plan");
+
+  // Device ops.
+  auto emit = [&](const std::vector<int> &ord, std::vector<DevOp> &dst) {
+    for (int id : ord) {
+      Node &n = B.nodes[id];
+      DevOp op = n.op;
+      if (n.kind == 2) { op = DevOp{}; op.code = OP_ZERO; op.flags = F_A_IMM | F_B_IMM; op.vcol = 0xFF; }
+      op.dst = (uint8_t)n.slot;
+      op.flags = 0;
+      if (n.kind == 1) {
+        if (n.a.imm) { op.flags |= F_A_IMM; op.imm_a = n.a.bits; } else op.a = (uint8_t)B.nodes[n.a.node].slot;
+        if (n.b.imm) { op.flags |= F_B_IMM; op.imm_b = n.b.bits; } else op.b = (uint8_t)B.nodes[n.b.node].slot;
+      } else {
+        op.flags = F_A_IMM | F_B_IMM;
+      }
+      dst.push_back(op);
+    }
+  };
+  std::vector<DevOp> ops_p, ops_q;
+  emit(order_p, ops_p);
+  emit(order_q, ops_q);
+
+  // Assemble the blob.
+  const int n_out = (int)req.outputs.size();
+  size_t off_in = (sizeof(ProgHeader) + 15) & ~size_t(15);
+  size_t off_out = off_in + sizeof(InCol) * n_dev;
+  off_out = (off_out + 15) & ~size_t(15);
+  size_t off_ops = off_out + sizeof(OutCol) * n_out;
+  off_ops = (off_ops + 15) & ~size_t(15);
+  size_t off_vops = off_ops + sizeof(DevOp) * (ops_p.size() + ops_q.size());
+  size_t total = off_vops + sizeof(VOp) * (vops_p.size() + vops_q.size());
+  total = (total + 15) & ~size_t(15);
+  if ((int)total > blob_capacity()) fail("plan too large for the on-chip program buffer");
+  out.blob.assign(total, 0);
+  ProgHeader h{};
+  h.n_rows = req.n_rows;
+  h.n_in = n_dev;
+  h.n_out = n_out;
+  h.n_ops_p = (int)ops_p.size();
+  h.n_ops_q = (int)ops_q.size();
+  h.n_vops_p = (int)vops_p.size();
+  h.n_vops_q = (int)vops_q.size();
+  h.n_vin = n_dev;
+  h.n_slots = n_slots;
+  h.n_vslots = n_vslots;
+  h.stencil_slot = stencil_node >= 0 ? B.nodes[stencil_node].slot : -1;
+  h.stencil_width = (req.compact && !req.pred) ? 8 : 1;
+  h.stencil_vsrc = vsrc(stencil_v);
+  int any_valid = 0;
+  for (auto &o : req.outputs) any_valid |= o.valid != nullptr;
+  h.any_valid_out = any_valid;
+  h.off_in = (int)off_in;
+  h.off_out = (int)off_out;
+  h.off_ops = (int)off_ops;
+  h.off_vops = (int)off_vops;
+  h.blob_bytes = (int)total;
+  put(out.blob, 0, h);
+  for (int c = 0; c < n_dev; c++) {
+    InCol ic{};
+    ic.data = req.inputs[c].data;
+    ic.valid = req.inputs[c].valid;
+    ic.load = load_kind(req.inputs[c].dtype);
+    ic.vslot = B.vnodes[c].needed ? c : -1;
+    put(out.blob, off_in + sizeof(InCol) * c, ic);
+  }
+  for (int o = 0; o < n_out; o++) {
+    const OutSpec &s = req.outputs[o];
+    OutCol oc{};
+    oc.data = s.data;
+    oc.valid = s.valid;
+    oc.store = s.kind == 0 ? store_kind_program(s.dtype) : store_kind_alias(s.dtype);
+    oc.slot = (int16_t)B.nodes[out_node[o]].slot;
+    oc.src_kind = (int16_t)s.kind;
+    oc.vsrc = vsrc(out_v[o]);
+    put(out.blob, off_out + sizeof(OutCol) * o, oc);
+  }
+  size_t p = off_ops;
+  for (auto &op : ops_p) { put(out.blob, p, op); p += sizeof(DevOp); }
+  for (auto &op : ops_q) { put(out.blob, p, op); p += sizeof(DevOp); }
+  p = off_vops;
+  for (auto &op : vops_p) { put(out.blob, p, op); p += sizeof(VOp); }
+  for (auto &op : vops_q) { put(out.blob, p, op); p += sizeof(VOp); }
+  out.S = S;
+  out.R = R;
+  out.n_slots = n_slots;
+  out.n_vslots = n_vslots;
+  out.n_ops = (int)(ops_p.size() + ops_q.size());
+  out.n_vops = (int)(vops_p.size() + vops_q.size());
+  return out;
+}
+
+}  // namespace blz

@@ -1,0 +1,126 @@
+/*
+ * program.h — the device program a reference plan is compiled into.
+ *
+ * The reference ships its plan to the GPU as ~16 parallel arrays and resolves the
+ * operand/result kernel types (int64 vs binary64) inside the kernel with three
+ * nested branches per op per row (interpreter_ops.cuh:473-505). Here the host
+ * resolves everything once (program.cpp) and the kernel receives one packed blob:
+ *
+ *   ProgHeader | InCol[n_in] | OutCol[n_out] | DevOp[n_ops_p + n_ops_q] | VOp[...]
+ *
+ * Each DevOp names a fully typed operation, its operand conversions and its result
+ * conversion, so the kernel's per-op dispatch is one wave-uniform switch. Validity
+ * is a separate, constant-folded program over 64-row words (VOp).
+ *
+ * Shared by host C++ and device HIP code: plain structs, fixed widths.
+ */
+#ifndef BLZ_PROGRAM_H_
+#define BLZ_PROGRAM_H_
+
+#include <stdint.h>
+
+namespace blz {
+
+// Device register file limits. Slots are 8-byte (int64 / binary64 bits).
+constexpr int kMaxSlots = 64;   // reference limit (interpreter_cpp.cu:86-217 handles <= 64)
+constexpr int kMaxVSlots = 64;
+constexpr int kMaxOps = 256;    // per program
+
+// Input load kinds: how a column widens into an 8-byte slot (interpreter_ops.cuh:386-452).
+enum LoadKind : int32_t {
+  LD_NONE = 0,  // dtype the reference never reads (slot holds 0)
+  LD_I8, LD_I16, LD_I32, LD_I64, LD_F32, LD_F64,
+  LD_U8,        // BOOL8 raw byte, used only for gathered (aliased) columns
+};
+
+// Output store kinds: narrowing writes (interpreter_ops.cuh:317-381).
+enum StoreKind : int32_t {
+  ST_NONE = 0,  // dtype the reference does not write
+  ST_I8, ST_I16, ST_I32, ST_I64, ST_F32, ST_F64,
+};
+
+// Operand / result conversions.
+enum Conv : uint8_t { CV_NONE = 0, CV_I2F = 1, CV_F2I = 2 };
+
+// Fully typed device opcodes.
+enum DevOpCode : uint8_t {
+  // binary, computed in int64 (_I) or binary64 (_F)
+  OP_ADD_I, OP_ADD_F, OP_SUB_I, OP_SUB_F, OP_MUL_I, OP_MUL_F, OP_DIV_I, OP_DIV_F,
+  OP_MOD_I, OP_POW_I, OP_POW_F,
+  OP_EQ_I, OP_EQ_F, OP_NE_I, OP_NE_F, OP_LT_I, OP_LT_F, OP_GT_I, OP_GT_F,
+  OP_LE_I, OP_LE_F, OP_GE_I, OP_GE_F,
+  OP_COALESCE,
+  // boolean AND of two operands known to be 0/1 (an `AND` = GDF_MUL whose operands
+  // are both comparison results: the product equals the bitwise and)
+  OP_AND_B,
+  // unary
+  OP_FLOOR, OP_CEIL, OP_SIN, OP_COS, OP_ASIN, OP_ACOS, OP_TAN, OP_COTAN, OP_ATAN,
+  OP_ABS, OP_LN, OP_LOG10, OP_NOT_I, OP_NOT_F,
+  OP_YEAR32, OP_MONTH32, OP_DAY32, OP_YEAR64, OP_MONTH64, OP_DAY64,
+  OP_HOUR64, OP_MINUTE64, OP_SECOND64, OP_ZERO,
+  OP_IS_NULL, OP_IS_NOT_NULL,
+  OP_IDENT,
+  OP_COUNT_
+};
+
+// One device op: 8 bytes of control + two 8-byte immediates.
+struct DevOp {
+  uint8_t code;   // DevOpCode
+  uint8_t dst;    // destination slot
+  uint8_t a, b;   // operand slots (ignored when the matching *_IMM flag is set)
+  uint8_t flags;  // bit0 A is immediate, bit1 B is immediate
+  uint8_t conv;   // bits 0-1 A conversion, 2-3 B conversion, 4-5 result conversion
+  uint8_t vcol;   // input column whose raw validity COALESCE / IS_[NOT_]NULL read; 0xFF = none
+  uint8_t pad;
+  uint64_t imm_a, imm_b;
+};
+static_assert(sizeof(DevOp) == 24, "DevOp layout");
+
+constexpr uint8_t F_A_IMM = 1, F_B_IMM = 2;
+
+// Validity op over 64-row words.
+enum VOpCode : uint8_t { VOP_AND = 0, VOP_OR = 1 };
+struct VOp {
+  uint8_t code, dst, a, b;
+};
+
+// Input column descriptor.
+struct InCol {
+  const void *data;
+  const uint8_t *valid;  // raw bitmap (may be null = all valid)
+  int32_t load;          // LoadKind
+  int32_t vslot;         // validity slot this column's words load into, or -1 (forced / unused)
+};
+
+// Output descriptor. `src_kind`: 0 = program slot (of the program that owns the
+// output), 1 = raw input column (a gathered alias).
+struct OutCol {
+  void *data;
+  uint8_t *valid;        // null = no validity written
+  int32_t store;         // StoreKind
+  int16_t slot;          // data slot (src_kind 0) or input column (src_kind 1)
+  int16_t src_kind;
+  int32_t vsrc;          // >= 0 validity slot; -1 = constant all-valid; -2 = constant all-null;
+                         // -3 = raw validity of input column `slot` (gathered alias)
+  int32_t pad;
+};
+
+struct ProgHeader {
+  int64_t n_rows;
+  int32_t n_in, n_out;
+  int32_t n_ops_p, n_ops_q;      // predicate ops (compaction mode), projection ops
+  int32_t n_vops_p, n_vops_q;
+  int32_t n_vin;                 // validity slots loaded from input bitmaps [0, n_vin)
+  int32_t n_slots, n_vslots;     // register-file sizes used
+  int32_t stencil_slot;          // compaction: slot holding the predicate value
+  int32_t stencil_width;         // compaction: bytes of the predicate value that decide "non-zero"
+  int32_t stencil_vsrc;          // compaction: validity of the predicate (same codes as OutCol::vsrc)
+  int32_t any_valid_out;         // some output has validity
+  int32_t pad;
+  // offsets (bytes from the blob start)
+  int32_t off_in, off_out, off_ops, off_vops;
+  int32_t blob_bytes, pad2;
+};
+
+}  // namespace blz
+#endif

@@ -1,0 +1,68 @@
+/*
+ * program_compiler.h — host compiler from reference-format plans to the device
+ * program blob of program.h.
+ */
+#ifndef BLZ_PROGRAM_COMPILER_H_
+#define BLZ_PROGRAM_COMPILER_H_
+
+#include <string>
+#include <vector>
+
+#include "../../include/blz/gdf_types.h"
+#include "../../include/blz/blz_ral.h"
+#include "program.h"
+
+namespace blz {
+
+// One plan in the reference's perform_operation format. Plan register c < n_in
+// holds device input `in_dev[c]`.
+struct PlanSpec {
+  std::vector<int> in_dev;
+  std::vector<int16_t> left, right, outs, final_pos;
+  std::vector<int32_t> binops, unops;
+  std::vector<gdf_scalar> lsc, rsc;
+};
+
+// What the compiler needs to know of a device input column.
+struct DevInputInfo {
+  void *data;
+  const uint8_t *valid;
+  int dtype;
+  int null_count;
+};
+
+struct OutSpec {
+  void *data;
+  uint8_t *valid;
+  int dtype;
+  int kind;   // 0: final output `index` of the projection plan; 1: gathered device input `index`
+  int index;
+};
+
+struct CompileRequest {
+  int64_t n_rows = 0;
+  std::vector<DevInputInfo> inputs;
+  bool compact = false;
+  const PlanSpec *pred = nullptr;   // compaction predicate; null -> bare-column stencil
+  int bare_stencil_input = -1;
+  const PlanSpec *proj = nullptr;
+  std::vector<OutSpec> outputs;
+  int scalar_mode = BLZ_SCALAR_INTENDED;
+  int force_s = 0, force_r = 0;     // register-file shape override (autotuner)
+};
+
+struct Compiled {
+  std::vector<uint8_t> blob;
+  int S = 8, R = 1;
+  int n_slots = 0, n_vslots = 0, n_ops = 0, n_vops = 0;
+};
+
+// Throws std::runtime_error with a message on an unsupported or malformed plan.
+Compiled compile_program(const CompileRequest &req);
+
+int unary_output_dtype(int l, int u);
+int binary_output_dtype(int l, int r, int op);
+uint64_t scalar_bits(const gdf_scalar &s, int mode);
+
+}  // namespace blz
+#endif

@@ -1,0 +1,186 @@
+/*
+ * blz_ral.h — C ABI of the MI355X expression-evaluation / filter / gather path.
+ *
+ * Plain C, plain pointers and sizes. Every device pointer is HBM memory of the
+ * current HIP device. Every entry point returns BLZ_OK (0) or a negative status and
+ * writes a message into `err` (when err != NULL, err_len > 0); nothing throws.
+ * `stream` is a hipStream_t (NULL = the default stream). Calls enqueue their work
+ * on `stream` and return when it is complete (the reference's calls are blocking:
+ * interpreter_cpp.cu:282-285), except blz_program_launch, which only enqueues.
+ *
+ * The C++ operator API of the reference (perform_operation, evaluate_expression,
+ * process_filter, execute_project_plan, materialize_column, gdf_sequence) is
+ * provided over this ABI by include/blz/ral.hpp.
+ */
+#ifndef BLZ_RAL_H_
+#define BLZ_RAL_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "gdf_types.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BLZ_OK 0
+#define BLZ_ERR_INVALID (-1)     /* malformed plan / arguments (std::runtime_error in the reference) */
+#define BLZ_ERR_HIP (-2)         /* HIP runtime error (CheckCudaErrors in Utils.cuh:25-55)          */
+#define BLZ_ERR_OOM (-3)         /* allocation failed or over budget (cuDF::Allocator OutOfMemory)   */
+#define BLZ_ERR_UNSUPPORTED (-4) /* operator / dtype outside this path (e.g. string literals)        */
+
+/* Float-literal widening (interpreter_ops.cuh:156-175). INTENDED keeps the literal's
+ * binary64 value; SHIPPED reproduces the shipped `static_cast<int64_t>(value)` whose
+ * bits the kernel then reads as a double. */
+#define BLZ_SCALAR_INTENDED 0
+#define BLZ_SCALAR_SHIPPED 1
+
+typedef void *blz_stream_t;
+
+/* ---- plan level ------------------------------------------------------------ */
+
+/* perform_operation (src/Interpreter/interpreter_cpp.h:25-37, implementation
+ * interpreter_cpp.cu:219-286): evaluate a register-file plan of n_operations ops
+ * over input_columns[0..n_inputs) (all input_columns[0]->size rows) into the
+ * caller-allocated output_columns. Register c < n_inputs holds input c; register
+ * -2 = scalar, -3 = null scalar, right register -1 = unary op. Writes data and, for
+ * outputs with a validity buffer, 64-row validity words. out_null_counts[k] (if
+ * non-NULL) receives output k's null count; output_columns[k]->null_count is left
+ * unchanged, as in the reference (interpreter_cpp.cu:280). Plans use at most 64
+ * registers (interpreter_cpp.cu:86-217). */
+int blz_perform_operation(gdf_column *const *output_columns, int n_outputs,
+                          gdf_column *const *input_columns, int n_inputs,
+                          const column_index_type *left_inputs,
+                          const column_index_type *right_inputs,
+                          const column_index_type *outputs, int n_operations,
+                          const column_index_type *final_output_positions,
+                          const gdf_binary_operator *operators,
+                          const gdf_unary_operator *unary_operators,
+                          const gdf_scalar *left_scalars, const gdf_scalar *right_scalars,
+                          int scalar_mode, int32_t *out_null_counts, blz_stream_t stream,
+                          char *err, size_t err_len);
+
+/* ---- expression level (host planner + kernels) ------------------------------- */
+
+/* evaluate_expression (src/LogicalFilter.h:20-23, LogicalFilter.cpp:271-371):
+ * plan `expression` (Calcite text, e.g. ">($1, 5)") over the frame's columns and
+ * evaluate it into the caller-allocated `output` (its dtype decides the narrowing).
+ * A bare "$k" expression copies column k into `output` (LogicalFilter.cpp:281-287;
+ * output must then have column k's dtype). *out_null_count (if non-NULL) receives
+ * the output's null count; output->null_count is not written (LogicalFilter.cpp:368). */
+int blz_evaluate_expression(gdf_column *const *frame, int n_frame, const char *expression,
+                            gdf_column *output, int scalar_mode, int32_t *out_null_count,
+                            blz_stream_t stream, char *err, size_t err_len);
+
+/* process_filter (src/CalciteInterpreter.cpp:509-573): `query_part` is a
+ * "LogicalFilter(condition=[...])" line (or the bare condition). Keeps the rows
+ * whose predicate is non-zero and valid, in row order, for EVERY frame column.
+ * out_columns[0..n_frame) are filled with callee-allocated columns (free each with
+ * blz_free_column): dtype of the input, size = rows kept, validity iff the input
+ * has validity, null_count set (ColumnManipulation.cu:105-113). *n_kept = rows kept. */
+int blz_process_filter(gdf_column *const *frame, int n_frame, const char *query_part,
+                       gdf_column *out_columns, int64_t *n_kept, int scalar_mode,
+                       blz_stream_t stream, char *err, size_t err_len);
+
+/* execute_project_plan (src/CalciteInterpreter.h:61-63, CalciteInterpreter.cpp:123-336):
+ * `query_part` is "LogicalProject(name=[expr], ...)". Evaluated expressions share
+ * one plan and one launch; "$k" outputs alias frame column k (no copy, is_alias[i]
+ * = 1; do not free those). Writes *n_out columns (<= max_out) with null counts set
+ * (CalciteInterpreter.cpp:332-335). */
+int blz_execute_project(gdf_column *const *frame, int n_frame, const char *query_part,
+                        gdf_column *out_columns, int32_t *is_alias, int max_out, int *n_out,
+                        int scalar_mode, blz_stream_t stream, char *err, size_t err_len);
+
+/* LogicalProject over LogicalFilter in ONE pass over the frame: the result equals
+ * blz_process_filter followed by blz_execute_project on its output (projection of
+ * row-local expressions commutes with the filter). Every output is callee-allocated
+ * (aliases are gathered copies, is_alias[i] = 0). */
+int blz_filter_project(gdf_column *const *frame, int n_frame, const char *filter_query_part,
+                       const char *project_query_part, gdf_column *out_columns, int max_out,
+                       int *n_out, int64_t *n_kept, int scalar_mode, blz_stream_t stream,
+                       char *err, size_t err_len);
+
+/* ---- gather / mask / sequence ------------------------------------------------ */
+
+/* materialize_column (src/ColumnManipulation.cuh:16-18, ColumnManipulation.cu:83-150):
+ * output[i] = input[row_indices[i]] for row_indices[i] >= 0; output validity bit i =
+ * input validity bit row_indices[i] (when output->valid != NULL); sets
+ * output->null_count. row_indices is an int32 column; output->size must equal
+ * row_indices->size. */
+int blz_materialize_column(const gdf_column *input, gdf_column *output,
+                           const gdf_column *row_indices, blz_stream_t stream, char *err,
+                           size_t err_len);
+
+/* gdf_sequence (src/Utils.cuh:187-190, Utils.cu:4-12): data[i] = init + i * step. */
+int blz_gdf_sequence(int32_t *data, size_t size, int32_t init_val, int32_t step,
+                     blz_stream_t stream, char *err, size_t err_len);
+
+/* cudf::apply_boolean_mask as process_filter uses it (CalciteInterpreter.cpp:543):
+ * keeps input rows whose stencil value is non-zero and valid, in order, into a
+ * callee-allocated `out` (free with blz_free_column). */
+int blz_apply_boolean_mask(const gdf_column *input, const gdf_column *stencil, gdf_column *out,
+                           blz_stream_t stream, char *err, size_t err_len);
+
+/* gdf_column_cpp::update_null_count (src/GDFColumn.cu:519-528). */
+int blz_update_null_count(gdf_column *column, blz_stream_t stream, char *err, size_t err_len);
+
+/* ---- prepared programs (no reference counterpart: plan once, launch many) ---- */
+
+typedef struct blz_program_s *blz_program;
+
+/* Compiles a fused filter+project (filter_query_part may be NULL for a pure
+ * projection; project_query_part NULL keeps every frame column) against fixed
+ * device buffers the program allocates and owns: out_columns[0..*n_out) receive
+ * them (capacity: every frame row; sizes valid after blz_program_result).
+ * s / r (0 = automatic) force the register-file shape (8,1) (8,2) (16,1) (32,1) (64,1). */
+int blz_program_create(blz_program *prog, gdf_column *const *frame, int n_frame,
+                       const char *filter_query_part, const char *project_query_part,
+                       gdf_column *out_columns, int max_out, int *n_out, int scalar_mode, int s,
+                       int r, char *err, size_t err_len);
+/* Enqueues one execution on `stream` (no allocation, no synchronisation). */
+int blz_program_launch(blz_program prog, blz_stream_t stream, char *err, size_t err_len);
+/* Waits for the last launch and reads back rows kept and per-output null counts. */
+int blz_program_result(blz_program prog, blz_stream_t stream, int64_t *n_kept,
+                       int32_t *null_counts, char *err, size_t err_len);
+/* Launches `iters` times with a HIP event pair around each kernel on `stream`;
+ * returns the mean kernel time in milliseconds. */
+int blz_program_time(blz_program prog, blz_stream_t stream, int iters, double *mean_ms,
+                     char *err, size_t err_len);
+/* Register-file shape and sizes chosen for the program. */
+int blz_program_info(blz_program prog, int *s, int *r, int *n_ops, int *n_vops, int *n_slots,
+                     int *grid);
+int blz_program_destroy(blz_program prog);
+
+/* ---- memory / streams (HIP plumbing for hosts without a HIP toolchain) ------- */
+
+int blz_device_malloc(void **ptr, size_t bytes);
+int blz_device_free(void *ptr);
+int blz_memcpy_htod(void *dst, const void *src, size_t bytes, blz_stream_t stream);
+int blz_memcpy_dtoh(void *dst, const void *src, size_t bytes, blz_stream_t stream);
+int blz_memset_device(void *dst, int value, size_t bytes, blz_stream_t stream);
+int blz_stream_create(blz_stream_t *stream);
+int blz_stream_synchronize(blz_stream_t stream);
+int blz_stream_destroy(blz_stream_t stream);
+/* Frees data and validity of a callee-allocated column and zeroes the struct. */
+int blz_free_column(gdf_column *column);
+int blz_device_count(void);
+const char *blz_version(void);
+
+/* ---- planner only (host; no device work) -------------------------------------- */
+
+/* Plans `expression` as evaluate_expression does (LogicalFilter.cpp:271-371) for a
+ * frame of the given dtypes: fills up to `cap` ops and returns the op count in
+ * *n_ops, the used frame columns (slot order) in in_cols / *n_in, the final
+ * register in *final_pos. */
+int blz_plan_expression(const int32_t *frame_dtypes, int n_frame, const char *expression,
+                        int cap, column_index_type *left, column_index_type *right,
+                        column_index_type *outputs, int32_t *binops, int32_t *unops,
+                        gdf_scalar *lscalars, gdf_scalar *rscalars, int *n_ops,
+                        int32_t *in_cols, int *n_in, column_index_type *final_pos,
+                        char *err, size_t err_len);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* BLZ_RAL_H_ */

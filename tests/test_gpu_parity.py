@@ -1,0 +1,222 @@
+"""GPU parity: the HIP path through the C ABI against the oracle, bit-exact.
+
+Every test here runs on an MI355X (marker `gpu`). Inputs are seeded; sizes keep
+the oracle within seconds.
+"""
+import numpy as np
+import pytest
+
+from oracle import planner as P
+from oracle import pyoracle as O
+from tests import exprgen
+from tests.helpers import (C1_FILTER, C1_PROJECT, assert_column_equal, c1_frame, gen_column,
+                           to_device)
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def B():
+    import blz_ral
+    if blz_ral.device_count() < 1:
+        pytest.fail("no HIP device visible")
+    return blz_ral
+
+
+def _print_case(**kw):
+    print("CASE", " ".join(f"{k}={v}" for k, v in kw.items()), flush=True)
+
+
+@pytest.mark.parametrize("n", [0, 1, 63, 64, 65, 1000, 4097, 1_000_000])
+def test_c1_filter_project(B, n):
+    _print_case(test="c1", n=n)
+    host = c1_frame(n, seed=11)
+    dev = [to_device(B, h) for h in host]
+    outs, kept = B.filter_project(dev, C1_FILTER, C1_PROJECT)
+    ref = O.execute_project(O.process_filter(host, C1_FILTER), C1_PROJECT)
+    assert kept == len(ref[0])
+    assert len(outs) == 2
+    for g, r in zip(outs, ref):
+        assert_column_equal(g, r, what="c1")
+
+
+def test_c1_shipped_scalar_mode(B):
+    n = 200_000
+    _print_case(test="c1_shipped", n=n)
+    host = c1_frame(n, seed=12)
+    dev = [to_device(B, h) for h in host]
+    outs, kept = B.filter_project(dev, C1_FILTER, C1_PROJECT, scalar_mode=B.SCALAR_SHIPPED)
+    ref = O.execute_project(O.process_filter(host, C1_FILTER, 1), C1_PROJECT, 1)
+    assert kept == len(ref[0])
+    for g, r in zip(outs, ref):
+        assert_column_equal(g, r, what="c1 shipped")
+
+
+def test_process_filter_then_project_equals_fused(B):
+    n = 300_001
+    _print_case(test="filter_then_project", n=n)
+    host = c1_frame(n, seed=13)
+    dev = [to_device(B, h) for h in host]
+    filtered = B.process_filter(dev, C1_FILTER)
+    ref_f = O.process_filter(host, C1_FILTER)
+    for g, r in zip(filtered, ref_f):
+        assert_column_equal(g, r, what="filter")
+    proj = B.execute_project(filtered, C1_PROJECT)
+    fused, _ = B.filter_project(dev, C1_FILTER, C1_PROJECT)
+    for g, f in zip(proj, fused):
+        assert g.data().tobytes() == f.data().tobytes()
+
+
+def test_interpreter_golden_test00(B):
+    """interpreter_tests.cu TEST_00 (:68-156): ($0+$1)*$2+$1 and $1+2 on int32."""
+    _print_case(test="golden_test00")
+    i = np.arange(10)
+    ins = [B.DeviceColumn.from_numpy((i * 2).astype(np.int32)),
+           B.DeviceColumn.from_numpy((i * 10).astype(np.int32)),
+           B.DeviceColumn.from_numpy((i * 20).astype(np.int32))]
+    outs = [B.DeviceColumn.from_numpy(np.zeros(10, np.int32)) for _ in range(2)]
+    junk = B.make_scalar(B.GDF_INT32, 0)
+    two = B.make_scalar(B.GDF_INT32, 2)
+    B.perform_operation(outs, ins, [0, 5, 5, 1], [1, 2, 1, -2], [5, 5, 3, 4], [3, 4],
+                        [B.GDF_ADD, B.GDF_MUL, B.GDF_ADD, B.GDF_ADD], [B.BLZ_INVALID_UNARY] * 4,
+                        [junk] * 4, [junk, junk, junk, two])
+    assert outs[0].data().tolist() == [0, 250, 980, 2190, 3880, 6050, 8700, 11830, 15440, 19530]
+    assert outs[1].data().tolist() == [2, 12, 22, 32, 42, 52, 62, 72, 82, 92]
+
+
+def _logical_filter_frame():
+    n = 32
+    i = np.arange(n)
+    x = np.where(i % 2 == 0, 1, i).astype(np.int8)
+    return [O.HostColumn(x, P.GDF_INT8), O.HostColumn(i.astype(np.int8), P.GDF_INT8),
+            O.HostColumn((i % 2).astype(np.int8), P.GDF_INT8),
+            O.HostColumn((i + 1).astype(np.int8), P.GDF_INT8)]
+
+
+LOGICAL_FILTER_EXPRS = [
+    ">($1, 5)", "+(*($1, $0), $2)", "=(=($1, $0), $2)", "*($0, $0)", "=(*($0, $0), 1))",
+    "=($1, $2)", "AND($0, $1)", "AND(=(*($0, $0), 1), =($1, 2))",
+    "AND(=(*($0, $0), 1), =($1, 2), >($2, 0))", "AND(=($0, 1), >($1, 5), >($2, 0))",
+    "AND(=($0, 1), >($1, 5), >($3, 10))", "AND(=($0, 1), >($1, 5))",
+    "AND(>($1, 5), >($2, 0)))", "AND(>($1, 50), >($2, 0)))", "AND(>($1, 5), >($2, 100)))",
+    "AND(=($0, 1), >($1, 5), =($2, 0), >($3, 5))", "AND(=($0, 1), >($1, 5), =($2, 1), >($3, 5))",
+]
+
+
+@pytest.mark.parametrize("expr", LOGICAL_FILTER_EXPRS)
+def test_logical_filter_expressions(B, expr):
+    """logical_filter-test.cu:93-366: evaluate_expression into an INT8 column."""
+    _print_case(test="logical_filter", expr=expr)
+    host = _logical_filter_frame()
+    dev = [to_device(B, h) for h in host]
+    out = B.DeviceColumn(B.GDF_INT8, 32, with_valid=True)
+    B.evaluate_expression(dev, expr, out)
+    ref = O.evaluate_expression(host, expr, P.GDF_INT8, True)
+    assert_column_equal(out, ref, what=expr)
+
+
+@pytest.mark.parametrize("seed", range(40))
+def test_random_expressions_with_nulls(B, seed):
+    rng = np.random.default_rng(1000 + seed)
+    k = int(rng.integers(2, 7))
+    dts = exprgen.rand_frame_dtypes(rng, k)
+    n = int(rng.choice([1, 17, 64, 129, 5000, 70_001]))
+    null_rates = [float(rng.choice([0.0, 0.0, 0.05, 0.5])) for _ in range(k)]
+    host = [gen_column(rng, dt, n, nr) for dt, nr in zip(dts, null_rates)]
+    nullable = [i for i, nr in enumerate(null_rates) if nr > 0]
+    expr = exprgen.rand_expr(rng, dts, int(rng.integers(1, 5)), allow_pow=False, nullable=nullable)
+    out_dt = int(rng.choice([P.GDF_INT8, P.GDF_INT16, P.GDF_INT32, P.GDF_INT64, P.GDF_FLOAT32,
+                             P.GDF_FLOAT64]))
+    if expr.startswith("$"):  # a bare column is cloned, so the output takes its dtype
+        out_dt = dts[P.get_index(expr)]
+    _print_case(test="random_expr", seed=seed, n=n, expr=expr, out=out_dt)
+    dev = [to_device(B, h) for h in host]
+    out = B.DeviceColumn(out_dt, n, with_valid=True)
+    try:
+        ref = O.evaluate_expression(host, expr, out_dt, True)
+    except (P.PlanError, ValueError) as e:
+        with pytest.raises(B.BlzError):
+            B.evaluate_expression(dev, expr, out)
+        return
+    nulls = B.evaluate_expression(dev, expr, out)
+    assert_column_equal(out, ref, check_valid=False, what=expr)
+    gv = out.valid()
+    assert np.array_equal(gv, ref.valid), f"validity differs: {expr}"
+    assert nulls == ref.null_count
+
+
+@pytest.mark.parametrize("seed", range(12))
+def test_random_filter_project_with_nulls(B, seed):
+    rng = np.random.default_rng(2000 + seed)
+    k = int(rng.integers(2, 6))
+    dts = exprgen.rand_frame_dtypes(rng, k)
+    n = int(rng.choice([100, 4096, 33_333, 200_000]))
+    null_rates = [float(rng.choice([0.0, 0.1])) for _ in range(k)]
+    host = [gen_column(rng, dt, n, nr) for dt, nr in zip(dts, null_rates)]
+    nullable = [i for i, nr in enumerate(null_rates) if nr > 0]
+    cond = f"<({exprgen.rand_expr(rng, dts, 2, False, nullable)}, {int(rng.integers(-5, 6))})"
+    exprs = [exprgen.rand_expr(rng, dts, 2, False, nullable) for _ in range(int(rng.integers(1, 4)))]
+    items = [f"E{i}=[{e}]" for i, e in enumerate(exprs)] + [f"A=[${int(rng.integers(0, k))}]"]
+    proj = "LogicalProject(" + ", ".join(items) + ")"
+    filt = f"LogicalFilter(condition=[{cond}])"
+    _print_case(test="random_filter_project", seed=seed, n=n, filt=filt, proj=proj)
+    dev = [to_device(B, h) for h in host]
+    try:
+        ref = O.execute_project(O.process_filter(host, filt), proj)
+    except (P.PlanError, ValueError):
+        with pytest.raises(B.BlzError):
+            B.filter_project(dev, filt, proj)
+        return
+    outs, kept = B.filter_project(dev, filt, proj)
+    assert kept == len(ref[0])
+    for g, r in zip(outs, ref):
+        assert_column_equal(g, r, what=f"{filt} {proj}")
+
+
+@pytest.mark.parametrize("dtype", [P.GDF_INT8, P.GDF_INT16, P.GDF_INT32, P.GDF_INT64,
+                                   P.GDF_FLOAT32, P.GDF_FLOAT64])
+def test_materialize_column(B, dtype):
+    rng = np.random.default_rng(dtype)
+    n, m = 10_000, 7_777
+    _print_case(test="gather", dtype=dtype)
+    col = gen_column(rng, dtype, n, 0.2)
+    idx = rng.integers(-3, n, m).astype(np.int32)
+    ref = O.gather(col, idx, True)
+    dcol = to_device(B, col)
+    didx = B.DeviceColumn.from_numpy(idx, B.GDF_INT32)
+    out = B.DeviceColumn(dtype, m, with_valid=True)
+    B.materialize_column(dcol, out, didx)
+    # rows with a negative index are left untouched by both: compare the rest
+    keep = idx >= 0
+    assert np.array_equal(out.data()[keep], ref.data[keep])
+    assert np.array_equal(out.valid(), ref.valid)
+    assert out.null_count == ref.null_count
+
+
+def test_sequence_and_boolean_mask(B):
+    n = 100_000
+    _print_case(test="sequence_mask")
+    seq = B.DeviceColumn(B.GDF_INT32, n)
+    B.gdf_sequence(seq, 0, 1)
+    assert np.array_equal(seq.data(), np.arange(n, dtype=np.int32))
+    rng = np.random.default_rng(5)
+    st = O.HostColumn((rng.random(n) < 0.3).astype(np.int8) * rng.integers(1, 3, n).astype(np.int8),
+                      P.GDF_INT8, rng.random(n) > 0.1)
+    got = B.apply_boolean_mask(seq, to_device(B, st))
+    assert np.array_equal(got.data(), O.stencil_select(st))
+
+
+def test_program_matches_one_shot(B):
+    n = 500_000
+    _print_case(test="program", n=n)
+    host = c1_frame(n, seed=14)
+    dev = [to_device(B, h) for h in host]
+    prog = B.Program(dev, C1_FILTER, C1_PROJECT)
+    for _ in range(3):
+        prog.launch()
+        kept, nulls = prog.result()
+    ref = O.execute_project(O.process_filter(host, C1_FILTER), C1_PROJECT)
+    assert kept == len(ref[0])
+    for g, r in zip(prog.outputs, ref):
+        assert g.data().tobytes() == r.data.tobytes()
+    prog.close()
