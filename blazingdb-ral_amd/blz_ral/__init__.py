@@ -98,7 +98,8 @@ def lib():
             "blz_apply_boolean_mask": [colp, colp, colp, P, ERR, SZ],
             "blz_update_null_count": [colp, P, ERR, SZ],
             "blz_program_create": [C.POINTER(P), colpp, I, C.c_char_p, C.c_char_p, colp, I,
-                                   C.POINTER(C.c_int), I, I, I, ERR, SZ],
+                                   C.POINTER(C.c_int), I, I, I, I, ERR, SZ],
+            "blz_program_tile": [P, C.POINTER(C.c_int), C.POINTER(C.c_int)],
             "blz_program_launch": [P, P, ERR, SZ],
             "blz_program_result": [P, P, C.POINTER(C.c_int64), I32P, ERR, SZ],
             "blz_program_time": [P, P, I, C.POINTER(C.c_double), ERR, SZ],
@@ -109,6 +110,7 @@ def lib():
             "blz_memcpy_htod": [P, P, SZ, P],
             "blz_memcpy_dtoh": [P, P, SZ, P],
             "blz_memset_device": [P, I, SZ, P],
+            "blz_device_copy": [P, P, SZ, P],
             "blz_stream_create": [C.POINTER(P)],
             "blz_stream_synchronize": [P],
             "blz_stream_destroy": [P],
@@ -401,7 +403,7 @@ class Program:
     """A planned and compiled filter+project bound to device buffers (launch many)."""
 
     def __init__(self, frame: Sequence[DeviceColumn], filter_query_part: Optional[str],
-                 project_query_part: Optional[str], scalar_mode=SCALAR_INTENDED, s=0, r=0,
+                 project_query_part: Optional[str], scalar_mode=SCALAR_INTENDED, s=0, r=0, t=0,
                  max_out=64):
         self.frame = list(frame)
         self.h = C.c_void_p(0)
@@ -410,10 +412,13 @@ class Program:
         _call(lib().blz_program_create, C.byref(self.h), _colpp(frame), len(frame),
               filter_query_part.encode() if filter_query_part else None,
               project_query_part.encode() if project_query_part else None, out, max_out,
-              C.byref(n_out), scalar_mode, s, r)
+              C.byref(n_out), scalar_mode, s, r, t)
         info = [C.c_int(0) for _ in range(6)]
         lib().blz_program_info(self.h, *[C.byref(x) for x in info])
         self.s, self.r, self.n_ops, self.n_vops, self.n_slots, self.grid = [x.value for x in info]
+        tr, lb = C.c_int(0), C.c_int(0)
+        lib().blz_program_tile(self.h, C.byref(tr), C.byref(lb))
+        self.tile_rows, self.lds_bytes = tr.value, lb.value
         self.outputs = [DeviceColumn.adopt(out[i], owned=False) for i in range(n_out.value)]
 
     def launch(self, stream=None):

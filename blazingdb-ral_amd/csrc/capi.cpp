@@ -84,9 +84,12 @@ int device_cus() {
 
 int64_t rows_per_tile(int r) { return (int64_t)block_threads() * r; }
 
-int interp_grid(const Compiled &c, int64_t n) {
-  const int64_t tiles = (n + rows_per_tile(c.R) - 1) / rows_per_tile(c.R);
-  const int64_t resident = (int64_t)device_cus() * 8;  // persistent workgroups
+int64_t tile_rows(const Compiled &c) { return c.T > 0 ? c.T : rows_per_tile(c.R); }
+
+// Persistent grid: every workgroup that fits on the device at once.
+int interp_grid(const Compiled &c, bool compact, int64_t n) {
+  const int64_t tiles = (n + tile_rows(c) - 1) / tile_rows(c);
+  const int64_t resident = (int64_t)device_cus() * interp_blocks_per_cu(c.S, c.R, compact, c.lds_bytes);
   return (int)std::max<int64_t>(1, std::min<int64_t>(tiles, resident));
 }
 
@@ -141,7 +144,7 @@ RunResult run_once(const Compiled &c, bool compact, int64_t n, int n_out, hipStr
   RunResult r;
   r.nulls.assign(n_out, 0);
   if (n == 0) return r;
-  const int64_t tiles = (n + rows_per_tile(c.R) - 1) / rows_per_tile(c.R);
+  const int64_t tiles = (n + tile_rows(c) - 1) / tile_rows(c);
   LaunchBuffers lb;
   layout(lb, c.blob.size(), n_out, compact ? tiles : 0);
   HIPCHECK(hipMallocAsync((void **)&lb.base, lb.bytes, st));
@@ -150,7 +153,8 @@ RunResult run_once(const Compiled &c, bool compact, int64_t n, int n_out, hipStr
   hipError_t e = hipMemcpyAsync(lb.blob, c.blob.data(), c.blob.size(), hipMemcpyHostToDevice, st);
   if (e == hipSuccess) e = launch_fill_zero(lb.base + lb.zero_off, (int64_t)lb.zero_bytes,
                                             (int)std::min<size_t>(1024, lb.zero_bytes / 2048 + 1), st);
-  if (e == hipSuccess) e = launch_interp(c.S, c.R, compact, interp_grid(c, n), st, lb.blob, lb.ws);
+  if (e == hipSuccess)
+    e = launch_interp(c.S, c.R, compact, interp_grid(c, compact, n), c.lds_bytes, st, lb.blob, lb.ws);
   if (e == hipSuccess && n_out > 0)
     e = hipMemcpyAsync(host.data(), lb.ws.null_counts, sizeof(unsigned long long) * n_out,
                        hipMemcpyDeviceToHost, st);
@@ -297,6 +301,8 @@ int blz_evaluate_expression(gdf_column *const *frame, int n_frame, const char *e
       if (bytes) HIPCHECK(hipMemcpyAsync(output->data, src->data, bytes, hipMemcpyDeviceToDevice, S(stream)));
       if (output->valid && src->valid && n > 0)
         HIPCHECK(hipMemcpyAsync(output->valid, src->valid, ((size_t)n + 7) / 8, hipMemcpyDeviceToDevice, S(stream)));
+      else if (output->valid && n > 0)  // a column without a bitmap is all valid
+        HIPCHECK(hipMemsetAsync(output->valid, 0xFF, ((size_t)n + 7) / 8, S(stream)));
       HIPCHECK(hipStreamSynchronize(S(stream)));
       if (out_null_count) *out_null_count = src->valid ? src->null_count : 0;
       return;
@@ -579,7 +585,7 @@ struct blz_program_s {
 
 int blz_program_create(blz_program *prog, gdf_column *const *frame, int n_frame, const char *filter_qp,
                        const char *proj_qp, gdf_column *out_columns, int n_out_cap, int *n_out,
-                       int scalar_mode, int s, int r, char *err, size_t err_len) {
+                       int scalar_mode, int s, int r, int t, char *err, size_t err_len) {
   return guarded(err, err_len, [&] {
     if (!prog) invalid("prog is NULL");
     *prog = nullptr;
@@ -612,6 +618,7 @@ int blz_program_create(blz_program *prog, gdf_column *const *frame, int n_frame,
       req.compact = has_filter;
       req.force_s = s;
       req.force_r = r;
+      req.force_t = t;
       for (size_t i = 0; i < outs.size(); i++) {
         const auto &o = outs[i];
         const bool with_valid = o.kind == 0 ? true : frame[o.index]->valid != nullptr;
@@ -631,8 +638,8 @@ int blz_program_create(blz_program *prog, gdf_column *const *frame, int n_frame,
       p->compact = has_filter;
       p->n = n;
       p->n_out = (int)outs.size();
-      p->grid = interp_grid(p->c, std::max<int64_t>(n, 1));
-      const int64_t tiles = (n + rows_per_tile(p->c.R) - 1) / rows_per_tile(p->c.R);
+      p->grid = interp_grid(p->c, has_filter, std::max<int64_t>(n, 1));
+      const int64_t tiles = (n + tile_rows(p->c) - 1) / tile_rows(p->c);
       layout(p->lb, p->c.blob.size(), p->n_out, has_filter ? tiles : 0);
       HIPCHECK(hipMalloc((void **)&p->lb.base, p->lb.bytes));
       bind(p->lb, p->n_out);
@@ -661,7 +668,8 @@ hipError_t program_enqueue(blz_program p, hipStream_t st) {
       const size_t vb = gdf_valid_allocation_size((gdf_size_type)p->n);
       e = launch_fill_zero(p->outs[i].valid, (int64_t)vb, (int)std::min<size_t>(1024, vb / 2048 + 1), st);
     }
-  if (e == hipSuccess) e = launch_interp(p->c.S, p->c.R, p->compact, p->grid, st, p->lb.blob, p->lb.ws);
+  if (e == hipSuccess)
+    e = launch_interp(p->c.S, p->c.R, p->compact, p->grid, p->c.lds_bytes, st, p->lb.blob, p->lb.ws);
   return e;
 }
 }  // namespace
@@ -710,8 +718,8 @@ int blz_program_time(blz_program prog, blz_stream_t stream, int iters, double *m
                                     S(stream)));
         }
       HIPCHECK(hipEventRecord(prog->ev0, S(stream)));
-      HIPCHECK(launch_interp(prog->c.S, prog->c.R, prog->compact, prog->grid, S(stream), prog->lb.blob,
-                             prog->lb.ws));
+      HIPCHECK(launch_interp(prog->c.S, prog->c.R, prog->compact, prog->grid, prog->c.lds_bytes, S(stream),
+                             prog->lb.blob, prog->lb.ws));
       HIPCHECK(hipEventRecord(prog->ev1, S(stream)));
       HIPCHECK(hipEventSynchronize(prog->ev1));
       float ms = 0;
@@ -730,6 +738,13 @@ int blz_program_info(blz_program prog, int *s, int *r, int *n_ops, int *n_vops, 
   if (n_vops) *n_vops = prog->c.n_vops;
   if (n_slots) *n_slots = prog->c.n_slots;
   if (grid) *grid = prog->grid;
+  return BLZ_OK;
+}
+
+int blz_program_tile(blz_program prog, int *tile_rows, int *lds_bytes) {
+  if (!prog) return BLZ_ERR_INVALID;
+  if (tile_rows) *tile_rows = prog->c.T;
+  if (lds_bytes) *lds_bytes = prog->c.lds_bytes;
   return BLZ_OK;
 }
 
@@ -765,6 +780,14 @@ int blz_memset_device(void *dst, int value, size_t bytes, blz_stream_t stream) {
   if (hipMemsetAsync(dst, value, bytes, S(stream)) != hipSuccess) return BLZ_ERR_HIP;
   return hipStreamSynchronize(S(stream)) == hipSuccess ? BLZ_OK : BLZ_ERR_HIP;
 }
+int blz_device_copy(void *dst, const void *src, size_t bytes, blz_stream_t stream) {
+  if (bytes == 0) return BLZ_OK;
+  if (bytes % 16 || (uintptr_t)dst % 16 || (uintptr_t)src % 16) return BLZ_ERR_INVALID;
+  const int grid = (int)std::min<size_t>(bytes / 16 / 256 + 1, (size_t)device_cus() * 16);
+  if (launch_copy(dst, src, (int64_t)bytes, grid, S(stream)) != hipSuccess) return BLZ_ERR_HIP;
+  return hipStreamSynchronize(S(stream)) == hipSuccess ? BLZ_OK : BLZ_ERR_HIP;
+}
+
 int blz_stream_create(blz_stream_t *stream) {
   hipStream_t s;
   if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) return BLZ_ERR_HIP;

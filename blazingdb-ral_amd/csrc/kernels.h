@@ -19,8 +19,11 @@ struct Workspace {
 
 // Register-file interpreter (project or fused filter+project/gather).
 // (s, r) in {(8,1), (8,2), (16,1), (32,1), (64,1)}: s slots, r rows per lane.
-hipError_t launch_interp(int s, int r, bool compact, int grid, hipStream_t st, const uint8_t *blob,
-                         Workspace ws);
+// lds_bytes > 0: the LDS-staged kernel (shapes (8,1), (8,2), (16,1)) with that much
+// dynamic LDS per workgroup; 0: the direct kernel.
+hipError_t launch_interp(int s, int r, bool compact, int grid, int lds_bytes, hipStream_t st,
+                         const uint8_t *blob, Workspace ws);
+int interp_blocks_per_cu(int s, int r, bool compact, int lds_bytes);
 // materialize_column: width in {1,2,4,8}.
 hipError_t launch_gather(int width, const void *in, const uint8_t *in_valid, int64_t n_in, const int32_t *idx,
                          int64_t n_idx, void *out, uint8_t *out_valid, unsigned long long *null_count,
@@ -29,6 +32,9 @@ hipError_t launch_sequence(int32_t *out, int64_t n, int32_t init, int32_t step, 
 hipError_t launch_fill_zero(void *p, int64_t bytes, int grid, hipStream_t st);
 hipError_t launch_null_count(const uint8_t *valid, int64_t n, unsigned long long *out, int grid,
                              hipStream_t st);
+
+// dst[0..bytes) = src[0..bytes); bytes a multiple of 16, both 16-byte aligned.
+hipError_t launch_copy(void *dst, const void *src, int64_t bytes, int grid, hipStream_t st);
 
 int blob_capacity();   // bytes of LDS the program blob may occupy
 int block_threads();   // threads per workgroup of every kernel

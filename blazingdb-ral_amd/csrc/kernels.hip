@@ -109,88 +109,80 @@ DEV uint64_t ipow(uint64_t base, int64_t e) {
   return r;
 }
 
-DEV bool valid_bit(const uint8_t *v, int64_t row) {
-  return v == nullptr ? true : ((v[row >> 3] >> (row & 7)) & 1);
-}
+// Explicit address spaces: column data is global memory (vector loads that do not
+// share a wait counter with LDS), the program blob is read through the scalar
+// cache (constant address space: uniform values land in SGPRs).
+#define GLOBAL __attribute__((address_space(1)))
+#define CONSTANT __attribute__((address_space(4)))
 
-// Uniform value from LDS.
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
 template <typename T>
-DEV T uni(T v) {
-  return (T)__builtin_amdgcn_readfirstlane((int)v);
+DEV T gload(const void *base, int64_t i) { return ((const GLOBAL T *)base)[i]; }
+template <typename T>
+DEV void gstore(void *base, int64_t i, T v) { ((GLOBAL T *)base)[i] = v; }
+
+DEV bool valid_bit(const void *v, int64_t row) {
+  return v == nullptr ? true : ((gload<uint8_t>(v, row >> 3) >> (row & 7)) & 1);
 }
-DEV uint64_t uni64(uint64_t v) {
-  uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
-  uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
-  return ((uint64_t)hi << 32) | lo;
-}
-DEV const void *uniptr(const void *p) { return (const void *)uni64((uint64_t)p); }
 
 // Validity word `w` of a bitmap covering n rows; the last partial word is read
 // byte by byte so a buffer of exactly ceil(n/8) bytes is never over-read.
-DEV uint64_t load_vword(const uint8_t *v, int64_t w, int64_t n) {
-  if ((w + 1) * 64 <= n) return *(const uint64_t *)(v + 8 * w);
+DEV uint64_t load_vword(const void *v, int64_t w, int64_t n) {
+  if ((w + 1) * 64 <= n) return gload<uint64_t>(v, w);
   uint64_t x = 0;
   const int64_t nb = (n - w * 64 + 7) >> 3;
-  for (int64_t b = 0; b < nb; b++) x |= (uint64_t)v[8 * w + b] << (8 * b);
+  for (int64_t b = 0; b < nb; b++) x |= (uint64_t)gload<uint8_t>(v, 8 * w + b) << (8 * b);
   return x;
 }
-DEV void store_vword(uint8_t *v, int64_t w, int64_t n, uint64_t x) {
-  if ((w + 1) * 64 <= n) { *(uint64_t *)(v + 8 * w) = x; return; }
+DEV void store_vword(void *v, int64_t w, int64_t n, uint64_t x) {
+  if ((w + 1) * 64 <= n) { gstore<uint64_t>(v, w, x); return; }
   const int64_t nb = (n - w * 64 + 7) >> 3;
-  for (int64_t b = 0; b < nb; b++) v[8 * w + b] = (uint8_t)(x >> (8 * b));
+  for (int64_t b = 0; b < nb; b++) gstore<uint8_t>(v, 8 * w + b, (uint8_t)(x >> (8 * b)));
 }
 DEV uint64_t low_mask(int64_t bits) { return bits >= 64 ? ~0ull : ((1ull << bits) - 1); }
+DEV uint64_t readlane64(uint64_t v, int lane) {
+  const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)v, lane);
+  const uint32_t hi = __builtin_amdgcn_readlane((uint32_t)(v >> 32), lane);
+  return ((uint64_t)hi << 32) | lo;
+}
 
 constexpr int kBlock = 256;
 constexpr int kWaves = kBlock / 64;
-constexpr int kLdsBlobBytes = 12288;
+constexpr int kLdsBlobBytes = 12288;  // program size limit (bytes)
 constexpr int kMaxUnrolledInputs = 16;
 
-// Raw load of one element (width class only; widened later).
-template <int R>
-DEV void load_raw(int kind, const void *base, const int64_t (&row)[R], uint64_t (&dst)[R]) {
+DEV bool is_wide(int kind) { return kind == LD_I64 || kind == LD_F64; }
+// Widens a 32-bit raw load (interpreter_ops.cuh:386-452).
+DEV uint64_t widen32(int kind, uint32_t w) {
   switch (kind) {
-    case LD_I8:
-#pragma unroll
-      for (int u = 0; u < R; u++) dst[u] = (uint64_t)(int64_t)((const int8_t *)base)[row[u]];
-      break;
-    case LD_U8:
-#pragma unroll
-      for (int u = 0; u < R; u++) dst[u] = (uint64_t)((const uint8_t *)base)[row[u]];
-      break;
-    case LD_I16:
-#pragma unroll
-      for (int u = 0; u < R; u++) dst[u] = (uint64_t)(int64_t)((const int16_t *)base)[row[u]];
-      break;
-    case LD_I32: case LD_F32:
-#pragma unroll
-      for (int u = 0; u < R; u++) dst[u] = (uint64_t)((const uint32_t *)base)[row[u]];
-      break;
-    case LD_I64: case LD_F64:
-#pragma unroll
-      for (int u = 0; u < R; u++) dst[u] = ((const uint64_t *)base)[row[u]];
-      break;
-    default:
-#pragma unroll
-      for (int u = 0; u < R; u++) dst[u] = 0;
-      break;
+    case LD_I8: return (uint64_t)(int64_t)(int8_t)(uint8_t)w;
+    case LD_U8: return (uint64_t)(uint8_t)w;
+    case LD_I16: return (uint64_t)(int64_t)(int16_t)(uint16_t)w;
+    case LD_I32: return (uint64_t)(int64_t)(int32_t)w;
+    case LD_F32: return d2b((double)__builtin_bit_cast(float, w));
+    default: return 0;
   }
 }
-DEV uint64_t widen(int kind, uint64_t raw) {
-  if (kind == LD_I32) return (uint64_t)(int64_t)(int32_t)(uint32_t)raw;
-  if (kind == LD_F32) return d2b((double)__builtin_bit_cast(float, (uint32_t)raw));
-  return raw;
+// Raw load of a narrow column element into 32 bits (no conversion: the widening
+// happens after every load of the tile is issued).
+DEV uint32_t load_narrow(int kind, const void *base, int64_t row) {
+  switch (kind) {
+    case LD_I32: case LD_F32: return gload<uint32_t>(base, row);
+    case LD_I16: return gload<uint16_t>(base, row);
+    case LD_I8: case LD_U8: return gload<uint8_t>(base, row);
+    default: return 0;
+  }
 }
 
-template <int R>
 DEV void store_narrow(int kind, void *base, int64_t row, uint64_t v) {
   switch (kind) {
-    case ST_I8: ((int8_t *)base)[row] = (int8_t)v; break;
-    case ST_I16: ((int16_t *)base)[row] = (int16_t)v; break;
-    case ST_I32: ((int32_t *)base)[row] = (int32_t)v; break;
-    case ST_I64: ((uint64_t *)base)[row] = v; break;
-    case ST_F32: ((float *)base)[row] = (float)b2d(v); break;
-    case ST_F64: ((uint64_t *)base)[row] = v; break;
+    case ST_I8: gstore<int8_t>(base, row, (int8_t)v); break;
+    case ST_I16: gstore<int16_t>(base, row, (int16_t)v); break;
+    case ST_I32: gstore<int32_t>(base, row, (int32_t)v); break;
+    case ST_I64: gstore<uint64_t>(base, row, v); break;
+    case ST_F32: gstore<float>(base, row, (float)b2d(v)); break;
+    case ST_F64: gstore<uint64_t>(base, row, v); break;
     default: break;
   }
 }
@@ -218,13 +210,13 @@ __device__ __attribute__((noinline)) uint64_t heavy_op(int code, uint64_t a, uin
 template <int R>
 DEV void exec_op(int code, const uint64_t (&A)[R], const uint64_t (&B)[R], const bool (&vb)[R],
                  uint64_t (&Z)[R]) {
-#define BLZ_CASE(CODE, EXPR)                     \
-  case CODE: {                                   \
+#define BLZ_CASE(CODE, EXPR)                        \
+  case CODE: {                                      \
     _Pragma("unroll") for (int u = 0; u < R; u++) { \
-      const uint64_t a = A[u], b = B[u];         \
-      (void)a; (void)b;                          \
-      Z[u] = (EXPR);                             \
-    }                                            \
+      const uint64_t a = A[u], b = B[u];            \
+      (void)a; (void)b;                             \
+      Z[u] = (EXPR);                                \
+    }                                               \
   } break;
   switch (code) {
     BLZ_CASE(OP_ADD_I, a + b)
@@ -237,7 +229,6 @@ DEV void exec_op(int code, const uint64_t (&A)[R], const uint64_t (&B)[R], const
     BLZ_CASE(OP_DIV_F, d2b(b2d(a) / b2d(b)))
     BLZ_CASE(OP_MOD_I, (uint64_t)imod((int64_t)a, (int64_t)b))
     BLZ_CASE(OP_POW_I, ipow(a, (int64_t)b))
-    case OP_POW_F:
     BLZ_CASE(OP_EQ_I, (uint64_t)(a == b))
     BLZ_CASE(OP_EQ_F, (uint64_t)(b2d(a) == b2d(b)))
     BLZ_CASE(OP_NE_I, (uint64_t)(a != b))
@@ -254,19 +245,7 @@ DEV void exec_op(int code, const uint64_t (&A)[R], const uint64_t (&B)[R], const
     BLZ_CASE(OP_AND_B, a & b)
     BLZ_CASE(OP_FLOOR, d2b(floor(b2d(a))))
     BLZ_CASE(OP_CEIL, d2b(ceil(b2d(a))))
-    case OP_SIN:
-    case OP_COS:
-    case OP_ASIN:
-    case OP_ACOS:
-    case OP_TAN:
-    case OP_COTAN:
-    case OP_ATAN:
     BLZ_CASE(OP_ABS, d2b(fabs(b2d(a))))
-    case OP_LN:
-    case OP_LOG10:
-#pragma nounroll
-      for (int u = 0; u < R; u++) Z[u] = heavy_op(code, A[u], B[u]);
-      break;
     BLZ_CASE(OP_NOT_I, (uint64_t)(a == 0))
     BLZ_CASE(OP_NOT_F, (uint64_t)(!(b2d(a) != 0.0)))
     BLZ_CASE(OP_YEAR32, (uint64_t)year_d((int64_t)a))
@@ -281,6 +260,11 @@ DEV void exec_op(int code, const uint64_t (&A)[R], const uint64_t (&B)[R], const
     BLZ_CASE(OP_ZERO, 0ull)
     BLZ_CASE(OP_IS_NULL, (uint64_t)(!vb[u]))
     BLZ_CASE(OP_IS_NOT_NULL, (uint64_t)(vb[u]))
+    case OP_POW_F: case OP_SIN: case OP_COS: case OP_ASIN: case OP_ACOS: case OP_TAN:
+    case OP_COTAN: case OP_ATAN: case OP_LN: case OP_LOG10:
+#pragma nounroll
+      for (int u = 0; u < R; u++) Z[u] = heavy_op(code, A[u], B[u]);
+      break;
     default:  // OP_IDENT
 #pragma unroll
       for (int u = 0; u < R; u++) Z[u] = A[u];
@@ -289,29 +273,20 @@ DEV void exec_op(int code, const uint64_t (&A)[R], const uint64_t (&B)[R], const
 #undef BLZ_CASE
 }
 
-struct LdsProg {
-  ProgHeader h;
-  const InCol *in;
-  const OutCol *out;
-  const DevOp *ops;
-  const VOp *vops;
-};
-
 // Runs `nops` device ops starting at `ops` over the register file.
 template <int S, int R>
-DEV void run_ops(const DevOp *ops, int nops, uint64_t (&rf)[S * R], const int64_t (&row)[R],
-                 const InCol *in) {
+DEV void run_ops(const CONSTANT DevOp *ops, int nops, uint64_t (&rf)[S * R], const int64_t (&row)[R],
+                 const CONSTANT InCol *in) {
   for (int k = 0; k < nops; k++) {
-    const DevOp &op = ops[k];
-    // code | dst | a | b | flags | conv | vcol | pad, one LDS read into SGPRs
-    const uint64_t ctl = uni64(*(const uint64_t *)&op);
+    // code | dst | a | b | flags | conv | vcol | pad: one scalar load
+    const uint64_t ctl = *(const CONSTANT uint64_t *)(ops + k);
     const int code = (int)(ctl & 0xFF), dst = (int)((ctl >> 8) & 0xFF);
     const int fa = (int)((ctl >> 32) & 0xFF), cv = (int)((ctl >> 40) & 0xFF);
     const int vcol = (int)((ctl >> 48) & 0xFF);
     uint64_t A[R], B[R], Z[R];
     bool vb[R];
     if (fa & F_A_IMM) {
-      const uint64_t ia = uni64(op.imm_a);
+      const uint64_t ia = ops[k].imm_a;
 #pragma unroll
       for (int u = 0; u < R; u++) A[u] = ia;
     } else {
@@ -320,7 +295,7 @@ DEV void run_ops(const DevOp *ops, int nops, uint64_t (&rf)[S * R], const int64_
       for (int u = 0; u < R; u++) A[u] = rf[a * R + u];
     }
     if (fa & F_B_IMM) {
-      const uint64_t ib = uni64(op.imm_b);
+      const uint64_t ib = ops[k].imm_b;
 #pragma unroll
       for (int u = 0; u < R; u++) B[u] = ib;
     } else {
@@ -338,7 +313,7 @@ DEV void run_ops(const DevOp *ops, int nops, uint64_t (&rf)[S * R], const int64_
       for (int u = 0; u < R; u++) B[u] = conv(B[u], cb);
     }
     if (vcol != 0xFF) {
-      const uint8_t *vp = (const uint8_t *)uniptr(in[vcol].valid);
+      const void *vp = in[vcol].valid;
 #pragma unroll
       for (int u = 0; u < R; u++) vb[u] = valid_bit(vp, row[u]);
     } else {
@@ -357,10 +332,10 @@ DEV void run_ops(const DevOp *ops, int nops, uint64_t (&rf)[S * R], const int64_
 
 // Validity program: one 64-row word per lane (lanes [0, R) of each wave).
 template <int S>
-DEV void run_vops(const VOp *vops, int nvops, uint64_t (&vrf)[S]) {
+DEV void run_vops(const CONSTANT VOp *vops, int nvops, uint64_t (&vrf)[S]) {
   for (int k = 0; k < nvops; k++) {
-    const VOp &op = vops[k];
-    const int code = uni(op.code), d = uni(op.dst), a = uni(op.a), b = uni(op.b);
+    const uint32_t w = *(const CONSTANT uint32_t *)(vops + k);
+    const int code = w & 0xFF, d = (w >> 8) & 0xFF, a = (w >> 16) & 0xFF, b = (w >> 24) & 0xFF;
     const uint64_t x = vrf[a], y = vrf[b];
     vrf[d] = code == VOP_OR ? (x | y) : (x & y);
   }
@@ -376,33 +351,29 @@ DEV uint64_t vsrc_word(int vsrc, const uint64_t (&vrf)[S]) {
 
 template <int S, int R, bool COMPACT>
 __global__ void __launch_bounds__(kBlock)
-interp_kernel(const uint8_t *__restrict__ blob, Workspace ws) {
-  __shared__ __attribute__((aligned(16))) uint8_t s_blob[kLdsBlobBytes];
+interp_kernel(const uint8_t *__restrict__ blob_arg, Workspace ws) {
   __shared__ int64_t s_tile;
   __shared__ uint64_t s_wave_cnt[kWaves];
   __shared__ uint64_t s_prefix;
   __shared__ uint64_t s_vbits[kWaves * R + 2];
 
-  // Stage the program in LDS.
-  const int blob_bytes = *(const int32_t *)(blob + offsetof(ProgHeader, blob_bytes));
-  for (int i = threadIdx.x * 16; i < blob_bytes; i += kBlock * 16)
-    *(uint4 *)(s_blob + i) = *(const uint4 *)(blob + i);
-  __syncthreads();
-  const ProgHeader &h = *(const ProgHeader *)s_blob;
-  const InCol *in = (const InCol *)(s_blob + uni(h.off_in));
-  const OutCol *out = (const OutCol *)(s_blob + uni(h.off_out));
-  const DevOp *ops = (const DevOp *)(s_blob + uni(h.off_ops));
-  const VOp *vops = (const VOp *)(s_blob + uni(h.off_vops));
-  const int64_t n = (int64_t)uni64((uint64_t)h.n_rows);
-  const int n_in = uni(h.n_in), n_out = uni(h.n_out);
-  const int n_ops_p = uni(h.n_ops_p), n_ops_q = uni(h.n_ops_q);
-  const int n_vops_p = uni(h.n_vops_p), n_vops_q = uni(h.n_vops_q);
-  const int any_valid_out = uni(h.any_valid_out);
+  const CONSTANT uint8_t *blob = (const CONSTANT uint8_t *)blob_arg;
+  const CONSTANT ProgHeader &h = *(const CONSTANT ProgHeader *)blob;
+  const CONSTANT InCol *in = (const CONSTANT InCol *)(blob + h.off_in);
+  const CONSTANT OutCol *out = (const CONSTANT OutCol *)(blob + h.off_out);
+  const CONSTANT DevOp *ops = (const CONSTANT DevOp *)(blob + h.off_ops);
+  const CONSTANT VOp *vops = (const CONSTANT VOp *)(blob + h.off_vops);
+  const int64_t n = h.n_rows;
+  const int n_in = h.n_in, n_out = h.n_out;
+  const int n_ops_p = h.n_ops_p, n_ops_q = h.n_ops_q;
+  const int n_vops_p = h.n_vops_p, n_vops_q = h.n_vops_q;
+  const int any_valid_out = h.any_valid_out;
 
   const int wave = threadIdx.x >> 6;
   const int lane = threadIdx.x & 63;
   const int64_t rows_per_tile = (int64_t)kBlock * R;
   const int64_t n_tiles = (n + rows_per_tile - 1) / rows_per_tile;
+  constexpr int MI = S < kMaxUnrolledInputs ? S : kMaxUnrolledInputs;
 
   uint64_t rf[S * R];
   uint64_t vrf[S];
@@ -426,77 +397,75 @@ interp_kernel(const uint8_t *__restrict__ blob, Workspace ws) {
       lrow[u] = row[u] < n ? row[u] : n - 1;  // clamped: loads stay in bounds
     }
 
-    // 1. Raw loads of every input column, all issued before any is widened.
+    // 1. Every input column's raw loads are issued before any is widened: wide
+    //    columns land in w64, narrow ones in w32 (distinct registers, so no move
+    //    and no wait sits between the loads).
+    {
+      uint64_t w64[MI][R];
+      uint32_t w32[MI][R];
 #pragma unroll
-    for (int c = 0; c < (S < kMaxUnrolledInputs ? S : kMaxUnrolledInputs); c++) {
-      if (c < n_in) {
-        const int kind = uni(in[c].load);
-        const void *base = uniptr(in[c].data);
-        uint64_t tmp[R];
-        load_raw<R>(kind, base, lrow, tmp);
+      for (int c = 0; c < MI; c++) {
+        if (c < n_in) {
+          const int kind = in[c].load;
+          const void *base = in[c].data;
+          if (is_wide(kind)) {
 #pragma unroll
-        for (int u = 0; u < R; u++) rf[c * R + u] = tmp[u];
+            for (int u = 0; u < R; u++) w64[c][u] = gload<uint64_t>(base, lrow[u]);
+          } else {
+#pragma unroll
+            for (int u = 0; u < R; u++) w32[c][u] = load_narrow(kind, base, lrow[u]);
+          }
+        }
+      }
+#pragma unroll
+      for (int c = 0; c < MI; c++) {
+        if (c < n_in) {
+          const int kind = in[c].load;
+#pragma unroll
+          for (int u = 0; u < R; u++) rf[c * R + u] = is_wide(kind) ? w64[c][u] : widen32(kind, w32[c][u]);
+        }
       }
     }
-    for (int c = kMaxUnrolledInputs; c < n_in; c++) {
-      const int kind = uni(in[c].load);
-      const void *base = uniptr(in[c].data);
-      uint64_t tmp[R];
-      load_raw<R>(kind, base, lrow, tmp);
+    for (int c = kMaxUnrolledInputs; c < n_in; c++) {  // plans with more than 16 inputs
+      const int kind = in[c].load;
+      const void *base = in[c].data;
 #pragma unroll
-      for (int u = 0; u < R; u++) rf[c * R + u] = tmp[u];
+      for (int u = 0; u < R; u++)
+        rf[c * R + u] = is_wide(kind) ? gload<uint64_t>(base, lrow[u]) : widen32(kind, load_narrow(kind, base, lrow[u]));
     }
-    // Validity words of the wave's R groups: lane u < R holds word of group u.
+    // Validity words of the wave's R groups: lane u < R holds the word of group u.
     const int64_t wave_word = wave_base >> 6;
     if (any_valid_out || COMPACT) {
       const int64_t w = wave_word + (lane < R ? lane : 0);
       for (int c = 0; c < n_in; c++) {
-        const int vs = uni(in[c].vslot);
+        const int vs = in[c].vslot;
         if (vs >= 0) {
-          const uint8_t *vp = (const uint8_t *)uniptr(in[c].valid);
+          const void *vp = in[c].valid;
           vrf[vs] = (lane < R && w * 64 < n) ? load_vword(vp, w, n) : 0ull;
         }
       }
     }
-    // 2. Widen.
-#pragma unroll
-    for (int c = 0; c < (S < kMaxUnrolledInputs ? S : kMaxUnrolledInputs); c++) {
-      if (c < n_in) {
-        const int kind = uni(in[c].load);
-        if (kind == LD_I32 || kind == LD_F32) {
-#pragma unroll
-          for (int u = 0; u < R; u++) rf[c * R + u] = widen(kind, rf[c * R + u]);
-        }
-      }
-    }
-    for (int c = kMaxUnrolledInputs; c < n_in; c++) {
-      const int kind = uni(in[c].load);
-#pragma unroll
-      for (int u = 0; u < R; u++) rf[c * R + u] = widen(kind, rf[c * R + u]);
-    }
 
     if (!COMPACT) {
-      // 3. Projection program, then narrowing stores of every output.
+      // 2. Projection program, then narrowing stores of every output.
       run_ops<S, R>(ops, n_ops_q, rf, lrow, in);
       if (any_valid_out) run_vops<S>(vops, n_vops_q, vrf);
       for (int o = 0; o < n_out; o++) {
-        const int kind = uni(out[o].store);
-        const int slot = uni(out[o].slot);
-        void *base = (void *)uniptr(out[o].data);
+        const int kind = out[o].store;
+        const int slot = out[o].slot;
+        void *base = out[o].data;
 #pragma unroll
         for (int u = 0; u < R; u++)
-          if (row[u] < n) store_narrow<R>(kind, base, row[u], rf[slot * R + u]);
-        uint8_t *vp = (uint8_t *)uniptr(out[o].valid);
+          if (row[u] < n) store_narrow(kind, base, row[u], rf[slot * R + u]);
+        void *vp = out[o].valid;
         if (vp != nullptr) {
-          const int vsrc = uni(out[o].vsrc);
-          const uint64_t word = vsrc_word<S>(vsrc, vrf);
+          const uint64_t word = vsrc_word<S>(out[o].vsrc, vrf);
           const int64_t w = wave_word + lane;
           uint64_t nulls = 0;
           if (lane < R && w * 64 < n) {
             store_vword(vp, w, n, word);
             nulls = __builtin_popcountll(~word & low_mask(n - w * 64));
           }
-          // wave reduction of the null count, one atomic per wave
           for (int off = 32; off > 0; off >>= 1) nulls += __shfl_xor(nulls, off);
           if (lane == 0 && nulls) atomicAdd(ws.null_counts + o, (unsigned long long)nulls);
         }
@@ -508,23 +477,18 @@ interp_kernel(const uint8_t *__restrict__ blob, Workspace ws) {
     // ---- COMPACT: filter + project/gather in one pass ----
     run_ops<S, R>(ops, n_ops_p, rf, lrow, in);
     run_vops<S>(vops, n_vops_p, vrf);
-    const int st_slot = uni(h.stencil_slot), st_width = uni(h.stencil_width);
-    const int st_vsrc = uni(h.stencil_vsrc);
+    const int st_slot = h.stencil_slot, st_width = h.stencil_width;
+    const int st_vsrc = h.stencil_vsrc;
     const uint64_t st_mask = low_mask(8 * st_width);
     bool keep[R];
     uint64_t kmask[R];
     uint64_t wave_cnt = 0;
     {
-      // Predicate validity: a validity slot, a constant, or (bare-column
-      // predicate) the raw bitmap of that column.
-      const uint64_t vstencil = vsrc_word<S>(st_vsrc == -3 ? -1 : st_vsrc, vrf);
-      const uint8_t *rawv = st_vsrc == -3 ? (const uint8_t *)uniptr(in[st_slot].valid) : nullptr;
+      // Predicate validity: a validity slot or a constant.
+      const uint64_t vstencil = vsrc_word<S>(st_vsrc, vrf);
 #pragma unroll
       for (int u = 0; u < R; u++) {
-        const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)vstencil, u);
-        const uint32_t hi = __builtin_amdgcn_readlane((uint32_t)(vstencil >> 32), u);
-        const uint64_t vw = ((uint64_t)hi << 32) | lo;
-        const bool vbit = st_vsrc == -3 ? valid_bit(rawv, lrow[u]) : ((vw >> lane) & 1);
+        const bool vbit = (readlane64(vstencil, u) >> lane) & 1;
         keep[u] = row[u] < n && ((rf[st_slot * R + u] & st_mask) != 0) && vbit;
         kmask[u] = __builtin_amdgcn_ballot_w64(keep[u]);
         wave_cnt += __builtin_popcountll(kmask[u]);
@@ -594,35 +558,28 @@ interp_kernel(const uint8_t *__restrict__ blob, Workspace ws) {
         run += __builtin_popcountll(kmask[u]);
       }
       // Projection program only if some row of this wave survives.
-      if (wave_cnt > 0) {
-        run_ops<S, R>(ops + n_ops_p, n_ops_q, rf, lrow, in);
-      }
+      if (wave_cnt > 0) run_ops<S, R>(ops + n_ops_p, n_ops_q, rf, lrow, in);
       if (any_valid_out) run_vops<S>(vops + n_vops_p, n_vops_q, vrf);
       for (int o = 0; o < n_out; o++) {
-        const int kind = uni(out[o].store);
-        const int slot = uni(out[o].slot);
-        void *base = (void *)uniptr(out[o].data);
+        const int kind = out[o].store;
+        const int slot = out[o].slot;
+        void *base = out[o].data;
         if (wave_cnt > 0) {
 #pragma unroll
           for (int u = 0; u < R; u++)
-            if (keep[u]) store_narrow<R>(kind, base, (int64_t)pos[u], rf[slot * R + u]);
+            if (keep[u]) store_narrow(kind, base, (int64_t)pos[u], rf[slot * R + u]);
         }
-        uint8_t *vp = (uint8_t *)uniptr(out[o].valid);
+        void *vp = out[o].valid;
         if (vp != nullptr) {
           // Compacted validity bits: tile-local bit buffer in LDS, then words.
-          const int vsrc = uni(out[o].vsrc);
-          const uint8_t *rawv = vsrc == -3 ? (const uint8_t *)uniptr(in[slot].valid) : nullptr;
-          const uint64_t vword = vsrc_word<S>(vsrc >= 0 ? vsrc : (vsrc == -3 ? -1 : vsrc), vrf);
+          const uint64_t vword = vsrc_word<S>(out[o].vsrc, vrf);
           const int nw = (int)(((tile_prefix & 63) + tile_cnt + 63) >> 6);
           for (int i = threadIdx.x; i < nw; i += kBlock) s_vbits[i] = 0;
           __syncthreads();
           uint64_t nulls = 0;
 #pragma unroll
           for (int u = 0; u < R; u++) {
-            const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)vword, u);
-            const uint32_t hi = __builtin_amdgcn_readlane((uint32_t)(vword >> 32), u);
-            const uint64_t vw = ((uint64_t)hi << 32) | lo;
-            const bool vbit = vsrc == -3 ? valid_bit(rawv, lrow[u]) : ((vw >> lane) & 1);
+            const bool vbit = (readlane64(vword, u) >> lane) & 1;
             if (keep[u]) {
               if (vbit) {
                 const uint64_t local = (tile_prefix & 63) + (pos[u] - tile_prefix);
@@ -639,9 +596,9 @@ interp_kernel(const uint8_t *__restrict__ blob, Workspace ws) {
             const int64_t w = w0 + i;
             const bool first_shared = (i == 0) && (tile_prefix & 63);
             const bool last_shared = (i == nw - 1) && (end & 63);
-            unsigned long long *gw = (unsigned long long *)(vp + 8 * w);
+            unsigned long long *gw = (unsigned long long *)((uint8_t *)vp + 8 * w);
             if (first_shared || last_shared) atomicOr(gw, (unsigned long long)s_vbits[i]);
-            else *gw = s_vbits[i];
+            else gstore<uint64_t>(gw, 0, s_vbits[i]);
           }
           for (int off = 32; off > 0; off >>= 1) nulls += __shfl_xor(nulls, off);
           if (lane == 0 && nulls) atomicAdd(ws.null_counts + o, (unsigned long long)nulls);
@@ -650,6 +607,377 @@ interp_kernel(const uint8_t *__restrict__ blob, Workspace ws) {
       }
     }
     // Next tile.
+    __syncthreads();
+    if (threadIdx.x == 0) s_tile = (int64_t)atomicAdd(ws.tile_counter, 1u);
+    __syncthreads();
+    tile = s_tile;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// staged_kernel: the same interpreter, with each tile's input columns (and the
+// bitmaps the validity program reads) first copied HBM -> LDS in 4 KiB pieces,
+// 16 bytes per thread, all pieces of a tile in flight together. Rows are then
+// interpreted from LDS, so the bytes a workgroup keeps in flight no longer depend
+// on the register file's size. Tile = T rows (a multiple of 256 * R); wave w owns
+// local rows [w*T/4, (w+1)*T/4), processed 64*R rows per pass.
+// COMPACT: pass A evaluates the predicate for the whole tile and keeps one
+// 64-row keep mask per group in LDS; after the look-back, pass B evaluates the
+// projection only for the groups with surviving rows and scatters them.
+
+DEV void stage_tile(const CONSTANT Chunk *chunks, int nch, int64_t tile, int64_t rows, int T,
+                    uint8_t *lds) {
+  const int tid = threadIdx.x;
+  const bool full = rows == T;
+  for (int k0 = 0; k0 < nch; k0 += 8) {
+    u32x4 v[8];
+    int64_t lim[8];
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+      const int k = k0 + j;
+      lim[j] = 0;
+      if (k < nch) {
+        const CONSTANT Chunk &ck = chunks[k];
+        const int sh = ck.shift;
+        const int64_t slice = full ? ck.tile_bytes : (sh >= 0 ? (rows << sh) : ((rows + 7) >> 3));
+        const int64_t off = (int64_t)ck.chunk_off + tid * 16;
+        lim[j] = slice - off;
+        const uint8_t *src = (const uint8_t *)ck.base + tile * (int64_t)ck.tile_bytes + off;
+        if (lim[j] >= 16) {
+          v[j] = gload<u32x4>(src, 0);
+        } else if (lim[j] > 0) {  // tail of the last tile: byte loads, never past the end
+          uint32_t w[4] = {0, 0, 0, 0};
+          for (int b = 0; b < (int)lim[j]; b++) w[b >> 2] |= (uint32_t)gload<uint8_t>(src, b) << (8 * (b & 3));
+          v[j] = u32x4{w[0], w[1], w[2], w[3]};
+        }
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+      const int k = k0 + j;
+      if (k < nch && lim[j] > 0)
+        *(u32x4 *)(lds + chunks[k].lds_off + tid * 16) = v[j];
+    }
+  }
+}
+
+// Inputs of one pass from LDS into the register file (all reads issued first).
+template <int S, int R>
+DEV void read_pass(const CONSTANT InCol *in, int n_in, const uint8_t *lds, const int (&lr)[R],
+                   uint64_t (&rf)[S * R]) {
+  constexpr int MI = S < kMaxUnrolledInputs ? S : kMaxUnrolledInputs;
+  uint64_t w64[MI][R];
+  uint32_t w32[MI][R];
+#pragma unroll
+  for (int c = 0; c < MI; c++) {
+    if (c < n_in) {
+      const int kind = in[c].load;
+      const uint8_t *p = lds + in[c].lds_off;
+      if (is_wide(kind)) {
+#pragma unroll
+        for (int u = 0; u < R; u++) w64[c][u] = *(const uint64_t *)(p + 8 * lr[u]);
+      } else if (kind == LD_I32 || kind == LD_F32) {
+#pragma unroll
+        for (int u = 0; u < R; u++) w32[c][u] = *(const uint32_t *)(p + 4 * lr[u]);
+      } else if (kind == LD_I16) {
+#pragma unroll
+        for (int u = 0; u < R; u++) w32[c][u] = *(const uint16_t *)(p + 2 * lr[u]);
+      } else {
+#pragma unroll
+        for (int u = 0; u < R; u++) w32[c][u] = p[lr[u]];
+      }
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < MI; c++) {
+    if (c < n_in) {
+      const int kind = in[c].load;
+#pragma unroll
+      for (int u = 0; u < R; u++) rf[c * R + u] = is_wide(kind) ? w64[c][u] : widen32(kind, w32[c][u]);
+    }
+  }
+}
+
+// Validity words of a pass: lane u < R loads group u's word of every staged bitmap.
+template <int S, int R>
+DEV void read_pass_valid(const CONSTANT InCol *in, int n_in, const uint8_t *lds, int group0,
+                         uint64_t (&vrf)[S]) {
+  const int lane = threadIdx.x & 63;
+  const int g = group0 + (lane < R ? lane : 0);
+  for (int c = 0; c < n_in; c++) {
+    const int vs = in[c].vslot;
+    if (vs >= 0) vrf[vs] = *(const uint64_t *)(lds + in[c].lds_voff + 8 * g);
+  }
+}
+
+template <int S, int R, bool COMPACT>
+__global__ void __launch_bounds__(kBlock)
+staged_kernel(const uint8_t *__restrict__ blob_arg, Workspace ws) {
+  extern __shared__ __attribute__((aligned(16))) uint8_t lds[];
+  __shared__ int64_t s_tile;
+  __shared__ uint64_t s_wave_cnt[kWaves];
+  __shared__ uint64_t s_prefix;
+  __shared__ uint64_t s_keep[64];  // keep mask per 64-row group (T <= 4096)
+
+  const CONSTANT uint8_t *blob = (const CONSTANT uint8_t *)blob_arg;
+  const CONSTANT ProgHeader &h = *(const CONSTANT ProgHeader *)blob;
+  const CONSTANT InCol *in = (const CONSTANT InCol *)(blob + h.off_in);
+  const CONSTANT OutCol *out = (const CONSTANT OutCol *)(blob + h.off_out);
+  const CONSTANT DevOp *ops = (const CONSTANT DevOp *)(blob + h.off_ops);
+  const CONSTANT VOp *vops = (const CONSTANT VOp *)(blob + h.off_vops);
+  const CONSTANT Chunk *chunks = (const CONSTANT Chunk *)(blob + h.off_chunks);
+  const int64_t n = h.n_rows;
+  const int n_in = h.n_in, n_out = h.n_out;
+  const int n_ops_p = h.n_ops_p, n_ops_q = h.n_ops_q;
+  const int n_vops_p = h.n_vops_p, n_vops_q = h.n_vops_q;
+  const int any_valid_out = h.any_valid_out;
+  const int T = h.tile_rows, nch = h.n_chunks;
+  const int passes = T / (kWaves * 64 * R);
+
+  const int wave = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  const int64_t n_tiles = (n + T - 1) / T;
+
+  uint64_t rf[S * R];
+  uint64_t vrf[S];
+#pragma unroll
+  for (int i = 0; i < S * R; i++) rf[i] = 0;
+#pragma unroll
+  for (int i = 0; i < S; i++) vrf[i] = 0;
+
+  int64_t tile = blockIdx.x;
+  if (COMPACT) {
+    if (threadIdx.x == 0) s_tile = (int64_t)atomicAdd(ws.tile_counter, 1u);
+    __syncthreads();
+    tile = s_tile;
+  }
+  while (tile < n_tiles) {
+    const int64_t row0 = tile * T;
+    const int64_t rows = (n - row0) < T ? (n - row0) : T;
+    stage_tile(chunks, nch, tile, rows, T, lds);
+    __syncthreads();
+
+    if (!COMPACT) {
+      for (int p = 0; p < passes; p++) {
+        const int lb = wave * (T / kWaves) + p * 64 * R;
+        int lr[R];
+        int64_t grow[R], lrow[R];
+#pragma unroll
+        for (int u = 0; u < R; u++) {
+          lr[u] = lb + u * 64 + lane;
+          grow[u] = row0 + lr[u];
+          lrow[u] = grow[u] < n ? grow[u] : n - 1;
+        }
+        if (row0 + lb >= n) break;  // wave-uniform: nothing left in this tile
+        read_pass<S, R>(in, n_in, lds, lr, rf);
+        run_ops<S, R>(ops, n_ops_q, rf, lrow, in);
+        if (any_valid_out) {
+          read_pass_valid<S, R>(in, n_in, lds, lb >> 6, vrf);
+          run_vops<S>(vops, n_vops_q, vrf);
+        }
+        for (int o = 0; o < n_out; o++) {
+          const int kind = out[o].store;
+          const int slot = out[o].slot;
+          void *base = out[o].data;
+#pragma unroll
+          for (int u = 0; u < R; u++)
+            if (grow[u] < n) store_narrow(kind, base, grow[u], rf[slot * R + u]);
+          void *vp = out[o].valid;
+          if (vp != nullptr) {
+            const uint64_t word = vsrc_word<S>(out[o].vsrc, vrf);
+            const int64_t w = ((row0 + lb) >> 6) + lane;
+            uint64_t nulls = 0;
+            if (lane < R && w * 64 < n) {
+              store_vword(vp, w, n, word);
+              nulls = __builtin_popcountll(~word & low_mask(n - w * 64));
+            }
+            for (int off = 32; off > 0; off >>= 1) nulls += __shfl_xor(nulls, off);
+            if (lane == 0 && nulls) atomicAdd(ws.null_counts + o, (unsigned long long)nulls);
+          }
+        }
+      }
+      __syncthreads();  // the tile buffer is reused by the next tile
+      tile += gridDim.x;
+      continue;
+    }
+
+    // ---- COMPACT, pass A: predicate over the whole tile -> keep masks ----
+    const int st_slot = h.stencil_slot, st_width = h.stencil_width, st_vsrc = h.stencil_vsrc;
+    const uint64_t st_mask = low_mask(8 * st_width);
+    uint64_t wave_cnt = 0;
+    for (int p = 0; p < passes; p++) {
+      const int lb = wave * (T / kWaves) + p * 64 * R;
+      int lr[R];
+      int64_t grow[R], lrow[R];
+#pragma unroll
+      for (int u = 0; u < R; u++) {
+        lr[u] = lb + u * 64 + lane;
+        grow[u] = row0 + lr[u];
+        lrow[u] = grow[u] < n ? grow[u] : n - 1;
+      }
+      if (row0 + lb >= n) {
+#pragma unroll
+        for (int u = 0; u < R; u++)
+          if (lane == 0) s_keep[(lb >> 6) + u] = 0;
+        continue;
+      }
+      read_pass<S, R>(in, n_in, lds, lr, rf);
+      run_ops<S, R>(ops, n_ops_p, rf, lrow, in);
+      read_pass_valid<S, R>(in, n_in, lds, lb >> 6, vrf);
+      run_vops<S>(vops, n_vops_p, vrf);
+      const uint64_t vstencil = vsrc_word<S>(st_vsrc, vrf);
+#pragma unroll
+      for (int u = 0; u < R; u++) {
+        const bool vbit = (readlane64(vstencil, u) >> lane) & 1;
+        const bool keep = grow[u] < n && ((rf[st_slot * R + u] & st_mask) != 0) && vbit;
+        const uint64_t km = __builtin_amdgcn_ballot_w64(keep);
+        if (lane == 0) s_keep[(lb >> 6) + u] = km;
+        wave_cnt += __builtin_popcountll(km);
+      }
+    }
+    if (lane == 0) s_wave_cnt[wave] = wave_cnt;
+    __syncthreads();
+    uint64_t tile_cnt = 0, wave_prefix = 0;
+#pragma unroll
+    for (int w = 0; w < kWaves; w++) {
+      const uint64_t c = s_wave_cnt[w];
+      if (w < wave) wave_prefix += c;
+      tile_cnt += c;
+    }
+    // Single-pass decoupled look-back (wave 0). State word: bits 62-63 flag
+    // (0 = not ready, 1 = aggregate, 2 = inclusive prefix), bits 0-61 value.
+    if (wave == 0) {
+      constexpr uint64_t FA = 1ull << 62, FP = 2ull << 62, VM = (1ull << 62) - 1;
+      uint64_t prefix = 0;
+      if (tile == 0) {
+        if (lane == 0)
+          __hip_atomic_store(ws.tile_state, FP | tile_cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      } else {
+        if (lane == 0)
+          __hip_atomic_store(ws.tile_state + tile, FA | tile_cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        int64_t look = tile - 1;
+        while (true) {
+          const int64_t idx = look - lane;
+          uint64_t st = FP;  // virtual inclusive prefix 0 before tile 0
+          if (idx >= 0)
+            st = __hip_atomic_load(ws.tile_state + idx, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          const uint64_t flag = st >> 62;
+          const uint64_t is_x = __builtin_amdgcn_ballot_w64(flag == 0);
+          const uint64_t is_p = __builtin_amdgcn_ballot_w64(flag == 2);
+          const uint64_t upto = is_p ? (((is_p & (~is_p + 1)) << 1) - 1) : ~0ull;
+          if (is_x & upto) {
+            __builtin_amdgcn_s_sleep(1);
+            continue;
+          }
+          uint64_t v = ((upto >> lane) & 1) ? (st & VM) : 0;
+          for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+          prefix += v;
+          if (is_p) break;
+          look -= 64;
+        }
+        if (lane == 0)
+          __hip_atomic_store(ws.tile_state + tile, FP | (prefix + tile_cnt), __ATOMIC_RELAXED,
+                             __HIP_MEMORY_SCOPE_AGENT);
+      }
+      if (lane == 0) {
+        s_prefix = prefix;
+        if (tile == n_tiles - 1) *ws.n_sel = prefix + tile_cnt;
+      }
+    }
+    __syncthreads();
+    const uint64_t tile_prefix = s_prefix;
+
+    if (tile_cnt > 0) {
+      // ---- pass B: projection / gather of the surviving rows, in row order ----
+      uint64_t *vbits = (uint64_t *)(lds + h.lds_vbits_off);
+      const int n_vout = h.n_valid_out;
+      const int nw = (int)(((tile_prefix & 63) + tile_cnt + 63) >> 6);
+      if (n_vout > 0) {
+        for (int i = threadIdx.x; i < n_vout * (T / 64 + 2); i += kBlock) vbits[i] = 0;
+        __syncthreads();
+      }
+      uint64_t run = tile_prefix + wave_prefix;
+      uint64_t vnulls[4] = {0, 0, 0, 0};  // per output with validity (first four), this lane
+      for (int p = 0; p < passes; p++) {
+        const int lb = wave * (T / kWaves) + p * 64 * R;
+        if (row0 + lb >= n) break;
+        uint64_t km[R], pos[R];
+        uint64_t any = 0;
+#pragma unroll
+        for (int u = 0; u < R; u++) {
+          km[u] = s_keep[(lb >> 6) + u];
+          const uint32_t below = __builtin_amdgcn_mbcnt_hi((uint32_t)(km[u] >> 32),
+                                     __builtin_amdgcn_mbcnt_lo((uint32_t)km[u], 0u));
+          pos[u] = run + below;
+          run += __builtin_popcountll(km[u]);
+          any |= km[u];
+        }
+        if (!any) continue;  // wave-uniform
+        int lr[R];
+        int64_t lrow[R];
+#pragma unroll
+        for (int u = 0; u < R; u++) {
+          lr[u] = lb + u * 64 + lane;
+          const int64_t g = row0 + lr[u];
+          lrow[u] = g < n ? g : n - 1;
+        }
+        read_pass<S, R>(in, n_in, lds, lr, rf);
+        run_ops<S, R>(ops + n_ops_p, n_ops_q, rf, lrow, in);
+        if (any_valid_out) {
+          read_pass_valid<S, R>(in, n_in, lds, lb >> 6, vrf);
+          run_vops<S>(vops + n_vops_p, n_vops_q, vrf);
+        }
+        for (int o = 0; o < n_out; o++) {
+          const int kind = out[o].store;
+          const int slot = out[o].slot;
+          void *base = out[o].data;
+#pragma unroll
+          for (int u = 0; u < R; u++)
+            if ((km[u] >> lane) & 1) store_narrow(kind, base, (int64_t)pos[u], rf[slot * R + u]);
+          if (out[o].valid != nullptr) {
+            const int vb = out[o].vbuf;
+            const uint64_t vword = vsrc_word<S>(out[o].vsrc, vrf);
+#pragma unroll
+            for (int u = 0; u < R; u++) {
+              if ((km[u] >> lane) & 1) {
+                if ((readlane64(vword, u) >> lane) & 1) {
+                  const uint64_t local = (tile_prefix & 63) + (pos[u] - tile_prefix);
+                  atomicOr((unsigned long long *)&vbits[vb * (T / 64 + 2) + (local >> 6)], 1ull << (local & 63));
+                } else if (vb < 4) {
+                  vnulls[vb]++;
+                } else {
+                  atomicAdd(ws.null_counts + o, 1ull);
+                }
+              }
+            }
+          }
+        }
+      }
+      if (n_vout > 0) {
+        __syncthreads();
+        const int64_t w0 = (int64_t)(tile_prefix >> 6);
+        const uint64_t end = tile_prefix + tile_cnt;
+        for (int o = 0; o < n_out; o++) {
+          void *vp = out[o].valid;
+          if (vp == nullptr) continue;
+          const int vb = out[o].vbuf;
+          for (int i = threadIdx.x; i < nw; i += kBlock) {
+            const bool first_shared = (i == 0) && (tile_prefix & 63);
+            const bool last_shared = (i == nw - 1) && (end & 63);
+            unsigned long long *gw = (unsigned long long *)((uint8_t *)vp + 8 * (w0 + i));
+            const uint64_t word = vbits[vb * (T / 64 + 2) + i];
+            if (first_shared || last_shared) atomicOr(gw, (unsigned long long)word);
+            else gstore<uint64_t>(gw, 0, word);
+          }
+          if (vb < 4) {
+            uint64_t nl = vnulls[vb];
+            for (int off = 32; off > 0; off >>= 1) nl += __shfl_xor(nl, off);
+            if (lane == 0 && nl) atomicAdd(ws.null_counts + o, (unsigned long long)nl);
+          }
+        }
+      }
+    }
+    // Next tile (the LDS tile buffer and keep masks are reused).
     __syncthreads();
     if (threadIdx.x == 0) s_tile = (int64_t)atomicAdd(ws.tile_counter, 1u);
     __syncthreads();
@@ -704,6 +1032,13 @@ fill_zero_kernel(uint64_t *p, int64_t n_words) {
     p[i] = 0;
 }
 
+// Plain 16-byte-wide copy (also the reference point for HBM byte counters).
+__global__ void __launch_bounds__(kBlock)
+copy_kernel(uint4 *__restrict__ dst, const uint4 *__restrict__ src, int64_t n16) {
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n16; i += (int64_t)gridDim.x * kBlock)
+    dst[i] = src[i];
+}
+
 // popcount of validity bits < n -> null count (gdf_count_nonzero_mask's role).
 __global__ void __launch_bounds__(kBlock)
 null_count_kernel(const uint8_t *valid, int64_t n, unsigned long long *nulls_out) {
@@ -729,16 +1064,53 @@ static hipError_t launch_sr(bool compact, int grid, hipStream_t st, const uint8_
   return hipGetLastError();
 }
 
+template <int S, int R>
+static hipError_t launch_staged_sr(bool compact, int grid, int lds, hipStream_t st, const uint8_t *blob,
+                                   Workspace ws) {
+  if (compact)
+    hipLaunchKernelGGL((staged_kernel<S, R, true>), dim3(grid), dim3(kBlock), lds, st, blob, ws);
+  else
+    hipLaunchKernelGGL((staged_kernel<S, R, false>), dim3(grid), dim3(kBlock), lds, st, blob, ws);
+  return hipGetLastError();
+}
+
 // Register-file shapes: S slots x R rows per lane. S*R <= 16 keeps the file in
 // VGPRs; S = 32 / 64 (R = 1) serve large plans with the file in scratch.
-hipError_t launch_interp(int s, int r, bool compact, int grid, hipStream_t st, const uint8_t *blob,
-                         Workspace ws) {
+// lds_bytes > 0 selects the LDS-staged kernel (shapes (8,1), (8,2), (16,1)).
+hipError_t launch_interp(int s, int r, bool compact, int grid, int lds_bytes, hipStream_t st,
+                         const uint8_t *blob, Workspace ws) {
+  if (lds_bytes > 0) {
+    if (s == 8 && r == 1) return launch_staged_sr<8, 1>(compact, grid, lds_bytes, st, blob, ws);
+    if (s == 8 && r == 2) return launch_staged_sr<8, 2>(compact, grid, lds_bytes, st, blob, ws);
+    if (s == 16 && r == 1) return launch_staged_sr<16, 1>(compact, grid, lds_bytes, st, blob, ws);
+    return hipErrorInvalidValue;
+  }
   if (s == 8 && r == 1) return launch_sr<8, 1>(compact, grid, st, blob, ws);
   if (s == 8 && r == 2) return launch_sr<8, 2>(compact, grid, st, blob, ws);
   if (s == 16 && r == 1) return launch_sr<16, 1>(compact, grid, st, blob, ws);
   if (s == 32 && r == 1) return launch_sr<32, 1>(compact, grid, st, blob, ws);
   if (s == 64 && r == 1) return launch_sr<64, 1>(compact, grid, st, blob, ws);
   return hipErrorInvalidValue;
+}
+
+// Workgroups of the kernel that fit on one CU at once.
+int interp_blocks_per_cu(int s, int r, bool compact, int lds_bytes) {
+  int blocks = 0;
+  hipError_t e = hipErrorInvalidValue;
+#define BLZ_OCC(KERNEL) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, KERNEL, kBlock, lds_bytes)
+  if (lds_bytes > 0) {
+    if (s == 8 && r == 1) { if (compact) BLZ_OCC((staged_kernel<8, 1, true>)); else BLZ_OCC((staged_kernel<8, 1, false>)); }
+    else if (s == 8 && r == 2) { if (compact) BLZ_OCC((staged_kernel<8, 2, true>)); else BLZ_OCC((staged_kernel<8, 2, false>)); }
+    else if (s == 16 && r == 1) { if (compact) BLZ_OCC((staged_kernel<16, 1, true>)); else BLZ_OCC((staged_kernel<16, 1, false>)); }
+  } else {
+    if (s == 8 && r == 1) { if (compact) BLZ_OCC((interp_kernel<8, 1, true>)); else BLZ_OCC((interp_kernel<8, 1, false>)); }
+    else if (s == 8 && r == 2) { if (compact) BLZ_OCC((interp_kernel<8, 2, true>)); else BLZ_OCC((interp_kernel<8, 2, false>)); }
+    else if (s == 16 && r == 1) { if (compact) BLZ_OCC((interp_kernel<16, 1, true>)); else BLZ_OCC((interp_kernel<16, 1, false>)); }
+    else if (s == 32 && r == 1) { if (compact) BLZ_OCC((interp_kernel<32, 1, true>)); else BLZ_OCC((interp_kernel<32, 1, false>)); }
+    else if (s == 64 && r == 1) { if (compact) BLZ_OCC((interp_kernel<64, 1, true>)); else BLZ_OCC((interp_kernel<64, 1, false>)); }
+  }
+#undef BLZ_OCC
+  return (e == hipSuccess && blocks > 0) ? blocks : 1;
 }
 
 hipError_t launch_gather(int width, const void *in, const uint8_t *in_valid, int64_t n_in, const int32_t *idx,
@@ -780,6 +1152,11 @@ hipError_t launch_fill_zero(void *p, int64_t bytes, int grid, hipStream_t st) {
 hipError_t launch_null_count(const uint8_t *valid, int64_t n, unsigned long long *out, int grid,
                              hipStream_t st) {
   hipLaunchKernelGGL(null_count_kernel, dim3(grid), dim3(kBlock), 0, st, valid, n, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_copy(void *dst, const void *src, int64_t bytes, int grid, hipStream_t st) {
+  hipLaunchKernelGGL(copy_kernel, dim3(grid), dim3(kBlock), 0, st, (uint4 *)dst, (const uint4 *)src, bytes / 16);
   return hipGetLastError();
 }
 

@@ -600,14 +600,77 @@ Compiled compile_program(const CompileRequest &req) {
   emit(order_p, ops_p);
   emit(order_q, ops_q);
 
-  // Assemble the blob.
+  // LDS staging layout (staged kernel): per tile of T rows, each input's values
+  // and each needed bitmap, 16-byte aligned, then the compacted-validity words.
   const int n_out = (int)req.outputs.size();
+  int n_valid_out = 0;
+  for (auto &o : req.outputs) n_valid_out += o.valid != nullptr;
+  std::vector<int> lds_off(n_dev, 0), lds_voff(n_dev, -1);
+  std::vector<Chunk> chunks;
+  int T = 0, lds_bytes = 0, vbits_off = 0;
+  bool staged_ok = (S == 8 || S == 16) && S * R <= 16 && req.force_t >= 0;
+  for (int c = 0; c < n_dev && staged_ok; c++) {
+    const int w = wid(req.inputs[c].dtype);
+    if (w == 0 && load_kind(req.inputs[c].dtype) != LD_NONE) staged_ok = false;
+    if ((uintptr_t)req.inputs[c].data % 16) staged_ok = false;
+    if (B.vnodes[c].needed && (uintptr_t)req.inputs[c].valid % 16) staged_ok = false;
+  }
+  if (staged_ok) {
+    auto layout_for = [&](int t) {
+      size_t off = 0;
+      for (int c = 0; c < n_dev; c++) {
+        lds_off[c] = (int)off;
+        off += (size_t)t * std::max(1, wid(req.inputs[c].dtype));
+        off = (off + 15) & ~size_t(15);
+      }
+      for (int c = 0; c < n_dev; c++) {
+        lds_voff[c] = -1;
+        if (B.vnodes[c].needed) {
+          lds_voff[c] = (int)off;
+          off += (size_t)t / 8;
+          off = (off + 15) & ~size_t(15);
+        }
+      }
+      vbits_off = (int)off;
+      off += (size_t)n_valid_out * (t / 64 + 2) * 8;
+      return off;
+    };
+    const int unit = 256 * R;
+    if (req.force_t > 0) {
+      T = req.force_t;
+      if (T % unit || T > 4096 || (int)layout_for(T) > req.lds_budget) fail("bad tile size for this plan");
+    } else {
+      // Largest tile whose staging stays within ~52 KiB (three workgroups per CU).
+      for (int t = 4096; t >= unit; t /= 2)
+        if (t % unit == 0 && layout_for(t) <= (size_t)std::min(req.lds_budget, 52 * 1024)) { T = t; break; }
+    }
+    if (T > 0) {
+      lds_bytes = (int)layout_for(T);
+      for (int c = 0; c < n_dev; c++) {
+        const int w = std::max(1, wid(req.inputs[c].dtype));
+        const int sh = w == 1 ? 0 : w == 2 ? 1 : w == 4 ? 2 : 3;
+        const int slice = T * w;
+        for (int off = 0; off < slice; off += kChunkBytes)
+          chunks.push_back(Chunk{req.inputs[c].data, slice, off, lds_off[c] + off, sh});
+      }
+      for (int c = 0; c < n_dev; c++) {
+        if (lds_voff[c] < 0) continue;
+        const int slice = T / 8;
+        for (int off = 0; off < slice; off += kChunkBytes)
+          chunks.push_back(Chunk{req.inputs[c].valid, slice, off, lds_voff[c] + off, -3});
+      }
+    }
+  }
+
+  // Assemble the blob.
   size_t off_in = (sizeof(ProgHeader) + 15) & ~size_t(15);
   size_t off_out = off_in + sizeof(InCol) * n_dev;
   off_out = (off_out + 15) & ~size_t(15);
   size_t off_ops = off_out + sizeof(OutCol) * n_out;
   off_ops = (off_ops + 15) & ~size_t(15);
-  size_t off_vops = off_ops + sizeof(DevOp) * (ops_p.size() + ops_q.size());
+  size_t off_chunks = off_ops + sizeof(DevOp) * (ops_p.size() + ops_q.size());
+  off_chunks = (off_chunks + 15) & ~size_t(15);
+  size_t off_vops = off_chunks + sizeof(Chunk) * chunks.size();
   size_t total = off_vops + sizeof(VOp) * (vops_p.size() + vops_q.size());
   total = (total + 15) & ~size_t(15);
   if ((int)total > blob_capacity()) fail("plan too large for the on-chip program buffer");
@@ -629,6 +692,12 @@ Compiled compile_program(const CompileRequest &req) {
   int any_valid = 0;
   for (auto &o : req.outputs) any_valid |= o.valid != nullptr;
   h.any_valid_out = any_valid;
+  h.tile_rows = T;
+  h.n_chunks = (int)chunks.size();
+  h.lds_vbits_off = vbits_off;
+  h.n_valid_out = n_valid_out;
+  h.lds_bytes = lds_bytes;
+  h.off_chunks = (int)off_chunks;
   h.off_in = (int)off_in;
   h.off_out = (int)off_out;
   h.off_ops = (int)off_ops;
@@ -641,8 +710,12 @@ Compiled compile_program(const CompileRequest &req) {
     ic.valid = req.inputs[c].valid;
     ic.load = load_kind(req.inputs[c].dtype);
     ic.vslot = B.vnodes[c].needed ? c : -1;
+    ic.lds_off = lds_off[c];
+    ic.lds_voff = lds_voff[c];
     put(out.blob, off_in + sizeof(InCol) * c, ic);
   }
+  for (size_t k = 0; k < chunks.size(); k++) put(out.blob, off_chunks + sizeof(Chunk) * k, chunks[k]);
+  int vbuf_next = 0;
   for (int o = 0; o < n_out; o++) {
     const OutSpec &s = req.outputs[o];
     OutCol oc{};
@@ -652,6 +725,7 @@ Compiled compile_program(const CompileRequest &req) {
     oc.slot = (int16_t)B.nodes[out_node[o]].slot;
     oc.src_kind = (int16_t)s.kind;
     oc.vsrc = vsrc(out_v[o]);
+    oc.vbuf = s.valid ? vbuf_next++ : -1;
     put(out.blob, off_out + sizeof(OutCol) * o, oc);
   }
   size_t p = off_ops;
@@ -666,6 +740,8 @@ Compiled compile_program(const CompileRequest &req) {
   out.n_vslots = n_vslots;
   out.n_ops = (int)(ops_p.size() + ops_q.size());
   out.n_vops = (int)(vops_p.size() + vops_q.size());
+  out.T = T;
+  out.lds_bytes = lds_bytes;
   return out;
 }
 

@@ -90,7 +90,20 @@ struct InCol {
   const uint8_t *valid;  // raw bitmap (may be null = all valid)
   int32_t load;          // LoadKind
   int32_t vslot;         // validity slot this column's words load into, or -1 (forced / unused)
+  int32_t lds_off;       // staged kernel: LDS offset of the tile's values
+  int32_t lds_voff;      // staged kernel: LDS offset of the tile's bitmap words (-1: none)
 };
+
+// Staged kernel: one 4 KiB piece of a tile's column (or bitmap) slice, copied
+// HBM -> LDS by the 256 threads of a workgroup, 16 bytes each.
+struct Chunk {
+  const void *base;      // column data / bitmap base
+  int32_t tile_bytes;    // bytes of the slice per full tile (T * width, or T / 8)
+  int32_t chunk_off;     // offset of this piece within the slice
+  int32_t lds_off;       // LDS destination of the piece
+  int32_t shift;         // log2(width) of a data slice; -3 for a bitmap slice
+};
+constexpr int kChunkBytes = 4096;
 
 // Output descriptor. `src_kind`: 0 = program slot (of the program that owns the
 // output), 1 = raw input column (a gathered alias).
@@ -100,9 +113,8 @@ struct OutCol {
   int32_t store;         // StoreKind
   int16_t slot;          // data slot (src_kind 0) or input column (src_kind 1)
   int16_t src_kind;
-  int32_t vsrc;          // >= 0 validity slot; -1 = constant all-valid; -2 = constant all-null;
-                         // -3 = raw validity of input column `slot` (gathered alias)
-  int32_t pad;
+  int32_t vsrc;          // >= 0 validity slot; -1 = constant all-valid; -2 = constant all-null
+  int32_t vbuf;          // staged compaction: index of this output's LDS validity buffer
 };
 
 struct ProgHeader {
@@ -116,10 +128,14 @@ struct ProgHeader {
   int32_t stencil_width;         // compaction: bytes of the predicate value that decide "non-zero"
   int32_t stencil_vsrc;          // compaction: validity of the predicate (same codes as OutCol::vsrc)
   int32_t any_valid_out;         // some output has validity
-  int32_t pad;
+  int32_t tile_rows;             // staged kernel: rows per tile (T)
+  int32_t n_chunks;              // staged kernel: HBM -> LDS pieces per tile
+  int32_t lds_vbits_off;         // staged kernel: LDS offset of compacted-validity words
+  int32_t n_valid_out;           // outputs with validity
+  int32_t lds_bytes;             // staged kernel: dynamic LDS per workgroup
   // offsets (bytes from the blob start)
-  int32_t off_in, off_out, off_ops, off_vops;
-  int32_t blob_bytes, pad2;
+  int32_t off_in, off_out, off_ops, off_vops, off_chunks;
+  int32_t blob_bytes;
 };
 
 }  // namespace blz

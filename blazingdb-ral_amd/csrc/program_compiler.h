@@ -49,12 +49,16 @@ struct CompileRequest {
   std::vector<OutSpec> outputs;
   int scalar_mode = BLZ_SCALAR_INTENDED;
   int force_s = 0, force_r = 0;     // register-file shape override (autotuner)
+  int force_t = 0;                  // staged tile rows override (autotuner); -1 = direct kernel
+  int lds_budget = 64 * 1024;       // dynamic LDS per workgroup the staged kernel may use
 };
 
 struct Compiled {
   std::vector<uint8_t> blob;
   int S = 8, R = 1;
   int n_slots = 0, n_vslots = 0, n_ops = 0, n_vops = 0;
+  int T = 0;          // staged kernel tile rows (0: direct kernel)
+  int lds_bytes = 0;  // staged kernel dynamic LDS per workgroup
 };
 
 // Throws std::runtime_error with a message on an unsupported or malformed plan.

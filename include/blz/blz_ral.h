@@ -133,11 +133,13 @@ typedef struct blz_program_s *blz_program;
  * projection; project_query_part NULL keeps every frame column) against fixed
  * device buffers the program allocates and owns: out_columns[0..*n_out) receive
  * them (capacity: every frame row; sizes valid after blz_program_result).
- * s / r (0 = automatic) force the register-file shape (8,1) (8,2) (16,1) (32,1) (64,1). */
+ * s / r (0 = automatic) force the register-file shape (8,1) (8,2) (16,1) (32,1) (64,1);
+ * t (0 = automatic) forces the LDS-staged tile rows (a multiple of 256*r, <= 4096) or,
+ * with t = -1, the unstaged kernel. */
 int blz_program_create(blz_program *prog, gdf_column *const *frame, int n_frame,
                        const char *filter_query_part, const char *project_query_part,
                        gdf_column *out_columns, int max_out, int *n_out, int scalar_mode, int s,
-                       int r, char *err, size_t err_len);
+                       int r, int t, char *err, size_t err_len);
 /* Enqueues one execution on `stream` (no allocation, no synchronisation). */
 int blz_program_launch(blz_program prog, blz_stream_t stream, char *err, size_t err_len);
 /* Waits for the last launch and reads back rows kept and per-output null counts. */
@@ -150,6 +152,8 @@ int blz_program_time(blz_program prog, blz_stream_t stream, int iters, double *m
 /* Register-file shape and sizes chosen for the program. */
 int blz_program_info(blz_program prog, int *s, int *r, int *n_ops, int *n_vops, int *n_slots,
                      int *grid);
+/* Tile rows (0 = unstaged kernel) and dynamic LDS bytes per workgroup. */
+int blz_program_tile(blz_program prog, int *tile_rows, int *lds_bytes);
 int blz_program_destroy(blz_program prog);
 
 /* ---- memory / streams (HIP plumbing for hosts without a HIP toolchain) ------- */
@@ -159,6 +163,9 @@ int blz_device_free(void *ptr);
 int blz_memcpy_htod(void *dst, const void *src, size_t bytes, blz_stream_t stream);
 int blz_memcpy_dtoh(void *dst, const void *src, size_t bytes, blz_stream_t stream);
 int blz_memset_device(void *dst, int value, size_t bytes, blz_stream_t stream);
+/* Device-to-device copy by a plain streaming kernel (bytes and both pointers 16-byte
+ * multiples); its hardware byte counters calibrate traffic measurements. */
+int blz_device_copy(void *dst, const void *src, size_t bytes, blz_stream_t stream);
 int blz_stream_create(blz_stream_t *stream);
 int blz_stream_synchronize(blz_stream_t stream);
 int blz_stream_destroy(blz_stream_t stream);

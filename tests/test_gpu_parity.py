@@ -141,7 +141,8 @@ def test_random_expressions_with_nulls(B, seed):
     nulls = B.evaluate_expression(dev, expr, out)
     assert_column_equal(out, ref, check_valid=False, what=expr)
     gv = out.valid()
-    assert np.array_equal(gv, ref.valid), f"validity differs: {expr}"
+    ref_valid = ref.valid if ref.valid is not None else np.ones(n, bool)
+    assert np.array_equal(gv, ref_valid), f"validity differs: {expr}"
     assert nulls == ref.null_count
 
 
