@@ -98,8 +98,8 @@ def lib():
             "blz_apply_boolean_mask": [colp, colp, colp, P, ERR, SZ],
             "blz_update_null_count": [colp, P, ERR, SZ],
             "blz_program_create": [C.POINTER(P), colpp, I, C.c_char_p, C.c_char_p, colp, I,
-                                   C.POINTER(C.c_int), I, I, I, I, ERR, SZ],
-            "blz_program_tile": [P, C.POINTER(C.c_int), C.POINTER(C.c_int)],
+                                   C.POINTER(C.c_int), I, I, I, I, I, ERR, SZ],
+            "blz_program_tile": [P, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)],
             "blz_program_launch": [P, P, ERR, SZ],
             "blz_program_result": [P, P, C.POINTER(C.c_int64), I32P, ERR, SZ],
             "blz_program_time": [P, P, I, C.POINTER(C.c_double), ERR, SZ],
@@ -404,7 +404,7 @@ class Program:
 
     def __init__(self, frame: Sequence[DeviceColumn], filter_query_part: Optional[str],
                  project_query_part: Optional[str], scalar_mode=SCALAR_INTENDED, s=0, r=0, t=0,
-                 max_out=64):
+                 kernel=0, max_out=64):
         self.frame = list(frame)
         self.h = C.c_void_p(0)
         out = (GdfColumn * max_out)()
@@ -412,13 +412,13 @@ class Program:
         _call(lib().blz_program_create, C.byref(self.h), _colpp(frame), len(frame),
               filter_query_part.encode() if filter_query_part else None,
               project_query_part.encode() if project_query_part else None, out, max_out,
-              C.byref(n_out), scalar_mode, s, r, t)
+              C.byref(n_out), scalar_mode, kernel, s, r, t)
         info = [C.c_int(0) for _ in range(6)]
         lib().blz_program_info(self.h, *[C.byref(x) for x in info])
         self.s, self.r, self.n_ops, self.n_vops, self.n_slots, self.grid = [x.value for x in info]
-        tr, lb = C.c_int(0), C.c_int(0)
-        lib().blz_program_tile(self.h, C.byref(tr), C.byref(lb))
-        self.tile_rows, self.lds_bytes = tr.value, lb.value
+        tr, lb, kk = C.c_int(0), C.c_int(0), C.c_int(0)
+        lib().blz_program_tile(self.h, C.byref(tr), C.byref(lb), C.byref(kk))
+        self.tile_rows, self.lds_bytes, self.kernel = tr.value, lb.value, kk.value
         self.outputs = [DeviceColumn.adopt(out[i], owned=False) for i in range(n_out.value)]
 
     def launch(self, stream=None):

@@ -89,7 +89,9 @@ int64_t tile_rows(const Compiled &c) { return c.T > 0 ? c.T : rows_per_tile(c.R)
 // Persistent grid: every workgroup that fits on the device at once.
 int interp_grid(const Compiled &c, bool compact, int64_t n) {
   const int64_t tiles = (n + tile_rows(c) - 1) / tile_rows(c);
-  const int64_t resident = (int64_t)device_cus() * interp_blocks_per_cu(c.S, c.R, compact, c.lds_bytes);
+  const int per_cu = c.acc ? acc_blocks_per_cu(c.S, c.R, compact, c.lds_bytes)
+                           : interp_blocks_per_cu(c.S, c.R, compact, c.lds_bytes);
+  const int64_t resident = (int64_t)device_cus() * per_cu;
   return (int)std::max<int64_t>(1, std::min<int64_t>(tiles, resident));
 }
 
@@ -134,6 +136,12 @@ void bind(LaunchBuffers &lb, int n_out) {
   lb.ws.tile_state = (unsigned long long *)(z + nc + 64);
 }
 
+hipError_t launch_program(const Compiled &c, bool compact, int grid, hipStream_t st, const uint8_t *blob,
+                          const Workspace &ws) {
+  if (c.acc) return launch_acc(c.S, c.R, compact, grid, c.lds_bytes, st, blob, ws);
+  return launch_interp(c.S, c.R, compact, grid, c.lds_bytes, st, blob, ws);
+}
+
 struct RunResult {
   int64_t n_sel = 0;
   std::vector<int32_t> nulls;
@@ -154,7 +162,7 @@ RunResult run_once(const Compiled &c, bool compact, int64_t n, int n_out, hipStr
   if (e == hipSuccess) e = launch_fill_zero(lb.base + lb.zero_off, (int64_t)lb.zero_bytes,
                                             (int)std::min<size_t>(1024, lb.zero_bytes / 2048 + 1), st);
   if (e == hipSuccess)
-    e = launch_interp(c.S, c.R, compact, interp_grid(c, compact, n), c.lds_bytes, st, lb.blob, lb.ws);
+    e = launch_program(c, compact, interp_grid(c, compact, n), st, lb.blob, lb.ws);
   if (e == hipSuccess && n_out > 0)
     e = hipMemcpyAsync(host.data(), lb.ws.null_counts, sizeof(unsigned long long) * n_out,
                        hipMemcpyDeviceToHost, st);
@@ -585,7 +593,7 @@ struct blz_program_s {
 
 int blz_program_create(blz_program *prog, gdf_column *const *frame, int n_frame, const char *filter_qp,
                        const char *proj_qp, gdf_column *out_columns, int n_out_cap, int *n_out,
-                       int scalar_mode, int s, int r, int t, char *err, size_t err_len) {
+                       int scalar_mode, int kernel, int s, int r, int t, char *err, size_t err_len) {
   return guarded(err, err_len, [&] {
     if (!prog) invalid("prog is NULL");
     *prog = nullptr;
@@ -619,6 +627,7 @@ int blz_program_create(blz_program *prog, gdf_column *const *frame, int n_frame,
       req.force_s = s;
       req.force_r = r;
       req.force_t = t;
+      req.kernel = kernel;
       for (size_t i = 0; i < outs.size(); i++) {
         const auto &o = outs[i];
         const bool with_valid = o.kind == 0 ? true : frame[o.index]->valid != nullptr;
@@ -669,7 +678,7 @@ hipError_t program_enqueue(blz_program p, hipStream_t st) {
       e = launch_fill_zero(p->outs[i].valid, (int64_t)vb, (int)std::min<size_t>(1024, vb / 2048 + 1), st);
     }
   if (e == hipSuccess)
-    e = launch_interp(p->c.S, p->c.R, p->compact, p->grid, p->c.lds_bytes, st, p->lb.blob, p->lb.ws);
+    e = launch_program(p->c, p->compact, p->grid, st, p->lb.blob, p->lb.ws);
   return e;
 }
 }  // namespace
@@ -718,8 +727,7 @@ int blz_program_time(blz_program prog, blz_stream_t stream, int iters, double *m
                                     S(stream)));
         }
       HIPCHECK(hipEventRecord(prog->ev0, S(stream)));
-      HIPCHECK(launch_interp(prog->c.S, prog->c.R, prog->compact, prog->grid, prog->c.lds_bytes, S(stream),
-                             prog->lb.blob, prog->lb.ws));
+      HIPCHECK(launch_program(prog->c, prog->compact, prog->grid, S(stream), prog->lb.blob, prog->lb.ws));
       HIPCHECK(hipEventRecord(prog->ev1, S(stream)));
       HIPCHECK(hipEventSynchronize(prog->ev1));
       float ms = 0;
@@ -741,10 +749,11 @@ int blz_program_info(blz_program prog, int *s, int *r, int *n_ops, int *n_vops, 
   return BLZ_OK;
 }
 
-int blz_program_tile(blz_program prog, int *tile_rows, int *lds_bytes) {
+int blz_program_tile(blz_program prog, int *tile_rows, int *lds_bytes, int *kernel) {
   if (!prog) return BLZ_ERR_INVALID;
   if (tile_rows) *tile_rows = prog->c.T;
   if (lds_bytes) *lds_bytes = prog->c.lds_bytes;
+  if (kernel) *kernel = prog->c.acc ? 1 : prog->c.T > 0 ? 2 : 3;
   return BLZ_OK;
 }
 

@@ -24,6 +24,10 @@ struct Workspace {
 hipError_t launch_interp(int s, int r, bool compact, int grid, int lds_bytes, hipStream_t st,
                          const uint8_t *blob, Workspace ws);
 int interp_blocks_per_cu(int s, int r, bool compact, int lds_bytes);
+// Accumulator-form staged kernel: (nt, r) in {(2,8), (4,4), (8,2)} temporaries x rows per lane.
+hipError_t launch_acc(int nt, int r, bool compact, int grid, int lds_bytes, hipStream_t st,
+                      const uint8_t *blob, Workspace ws);
+int acc_blocks_per_cu(int nt, int r, bool compact, int lds_bytes);
 // materialize_column: width in {1,2,4,8}.
 hipError_t launch_gather(int width, const void *in, const uint8_t *in_valid, int64_t n_in, const int32_t *idx,
                          int64_t n_idx, void *out, uint8_t *out_valid, unsigned long long *null_count,

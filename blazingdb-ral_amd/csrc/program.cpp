@@ -23,6 +23,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <functional>
 #include <map>
 #include <stdexcept>
 
@@ -572,12 +573,169 @@ Compiled compile_program(const CompileRequest &req) {
   if (n_vslots > kMaxVSlots) fail("plan needs more than 64 validity registers");
   auto vsrc = [&](int v) { return v == V_ALL ? -1 : v == V_NONE ? -2 : B.vnodes[v].vslot; };
 
+  // Accumulator form: expression trees lowered to acc = acc OP operand code.
+  std::vector<AccOp> acc_p, acc_q;
+  int acc_temps = 0;
+  bool acc_ok = n_vslots <= 8 && req.kernel != 2 && req.kernel != 3;
+  if (acc_ok) {
+    std::vector<int> uses(B.nodes.size(), 0);
+    for (auto &n : B.nodes)
+      if (n.kind == 1) {
+        if (!n.a.imm && n.a.node >= 0) uses[n.a.node]++;
+        if (!n.b.imm && n.b.node >= 0) uses[n.b.node]++;
+      }
+    std::vector<int> mat(B.nodes.size(), -1);  // materialised node -> temp
+    std::vector<bool> tbusy(8, false);
+    std::vector<AccOp> *code = nullptr;
+    auto alloc_t = [&]() {
+      for (int t = 0; t < 8; t++)
+        if (!tbusy[t]) { tbusy[t] = true; acc_temps = std::max(acc_temps, t + 1); return t; }
+      throw std::runtime_error("accumulator form needs more than 8 temporaries");
+    };
+    auto is_leaf = [&](int id) { return B.nodes[id].kind != 1 || mat[id] >= 0; };
+    auto set_operand = [&](AccOp &a, const Operand &o) {
+      if (o.imm) { a.okind = OK_IMM; a.imm = o.bits; return; }
+      const Node &n = B.nodes[o.node];
+      if (mat[o.node] >= 0) { a.okind = OK_TEMP; a.oidx = (uint8_t)mat[o.node]; }
+      else if (n.kind == 0) { a.okind = OK_INPUT; a.oidx = (uint8_t)n.aux; }
+      else { a.okind = OK_IMM; a.imm = 0; }  // constant zero
+    };
+    auto release = [&](const Operand &o) {  // a use of a materialised node is done
+      if (o.imm || o.node < 0 || mat[o.node] < 0) return;
+      if (--uses[o.node] == 0) tbusy[mat[o.node]] = false;
+    };
+    auto new_op = [](int code) { AccOp a{}; a.code = (uint8_t)code; a.vcol = 0xFF; return a; };
+    auto is_cmp = [](int c) { return c >= OP_EQ_I && c <= OP_GE_F; };
+    auto flip = [](int c) {  // imm OP x  ==  x FLIP(OP) imm
+      switch (c) {
+        case OP_LT_I: return (int)OP_GT_I; case OP_GT_I: return (int)OP_LT_I;
+        case OP_LE_I: return (int)OP_GE_I; case OP_GE_I: return (int)OP_LE_I;
+        case OP_LT_F: return (int)OP_GT_F; case OP_GT_F: return (int)OP_LT_F;
+        case OP_LE_F: return (int)OP_GE_F; case OP_GE_F: return (int)OP_LE_F;
+        default: return c;
+      }
+    };
+    std::function<void(int)> gen = [&](int id) {
+      Node &n = B.nodes[id];
+      if (is_leaf(id)) {
+        AccOp a = new_op(A_LOAD);
+        Operand o; o.node = id;
+        set_operand(a, o);
+        code->push_back(a);
+        return;
+      }
+      const int op = n.op.code;
+      const int ca = n.op.conv & 3, cb = (n.op.conv >> 2) & 3, co = (n.op.conv >> 4) & 3;
+      const bool unary = op >= OP_FLOOR;
+      if (unary) {
+        if (op != OP_IS_NULL && op != OP_IS_NOT_NULL) {
+          if (n.a.imm) { AccOp l = new_op(A_LOAD); set_operand(l, n.a); code->push_back(l); }
+          else gen(n.a.node);
+        }
+        release(n.a);
+        AccOp a = new_op(A_UN);
+        a.sub = (uint8_t)op;
+        a.conv = (uint8_t)(ca | (co << 4));
+        a.vcol = n.op.vcol;
+        code->push_back(a);
+      } else {
+        // AND of 0/1 values with a comparison against an immediate: fused.
+        if (op == OP_AND_B) {
+          for (int side = 0; side < 2; side++) {
+            const Operand &c = side ? n.a : n.b, &x = side ? n.b : n.a;
+            if (c.imm || mat[c.node] >= 0 || uses[c.node] > 1) continue;
+            const Node &cn = B.nodes[c.node];
+            if (cn.kind != 1 || !is_cmp(cn.op.code) || ((cn.op.conv >> 4) & 3)) continue;
+            const int cca = cn.op.conv & 3, ccb = (cn.op.conv >> 2) & 3;
+            const Operand *leaf = nullptr, *imm = nullptr;
+            int sub = cn.op.code, lconv = 0;
+            if (cn.b.imm && !cn.a.imm && is_leaf(cn.a.node)) { leaf = &cn.a; imm = &cn.b; lconv = cca; }
+            else if (cn.a.imm && !cn.b.imm && is_leaf(cn.b.node)) { leaf = &cn.b; imm = &cn.a; lconv = ccb; sub = flip(sub); }
+            if (!leaf) continue;
+            if (x.imm) { AccOp l = new_op(A_LOAD); set_operand(l, x); code->push_back(l); }
+            else gen(x.node);
+            release(x);
+            AccOp a = new_op(A_CMPAND);
+            a.sub = (uint8_t)sub;
+            set_operand(a, *leaf);
+            a.conv = (uint8_t)(lconv << 2);
+            a.imm2 = imm->bits;
+            code->push_back(a);
+            release(*leaf);
+            goto done;
+          }
+        }
+        {
+          const bool a_simple = n.a.imm || is_leaf(n.a.node);
+          const bool b_simple = n.b.imm || is_leaf(n.b.node);
+          AccOp a = new_op(A_BIN);
+          a.sub = (uint8_t)op;
+          a.vcol = n.op.vcol;
+          if (b_simple) {
+            if (n.a.imm) { AccOp l = new_op(A_LOAD); set_operand(l, n.a); code->push_back(l); }
+            else gen(n.a.node);
+            set_operand(a, n.b);
+            a.conv = (uint8_t)(ca | (cb << 2) | (co << 4));
+          } else if (a_simple) {
+            gen(n.b.node);
+            set_operand(a, n.a);
+            a.flags = F_REV;
+            a.conv = (uint8_t)(cb | (ca << 2) | (co << 4));
+          } else {
+            const bool a_first = B.nodes[n.a.node].need >= B.nodes[n.b.node].need;
+            const int first = a_first ? n.a.node : n.b.node, second = a_first ? n.b.node : n.a.node;
+            gen(first);
+            const int t = alloc_t();
+            AccOp st = new_op(A_STORE);
+            st.oidx = (uint8_t)t;
+            code->push_back(st);
+            gen(second);
+            a.okind = OK_TEMP;
+            a.oidx = (uint8_t)t;
+            if (a_first) { a.flags = F_REV; a.conv = (uint8_t)(cb | (ca << 2) | (co << 4)); }
+            else a.conv = (uint8_t)(ca | (cb << 2) | (co << 4));
+            tbusy[t] = false;
+          }
+          code->push_back(a);
+          release(n.a);
+          release(n.b);
+        }
+      done:;
+      }
+      if (uses[id] > 1) {  // shared value: keep it in a temporary for its other uses
+        const int t = alloc_t();
+        AccOp st = new_op(A_STORE);
+        st.oidx = (uint8_t)t;
+        code->push_back(st);
+        mat[id] = t;
+      }
+    };
+    try {
+      for (auto &n : B.nodes) n.need = 0;
+      for (size_t i = 0; i < B.nodes.size(); i++) compute_need(B, (int)i);
+      code = &acc_p;
+      if (stencil_node >= 0) gen(stencil_node);
+      std::fill(tbusy.begin(), tbusy.end(), false);
+      std::fill(mat.begin(), mat.end(), -1);
+      code = &acc_q;
+      for (size_t o = 0; o < req.outputs.size(); o++) {
+        gen(out_node[o]);
+        AccOp e = new_op(A_EMIT);
+        e.oidx = (uint8_t)o;
+        code->push_back(e);
+      }
+      if ((int)(acc_p.size() + acc_q.size()) > kMaxOps) throw std::runtime_error("too many ops");
+    } catch (const std::runtime_error &) {
+      acc_ok = false;
+    }
+  }
+
   // Register-file shape.
   const int need = std::max(n_slots, n_vslots);
   int S = need <= 8 ? 8 : need <= 16 ? 16 : need <= 32 ? 32 : 64;
   int R = S == 8 ? 2 : 1;
-  if (req.force_s > 0) { S = req.force_s; R = req.force_r > 0 ? req.force_r : 1; }
-  if (S < need) fail("requested register file too small for the plan");
+  if (req.force_s > 0 && req.kernel != 1) { S = req.force_s; R = req.force_r > 0 ? req.force_r : 1; }
+  if (req.kernel == 1 && !acc_ok) fail("plan does not fit the accumulator kernel");
 
   // Device ops.
   auto emit = [&](const std::vector<int> &ord, std::vector<DevOp> &dst) {
@@ -608,41 +766,68 @@ Compiled compile_program(const CompileRequest &req) {
   std::vector<int> lds_off(n_dev, 0), lds_voff(n_dev, -1);
   std::vector<Chunk> chunks;
   int T = 0, lds_bytes = 0, vbits_off = 0;
-  bool staged_ok = (S == 8 || S == 16) && S * R <= 16 && req.force_t >= 0;
-  for (int c = 0; c < n_dev && staged_ok; c++) {
+  bool layout_ok = req.force_t >= 0 && req.kernel != 3;
+  for (int c = 0; c < n_dev && layout_ok; c++) {
     const int w = wid(req.inputs[c].dtype);
-    if (w == 0 && load_kind(req.inputs[c].dtype) != LD_NONE) staged_ok = false;
-    if ((uintptr_t)req.inputs[c].data % 16) staged_ok = false;
-    if (B.vnodes[c].needed && (uintptr_t)req.inputs[c].valid % 16) staged_ok = false;
+    if (w == 0 && load_kind(req.inputs[c].dtype) != LD_NONE) layout_ok = false;
+    if ((uintptr_t)req.inputs[c].data % 16) layout_ok = false;
+    if (B.vnodes[c].needed && (uintptr_t)req.inputs[c].valid % 16) layout_ok = false;
   }
-  if (staged_ok) {
-    auto layout_for = [&](int t) {
-      size_t off = 0;
-      for (int c = 0; c < n_dev; c++) {
-        lds_off[c] = (int)off;
-        off += (size_t)t * std::max(1, wid(req.inputs[c].dtype));
+  auto layout_for = [&](int t) {
+    size_t off = 0;
+    for (int c = 0; c < n_dev; c++) {
+      lds_off[c] = (int)off;
+      off += (size_t)t * std::max(1, wid(req.inputs[c].dtype));
+      off = (off + 15) & ~size_t(15);
+    }
+    for (int c = 0; c < n_dev; c++) {
+      lds_voff[c] = -1;
+      if (B.vnodes[c].needed) {
+        lds_voff[c] = (int)off;
+        off += (size_t)t / 8;
         off = (off + 15) & ~size_t(15);
       }
-      for (int c = 0; c < n_dev; c++) {
-        lds_voff[c] = -1;
-        if (B.vnodes[c].needed) {
-          lds_voff[c] = (int)off;
-          off += (size_t)t / 8;
-          off = (off + 15) & ~size_t(15);
-        }
-      }
-      vbits_off = (int)off;
-      off += (size_t)n_valid_out * (t / 64 + 2) * 8;
-      return off;
-    };
-    const int unit = 256 * R;
+    }
+    vbits_off = (int)off;
+    off += (size_t)n_valid_out * (t / 64 + 2) * 8;
+    return off;
+  };
+  // Tile rows for rows-per-lane `r`: the forced size, or the largest multiple of
+  // 256 r (<= 4096) whose staging stays within ~52 KiB (three workgroups per CU).
+  auto pick_tile = [&](int r) {
+    const int unit = 256 * r;
     if (req.force_t > 0) {
-      T = req.force_t;
-      if (T % unit || T > 4096 || (int)layout_for(T) > req.lds_budget) fail("bad tile size for this plan");
-    } else {
-      // Largest tile whose staging stays within ~52 KiB (three workgroups per CU).
-      for (int t = 4096; t >= unit; t /= 2)
-        if (t % unit == 0 && layout_for(t) <= (size_t)std::min(req.lds_budget, 52 * 1024)) { T = t; break; }
+      const int t = req.force_t;
+      return (t % unit || t > 4096 || (int)layout_for(t) > req.lds_budget) ? 0 : t;
+    }
+    for (int t = 4096; t >= unit; t /= 2)
+      if (t % unit == 0 && layout_for(t) <= (size_t)std::min(req.lds_budget, 52 * 1024)) return t;
+    return 0;
+  };
+  bool use_acc = false;
+  if (!layout_ok) acc_ok = false;
+  if (acc_ok) {
+    // Most rows per lane whose temporaries and tile fit: (NT, R) = (2,8), (4,4), (8,2).
+    const int shapes[3][2] = {{2, 8}, {4, 4}, {8, 2}};
+    for (auto &sh : shapes) {
+      if (acc_temps > sh[0]) continue;
+      if (req.kernel == 1 && req.force_s > 0 && (req.force_s != sh[0] || req.force_r != sh[1])) continue;
+      const int t = pick_tile(sh[1]);
+      if (t == 0) continue;
+      S = sh[0];
+      R = sh[1];
+      T = t;
+      use_acc = true;
+      break;
+    }
+    if (!use_acc && req.kernel == 1) fail("requested accumulator shape or tile does not fit the plan");
+  }
+  if (!use_acc && S < need) fail("register file too small for the plan");
+  bool staged_ok = layout_ok && (use_acc || ((S == 8 || S == 16) && S * R <= 16));
+  if (staged_ok) {
+    if (!use_acc) {
+      T = pick_tile(R);
+      if (T == 0 && req.force_t > 0) fail("bad tile size for this plan");
     }
     if (T > 0) {
       lds_bytes = (int)layout_for(T);
@@ -668,7 +853,9 @@ Compiled compile_program(const CompileRequest &req) {
   off_out = (off_out + 15) & ~size_t(15);
   size_t off_ops = off_out + sizeof(OutCol) * n_out;
   off_ops = (off_ops + 15) & ~size_t(15);
-  size_t off_chunks = off_ops + sizeof(DevOp) * (ops_p.size() + ops_q.size());
+  if (use_acc && T == 0) use_acc = false;
+  const size_t n_ops_total = use_acc ? acc_p.size() + acc_q.size() : ops_p.size() + ops_q.size();
+  size_t off_chunks = off_ops + sizeof(DevOp) * n_ops_total;
   off_chunks = (off_chunks + 15) & ~size_t(15);
   size_t off_vops = off_chunks + sizeof(Chunk) * chunks.size();
   size_t total = off_vops + sizeof(VOp) * (vops_p.size() + vops_q.size());
@@ -679,8 +866,10 @@ Compiled compile_program(const CompileRequest &req) {
   h.n_rows = req.n_rows;
   h.n_in = n_dev;
   h.n_out = n_out;
-  h.n_ops_p = (int)ops_p.size();
-  h.n_ops_q = (int)ops_q.size();
+  h.n_ops_p = (int)(use_acc ? acc_p.size() : ops_p.size());
+  h.n_ops_q = (int)(use_acc ? acc_q.size() : ops_q.size());
+  h.acc_mode = use_acc ? 1 : 0;
+  h.n_temps = acc_temps;
   h.n_vops_p = (int)vops_p.size();
   h.n_vops_q = (int)vops_q.size();
   h.n_vin = n_dev;
@@ -729,8 +918,13 @@ Compiled compile_program(const CompileRequest &req) {
     put(out.blob, off_out + sizeof(OutCol) * o, oc);
   }
   size_t p = off_ops;
-  for (auto &op : ops_p) { put(out.blob, p, op); p += sizeof(DevOp); }
-  for (auto &op : ops_q) { put(out.blob, p, op); p += sizeof(DevOp); }
+  if (use_acc) {
+    for (auto &op : acc_p) { put(out.blob, p, op); p += sizeof(AccOp); }
+    for (auto &op : acc_q) { put(out.blob, p, op); p += sizeof(AccOp); }
+  } else {
+    for (auto &op : ops_p) { put(out.blob, p, op); p += sizeof(DevOp); }
+    for (auto &op : ops_q) { put(out.blob, p, op); p += sizeof(DevOp); }
+  }
   p = off_vops;
   for (auto &op : vops_p) { put(out.blob, p, op); p += sizeof(VOp); }
   for (auto &op : vops_q) { put(out.blob, p, op); p += sizeof(VOp); }
@@ -742,6 +936,8 @@ Compiled compile_program(const CompileRequest &req) {
   out.n_vops = (int)(vops_p.size() + vops_q.size());
   out.T = T;
   out.lds_bytes = lds_bytes;
+  out.acc = use_acc;
+  if (use_acc) out.n_ops = (int)n_ops_total;
   return out;
 }
 

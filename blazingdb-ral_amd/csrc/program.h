@@ -78,6 +78,33 @@ static_assert(sizeof(DevOp) == 24, "DevOp layout");
 
 constexpr uint8_t F_A_IMM = 1, F_B_IMM = 2;
 
+// Accumulator form (staged kernel): each op combines the accumulator with at most
+// one operand, so results need no write-back and inputs are read straight from the
+// LDS tile. Same 24-byte footprint as DevOp.
+enum AccCode : uint8_t {
+  A_LOAD = 0,   // acc = operand
+  A_STORE,      // temp[oidx] = acc
+  A_EMIT,       // output oidx <- acc (narrowing store at the row's output position)
+  A_BIN,        // acc = sub(acc, operand), or sub(operand, acc) when F_REV
+  A_UN,         // acc = sub(acc)
+  A_CMPAND,     // acc = acc & (operand sub imm2), acc and the comparison both 0/1
+};
+enum OperandKind : uint8_t { OK_NONE = 0, OK_IMM = 1, OK_INPUT = 2, OK_TEMP = 3 };
+constexpr uint8_t F_REV = 1;
+struct AccOp {
+  uint8_t code;   // AccCode
+  uint8_t sub;    // DevOpCode of the operation (A_BIN / A_UN / comparison of A_CMPAND)
+  uint8_t okind;  // OperandKind
+  uint8_t oidx;   // input column / temp index (operand), temp (A_STORE), output (A_EMIT)
+  uint8_t conv;   // bits 0-1 accumulator conversion, 2-3 operand conversion, 4-5 result conversion
+  uint8_t flags;  // F_REV: the operand is the left-hand side
+  uint8_t vcol;   // input column whose raw validity COALESCE / IS_[NOT_]NULL read; 0xFF = none
+  uint8_t pad;
+  uint64_t imm;   // immediate operand
+  uint64_t imm2;  // A_CMPAND: the comparison's immediate
+};
+static_assert(sizeof(AccOp) == 24, "AccOp layout");
+
 // Validity op over 64-row words.
 enum VOpCode : uint8_t { VOP_AND = 0, VOP_OR = 1 };
 struct VOp {
@@ -133,6 +160,8 @@ struct ProgHeader {
   int32_t lds_vbits_off;         // staged kernel: LDS offset of compacted-validity words
   int32_t n_valid_out;           // outputs with validity
   int32_t lds_bytes;             // staged kernel: dynamic LDS per workgroup
+  int32_t acc_mode;              // ops are AccOp (accumulator form) instead of DevOp
+  int32_t n_temps;               // accumulator form: temporaries used
   // offsets (bytes from the blob start)
   int32_t off_in, off_out, off_ops, off_vops, off_chunks;
   int32_t blob_bytes;

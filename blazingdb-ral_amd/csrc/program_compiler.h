@@ -50,6 +50,7 @@ struct CompileRequest {
   int scalar_mode = BLZ_SCALAR_INTENDED;
   int force_s = 0, force_r = 0;     // register-file shape override (autotuner)
   int force_t = 0;                  // staged tile rows override (autotuner); -1 = direct kernel
+  int kernel = 0;                   // 0 automatic, 1 accumulator, 2 staged register file, 3 direct
   int lds_budget = 64 * 1024;       // dynamic LDS per workgroup the staged kernel may use
 };
 
@@ -59,6 +60,7 @@ struct Compiled {
   int n_slots = 0, n_vslots = 0, n_ops = 0, n_vops = 0;
   int T = 0;          // staged kernel tile rows (0: direct kernel)
   int lds_bytes = 0;  // staged kernel dynamic LDS per workgroup
+  bool acc = false;   // accumulator-form program (S = temporaries)
 };
 
 // Throws std::runtime_error with a message on an unsupported or malformed plan.

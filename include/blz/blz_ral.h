@@ -133,13 +133,15 @@ typedef struct blz_program_s *blz_program;
  * projection; project_query_part NULL keeps every frame column) against fixed
  * device buffers the program allocates and owns: out_columns[0..*n_out) receive
  * them (capacity: every frame row; sizes valid after blz_program_result).
- * s / r (0 = automatic) force the register-file shape (8,1) (8,2) (16,1) (32,1) (64,1);
- * t (0 = automatic) forces the LDS-staged tile rows (a multiple of 256*r, <= 4096) or,
- * with t = -1, the unstaged kernel. */
+ * kernel: 0 automatic, 1 accumulator form (LDS-staged), 2 LDS-staged register file,
+ * 3 unstaged register file. s / r (0 = automatic) force the shape: temporaries x rows
+ * per lane (2,8) (4,4) (8,2) for kernel 1, slots x rows (8,1) (8,2) (16,1) (32,1) (64,1)
+ * otherwise; t (0 = automatic) forces the staged tile rows (a multiple of 256*r,
+ * <= 4096). */
 int blz_program_create(blz_program *prog, gdf_column *const *frame, int n_frame,
                        const char *filter_query_part, const char *project_query_part,
-                       gdf_column *out_columns, int max_out, int *n_out, int scalar_mode, int s,
-                       int r, int t, char *err, size_t err_len);
+                       gdf_column *out_columns, int max_out, int *n_out, int scalar_mode,
+                       int kernel, int s, int r, int t, char *err, size_t err_len);
 /* Enqueues one execution on `stream` (no allocation, no synchronisation). */
 int blz_program_launch(blz_program prog, blz_stream_t stream, char *err, size_t err_len);
 /* Waits for the last launch and reads back rows kept and per-output null counts. */
@@ -152,8 +154,9 @@ int blz_program_time(blz_program prog, blz_stream_t stream, int iters, double *m
 /* Register-file shape and sizes chosen for the program. */
 int blz_program_info(blz_program prog, int *s, int *r, int *n_ops, int *n_vops, int *n_slots,
                      int *grid);
-/* Tile rows (0 = unstaged kernel) and dynamic LDS bytes per workgroup. */
-int blz_program_tile(blz_program prog, int *tile_rows, int *lds_bytes);
+/* Tile rows (0 = unstaged kernel), dynamic LDS bytes per workgroup and the kernel
+ * kind chosen (1 accumulator, 2 staged register file, 3 unstaged). */
+int blz_program_tile(blz_program prog, int *tile_rows, int *lds_bytes, int *kernel);
 int blz_program_destroy(blz_program prog);
 
 /* ---- memory / streams (HIP plumbing for hosts without a HIP toolchain) ------- */

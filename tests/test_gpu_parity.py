@@ -221,3 +221,56 @@ def test_program_matches_one_shot(B):
     for g, r in zip(prog.outputs, ref):
         assert g.data().tobytes() == r.data.tobytes()
     prog.close()
+
+
+@pytest.mark.parametrize("kernel", [1, 2, 3])
+@pytest.mark.parametrize("seed", range(8))
+def test_kernel_kinds_random_filter_project(B, kernel, seed):
+    """Every kernel kind (accumulator / staged register file / direct) on random plans."""
+    rng = np.random.default_rng(3000 + seed)
+    k = int(rng.integers(2, 6))
+    dts = exprgen.rand_frame_dtypes(rng, k)
+    n = int(rng.choice([1000, 5000, 70_001]))
+    null_rates = [float(rng.choice([0.0, 0.1])) for _ in range(k)]
+    host = [gen_column(rng, dt, n, nr) for dt, nr in zip(dts, null_rates)]
+    nullable = [i for i, nr in enumerate(null_rates) if nr > 0]
+    cond = f"<({exprgen.rand_expr(rng, dts, 2, False, nullable)}, {int(rng.integers(-5, 6))})"
+    exprs = [exprgen.rand_expr(rng, dts, 2, False, nullable) for _ in range(int(rng.integers(1, 4)))]
+    items = [f"E{i}=[{e}]" for i, e in enumerate(exprs)] + [f"A=[${int(rng.integers(0, k))}]"]
+    proj = "LogicalProject(" + ", ".join(items) + ")"
+    filt = f"LogicalFilter(condition=[{cond}])"
+    _print_case(test="kernel_kinds", kernel=kernel, seed=seed, n=n, filt=filt, proj=proj)
+    try:
+        ref = O.execute_project(O.process_filter(host, filt), proj)
+    except (P.PlanError, ValueError):
+        return
+    dev = [to_device(B, h) for h in host]
+    try:
+        prog = B.Program(dev, filt, proj, kernel=kernel)
+    except B.BlzError as e:
+        pytest.skip(f"plan does not fit kernel {kernel}: {e}")
+    assert prog.kernel == kernel
+    prog.launch()
+    kept, nulls = prog.result()
+    assert kept == len(ref[0])
+    for g, r, nc in zip(prog.outputs, ref, nulls):
+        g.col.null_count = nc
+        assert_column_equal(g, r, what=f"kernel {kernel}: {filt} {proj}")
+    prog.close()
+
+
+@pytest.mark.parametrize("kernel", [1, 2, 3])
+def test_kernel_kinds_c1(B, kernel):
+    n = 1_000_003
+    _print_case(test="kernel_kinds_c1", kernel=kernel, n=n)
+    host = c1_frame(n, seed=15)
+    dev = [to_device(B, h) for h in host]
+    prog = B.Program(dev, C1_FILTER, C1_PROJECT, kernel=kernel)
+    assert prog.kernel == kernel
+    prog.launch()
+    kept, _ = prog.result()
+    ref = O.execute_project(O.process_filter(host, C1_FILTER), C1_PROJECT)
+    assert kept == len(ref[0])
+    for g, r in zip(prog.outputs, ref):
+        assert g.data().tobytes() == r.data.tobytes()
+    prog.close()
