@@ -9,7 +9,9 @@
 #include <hip/hip_runtime.h>
 
 #include <cstring>
+#include <map>
 #include <mutex>
+#include <tuple>
 #include <new>
 #include <stdexcept>
 #include <string>
@@ -86,11 +88,25 @@ int64_t rows_per_tile(int r) { return (int64_t)block_threads() * r; }
 
 int64_t tile_rows(const Compiled &c) { return c.T > 0 ? c.T : rows_per_tile(c.R); }
 
+// Workgroups per CU of a tile-VM kernel, cached per (shape, mode, LDS).
+int vm_per_cu(const Compiled &c, int mode, int lds) {
+  static std::mutex mu;
+  static std::map<std::tuple<int, int, int, bool, int>, int> cache;
+  const auto key = std::make_tuple(c.R, c.S, mode, c.vm_slow, lds);
+  std::lock_guard<std::mutex> g(mu);
+  auto it = cache.find(key);
+  if (it != cache.end()) return it->second;
+  const int b = std::max(1, vm_blocks_per_cu(c.R, c.S, mode, c.vm_slow, lds));
+  cache[key] = b;
+  return b;
+}
+
 // Persistent grid: every workgroup that fits on the device at once.
 int interp_grid(const Compiled &c, bool compact, int64_t n) {
   const int64_t tiles = (n + tile_rows(c) - 1) / tile_rows(c);
-  const int per_cu = c.acc ? acc_blocks_per_cu(c.S, c.R, compact, c.lds_bytes)
-                           : interp_blocks_per_cu(c.S, c.R, compact, c.lds_bytes);
+  const int per_cu = c.vm ? vm_per_cu(c, compact ? 1 : 0, c.lds_bytes)
+                     : c.acc ? acc_blocks_per_cu(c.S, c.R, compact, c.lds_bytes)
+                             : interp_blocks_per_cu(c.S, c.R, compact, c.lds_bytes);
   const int64_t resident = (int64_t)device_cus() * per_cu;
   return (int)std::max<int64_t>(1, std::min<int64_t>(tiles, resident));
 }
@@ -105,25 +121,38 @@ void check_column(const gdf_column *c, const char *what, int64_t n) {
   if (c->valid && ((uintptr_t)c->valid % 8) != 0) invalid(std::string(what) + " validity is not 8-byte aligned");
 }
 
-// Device scratch of one launch: blob | null counts | counters | tile states.
+// Device scratch of one launch:
+//   blob | null counts | counters | tile states  (zeroed before each launch)
+//   | keep words | slot counts | slot prefixes   (tile VM filter; written first)
 struct LaunchBuffers {
   uint8_t *base = nullptr;
   size_t bytes = 0;
   size_t zero_off = 0, zero_bytes = 0;
+  size_t keep_off = 0, count_off = 0, prefix_off = 0;
   Workspace ws{};
   uint8_t *blob = nullptr;
 };
 
 size_t round_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 
-void layout(LaunchBuffers &lb, size_t blob_bytes, int n_out, int64_t n_tiles) {
-  const size_t blob_sz = round_up(blob_bytes, 256);
+void layout(LaunchBuffers &lb, const Compiled &c, int n_out, bool compact, int64_t n) {
+  const int64_t n_tiles = compact ? (n + tile_rows(c) - 1) / tile_rows(c) : 0;
+  const size_t blob_sz = round_up(c.blob.size(), 256);
   const size_t nc = round_up(sizeof(unsigned long long) * std::max(n_out, 1), 64);
   const size_t cnt = 64;  // tile counter + n_sel
   const size_t ts = round_up(sizeof(unsigned long long) * std::max<int64_t>(n_tiles, 1), 64);
-  lb.bytes = blob_sz + nc + cnt + ts;
   lb.zero_off = blob_sz;
   lb.zero_bytes = nc + cnt + ts;
+  size_t off = blob_sz + lb.zero_bytes;
+  const bool vmf = c.vm && compact;
+  const int64_t slots = vmf ? n_tiles * 4 : 0;
+  lb.keep_off = off;
+  off += round_up(vmf ? sizeof(uint64_t) * n_tiles * (c.T / 64) : 0, 256);
+  lb.count_off = off;
+  off += round_up(sizeof(uint32_t) * slots, 256);
+  lb.prefix_off = off;
+  off += round_up(sizeof(uint32_t) * slots, 256);
+  lb.bytes = std::max<size_t>(off, 256);
 }
 void bind(LaunchBuffers &lb, int n_out) {
   uint8_t *p = lb.base;
@@ -134,10 +163,26 @@ void bind(LaunchBuffers &lb, int n_out) {
   lb.ws.tile_counter = (unsigned int *)(z + nc);
   lb.ws.n_sel = (unsigned long long *)(z + nc + 8);
   lb.ws.tile_state = (unsigned long long *)(z + nc + 64);
+  lb.ws.keep = (uint64_t *)(p + lb.keep_off);
+  lb.ws.wave_count = (uint32_t *)(p + lb.count_off);
+  lb.ws.wave_prefix = (uint32_t *)(p + lb.prefix_off);
 }
 
 hipError_t launch_program(const Compiled &c, bool compact, int grid, hipStream_t st, const uint8_t *blob,
                           const Workspace &ws) {
+  if (c.vm) {
+    const int64_t tiles = (c.n_rows + c.T - 1) / c.T;
+    auto grid_for = [&](int mode, int lds) {
+      const int64_t resident = (int64_t)device_cus() * vm_per_cu(c, mode, lds);
+      return (int)std::max<int64_t>(1, std::min<int64_t>(tiles, resident));
+    };
+    if (!compact) return launch_vm(c.R, c.S, 0, c.vm_slow, grid_for(0, c.lds_bytes), c.lds_bytes, st, blob, ws);
+    hipError_t e = launch_vm(c.R, c.S, 1, c.vm_slow, grid_for(1, c.lds_bytes), c.lds_bytes, st, blob, ws);
+    if (e == hipSuccess) e = launch_slot_scan(ws.wave_count, ws.wave_prefix, tiles * 4, st, ws);
+    if (e == hipSuccess)
+      e = launch_vm(c.R, c.S, 2, c.vm_slow, grid_for(2, c.lds_gather), c.lds_gather, st, blob, ws);
+    return e;
+  }
   if (c.acc) return launch_acc(c.S, c.R, compact, grid, c.lds_bytes, st, blob, ws);
   return launch_interp(c.S, c.R, compact, grid, c.lds_bytes, st, blob, ws);
 }
@@ -154,7 +199,7 @@ RunResult run_once(const Compiled &c, bool compact, int64_t n, int n_out, hipStr
   if (n == 0) return r;
   const int64_t tiles = (n + tile_rows(c) - 1) / tile_rows(c);
   LaunchBuffers lb;
-  layout(lb, c.blob.size(), n_out, compact ? tiles : 0);
+  layout(lb, c, n_out, compact, n);
   HIPCHECK(hipMallocAsync((void **)&lb.base, lb.bytes, st));
   bind(lb, n_out);
   std::vector<unsigned long long> host(n_out + 1, 0);
@@ -649,7 +694,7 @@ int blz_program_create(blz_program *prog, gdf_column *const *frame, int n_frame,
       p->n_out = (int)outs.size();
       p->grid = interp_grid(p->c, has_filter, std::max<int64_t>(n, 1));
       const int64_t tiles = (n + tile_rows(p->c) - 1) / tile_rows(p->c);
-      layout(p->lb, p->c.blob.size(), p->n_out, has_filter ? tiles : 0);
+      layout(p->lb, p->c, p->n_out, has_filter, n);
       HIPCHECK(hipMalloc((void **)&p->lb.base, p->lb.bytes));
       bind(p->lb, p->n_out);
       HIPCHECK(hipMemcpy(p->lb.blob, p->c.blob.data(), p->c.blob.size(), hipMemcpyHostToDevice));
@@ -753,7 +798,7 @@ int blz_program_tile(blz_program prog, int *tile_rows, int *lds_bytes, int *kern
   if (!prog) return BLZ_ERR_INVALID;
   if (tile_rows) *tile_rows = prog->c.T;
   if (lds_bytes) *lds_bytes = prog->c.lds_bytes;
-  if (kernel) *kernel = prog->c.acc ? 1 : prog->c.T > 0 ? 2 : 3;
+  if (kernel) *kernel = prog->c.vm ? 4 : prog->c.acc ? 1 : prog->c.T > 0 ? 2 : 3;
   return BLZ_OK;
 }
 

@@ -455,10 +455,342 @@ void put(std::vector<uint8_t> &blob, size_t off, const T &v) {
   std::memcpy(blob.data() + off, &v, sizeof(T));
 }
 
+int flip_cmp(int c) {  // a OP b  ==  b FLIP(OP) a
+  switch (c) {
+    case OP_LT_I: return OP_GT_I; case OP_GT_I: return OP_LT_I;
+    case OP_LE_I: return OP_GE_I; case OP_GE_I: return OP_LE_I;
+    case OP_LT_F: return OP_GT_F; case OP_GT_F: return OP_LT_F;
+    case OP_LE_F: return OP_GE_F; case OP_GE_F: return OP_LE_F;
+    default: return c;
+  }
+}
+bool is_cmp_code(int c) { return c >= OP_EQ_I && c <= OP_GE_F; }
+
+// Lowers the accumulator form to VmOps and lays out the double-buffered tile.
+// Returns false (with the reason) when the plan does not fit the tile VM.
+bool compile_vm(const CompileRequest &req, Builder &B, const std::vector<AccOp> &acc_p,
+                const std::vector<AccOp> &acc_q, int acc_temps, const std::vector<VOp> &vops_p,
+                const std::vector<VOp> &vops_q, const std::vector<int> &out_node, const std::vector<int> &out_v,
+                int stencil_v, int n_vslots, Compiled &out, std::string &why) {
+  const int n_dev = (int)req.inputs.size();
+  const int n_out = (int)req.outputs.size();
+  if (n_vslots > 8) { why = "plan needs more than 8 validity registers"; return false; }
+  if (req.n_rows > (int64_t)INT32_MAX * 256) { why = "table too large"; return false; }
+
+  // Which inputs the predicate reads (staged) and which only the projection of a
+  // filter reads (fetched from HBM for kept rows).
+  std::vector<bool> used_p(n_dev, false), used_q(n_dev, false);
+  for (auto &a : acc_p) if (a.okind == OK_INPUT) used_p[a.oidx] = true;
+  for (auto &a : acc_q) if (a.okind == OK_INPUT) used_q[a.oidx] = true;
+  std::vector<bool> staged(n_dev, false);
+  for (int c = 0; c < n_dev; c++) staged[c] = used_p[c] || (used_q[c] && !req.compact);
+  for (int c = 0; c < n_dev; c++) {
+    if (staged[c] && (uintptr_t)req.inputs[c].data % 16) { why = "unaligned column"; return false; }
+    if (B.vnodes[c].needed && (uintptr_t)req.inputs[c].valid % 16) { why = "unaligned bitmap"; return false; }
+  }
+
+  bool gather_pass = false;  // lowering the projection of a filter: operands from HBM
+  auto operand = [&](VmOp &v, const AccOp &a, int conv_b) {
+    switch (a.okind) {
+      case OK_IMM: v.cls = VC_IMM; v.imm = host_conv(a.imm, conv_b); return;
+      case OK_TEMP: v.cls = VC_TMP; v.arg2 = a.oidx; v.cv = (uint8_t)conv_b; return;
+      case OK_INPUT: {
+        const int c = a.oidx;
+        const int lk = load_kind(req.inputs[c].dtype);
+        const bool g = gather_pass || !staged[c];
+        v.arg2 = (uint8_t)c;
+        v.cv = (uint8_t)conv_b;
+        switch (lk) {
+          case LD_I64: case LD_F64: v.cls = g ? VC_G64 : VC_L64; break;
+          case LD_I32: v.cls = g ? VC_G32I : VC_L32I; break;
+          case LD_F32: v.cls = g ? VC_G32F : VC_L32F; break;
+          case LD_I16: v.cls = g ? VC_G16I : VC_L16I; break;
+          case LD_I8: v.cls = g ? VC_G8I : VC_L8I; break;
+          case LD_U8: v.cls = g ? VC_G8U : VC_L8U; break;
+          default: v.cls = VC_IMM; v.imm = host_conv(0, conv_b); v.cv = 0; break;  // never read: 0
+        }
+        return;
+      }
+      default: v.cls = VC_IMM; v.imm = 0; return;
+    }
+  };
+  auto new_vm = [](int code) { VmOp v{}; v.code = (uint8_t)code; v.vcol = 0xFF; return v; };
+  bool slow = false, vcol_needed = false;
+  auto conv_op = [&](std::vector<VmOp> &dst, int cv) {
+    if (cv == CV_I2F) dst.push_back(new_vm(VM_I2F));
+    else if (cv == CV_F2I) dst.push_back(new_vm(VM_F2I));
+  };
+  auto lower = [&](const std::vector<AccOp> &src, std::vector<VmOp> &dst) {
+    for (const AccOp &a : src) {
+      const int ca = a.conv & 3, cb = (a.conv >> 2) & 3, co = (a.conv >> 4) & 3;
+      switch (a.code) {
+        case A_LOAD: { VmOp v = new_vm(VM_LOAD); operand(v, a, cb); dst.push_back(v); break; }
+        case A_STORE: { VmOp v = new_vm(VM_STORE); v.arg = a.oidx; dst.push_back(v); break; }
+        case A_EMIT: {
+          const OutSpec &s = req.outputs[a.oidx];
+          VmOp v = new_vm(VM_EMIT);
+          v.arg = a.oidx;
+          v.cls = (uint8_t)(s.kind == 0 ? store_kind_program(s.dtype) : store_kind_alias(s.dtype));
+          dst.push_back(v);
+          break;
+        }
+        case A_CMPAND: {
+          VmOp v = new_vm(VM_CMPAND);
+          operand(v, a, cb);
+          v.arg = a.sub;
+          v.imm2 = a.imm2;
+          dst.push_back(v);
+          break;
+        }
+        case A_BIN: {
+          const int sub = a.sub;
+          const bool rev = a.flags & F_REV;
+          int hot = -1, harg = 0;
+          switch (sub) {
+            case OP_ADD_I: hot = VM_ADD_I; break;
+            case OP_ADD_F: hot = VM_ADD_F; break;
+            case OP_SUB_I: hot = rev ? VM_RSUB_I : VM_SUB_I; break;
+            case OP_SUB_F: hot = rev ? VM_RSUB_F : VM_SUB_F; break;
+            case OP_MUL_I: hot = VM_MUL_I; break;
+            case OP_MUL_F: hot = VM_MUL_F; break;
+            case OP_AND_B: hot = VM_AND_B; break;
+            case OP_DIV_F: hot = rev ? VM_RDIV_F : VM_DIV_F; break;
+            case OP_DIV_I: hot = rev ? VM_RDIV_I : VM_DIV_I; break;
+            case OP_MOD_I: hot = rev ? VM_RMOD_I : VM_MOD_I; break;
+            case OP_COALESCE: hot = rev ? VM_RCOALESCE : VM_COALESCE; break;
+            default:
+              if (is_cmp_code(sub)) { hot = VM_CMP; harg = rev ? flip_cmp(sub) : sub; }
+          }
+          if (hot == VM_COALESCE || hot == VM_RCOALESCE) vcol_needed = true;
+          if (hot >= 0) {
+            conv_op(dst, ca);
+            VmOp v = new_vm(hot);
+            operand(v, a, cb);
+            v.arg = (uint8_t)harg;
+            v.vcol = a.vcol;
+            dst.push_back(v);
+            conv_op(dst, co);
+          } else {
+            VmOp v = new_vm(VM_GENERIC);
+            operand(v, a, 0);
+            v.cv = 0;
+            v.arg = (uint8_t)sub;
+            v.gen = (uint8_t)(cb | (ca << 2) | (co << 4) | (rev ? VG_REV : 0));
+            v.vcol = a.vcol;
+            dst.push_back(v);
+            slow = true;
+          }
+          break;
+        }
+        case A_UN: {
+          const int sub = a.sub;
+          int hot = -1;
+          switch (sub) {
+            case OP_FLOOR: hot = VM_FLOOR; break;
+            case OP_CEIL: hot = VM_CEIL; break;
+            case OP_ABS: hot = VM_ABS_F; break;
+            case OP_NOT_I: hot = VM_NOT_I; break;
+            case OP_NOT_F: hot = VM_NOT_F; break;
+            case OP_IS_NULL: hot = VM_IS_NULL; break;
+            case OP_IS_NOT_NULL: hot = VM_IS_NOT_NULL; break;
+            case OP_IDENT: hot = VM_COUNT_; break;
+            case OP_ZERO: hot = VM_LOAD; break;
+            default: break;
+          }
+          if (hot >= 0) {
+            conv_op(dst, ca);
+            if (hot == VM_LOAD) { VmOp v = new_vm(VM_LOAD); v.cls = VC_IMM; v.imm = 0; dst.push_back(v); }
+            else if (hot != VM_COUNT_) { VmOp v = new_vm(hot); v.vcol = a.vcol; dst.push_back(v); }
+            conv_op(dst, co);
+          } else {
+            VmOp v = new_vm(VM_GENERIC);
+            v.cls = VC_NONE;
+            v.arg = (uint8_t)sub;
+            v.gen = (uint8_t)((ca << 2) | (co << 4) | VG_UNARY);
+            v.vcol = a.vcol;
+            dst.push_back(v);
+            slow = true;
+          }
+          break;
+        }
+        default: why = "unknown accumulator op"; return false;
+      }
+    }
+    return true;
+  };
+  std::vector<VmOp> vm_p, vm_q;
+  if (!lower(acc_p, vm_p)) return false;
+  gather_pass = req.compact;
+  if (!lower(acc_q, vm_q)) return false;
+  // A VM_GENERIC with no operand still runs `fetch` on VC_NONE: give it an
+  // immediate so no memory is touched.
+  for (auto *v : {&vm_p, &vm_q})
+    for (auto &op : *v)
+      if (op.cls == VC_NONE && op.code != VM_EMIT) { op.cls = VC_IMM; op.imm = 0; }
+
+  // Tile buffers: staged column slices, then needed bitmaps, each 16-byte aligned.
+  int n_valid_out = 0;
+  for (auto &o : req.outputs) n_valid_out += o.valid != nullptr;
+  std::vector<int> lds_off(n_dev, 0), lds_voff(n_dev, -1);
+  auto layout = [&](int t, int &buf) {
+    size_t off = 0;
+    for (int c = 0; c < n_dev; c++) {
+      lds_off[c] = 0;
+      if (!staged[c]) continue;
+      lds_off[c] = (int)off;
+      off += (size_t)t * std::max(1, wid(req.inputs[c].dtype));
+      off = (off + 15) & ~size_t(15);
+    }
+    for (int c = 0; c < n_dev; c++) {
+      lds_voff[c] = -1;
+      if (!B.vnodes[c].needed) continue;
+      lds_voff[c] = (int)off;
+      off += (size_t)t / 8;
+      off = (off + 15) & ~size_t(15);
+    }
+    buf = (int)std::max<size_t>(off, 16);
+    return 2 * (size_t)buf;
+  };
+  // Shape: rows per lane R (tile 256 R) and temporaries NT. The default keeps
+  // two tile buffers within ~80 KiB so at least two workgroups share a CU.
+  static const int shapes[][2] = {{8, 2}, {8, 4}, {4, 2}, {4, 4}, {4, 8}, {2, 4}, {2, 8}};
+  const int budget = std::min(req.lds_budget > 0 ? req.lds_budget : 80 * 1024, 160 * 1024 - 4096);
+  int R = 0, NT = 0, buf = 0;
+  size_t lds_total = 0;
+  for (auto &sh : shapes) {
+    if (acc_temps > sh[1]) continue;
+    if (req.force_r > 0 && req.force_r != sh[0]) continue;
+    if (req.force_s > 0 && req.force_s != sh[1]) continue;
+    int b = 0;
+    const size_t tot = layout(256 * sh[0], b);
+    if (tot > (size_t)budget) continue;
+    if (req.force_t > 0 && req.force_t != 256 * sh[0]) continue;
+    R = sh[0]; NT = sh[1]; buf = b; lds_total = tot;
+    break;
+  }
+  if (R == 0) { why = "no tile VM shape fits the plan"; return false; }
+  const int T = 256 * R;
+  layout(T, buf);
+  for (auto *v : {&vm_p, &vm_q})
+    for (auto &op : *v)
+      if (op.cls >= VC_L64 && op.cls <= VC_L8U) op.off = lds_off[op.arg2];
+
+  std::vector<Piece> pieces;
+  for (int c = 0; c < n_dev; c++) {
+    if (!staged[c]) continue;
+    const int w = std::max(1, wid(req.inputs[c].dtype));
+    const int slice = T * w;
+    for (int off = 0; off < slice; off += kPieceBytes)
+      pieces.push_back(Piece{req.inputs[c].data, slice, off, lds_off[c] + off,
+                             std::min(kPieceBytes, slice - off), 0, w});
+  }
+  for (int c = 0; c < n_dev; c++) {
+    if (lds_voff[c] < 0) continue;
+    const int slice = T / 8;
+    for (int off = 0; off < slice; off += kPieceBytes)
+      pieces.push_back(Piece{req.inputs[c].valid, slice, off, lds_voff[c] + off,
+                             std::min(kPieceBytes, slice - off), 1, 0});
+  }
+
+  // Blob: header | InCol | OutCol | VmOp | Piece | VOp.
+  auto al = [](size_t x) { return (x + 15) & ~size_t(15); };
+  const size_t off_in = al(sizeof(ProgHeader));
+  const size_t off_out = al(off_in + sizeof(InCol) * n_dev);
+  const size_t off_ops = al(off_out + sizeof(OutCol) * n_out);
+  const size_t off_pieces = al(off_ops + sizeof(VmOp) * (vm_p.size() + vm_q.size()));
+  const size_t off_vops = al(off_pieces + sizeof(Piece) * pieces.size());
+  const size_t total = al(off_vops + sizeof(VOp) * (vops_p.size() + vops_q.size()));
+  if ((int)total > blob_capacity()) { why = "plan too large for the program buffer"; return false; }
+  auto vsrc = [&](int v) { return v == V_ALL ? -1 : v == V_NONE ? -2 : B.vnodes[v].vslot; };
+
+  out.blob.assign(total, 0);
+  ProgHeader h{};
+  h.n_rows = req.n_rows;
+  h.n_in = n_dev;
+  h.n_out = n_out;
+  h.n_ops_p = (int)vm_p.size();
+  h.n_ops_q = (int)vm_q.size();
+  h.n_vops_p = (int)vops_p.size();
+  h.n_vops_q = (int)vops_q.size();
+  h.n_vin = n_dev;
+  h.n_slots = 0;
+  h.n_vslots = n_vslots;
+  h.stencil_slot = 0;
+  // The predicate's value is the INT8 stencil column the reference writes
+  // (narrowed); a bare-column stencil is the widened column value.
+  h.stencil_width = (req.compact && !req.pred) ? 8 : 1;
+  h.stencil_vsrc = vsrc(stencil_v);
+  int any_valid = 0;
+  for (auto &o : req.outputs) any_valid |= o.valid != nullptr;
+  h.any_valid_out = any_valid;
+  h.tile_rows = T;
+  h.n_chunks = 0;
+  h.lds_vbits_off = 0;
+  h.n_valid_out = n_valid_out;
+  h.lds_bytes = (int)lds_total;
+  h.acc_mode = 2;
+  h.n_temps = acc_temps;
+  h.buf_bytes = buf;
+  h.n_pieces = (int)pieces.size();
+  h.off_pieces = (int)off_pieces;
+  h.off_chunks = (int)off_pieces;
+  h.off_in = (int)off_in;
+  h.off_out = (int)off_out;
+  h.off_ops = (int)off_ops;
+  h.off_vops = (int)off_vops;
+  h.blob_bytes = (int)total;
+  put(out.blob, 0, h);
+  for (int c = 0; c < n_dev; c++) {
+    InCol ic{};
+    ic.data = req.inputs[c].data;
+    ic.valid = req.inputs[c].valid;
+    ic.load = load_kind(req.inputs[c].dtype);
+    ic.vslot = B.vnodes[c].needed ? c : -1;
+    ic.lds_off = lds_off[c];
+    ic.lds_voff = lds_voff[c];
+    put(out.blob, off_in + sizeof(InCol) * c, ic);
+  }
+  int vbuf_next = 0;
+  for (int o = 0; o < n_out; o++) {
+    const OutSpec &s = req.outputs[o];
+    OutCol oc{};
+    oc.data = s.data;
+    oc.valid = s.valid;
+    oc.store = s.kind == 0 ? store_kind_program(s.dtype) : store_kind_alias(s.dtype);
+    oc.slot = 0;
+    oc.src_kind = (int16_t)s.kind;
+    oc.vsrc = vsrc(out_v[o]);
+    oc.vbuf = s.valid ? vbuf_next++ : -1;
+    put(out.blob, off_out + sizeof(OutCol) * o, oc);
+  }
+  size_t p = off_ops;
+  for (auto &op : vm_p) { put(out.blob, p, op); p += sizeof(VmOp); }
+  for (auto &op : vm_q) { put(out.blob, p, op); p += sizeof(VmOp); }
+  for (size_t k = 0; k < pieces.size(); k++) put(out.blob, off_pieces + sizeof(Piece) * k, pieces[k]);
+  p = off_vops;
+  for (auto &op : vops_p) { put(out.blob, p, op); p += sizeof(VOp); }
+  for (auto &op : vops_q) { put(out.blob, p, op); p += sizeof(VOp); }
+  out.S = NT;
+  out.R = R;
+  out.n_slots = 0;
+  out.n_vslots = n_vslots;
+  out.n_ops = (int)(vm_p.size() + vm_q.size());
+  out.n_vops = (int)(vops_p.size() + vops_q.size());
+  out.T = T;
+  out.lds_bytes = (int)lds_total;
+  out.acc = false;
+  out.vm = true;
+  out.vm_slow = slow;
+  out.lds_gather = n_valid_out > 0 ? 4 * (R + 2) * 8 : 0;
+  (void)vcol_needed;
+  return true;
+}
+
 }  // namespace
 
 Compiled compile_program(const CompileRequest &req) {
   Compiled out;
+  out.n_rows = req.n_rows;
   const int n_dev = (int)req.inputs.size();
   if (n_dev > kMaxSlots) fail("more than 64 input columns");
   Builder B;
@@ -728,6 +1060,16 @@ Compiled compile_program(const CompileRequest &req) {
     } catch (const std::runtime_error &) {
       acc_ok = false;
     }
+  }
+
+  // ---- Tile VM: the accumulator form, pre-decoded, over double-buffered tiles ----
+  if (req.kernel == 4 && !acc_ok) fail("plan does not fit the tile VM");
+  if (acc_ok && (req.kernel == 0 || req.kernel == 4)) {
+    std::string why;
+    if (compile_vm(req, B, acc_p, acc_q, acc_temps, vops_p, vops_q, out_node, out_v, stencil_v, n_vslots,
+                   out, why))
+      return out;
+    if (req.kernel == 4) fail(why);
   }
 
   // Register-file shape.

@@ -105,6 +105,67 @@ struct AccOp {
 };
 static_assert(sizeof(AccOp) == 24, "AccOp layout");
 
+// Tile VM (vm_kernel): the accumulator form pre-decoded on the host so the
+// kernel's dispatch is one switch per op with no further field decoding. The
+// operand of an op is fetched by class: an immediate, a temporary, a column slice
+// of the LDS tile (byte offset `off` within the tile buffer), or a column read
+// straight from HBM for the rows a compaction keeps (input `arg2`).
+// VM_EMIT's `arg` is the output, VM_STORE's the temporary.
+enum VmCode : uint8_t {
+  VM_LOAD = 0,  // acc = B
+  VM_STORE,     // tmp[arg] = acc
+  VM_EMIT,      // output arg <- acc; cls = StoreKind
+  VM_ADD_I, VM_ADD_F, VM_SUB_I, VM_SUB_F, VM_RSUB_I, VM_RSUB_F, VM_MUL_I, VM_MUL_F,
+  VM_AND_B,     // acc &= B (both 0/1)
+  VM_CMP,       // acc = acc CMP B; arg = comparison DevOpCode
+  VM_CMPAND,    // acc &= (B CMP imm2); arg = comparison DevOpCode
+  VM_I2F, VM_F2I,                   // acc conversion
+  VM_FLOOR, VM_CEIL, VM_ABS_F, VM_NOT_I, VM_NOT_F,
+  VM_DIV_F, VM_RDIV_F, VM_DIV_I, VM_RDIV_I, VM_MOD_I, VM_RMOD_I,
+  VM_COALESCE, VM_RCOALESCE,        // acc = valid(vcol) ? acc : B  /  valid(vcol) ? B : acc
+  VM_IS_NULL, VM_IS_NOT_NULL,       // acc = !valid(vcol)  /  valid(vcol)
+  VM_GENERIC,   // transcendentals, POWER, date parts: row by row, out of line;
+                // arg = DevOpCode, gen = conversions / flags
+  VM_COUNT_
+};
+enum VmCls : uint8_t {
+  VC_NONE = 0, VC_IMM, VC_TMP,
+  VC_L64, VC_L32I, VC_L32F, VC_L16I, VC_L8I, VC_L8U,   // LDS tile slice
+  VC_G64, VC_G32I, VC_G32F, VC_G16I, VC_G8I, VC_G8U,   // HBM column (kept rows only)
+};
+// VM_GENERIC flag bits (`gen`): 0-1 operand conversion, 2-3 accumulator conversion,
+// 4-5 result conversion, 6 operand is the left-hand side, 7 unary.
+constexpr uint8_t VG_REV = 0x40, VG_UNARY = 0x80;
+struct VmOp {
+  uint8_t code;   // VmCode
+  uint8_t arg;    // temp / output / comparison / DevOpCode
+  uint8_t cls;    // VmCls of the operand (StoreKind for VM_EMIT)
+  uint8_t cv;     // operand conversion (Conv) applied after the fetch
+  uint8_t gen;    // VM_GENERIC flags
+  uint8_t vcol;   // input whose raw validity COALESCE / IS_[NOT_]NULL read; 0xFF none
+  uint8_t arg2;   // VC_G* / VC_L*: input column; VC_TMP: temporary
+  uint8_t pad;
+  int32_t off;    // VC_L*: byte offset of the column slice in the tile buffer
+  int32_t pad2;
+  uint64_t imm;   // VC_IMM operand
+  uint64_t imm2;  // VM_CMPAND immediate
+};
+static_assert(sizeof(VmOp) == 32, "VmOp layout");
+
+// Tile VM staging: one piece of up to 1 KiB of a tile's column (or bitmap) slice,
+// moved HBM -> LDS by one wave (64 lanes x 16 bytes, direct to LDS).
+struct Piece {
+  const void *base;      // column data / bitmap base
+  int32_t slice;         // bytes of the slice per full tile (T * width, or T / 8)
+  int32_t off;           // offset of the piece within the slice
+  int32_t lds;           // LDS offset of the piece within a tile buffer
+  int32_t bytes;         // bytes of the piece (<= 1024)
+  int32_t bitmap;        // 1: a bitmap slice (partial tile: ceil(rows / 8) bytes)
+  int32_t width;         // data slice: bytes per row
+};
+static_assert(sizeof(Piece) == 32, "Piece layout");
+constexpr int kPieceBytes = 1024;
+
 // Validity op over 64-row words.
 enum VOpCode : uint8_t { VOP_AND = 0, VOP_OR = 1 };
 struct VOp {
@@ -160,8 +221,11 @@ struct ProgHeader {
   int32_t lds_vbits_off;         // staged kernel: LDS offset of compacted-validity words
   int32_t n_valid_out;           // outputs with validity
   int32_t lds_bytes;             // staged kernel: dynamic LDS per workgroup
-  int32_t acc_mode;              // ops are AccOp (accumulator form) instead of DevOp
+  int32_t acc_mode;              // ops are AccOp (1: accumulator form) / VmOp (2: tile VM) instead of DevOp
   int32_t n_temps;               // accumulator form: temporaries used
+  int32_t buf_bytes;             // tile VM: bytes of one of the two tile buffers
+  int32_t n_pieces;              // tile VM: staging pieces per tile
+  int32_t off_pieces;            // tile VM: offset of the Piece table
   // offsets (bytes from the blob start)
   int32_t off_in, off_out, off_ops, off_vops, off_chunks;
   int32_t blob_bytes;

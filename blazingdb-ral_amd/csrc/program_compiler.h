@@ -50,17 +50,21 @@ struct CompileRequest {
   int scalar_mode = BLZ_SCALAR_INTENDED;
   int force_s = 0, force_r = 0;     // register-file shape override (autotuner)
   int force_t = 0;                  // staged tile rows override (autotuner); -1 = direct kernel
-  int kernel = 0;                   // 0 automatic, 1 accumulator, 2 staged register file, 3 direct
+  int kernel = 0;                   // 0 automatic, 1 accumulator, 2 staged register file, 3 direct, 4 tile VM
   int lds_budget = 64 * 1024;       // dynamic LDS per workgroup the staged kernel may use
 };
 
 struct Compiled {
   std::vector<uint8_t> blob;
+  int64_t n_rows = 0;
   int S = 8, R = 1;
   int n_slots = 0, n_vslots = 0, n_ops = 0, n_vops = 0;
   int T = 0;          // staged kernel tile rows (0: direct kernel)
   int lds_bytes = 0;  // staged kernel dynamic LDS per workgroup
   bool acc = false;   // accumulator-form program (S = temporaries)
+  bool vm = false;    // tile VM program (S = temporaries, T = 256 R)
+  bool vm_slow = false;  // the VM program has ops of the out-of-line set
+  int lds_gather = 0;    // tile VM filter: LDS of the gather kernel
 };
 
 // Throws std::runtime_error with a message on an unsupported or malformed plan.

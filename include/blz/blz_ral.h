@@ -134,10 +134,12 @@ typedef struct blz_program_s *blz_program;
  * device buffers the program allocates and owns: out_columns[0..*n_out) receive
  * them (capacity: every frame row; sizes valid after blz_program_result).
  * kernel: 0 automatic, 1 accumulator form (LDS-staged), 2 LDS-staged register file,
- * 3 unstaged register file. s / r (0 = automatic) force the shape: temporaries x rows
- * per lane (2,8) (4,4) (8,2) for kernel 1, slots x rows (8,1) (8,2) (16,1) (32,1) (64,1)
- * otherwise; t (0 = automatic) forces the staged tile rows (a multiple of 256*r,
- * <= 4096). */
+ * 3 unstaged register file, 4 tile VM (double-buffered direct-to-LDS staging).
+ * s / r (0 = automatic) force the shape: temporaries x rows per lane (2,8) (4,4)
+ * (8,2) for kernel 1, (2,8) (4,8) (2,4) (4,4) (8,4) (2,2) ... as listed in
+ * vm_kernel.hip for kernel 4, slots x rows (8,1) (8,2) (16,1) (32,1) (64,1)
+ * otherwise; t (0 = automatic) forces the tile rows (a multiple of 256*r, <= 4096;
+ * exactly 256*r for kernel 4). */
 int blz_program_create(blz_program *prog, gdf_column *const *frame, int n_frame,
                        const char *filter_query_part, const char *project_query_part,
                        gdf_column *out_columns, int max_out, int *n_out, int scalar_mode,

@@ -223,7 +223,7 @@ def test_program_matches_one_shot(B):
     prog.close()
 
 
-@pytest.mark.parametrize("kernel", [1, 2, 3])
+@pytest.mark.parametrize("kernel", [1, 2, 3, 4])
 @pytest.mark.parametrize("seed", range(8))
 def test_kernel_kinds_random_filter_project(B, kernel, seed):
     """Every kernel kind (accumulator / staged register file / direct) on random plans."""
@@ -259,7 +259,7 @@ def test_kernel_kinds_random_filter_project(B, kernel, seed):
     prog.close()
 
 
-@pytest.mark.parametrize("kernel", [1, 2, 3])
+@pytest.mark.parametrize("kernel", [1, 2, 3, 4])
 def test_kernel_kinds_c1(B, kernel):
     n = 1_000_003
     _print_case(test="kernel_kinds_c1", kernel=kernel, n=n)
@@ -274,3 +274,51 @@ def test_kernel_kinds_c1(B, kernel):
     for g, r in zip(prog.outputs, ref):
         assert g.data().tobytes() == r.data.tobytes()
     prog.close()
+
+
+VM_SHAPES = [(8, 2), (8, 4), (4, 2), (4, 4), (4, 8), (2, 4), (2, 8)]
+
+
+@pytest.mark.parametrize("shape", VM_SHAPES, ids=[f"r{r}nt{t}" for r, t in VM_SHAPES])
+@pytest.mark.parametrize("mode", ["filter", "project"])
+def test_vm_shapes(B, shape, mode):
+    """Every tile-VM shape, filter+project and pure projection, with nulls, on row
+    counts below, at and just past whole tiles."""
+    r, nt = shape
+    T = 256 * r
+    for seed, n in enumerate([1, T - 1, T, 3 * T + 17, 40 * T + 5]):
+        rng = np.random.default_rng(7100 + 17 * seed + r * 3 + nt)
+        k = int(rng.integers(2, 6))
+        dts = exprgen.rand_frame_dtypes(rng, k)
+        null_rates = [float(rng.choice([0.0, 0.1])) for _ in range(k)]
+        host = [gen_column(rng, dt, n, nr) for dt, nr in zip(dts, null_rates)]
+        nullable = [i for i, nr in enumerate(null_rates) if nr > 0]
+        exprs = [exprgen.rand_expr(rng, dts, 2, False, nullable) for _ in range(int(rng.integers(1, 4)))]
+        items = [f"E{i}=[{e}]" for i, e in enumerate(exprs)] + [f"A=[${int(rng.integers(0, k))}]"]
+        proj = "LogicalProject(" + ", ".join(items) + ")"
+        filt = None
+        if mode == "filter":
+            cond = f"<({exprgen.rand_expr(rng, dts, 2, False, nullable)}, {int(rng.integers(-5, 6))})"
+            filt = f"LogicalFilter(condition=[{cond}])"
+        _print_case(test="vm_shapes", r=r, nt=nt, n=n, filt=filt, proj=proj)
+        try:
+            mid = O.process_filter(host, filt) if filt else host
+            ref = O.execute_project(mid, proj)
+        except (P.PlanError, ValueError):
+            continue
+        dev = [to_device(B, h) for h in host]
+        try:
+            prog = B.Program(dev, filt, proj, kernel=4, s=nt, r=r)
+        except B.BlzError as e:
+            if "fit" in str(e):
+                continue
+            raise
+        assert prog.kernel == 4 and prog.r == r and prog.s == nt and prog.tile_rows == T
+        for _ in range(2):  # a second launch must give the same result
+            prog.launch()
+            kept, nulls = prog.result()
+            assert kept == len(ref[0])
+            for g, rc, nc in zip(prog.outputs, ref, nulls):
+                g.col.null_count = nc
+                assert_column_equal(g, rc, what=f"vm r={r} nt={nt} n={n}: {filt} {proj}")
+        prog.close()

@@ -1,0 +1,142 @@
+"""Host-side checks that need no GPU.
+
+- The C-ABI library loads and exports every entry point include/blz/*.h declares.
+- The C++ planner (the product's, behind blz_plan_expression) produces the same
+  plan vectors as the oracle's restatement of add_expression_to_plan
+  (LogicalFilter.cpp:44-267) on seeded random expressions, and both reject the
+  plans the reference cannot run.
+- The C1 predicate's plan matches its hand derivation from the reference's
+  register allocation (inputs in first-use order, final output at n_in, temps
+  lowest-free from n_in + 1).
+"""
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+from oracle import planner as P
+
+from . import exprgen
+from .conftest import ROOT
+
+HEADERS = [os.path.join(ROOT, "include", "blz", h) for h in ("blz_ral.h", "gdf_types.h")]
+
+
+@pytest.fixture(scope="module")
+def B():
+    import blz_ral
+    blz_ral.lib()  # loads on a host without a GPU; no compute is called here
+    return blz_ral
+
+
+def declared_entry_points():
+    names = set()
+    for h in HEADERS:
+        text = open(h).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        for m in re.finditer(r"^\s*(?:[\w\s\*]+?)\b(blz_\w+)\s*\(", text, flags=re.M):
+            names.add(m.group(1))
+    return names
+
+
+def test_library_exports_every_declared_entry_point(B):
+    names = declared_entry_points()
+    assert len(names) >= 25, names
+    out = subprocess.run(["nm", "-D", "--defined-only", B.LIB_PATH], capture_output=True, text=True,
+                         check=True).stdout
+    exported = {l.split()[-1] for l in out.splitlines() if l.strip()}
+    missing = sorted(n for n in names if n not in exported and not n.startswith("blz_dtype"))
+    assert not missing, missing
+    for n in names:
+        if n in exported:
+            getattr(B.lib(), n)  # resolvable through ctypes
+
+
+def test_version_and_no_device(B):
+    assert B.lib().blz_version() is not None
+
+
+def _scalar_value(s):
+    field = {P.GDF_FLOAT32: "fp32", P.GDF_FLOAT64: "fp64", P.GDF_INT8: "si08", P.GDF_INT16: "si16",
+             P.GDF_INT32: "si32"}.get(s.dtype, "si64")
+    v = getattr(s.data, field)
+    return float(v) if field.startswith("fp") else int(v)
+
+
+def _py_scalar_value(s):
+    if s.dtype == P.GDF_FLOAT32:
+        return float(np.float32(s.value))
+    if s.dtype == P.GDF_FLOAT64:
+        return float(s.value)
+    return int(s.value)
+
+
+def compare_plans(B, dtypes, expr):
+    try:
+        want = P.plan_expression(dtypes, expr)
+    except P.PlanError:
+        with pytest.raises(B.BlzError):
+            B.plan_expression(dtypes, expr)
+        return "rejected"
+    if want is None:
+        return "bare"
+    got = B.plan_expression(dtypes, expr)
+    assert got["left"] == want.left, expr
+    assert got["right"] == want.right, expr
+    assert got["outputs"] == want.outputs, expr
+    assert got["binops"] == want.binops, expr
+    assert got["unops"] == want.unops, expr
+    assert got["input_columns"] == want.input_columns, expr
+    assert got["final_position"] == want.final_positions[0], expr
+    for side, wl in (("lscalars", want.lscalars), ("rscalars", want.rscalars)):
+        for k, (g, w) in enumerate(zip(got[side], wl)):
+            uses = (want.left[k] if side == "lscalars" else want.right[k]) in (-2, -3)
+            if not uses:
+                continue
+            assert g.dtype == w.dtype, (expr, side, k)
+            assert bool(g.is_valid) == bool(w.is_valid), (expr, side, k)
+            if w.is_valid:
+                assert _scalar_value(g) == _py_scalar_value(w), (expr, side, k)
+    return "ok"
+
+
+@pytest.mark.parametrize("seed", range(40))
+def test_cpp_planner_matches_oracle_planner(B, seed):
+    rng = np.random.default_rng(900 + seed)
+    k = int(rng.integers(1, 7))
+    dtypes = exprgen.rand_frame_dtypes(rng, k)
+    seen = set()
+    for _ in range(25):
+        expr = exprgen.rand_expr(rng, dtypes, int(rng.integers(1, 5)), True, list(range(k)))
+        seen.add(compare_plans(B, dtypes, expr))
+    assert "ok" in seen
+
+
+@pytest.mark.parametrize("expr", [
+    "+(1, 2)",               # both operands literal (LogicalFilter.cpp:121-129)
+    "SIN(1.5)",              # unary on a literal (:234-235)
+    "=($0, 'abc')",          # string literal (NVCategory path)
+    "BITWISE_AND($0, $1)",   # no kernel branch
+])
+def test_planner_rejections_agree(B, expr):
+    assert compare_plans(B, [P.GDF_INT32, P.GDF_INT32], expr) == "rejected"
+
+
+def test_c1_predicate_plan_by_hand(B):
+    """AND(>=($3, 730), <($3, 1095), >=($2, 0.03), <=($2, 0.05), <($0, 30)) over
+    (a i32, b f64, c f64, d i32): expand_if_logical_op makes it left-deep
+    AND(AND(AND(AND(p1, p2), p3), p4), p5); tokens are processed right to left, so
+    p5 (<($0, 30)) is planned first. Inputs are the used columns in frame order
+    (a, c, d -> slots 0, 1, 2), the result slot is 3, temporaries start at 4."""
+    dtypes = [P.GDF_INT32, P.GDF_FLOAT64, P.GDF_FLOAT64, P.GDF_INT32]
+    expr = "AND(>=($3, 730), <($3, 1095), >=($2, 0.03), <=($2, 0.05), <($0, 30))"
+    got = B.plan_expression(dtypes, expr)
+    assert got["input_columns"] == [0, 2, 3]
+    assert got["final_position"] == 3
+    assert len(got["binops"]) == 9
+    assert sorted(got["binops"]).count(P.GDF_MUL) == 4  # AND -> GDF_MUL
+    assert got["outputs"][-1] == 3
+    assert all(o == 3 or o >= 4 for o in got["outputs"])
+    assert compare_plans(B, dtypes, expr) == "ok"
