@@ -96,7 +96,7 @@ int vm_per_cu(const Compiled &c, int mode, int lds) {
   std::lock_guard<std::mutex> g(mu);
   auto it = cache.find(key);
   if (it != cache.end()) return it->second;
-  const int b = std::max(1, vm_blocks_per_cu(c.R, c.S, mode, c.vm_slow, lds));
+  const int b = std::max(1, vm_blocks_per_cu(c.R, mode, c.vm_slow, lds));
   cache[key] = b;
   return b;
 }
@@ -176,11 +176,11 @@ hipError_t launch_program(const Compiled &c, bool compact, int grid, hipStream_t
       const int64_t resident = (int64_t)device_cus() * vm_per_cu(c, mode, lds);
       return (int)std::max<int64_t>(1, std::min<int64_t>(tiles, resident));
     };
-    if (!compact) return launch_vm(c.R, c.S, 0, c.vm_slow, grid_for(0, c.lds_bytes), c.lds_bytes, st, blob, ws);
-    hipError_t e = launch_vm(c.R, c.S, 1, c.vm_slow, grid_for(1, c.lds_bytes), c.lds_bytes, st, blob, ws);
+    if (!compact) return launch_vm(c.R, 0, c.vm_slow, grid_for(0, c.lds_bytes), c.lds_bytes, st, blob, ws);
+    hipError_t e = launch_vm(c.R, 1, c.vm_slow, grid_for(1, c.lds_bytes), c.lds_bytes, st, blob, ws);
     if (e == hipSuccess) e = launch_slot_scan(ws.wave_count, ws.wave_prefix, tiles * 4, st, ws);
     if (e == hipSuccess)
-      e = launch_vm(c.R, c.S, 2, c.vm_slow, grid_for(2, c.lds_gather), c.lds_gather, st, blob, ws);
+      e = launch_vm(c.R, 2, c.vm_slow, grid_for(2, c.lds_gather), c.lds_gather, st, blob, ws);
     return e;
   }
   if (c.acc) return launch_acc(c.S, c.R, compact, grid, c.lds_bytes, st, blob, ws);

@@ -32,21 +32,20 @@ int interp_blocks_per_cu(int s, int r, bool compact, int lds_bytes);
 hipError_t launch_acc(int nt, int r, bool compact, int grid, int lds_bytes, hipStream_t st,
                       const uint8_t *blob, Workspace ws);
 int acc_blocks_per_cu(int nt, int r, bool compact, int lds_bytes);
-// Tile VM (vm_kernel.hip): (r, nt) rows per lane x temporaries in
-// {(8,2), (8,4), (4,2), (4,4), (4,8), (2,4), (2,8)}; tile = 256 r rows, lds_bytes
-// holds two tile buffers and the compacted-validity words. slow: the program has
-// ops of the out-of-line set (transcendentals, POWER, date parts).
+// Tile VM (vm_kernel.hip): r rows per lane in {8, 4, 2}; tile = 256 r P rows;
+// lds_bytes: two tile buffers + temporaries (modes 0, 1), or the gather kernel's
+// keep words, pass buffer, validity words and temporaries (mode 2). slow: the
+// program has ops of the out-of-line set (transcendentals, POWER, date parts).
 // mode: 0 projection, 1 predicate (keep words + slot counts), 2 gather of the kept
 // rows (after launch_slot_scan).
-hipError_t launch_vm(int r, int nt, int mode, bool slow, int grid, int lds_bytes, hipStream_t st,
-                     const uint8_t *blob, Workspace ws);
-int vm_blocks_per_cu(int r, int nt, int mode, bool slow, int lds_bytes);
+hipError_t launch_vm(int r, int mode, bool slow, int grid, int lds_bytes, hipStream_t st, const uint8_t *blob,
+                     Workspace ws);
+int vm_blocks_per_cu(int r, int mode, bool slow, int lds_bytes);
 // Exclusive prefix of n_slots counts (single-pass decoupled look-back over
 // ws.tile_state / ws.tile_counter, which must be zero); total to ws.n_sel.
 hipError_t launch_slot_scan(const uint32_t *counts, uint32_t *prefix, int64_t n_slots, hipStream_t st,
                             Workspace ws);
 int64_t slot_scan_blocks(int64_t n_slots);
-bool vm_shape_ok(int r, int nt);
 // materialize_column: width in {1,2,4,8}.
 hipError_t launch_gather(int width, const void *in, const uint8_t *in_valid, int64_t n_in, const int32_t *idx,
                          int64_t n_idx, void *out, uint8_t *out_valid, unsigned long long *null_count,

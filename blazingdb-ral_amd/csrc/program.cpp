@@ -497,7 +497,7 @@ bool compile_vm(const CompileRequest &req, Builder &B, const std::vector<AccOp> 
       case OK_INPUT: {
         const int c = a.oidx;
         const int lk = load_kind(req.inputs[c].dtype);
-        const bool g = gather_pass || !staged[c];
+        const bool g = !gather_pass && !staged[c];
         v.arg2 = (uint8_t)c;
         v.cv = (uint8_t)conv_b;
         switch (lk) {
@@ -622,6 +622,9 @@ bool compile_vm(const CompileRequest &req, Builder &B, const std::vector<AccOp> 
   if (!lower(acc_p, vm_p)) return false;
   gather_pass = req.compact;
   if (!lower(acc_q, vm_q)) return false;
+  // The gather pass of a filter stages the kept rows of the columns its ops read
+  // into a pass buffer (256 R rows x each width) before evaluating them.
+  std::vector<int> goff(n_dev, -1);
   // A VM_GENERIC with no operand still runs `fetch` on VC_NONE: give it an
   // immediate so no memory is touched.
   for (auto *v : {&vm_p, &vm_q})
@@ -651,29 +654,54 @@ bool compile_vm(const CompileRequest &req, Builder &B, const std::vector<AccOp> 
     buf = (int)std::max<size_t>(off, 16);
     return 2 * (size_t)buf;
   };
-  // Shape: rows per lane R (tile 256 R) and temporaries NT. The default keeps
-  // two tile buffers within ~80 KiB so at least two workgroups share a CU.
-  static const int shapes[][2] = {{8, 2}, {8, 4}, {4, 2}, {4, 4}, {4, 8}, {2, 4}, {2, 8}};
-  const int budget = std::min(req.lds_budget > 0 ? req.lds_budget : 80 * 1024, 160 * 1024 - 4096);
-  int R = 0, NT = 0, buf = 0;
+  // Shape: rows per lane R and tile rows T = 256 R P (P passes of the four waves).
+  // Temporaries live in LDS (256 R rows each). The default is R = 4 and one pass
+  // per tile (T = 1024): small double buffers keep four workgroups per CU, which
+  // hides more latency than longer tiles do; the autotuner may choose otherwise.
+  static const int shapes[] = {4, 8, 2};
+  const int budget = std::min(req.lds_budget > 0 ? req.lds_budget : 64 * 1024, 160 * 1024 - 4096);
+  int R = 0, T = 0, buf = 0;
   size_t lds_total = 0;
-  for (auto &sh : shapes) {
-    if (acc_temps > sh[1]) continue;
-    if (req.force_r > 0 && req.force_r != sh[0]) continue;
-    if (req.force_s > 0 && req.force_s != sh[1]) continue;
-    int b = 0;
-    const size_t tot = layout(256 * sh[0], b);
-    if (tot > (size_t)budget) continue;
-    if (req.force_t > 0 && req.force_t != 256 * sh[0]) continue;
-    R = sh[0]; NT = sh[1]; buf = b; lds_total = tot;
-    break;
+  int tmp_bytes = 0;
+  auto al16 = [](int x) { return (x + 15) & ~15; };
+  for (int r : shapes) {
+    if (req.force_r > 0 && req.force_r != r) continue;
+    const int tb = acc_temps * 256 * r * 8;
+    int g = 0;  // gather pass buffer of a filter
+    for (int c = 0; c < n_dev && req.compact; c++)
+      if (used_q[c]) g = al16(g + 256 * r * std::max(1, wid(req.inputs[c].dtype)));
+    for (int p = 64 / r; p >= 1; p /= 2) {
+      const int t = 256 * r * p;
+      if (req.force_t > 0 ? t != req.force_t : t > 256 * r) continue;
+      int b = 0;
+      const size_t tot = layout(t, b) + tb;
+      if (tot > (size_t)budget) continue;
+      const int gl = g + 4 * n_valid_out * 66 * 8 + tb;
+      if (req.compact && gl > budget) continue;
+      R = r; T = t; buf = b; lds_total = tot; tmp_bytes = tb;
+      break;
+    }
+    if (R) break;
   }
   if (R == 0) { why = "no tile VM shape fits the plan"; return false; }
-  const int T = 256 * R;
+  const int NT = acc_temps;
   layout(T, buf);
-  for (auto *v : {&vm_p, &vm_q})
-    for (auto &op : *v)
-      if (op.cls >= VC_L64 && op.cls <= VC_L8U) op.off = lds_off[op.arg2];
+  for (auto &op : vm_p)
+    if (op.cls >= VC_L64 && op.cls <= VC_L8U) op.off = lds_off[op.arg2];
+  std::vector<Piece> gcols;
+  int gbuf = 0;
+  if (req.compact) {
+    for (int c = 0; c < n_dev; c++) {
+      if (!used_q[c]) continue;
+      const int w = std::max(1, wid(req.inputs[c].dtype));
+      goff[c] = gbuf;
+      gcols.push_back(Piece{req.inputs[c].data, 0, 0, gbuf, 0, 0, w});
+      gbuf += 256 * R * w;
+      gbuf = (gbuf + 15) & ~15;
+    }
+  }
+  for (auto &op : vm_q)
+    if (op.cls >= VC_L64 && op.cls <= VC_L8U) op.off = req.compact ? goff[op.arg2] : lds_off[op.arg2];
 
   std::vector<Piece> pieces;
   for (int c = 0; c < n_dev; c++) {
@@ -698,7 +726,8 @@ bool compile_vm(const CompileRequest &req, Builder &B, const std::vector<AccOp> 
   const size_t off_out = al(off_in + sizeof(InCol) * n_dev);
   const size_t off_ops = al(off_out + sizeof(OutCol) * n_out);
   const size_t off_pieces = al(off_ops + sizeof(VmOp) * (vm_p.size() + vm_q.size()));
-  const size_t off_vops = al(off_pieces + sizeof(Piece) * pieces.size());
+  const size_t off_gcols = al(off_pieces + sizeof(Piece) * pieces.size());
+  const size_t off_vops = al(off_gcols + sizeof(Piece) * gcols.size());
   const size_t total = al(off_vops + sizeof(VOp) * (vops_p.size() + vops_q.size()));
   if ((int)total > blob_capacity()) { why = "plan too large for the program buffer"; return false; }
   auto vsrc = [&](int v) { return v == V_ALL ? -1 : v == V_NONE ? -2 : B.vnodes[v].vslot; };
@@ -725,7 +754,19 @@ bool compile_vm(const CompileRequest &req, Builder &B, const std::vector<AccOp> 
   h.any_valid_out = any_valid;
   h.tile_rows = T;
   h.n_chunks = 0;
-  h.lds_vbits_off = 0;
+  // gather kernel LDS: pass buffer | validity words (per wave) | temporaries
+  const int g_kw = 0;
+  const int g_buf = 0;
+  const int g_vb = g_buf + gbuf;
+  const int g_tmp = g_vb + 4 * n_valid_out * 66 * 8;  // 64 + 2 words per wave and output
+  h.lds_vbits_off = g_vb;
+  h.n_gcols = (int)gcols.size();
+  h.off_gcols = (int)off_gcols;
+  h.gbuf_bytes = gbuf;
+  h.g_kw_off = g_kw;
+  h.g_buf_off = g_buf;
+  h.g_tmp_off = g_tmp;
+  h.lds_tmp_off = 2 * buf;
   h.n_valid_out = n_valid_out;
   h.lds_bytes = (int)lds_total;
   h.acc_mode = 2;
@@ -767,6 +808,7 @@ bool compile_vm(const CompileRequest &req, Builder &B, const std::vector<AccOp> 
   for (auto &op : vm_p) { put(out.blob, p, op); p += sizeof(VmOp); }
   for (auto &op : vm_q) { put(out.blob, p, op); p += sizeof(VmOp); }
   for (size_t k = 0; k < pieces.size(); k++) put(out.blob, off_pieces + sizeof(Piece) * k, pieces[k]);
+  for (size_t k = 0; k < gcols.size(); k++) put(out.blob, off_gcols + sizeof(Piece) * k, gcols[k]);
   p = off_vops;
   for (auto &op : vops_p) { put(out.blob, p, op); p += sizeof(VOp); }
   for (auto &op : vops_q) { put(out.blob, p, op); p += sizeof(VOp); }
@@ -781,7 +823,7 @@ bool compile_vm(const CompileRequest &req, Builder &B, const std::vector<AccOp> 
   out.acc = false;
   out.vm = true;
   out.vm_slow = slow;
-  out.lds_gather = n_valid_out > 0 ? 4 * (R + 2) * 8 : 0;
+  out.lds_gather = g_tmp + tmp_bytes;
   (void)vcol_needed;
   return true;
 }

@@ -226,6 +226,13 @@ struct ProgHeader {
   int32_t buf_bytes;             // tile VM: bytes of one of the two tile buffers
   int32_t n_pieces;              // tile VM: staging pieces per tile
   int32_t off_pieces;            // tile VM: offset of the Piece table
+  int32_t n_gcols;               // tile VM filter: columns the gather pass stages (Piece: base, width, lds)
+  int32_t off_gcols;             // tile VM filter: offset of that table
+  int32_t gbuf_bytes;            // tile VM filter: bytes of the gather pass buffer (256 R rows)
+  int32_t lds_tmp_off;           // tile VM: LDS temporaries (projection / predicate kernels)
+  int32_t g_tmp_off;             // tile VM filter: LDS temporaries of the gather kernel
+  int32_t g_kw_off;              // tile VM filter: gather kernel's keep words + counts
+  int32_t g_buf_off;             // tile VM filter: gather kernel's pass buffer
   // offsets (bytes from the blob start)
   int32_t off_in, off_out, off_ops, off_vops, off_chunks;
   int32_t blob_bytes;

@@ -15,15 +15,17 @@
  * Evaluation. The plan is in accumulator form (every op combines the running
  * value with at most one operand) pre-decoded on the host into 32-byte VmOps read
  * through the scalar cache, so the per-op cost is one uniform switch amortised
- * over 64 R rows. Operands come from the LDS tile, an immediate, one of NT
- * temporaries or, in the projection pass of a filter, straight from HBM for the
- * rows the filter keeps (columns the predicate does not read are never staged).
- * Ops outside the hot set run out of line, row by row.
+ * over 64 R rows. Operands come from the LDS tile, an immediate or an LDS
+ * temporary. Ops outside the hot set run out of line, row by row.
  *
- * Filter. Pass A evaluates the predicate into per-group keep masks (ballot);
- * the tile's count is combined across workgroups by a single-pass decoupled
- * look-back over dynamically assigned tile ids; pass B evaluates the projection
- * and scatters the kept rows, in row order, and their validity bits.
+ * Filter (three launches, no inter-workgroup waiting inside a tile loop). The
+ * predicate kernel writes one keep word per 64-row group (ballot) and the kept
+ * count of every (tile, wave) slot; one scan kernel turns the counts into output
+ * offsets with a single-pass decoupled look-back across workgroups; the gather
+ * kernel packs each tile's kept rows densely into lanes (output j is the tile's
+ * j-th kept row, found by a search over the keep words), stages just those rows
+ * of the columns the projection reads, evaluates the projection and stores the
+ * outputs and their validity words in row order.
  */
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -78,25 +80,69 @@ DEV void stage_partial(const CONSTANT Piece *pieces, int np, int64_t tile, int64
   }
 }
 
+// Raw element `i` of a column of `width` bytes (no widening; staged as is).
+DEV uint64_t load_raw(int width, const void *base, int64_t i) {
+  switch (width) {
+    case 8: return gload<uint64_t>(base, i);
+    case 4: return gload<uint32_t>(base, i);
+    case 2: return gload<uint16_t>(base, i);
+    default: return gload<uint8_t>(base, i);
+  }
+}
+DEV void store_raw(int width, lds_ptr p, uint64_t v) {
+  switch (width) {
+    case 8: *(LDSAS uint64_t *)p = v; break;
+    case 4: *(LDSAS uint32_t *)p = (uint32_t)v; break;
+    case 2: *(LDSAS uint16_t *)p = (uint16_t)v; break;
+    default: *p = (uint8_t)v; break;
+  }
+}
+
+// Gather staging: the kept rows grow[u] of every column the projection reads,
+// into this lane's rows of the pass buffer (row plrow + 64 u). Up to four
+// columns' loads are in flight before the first LDS store waits for them.
+template <int R>
+DEV void stage_kept(const CONSTANT Piece *gcols, int ng, lds_ptr gb, int plrow, const int64_t (&grow)[R],
+                    const bool (&keep)[R]) {
+  for (int g0 = 0; g0 < ng; g0 += 4) {
+    uint64_t v[4][R];
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      if (g0 + j < ng) {
+        const int w = gcols[g0 + j].width;
+        const void *base = gcols[g0 + j].base;
+#pragma unroll
+        for (int u = 0; u < R; u++) v[j][u] = keep[u] ? load_raw(w, base, grow[u]) : 0;
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      if (g0 + j < ng) {
+        const int w = gcols[g0 + j].width;
+        lds_ptr d = gb + gcols[g0 + j].lds + w * plrow;
+#pragma unroll
+        for (int u = 0; u < R; u++) store_raw(w, d + w * 64 * u, v[j][u]);
+      }
+    }
+  }
+}
+
 // Operand rows (lane + 64 u) of the op, widened to 8-byte slots.
-template <int R, int NT>
+template <int R>
 DEV void fetch(int cls, int off, int arg, int arg2, const CONSTANT VmOp &op, clds_ptr tb, int lrow,
-               const uint64_t (&tmp)[NT][R], const CONSTANT InCol *in, int64_t grow, const bool (&live)[R],
-               uint64_t (&B)[R]) {
+               clds_ptr tmpb, int tstride, const CONSTANT InCol *in, const int64_t (&grow)[R],
+               const bool (&live)[R], uint64_t (&B)[R]) {
   switch (cls) {
     case VC_IMM: {
       const uint64_t im = op.imm;
 #pragma unroll
       for (int u = 0; u < R; u++) B[u] = im;
     } break;
-    case VC_TMP:
+    case VC_TMP: {
+      clds_ptr p = tmpb + arg2 * tstride;
 #pragma unroll
-      for (int t = 0; t < NT; t++)
-        if (t == arg2) {
-#pragma unroll
-          for (int u = 0; u < R; u++) B[u] = tmp[t][u];
-        }
-      break;
+      for (int u = 0; u < R; u++) B[u] = *(const LDSAS uint64_t *)(p + 512 * u);
+    } break;
     case VC_L64: {
       clds_ptr p = tb + off + 8 * lrow;
 #pragma unroll
@@ -134,7 +180,7 @@ DEV void fetch(int cls, int off, int arg, int arg2, const CONSTANT VmOp &op, cld
       for (int u = 0; u < R; u++) {
         uint64_t v = 0;
         if (live[u]) {
-          const int64_t g = grow + 64 * u;
+          const int64_t g = grow[u];
           switch (cls) {
             case VC_G64: v = gload<uint64_t>(base, g); break;
             case VC_G32I: v = (uint64_t)(int64_t)gload<int32_t>(base, g); break;
@@ -230,35 +276,36 @@ DEV uint64_t vm_imod(uint64_t a, uint64_t b) { return (uint64_t)imod((int64_t)a,
 // Raw validity bits of input `vcol` for rows grow + 64 u (COALESCE, IS_[NOT_]NULL:
 // interpreter_ops.cuh:633-645, :820-829 read the left input's bitmap).
 template <int R>
-DEV void vcol_bits(const CONSTANT InCol *in, int vcol, int64_t grow, int64_t n, bool (&vb)[R]) {
+DEV void vcol_bits(const CONSTANT InCol *in, int vcol, const int64_t (&grow)[R], int64_t n, bool (&vb)[R]) {
   const void *vp = vcol != 0xFF ? in[vcol].valid : nullptr;
 #pragma unroll
   for (int u = 0; u < R; u++) {
-    const int64_t g = grow + 64 * u;
+    const int64_t g = grow[u];
     vb[u] = valid_bit(vp, g < n ? g : n - 1);
   }
 }
 
-// Runs `nops` VmOps over rows lrow + 64 u of the tile buffer `tb` (global row
-// grow + 64 u). VM_EMIT stores row u at output position obase + orel[u] when
-// live[u]. SLOW: the program has ops of the out-of-line set (VM_GENERIC).
-template <int R, int NT, bool SLOW>
-DEV void run_vm(const CONSTANT VmOp *ops, int nops, uint64_t (&acc)[R], uint64_t (&tmp)[NT][R],
-                const CONSTANT InCol *in, const CONSTANT OutCol *out, clds_ptr tb, int lrow, int64_t grow,
-                int64_t n, int64_t obase, const int (&orel)[R], const bool (&live)[R]) {
+// Runs `nops` VmOps over rows lrow + 64 u of the tile buffer `tb` (global rows
+// grow[u]). Temporaries live in LDS: temporary t of row u at
+// tmpb + t * tstride + 512 u. VM_EMIT stores row u at output position
+// obase + orel[u] when live[u]. SLOW: the program has ops of the out-of-line set
+// (VM_GENERIC).
+template <int R, bool SLOW>
+DEV void run_vm(const CONSTANT VmOp *ops, int nops, uint64_t (&acc)[R], lds_ptr tmpb, int tstride,
+                const CONSTANT InCol *in, const CONSTANT OutCol *out, clds_ptr tb, int lrow,
+                const int64_t (&grow)[R], int64_t n, int64_t obase, const int (&orel)[R],
+                const bool (&live)[R]) {
   for (int k = 0; k < nops; k++) {
     const CONSTANT VmOp &op = ops[k];
     const u32x4 w = *(const CONSTANT u32x4 *)&op;
     const int code = (int)(w.x & 0xFF), arg = (int)((w.x >> 8) & 0xFF);
     const int cls = (int)((w.x >> 16) & 0xFF), cv = (int)(w.x >> 24);
     switch (code) {
-      case VM_STORE:
+      case VM_STORE: {
+        lds_ptr p = tmpb + arg * tstride;
 #pragma unroll
-        for (int t = 0; t < NT; t++)
-          if (t == arg) {
-#pragma unroll
-            for (int u = 0; u < R; u++) tmp[t][u] = acc[u];
-          }
+        for (int u = 0; u < R; u++) *(LDSAS uint64_t *)(p + 512 * u) = acc[u];
+      }
         continue;
       case VM_EMIT: {
         void *base = out[arg].data;
@@ -330,7 +377,7 @@ DEV void run_vm(const CONSTANT VmOp *ops, int nops, uint64_t (&acc)[R], uint64_t
     const int off = (int)w.z;
     const int arg2 = (int)((w.y >> 16) & 0xFF);
     uint64_t B[R];
-    fetch<R, NT>(cls, off, arg, arg2, op, tb, lrow, tmp, in, grow, live, B);
+    fetch<R>(cls, off, arg, arg2, op, tb, lrow, tmpb, tstride, in, grow, live, B);
     if (cv == CV_I2F) {
 #pragma unroll
       for (int u = 0; u < R; u++) B[u] = d2b((double)(int64_t)B[u]);
@@ -426,7 +473,7 @@ DEV void run_vm(const CONSTANT VmOp *ops, int nops, uint64_t (&acc)[R], uint64_t
           const void *vp = vcol != 0xFF ? in[vcol].valid : nullptr;
 #pragma nounroll
           for (int u = 0; u < R; u++) {
-            const int64_t g = grow + 64 * u;
+            const int64_t g = grow[u];
             const bool vb = valid_bit(vp, g < n ? g : n - 1);
             acc[u] = vm_generic_row(arg, gen, acc[u], B[u], vb);
           }
@@ -455,16 +502,31 @@ DEV void vm_read_valid(const CONSTANT InCol *in, int n_in, clds_ptr tb, int grou
 //   VM_PRED     evaluate the predicate: one keep word per 64-row group and one
 //               kept-row count per wave slot (tile, wave);
 //   VM_GATHER   from the keep words and the exclusive prefix of the counts, evaluate
-//               the projection of the kept rows only (operands read from HBM for those
-//               rows) and store them, and their validity bits, in row order.
+//               the projection of the kept rows only, packed densely into lanes
+//               (output j of a tile is its j-th kept row), and store them, and their
+//               validity bits, in row order.
 enum VmMode { VM_PROJECT = 0, VM_PRED = 1, VM_GATHER = 2 };
 
-template <int R, int NT, int MODE, bool SLOW>
+// Position of the r-th (0-based) set bit of m (r < popcount(m)).
+DEV int select_bit(uint64_t m, int r) {
+  int p = 0;
+#pragma unroll
+  for (int s = 32; s > 0; s >>= 1) {
+    const int lo = __builtin_popcountll(m & ((1ull << s) - 1));
+    if (r >= lo) {
+      r -= lo;
+      m >>= s;
+      p += s;
+    }
+  }
+  return p;
+}
+
+template <int R, int MODE, bool SLOW>
 __global__ void __launch_bounds__(kBlock)
 vm_kernel(const uint8_t *__restrict__ blob_arg, Workspace ws) {
   extern __shared__ __attribute__((aligned(16))) uint8_t lds_raw[];
   constexpr int VS = 8;  // validity registers
-  constexpr int T = 256 * R;
   constexpr bool STAGED = MODE != VM_GATHER;
 
   lds_ptr lds = (lds_ptr)lds_raw;
@@ -475,26 +537,32 @@ vm_kernel(const uint8_t *__restrict__ blob_arg, Workspace ws) {
   const CONSTANT VmOp *ops = (const CONSTANT VmOp *)(blob + h.off_ops);
   const CONSTANT VOp *vops = (const CONSTANT VOp *)(blob + h.off_vops);
   const CONSTANT Piece *pieces = (const CONSTANT Piece *)(blob + h.off_pieces);
+  const CONSTANT Piece *gcols = (const CONSTANT Piece *)(blob + h.off_gcols);
+  const int n_gcols = h.n_gcols;
   const int64_t n = h.n_rows;
   const int n_in = h.n_in, n_out = h.n_out;
   const int n_ops_p = h.n_ops_p, n_ops_q = h.n_ops_q;
   const int n_vops_p = h.n_vops_p, n_vops_q = h.n_vops_q;
   const int any_valid_out = h.any_valid_out;
   const int np = h.n_pieces, bufb = h.buf_bytes;
+  const int T = h.tile_rows;            // 256 R P rows
+  const int P = T / (kBlock * R);       // passes per tile
+  const int GW = P * R;                 // 64-row groups per wave slot
 
   const int wave = threadIdx.x >> 6;
   const int lane = threadIdx.x & 63;
-  const int lb = wave * 64 * R;  // first row of this wave in the tile
-  const int lrow = lb + lane;
+  const int lbw = wave * 64 * GW;       // first row of this wave's slot in the tile
   const int64_t n_tiles = (n + T - 1) / T;
+  // Temporaries: LDS rows of this lane (one pass of the workgroup: 256 R rows).
+  const int tstride = 8 * kBlock * R;
+  lds_ptr tmpb = lds + (MODE == VM_GATHER ? h.g_tmp_off : h.lds_tmp_off) + 8 * (wave * 64 * R + lane);
 
   // Every register below is written by the program before it is read (the host
-  // compiler guarantees it); none is live across tiles.
+  // compiler guarantees it); none is live across passes.
   uint64_t acc[R];
-  uint64_t tmp[NT][R];
   uint64_t vrf[VS];
 
-  int64_t cur = blockIdx.x;
+  int64_t cur = MODE == VM_GATHER ? n_tiles : (int64_t)blockIdx.x;
   int buf = 0;
   if (STAGED && cur < n_tiles && (cur + 1) * T <= n) dma_tile(pieces, np, cur, lds);
 
@@ -516,31 +584,37 @@ vm_kernel(const uint8_t *__restrict__ blob_arg, Workspace ws) {
     }
 
     if (MODE == VM_PROJECT) {
-      if (lb >= rows) continue;  // wave-uniform
-      int orel[R];
-      bool put[R];
+      for (int p = 0; p < P; p++) {
+        const int lb = lbw + p * 64 * R;
+        if (lb >= rows) break;  // wave-uniform
+        const int lrow = lb + lane;
+        int orel[R];
+        bool put[R];
+        int64_t grow[R];
 #pragma unroll
-      for (int u = 0; u < R; u++) {
-        orel[u] = lrow + 64 * u;
-        put[u] = orel[u] < rows;
-        acc[u] = 0;
-      }
-      run_vm<R, NT, SLOW>(ops, n_ops_q, acc, tmp, in, out, tb, lrow, row0 + lrow, n, row0, orel, put);
-      if (any_valid_out) {
-        vm_read_valid<VS, R>(in, n_in, tb, lb >> 6, vrf);
-        run_vops<VS>(vops, n_vops_q, vrf);
-        for (int o = 0; o < n_out; o++) {
-          void *vp = out[o].valid;
-          if (vp == nullptr) continue;
-          const uint64_t word = vsrc_word<VS>(out[o].vsrc, vrf);
-          const int64_t wi = ((row0 + lb) >> 6) + lane;
-          uint64_t nulls = 0;
-          if (lane < R && wi * 64 < n) {
-            store_vword(vp, wi, n, word);
-            nulls = __builtin_popcountll(~word & low_mask(n - wi * 64));
+        for (int u = 0; u < R; u++) {
+          orel[u] = lrow + 64 * u;
+          put[u] = orel[u] < rows;
+          grow[u] = row0 + orel[u];
+          acc[u] = 0;
+        }
+        run_vm<R, SLOW>(ops, n_ops_q, acc, tmpb, tstride, in, out, tb, lrow, grow, n, row0, orel, put);
+        if (any_valid_out) {
+          vm_read_valid<VS, R>(in, n_in, tb, lb >> 6, vrf);
+          run_vops<VS>(vops, n_vops_q, vrf);
+          for (int o = 0; o < n_out; o++) {
+            void *vp = out[o].valid;
+            if (vp == nullptr) continue;
+            const uint64_t word = vsrc_word<VS>(out[o].vsrc, vrf);
+            const int64_t wi = ((row0 + lb) >> 6) + lane;
+            uint64_t nulls = 0;
+            if (lane < R && wi * 64 < n) {
+              store_vword(vp, wi, n, word);
+              nulls = __builtin_popcountll(~word & low_mask(n - wi * 64));
+            }
+            for (int s = 32; s > 0; s >>= 1) nulls += __shfl_xor(nulls, s);
+            if (lane == 0 && nulls) atomicAdd(ws.null_counts + o, (unsigned long long)nulls);
           }
-          for (int s = 32; s > 0; s >>= 1) nulls += __shfl_xor(nulls, s);
-          if (lane == 0 && nulls) atomicAdd(ws.null_counts + o, (unsigned long long)nulls);
         }
       }
       continue;
@@ -550,16 +624,21 @@ vm_kernel(const uint8_t *__restrict__ blob_arg, Workspace ws) {
       // ---- predicate -> keep word per 64-row group, kept rows per wave slot ----
       const uint64_t st_mask = low_mask(8 * h.stencil_width);
       uint64_t myword = 0, cnt = 0;
-      if (lb < rows) {
+      for (int p = 0; p < P; p++) {
+        const int lb = lbw + p * 64 * R;
+        if (lb >= rows) break;  // wave-uniform
+        const int lrow = lb + lane;
         int orel[R];
         bool none[R];
+        int64_t grow[R];
 #pragma unroll
         for (int u = 0; u < R; u++) {
           orel[u] = 0;
           none[u] = false;
+          grow[u] = row0 + lrow + 64 * u;
           acc[u] = 0;
         }
-        run_vm<R, NT, SLOW>(ops, n_ops_p, acc, tmp, in, out, tb, lrow, row0 + lrow, n, 0, orel, none);
+        run_vm<R, SLOW>(ops, n_ops_p, acc, tmpb, tstride, in, out, tb, lrow, grow, n, 0, orel, none);
         vm_read_valid<VS, R>(in, n_in, tb, lb >> 6, vrf);
         run_vops<VS>(vops, n_vops_p, vrf);
         const uint64_t vstencil = vsrc_word<VS>(h.stencil_vsrc, vrf);
@@ -568,87 +647,149 @@ vm_kernel(const uint8_t *__restrict__ blob_arg, Workspace ws) {
           const bool vbit = (readlane64(vstencil, u) >> lane) & 1;
           const bool keep = lrow + 64 * u < rows && ((acc[u] & st_mask) != 0) && vbit;
           const uint64_t km = __builtin_amdgcn_ballot_w64(keep);
-          if (lane == u) myword = km;
+          if (lane == p * R + u) myword = km;
           cnt += __builtin_popcountll(km);
         }
       }
-      const int64_t g0 = cur * (T / 64) + wave * R;  // this wave's first group
-      if (lane < R) gstore<uint64_t>(ws.keep, g0 + lane, myword);
+      const int64_t g0 = cur * (T / 64) + wave * GW;  // this wave's first group
+      if (lane < GW) gstore<uint64_t>(ws.keep, g0 + lane, myword);
       if (lane == 0) gstore<uint32_t>(ws.wave_count, cur * kWaves + wave, (uint32_t)cnt);
       continue;
     }
 
-    // ---- VM_GATHER: the kept rows of this wave slot, in row order ----
-    if (lb >= rows) continue;
-    const int64_t slot = cur * kWaves + wave;
-    const uint32_t wcnt = ws.wave_count[slot];
-    if (wcnt == 0) continue;  // wave-uniform
-    const uint64_t obase = ws.wave_prefix[slot];
-    const int64_t g0 = cur * (T / 64) + wave * R;
-    int rel[R];
-    bool keep[R];
-    uint64_t run = 0;
-#pragma unroll
-    for (int u = 0; u < R; u++) {
-      const uint64_t m = ((const CONSTANT uint64_t *)ws.keep)[g0 + u];
-      const uint32_t below = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
-                                 __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
-      rel[u] = (int)run + (int)below;
-      keep[u] = (m >> lane) & 1;
-      run += __builtin_popcountll(m);
-      acc[u] = 0;
+    (void)row0;
+  }
+  if (MODE != VM_GATHER) return;
+
+  // ---- VM_GATHER: every wave on its own spans of 64 keep words (4096 rows), no
+  // barriers. Output j of a span is its j-th kept row: found by a binary search
+  // over the span's per-word prefix counts (one word per lane, read with
+  // cross-lane shuffles) and a bit select within the word. The next span's keep
+  // words and output offset are loaded while this one is evaluated.
+  const int64_t n_words = n_tiles * (T / 64);
+  const int64_t n_spans = (n_words + 63) / 64;
+  const int n_vout = h.n_valid_out;
+  constexpr int VW = 66;  // validity words of a span's outputs (64 + 2)
+  LDSAS uint64_t *vbits = (LDSAS uint64_t *)(lds + h.lds_vbits_off) + wave * VW * n_vout;
+  lds_ptr gbuf = lds + h.g_buf_off;
+  const int plrow = wave * 64 * R + lane;  // this lane's row in the pass buffer
+  const int64_t stride = (int64_t)gridDim.x * kWaves;
+  int64_t span = (int64_t)blockIdx.x * kWaves + wave;
+  // wave_prefix[slot] of the span's first slot (a span holds 64 / GW whole slots)
+  auto span_words = [&](int64_t sp) -> uint64_t {
+    const int64_t wi = sp * 64 + lane;
+    return (sp < n_spans && wi < n_words) ? ws.keep[wi] : 0ull;
+  };
+  auto span_base = [&](int64_t sp) -> uint64_t {
+    return sp < n_spans ? (uint64_t)ws.wave_prefix[sp * 64 / GW] : 0ull;
+  };
+  uint64_t myw = span_words(span);
+  uint64_t obase = span_base(span);
+  for (; span < n_spans; span += stride) {
+    const uint64_t myw_next = span_words(span + stride);
+    const uint64_t obase_next = span_base(span + stride);
+    const uint32_t c = (uint32_t)__builtin_popcountll(myw);
+    uint32_t inc = c;
+    for (int off = 1; off < 64; off <<= 1) {
+      const uint32_t y = __shfl_up(inc, off);
+      if (lane >= off) inc += y;
     }
-    run_vm<R, NT, SLOW>(ops + n_ops_p, n_ops_q, acc, tmp, in, out, tb, lrow, row0 + lrow, n, (int64_t)obase,
-                        rel, keep);
-    if (any_valid_out) {
-      // Validity words of the wave's groups straight from the input bitmaps.
-      const int64_t gw = ((row0 + lb) >> 6) + (lane < R ? lane : 0);
-      for (int c = 0; c < n_in; c++) {
-        const int vs = in[c].vslot;
-        if (vs >= 0) vrf[vs] = load_vword(in[c].valid, gw, n);
-      }
-      run_vops<VS>(vops + n_vops_p, n_vops_q, vrf);
-      // The wave's kept rows are output positions [obase, obase + wcnt): their bits
-      // are gathered in this wave's LDS words, then stored; the two end words may
-      // be shared with the neighbouring wave slots and are merged atomically.
-      constexpr int VW = R + 2;
-      LDSAS uint64_t *vbits = (LDSAS uint64_t *)(lds + h.lds_vbits_off) + wave * VW;
-      const int nw = (int)(((obase & 63) + wcnt + 63) >> 6);
-      const uint64_t end = obase + wcnt;
-      const int64_t w0 = (int64_t)(obase >> 6);
-      for (int o = 0; o < n_out; o++) {
-        void *vp = out[o].valid;
-        if (vp == nullptr) continue;
-        if (lane < VW) vbits[lane] = 0;
+    const uint32_t cnt = __shfl(inc, 63);
+    const uint32_t mypre = inc - c;  // kept rows of the span before word `lane`
+    if (cnt > 0) {
+      const int64_t srow0 = span * 4096;  // first row of the span
+      const int lo = (int)(obase & 63);
+      if (n_vout > 0) {
+        for (int i = lane; i < VW * n_vout; i += 64) vbits[i] = 0;
         __builtin_amdgcn_wave_barrier();
-        const uint64_t vword = vsrc_word<VS>(out[o].vsrc, vrf);
-        unsigned long long nl = 0;
+      }
+      unsigned long long nl[4] = {0, 0, 0, 0};
+      for (int g = 0; g * 64 * R < (int)cnt; g++) {
+        int orel[R];
+        bool live[R];
+        int64_t grow[R];
 #pragma unroll
         for (int u = 0; u < R; u++) {
-          if (keep[u]) {
-            if ((readlane64(vword, u) >> lane) & 1) {
-              const uint64_t local = (obase & 63) + (uint64_t)rel[u];
-              __hip_atomic_fetch_or(vbits + (local >> 6), 1ull << (local & 63), __ATOMIC_RELAXED,
-                                    __HIP_MEMORY_SCOPE_WORKGROUP);
-            } else {
-              nl++;
+          const int j = g * 64 * R + 64 * u + lane;
+          live[u] = j < (int)cnt;
+          // word holding the j-th kept row: largest w with pre[w] <= j
+          int w = 0;
+#pragma unroll
+          for (int s = 32; s > 0; s >>= 1) {
+            const uint32_t pc = __shfl(mypre, w + s);
+            if ((int)pc <= j) w += s;
+          }
+          const uint64_t m = (uint64_t)__shfl((long long)myw, w);
+          const int pw = (int)__shfl(mypre, w);
+          const int src = live[u] ? w * 64 + select_bit(m, j - pw) : 0;
+          orel[u] = j;
+          grow[u] = srow0 + src;
+          acc[u] = 0;
+        }
+        stage_kept<R>(gcols, n_gcols, gbuf, plrow, grow, live);
+        run_vm<R, SLOW>(ops + n_ops_p, n_ops_q, acc, tmpb, tstride, in, out, gbuf, plrow, grow, n,
+                        (int64_t)obase, orel, live);
+        if (any_valid_out) {
+          // Validity of the 64 outputs of group u as one word: lane u holds it.
+#pragma unroll
+          for (int u = 0; u < R; u++) {
+            for (int ci = 0; ci < n_in; ci++) {
+              const int vs = in[ci].vslot;
+              if (vs < 0) continue;
+              const uint64_t word = __builtin_amdgcn_ballot_w64(live[u] && valid_bit(in[ci].valid, grow[u]));
+              if (lane == u) vrf[vs] = word;
+            }
+          }
+          run_vops<VS>(vops + n_vops_p, n_vops_q, vrf);
+          for (int o = 0; o < n_out; o++) {
+            if (out[o].valid == nullptr) continue;
+            const uint64_t vword = vsrc_word<VS>(out[o].vsrc, vrf);
+            const int vb = out[o].vbuf;
+#pragma unroll
+            for (int u = 0; u < R; u++) {
+              const int j0 = g * 64 * R + 64 * u;
+              if (j0 >= (int)cnt) break;  // wave-uniform
+              const uint64_t lm = low_mask((int64_t)cnt - j0);
+              const uint64_t bits = readlane64(vword, u) & lm;
+              if (o < 4) nl[o] += __builtin_popcountll(~bits & lm);
+              else if (lane == 0) atomicAdd(ws.null_counts + o, (unsigned long long)__builtin_popcountll(~bits & lm));
+              const int L = lo + j0;  // span-local output position of the group's first row
+              if (lane == 0 && bits) {
+                LDSAS uint64_t *wp = vbits + vb * VW + (L >> 6);
+                wp[0] |= bits << (L & 63);
+                if (L & 63) wp[1] |= bits >> (64 - (L & 63));
+              }
             }
           }
         }
-        __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0): the wave's LDS atomics are done
+      }
+      if (n_vout > 0) {
+        // The span's outputs are [obase, obase + cnt); its first and last words may
+        // be shared with the neighbouring spans and are merged atomically.
+        __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0)
         __builtin_amdgcn_wave_barrier();
-        if (lane < nw) {
-          unsigned long long *gp = (unsigned long long *)((uint8_t *)vp + 8 * (w0 + lane));
-          const uint64_t word = vbits[lane];
-          const bool shared = (lane == 0 && (obase & 63)) || (lane == nw - 1 && (end & 63));
-          if (shared) atomicOr(gp, (unsigned long long)word);
-          else gstore<uint64_t>(gp, 0, word);
+        const int nw = (lo + (int)cnt + 63) >> 6;
+        const int64_t w0 = (int64_t)(obase >> 6);
+        const int end = lo + (int)cnt;
+        for (int o = 0; o < n_out; o++) {
+          void *vp = out[o].valid;
+          if (vp == nullptr) continue;
+          const int vb = out[o].vbuf;
+          for (int i = lane; i < nw; i += 64) {
+            unsigned long long *gp = (unsigned long long *)((uint8_t *)vp + 8 * (w0 + i));
+            const uint64_t word = vbits[vb * VW + i];
+            const bool shared = (i == 0 && lo) || (i == nw - 1 && (end & 63));
+            if (shared) atomicOr(gp, (unsigned long long)word);
+            else gstore<uint64_t>(gp, 0, word);
+          }
         }
-        for (int s = 32; s > 0; s >>= 1) nl += __shfl_xor(nl, s);
-        if (lane == 0 && nl) atomicAdd(ws.null_counts + o, nl);
         __builtin_amdgcn_wave_barrier();
       }
+      for (int o = 0; o < n_out && o < 4; o++)
+        if (lane == 0 && nl[o]) atomicAdd(ws.null_counts + o, nl[o]);
     }
+    myw = myw_next;
+    obase = obase_next;
   }
 }
 
@@ -738,66 +879,59 @@ slot_scan_kernel(const uint32_t *__restrict__ counts, uint32_t *__restrict__ pre
 // ---------------------------------------------------------------------------
 // launchers
 
-#define BLZ_VM_SHAPES(X) X(8, 2) X(8, 4) X(4, 2) X(4, 4) X(4, 8) X(2, 4) X(2, 8)
+#define BLZ_VM_SHAPES(X) X(8) X(4) X(2)
 
-template <int RR, int NN, int MODE>
+template <int RR, int MODE>
 hipError_t launch_vm_mode(bool slow, int grid, int lds_bytes, hipStream_t st, const uint8_t *blob, Workspace ws) {
-  if (slow) hipLaunchKernelGGL((vm_kernel<RR, NN, MODE, true>), dim3(grid), dim3(kBlock), lds_bytes, st, blob, ws);
-  else hipLaunchKernelGGL((vm_kernel<RR, NN, MODE, false>), dim3(grid), dim3(kBlock), lds_bytes, st, blob, ws);
+  if (slow) hipLaunchKernelGGL((vm_kernel<RR, MODE, true>), dim3(grid), dim3(kBlock), lds_bytes, st, blob, ws);
+  else hipLaunchKernelGGL((vm_kernel<RR, MODE, false>), dim3(grid), dim3(kBlock), lds_bytes, st, blob, ws);
   return hipGetLastError();
 }
 
-template <int RR, int NN, int MODE>
+template <int RR, int MODE>
 int vm_occupancy_mode(bool slow, int lds_bytes) {
   int b = 0;
-  if (slow) (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, vm_kernel<RR, NN, MODE, true>, kBlock, lds_bytes);
-  else (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, vm_kernel<RR, NN, MODE, false>, kBlock, lds_bytes);
+  if (slow) (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, vm_kernel<RR, MODE, true>, kBlock, lds_bytes);
+  else (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, vm_kernel<RR, MODE, false>, kBlock, lds_bytes);
   return b;
 }
 
-template <int RR, int NN>
+template <int RR>
 hipError_t launch_vm_shape(int mode, bool slow, int grid, int lds_bytes, hipStream_t st, const uint8_t *blob,
                            Workspace ws) {
   switch (mode) {
-    case VM_PROJECT: return launch_vm_mode<RR, NN, VM_PROJECT>(slow, grid, lds_bytes, st, blob, ws);
-    case VM_PRED: return launch_vm_mode<RR, NN, VM_PRED>(slow, grid, lds_bytes, st, blob, ws);
-    case VM_GATHER: return launch_vm_mode<RR, NN, VM_GATHER>(slow, grid, lds_bytes, st, blob, ws);
+    case VM_PROJECT: return launch_vm_mode<RR, VM_PROJECT>(slow, grid, lds_bytes, st, blob, ws);
+    case VM_PRED: return launch_vm_mode<RR, VM_PRED>(slow, grid, lds_bytes, st, blob, ws);
+    case VM_GATHER: return launch_vm_mode<RR, VM_GATHER>(slow, grid, lds_bytes, st, blob, ws);
     default: return hipErrorInvalidValue;
   }
 }
 
-template <int RR, int NN>
+template <int RR>
 int vm_occupancy(int mode, bool slow, int lds_bytes) {
   switch (mode) {
-    case VM_PROJECT: return vm_occupancy_mode<RR, NN, VM_PROJECT>(slow, lds_bytes);
-    case VM_PRED: return vm_occupancy_mode<RR, NN, VM_PRED>(slow, lds_bytes);
-    case VM_GATHER: return vm_occupancy_mode<RR, NN, VM_GATHER>(slow, lds_bytes);
+    case VM_PROJECT: return vm_occupancy_mode<RR, VM_PROJECT>(slow, lds_bytes);
+    case VM_PRED: return vm_occupancy_mode<RR, VM_PRED>(slow, lds_bytes);
+    case VM_GATHER: return vm_occupancy_mode<RR, VM_GATHER>(slow, lds_bytes);
     default: return 0;
   }
 }
 
-hipError_t launch_vm(int r, int nt, int mode, bool slow, int grid, int lds_bytes, hipStream_t st,
-                     const uint8_t *blob, Workspace ws) {
-#define BLZ_VM_LAUNCH(RR, NN) \
-  if (r == RR && nt == NN) return launch_vm_shape<RR, NN>(mode, slow, grid, lds_bytes, st, blob, ws);
+hipError_t launch_vm(int r, int mode, bool slow, int grid, int lds_bytes, hipStream_t st, const uint8_t *blob,
+                     Workspace ws) {
+#define BLZ_VM_LAUNCH(RR) \
+  if (r == RR) return launch_vm_shape<RR>(mode, slow, grid, lds_bytes, st, blob, ws);
   BLZ_VM_SHAPES(BLZ_VM_LAUNCH)
 #undef BLZ_VM_LAUNCH
   return hipErrorInvalidValue;
 }
 
-int vm_blocks_per_cu(int r, int nt, int mode, bool slow, int lds_bytes) {
-#define BLZ_VM_OCC(RR, NN) \
-  if (r == RR && nt == NN) return vm_occupancy<RR, NN>(mode, slow, lds_bytes);
+int vm_blocks_per_cu(int r, int mode, bool slow, int lds_bytes) {
+#define BLZ_VM_OCC(RR) \
+  if (r == RR) return vm_occupancy<RR>(mode, slow, lds_bytes);
   BLZ_VM_SHAPES(BLZ_VM_OCC)
 #undef BLZ_VM_OCC
   return 0;
-}
-
-bool vm_shape_ok(int r, int nt) {
-#define BLZ_VM_OK(RR, NN) if (r == RR && nt == NN) return true;
-  BLZ_VM_SHAPES(BLZ_VM_OK)
-#undef BLZ_VM_OK
-  return false;
 }
 
 hipError_t launch_slot_scan(const uint32_t *counts, uint32_t *prefix, int64_t n_slots, hipStream_t st,

@@ -276,18 +276,18 @@ def test_kernel_kinds_c1(B, kernel):
     prog.close()
 
 
-VM_SHAPES = [(8, 2), (8, 4), (4, 2), (4, 4), (4, 8), (2, 4), (2, 8)]
+VM_SHAPES = [(8, 1), (8, 2), (4, 1), (4, 4), (2, 1), (2, 8)]
 
 
-@pytest.mark.parametrize("shape", VM_SHAPES, ids=[f"r{r}nt{t}" for r, t in VM_SHAPES])
+@pytest.mark.parametrize("shape", VM_SHAPES, ids=[f"r{r}p{p}" for r, p in VM_SHAPES])
 @pytest.mark.parametrize("mode", ["filter", "project"])
 def test_vm_shapes(B, shape, mode):
     """Every tile-VM shape, filter+project and pure projection, with nulls, on row
     counts below, at and just past whole tiles."""
-    r, nt = shape
-    T = 256 * r
+    r, p = shape
+    T = 256 * r * p
     for seed, n in enumerate([1, T - 1, T, 3 * T + 17, 40 * T + 5]):
-        rng = np.random.default_rng(7100 + 17 * seed + r * 3 + nt)
+        rng = np.random.default_rng(7100 + 17 * seed + r * 3 + p)
         k = int(rng.integers(2, 6))
         dts = exprgen.rand_frame_dtypes(rng, k)
         null_rates = [float(rng.choice([0.0, 0.1])) for _ in range(k)]
@@ -300,7 +300,7 @@ def test_vm_shapes(B, shape, mode):
         if mode == "filter":
             cond = f"<({exprgen.rand_expr(rng, dts, 2, False, nullable)}, {int(rng.integers(-5, 6))})"
             filt = f"LogicalFilter(condition=[{cond}])"
-        _print_case(test="vm_shapes", r=r, nt=nt, n=n, filt=filt, proj=proj)
+        _print_case(test="vm_shapes", r=r, p=p, n=n, filt=filt, proj=proj)
         try:
             mid = O.process_filter(host, filt) if filt else host
             ref = O.execute_project(mid, proj)
@@ -308,17 +308,17 @@ def test_vm_shapes(B, shape, mode):
             continue
         dev = [to_device(B, h) for h in host]
         try:
-            prog = B.Program(dev, filt, proj, kernel=4, s=nt, r=r)
+            prog = B.Program(dev, filt, proj, kernel=4, r=r, t=T)
         except B.BlzError as e:
             if "fit" in str(e):
                 continue
             raise
-        assert prog.kernel == 4 and prog.r == r and prog.s == nt and prog.tile_rows == T
+        assert prog.kernel == 4 and prog.r == r and prog.tile_rows == T
         for _ in range(2):  # a second launch must give the same result
             prog.launch()
             kept, nulls = prog.result()
             assert kept == len(ref[0])
             for g, rc, nc in zip(prog.outputs, ref, nulls):
                 g.col.null_count = nc
-                assert_column_equal(g, rc, what=f"vm r={r} nt={nt} n={n}: {filt} {proj}")
+                assert_column_equal(g, rc, what=f"vm r={r} T={T} n={n}: {filt} {proj}")
         prog.close()
