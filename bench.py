@@ -4,14 +4,20 @@ Workload (the metric's "filter+project at 100M rows"): 100,000,000 rows of
 a int32 U[1,50], b float64 U[900,105000], c float64 in {0.00,...,0.10},
 d int32 U[0,2555] (no nulls), filter
   AND(>=($3, 730), <($3, 1095), >=($2, 0.03), <=($2, 0.05), <($0, 30))
-and project  b*(1-c), a+1  over the rows kept. One step = one fused
-filter+project launch over the device-resident table (plan compiled once).
+and project  b*(1-c), a+1  over the rows kept. One step = one execution of the
+compiled filter+project program over the device-resident table: the predicate
+kernel (keep words + per-slot counts), the slot scan, and the gather kernel that
+evaluates the projection on the kept rows.
 
 value = algorithmic bytes per step / wall time per step (GB/s), where algorithmic
 bytes = 24 B per input row (a, b, c, d read once) + 12 B per kept row (f64 + i32
-written). roofline.achieved uses the same bytes over the kernel's own duration
-(HIP events on the launch stream). cpu_baseline times the scalar oracle on a
-10M-row sample of the same workload on this host.
+written). roofline is for the dominant kernel (the predicate kernel): its
+algorithmic bytes (predicate columns a, c, d read once: 16 B/row, keep words
+written: 1/8 B/row, slot counts: 16 B per 1024-row tile) over its own duration
+(HIP events on the launch stream); "pipeline" gives the same figure for all three
+kernels together. traffic is the dominant kernel's HBM bytes per launch from the
+rocprofv3 FETCH_SIZE / WRITE_SIZE summary in profiles/. cpu_baseline times the
+scalar oracle on a 10M-row sample of the same workload on this host.
 
 python bench.py [--gpus 1] [--steps K] [--warmup W] [--rows N] [--calibrate]
 """
@@ -33,7 +39,8 @@ HBM_PEAK_GBS = 8000.0
 FILTER = ("LogicalFilter(condition=[AND(>=($3, 730), <($3, 1095), >=($2, 0.03), "
           "<=($2, 0.05), <($0, 30))])")
 PROJECT = "LogicalProject(EXPR$0=[*($1, -(1, $2))], EXPR$1=[+($0, 1)])"
-PROFILE = os.path.join(ROOT, "profiles", "interp_kernel_c1_filter_project_100M.txt")
+PROFILE = os.path.join(ROOT, "profiles", "vm_kernel_c1_filter_project_100M.txt")
+STAGES = ["predicate", "slot_scan", "gather"]
 
 
 def gen_c1(n, seed):
@@ -49,12 +56,24 @@ def algorithmic_bytes(n, kept):
     return 24 * n + 12 * kept
 
 
-def traffic_from_profile():
-    """HBM bytes per launch measured with rocprofv3 counters (see the profile file)."""
+def stage_bytes(n, kept, tile_rows):
+    """Algorithmic bytes of each kernel of the filter+project program."""
+    tiles = (n + tile_rows - 1) // tile_rows
+    pred = 16 * n + n / 8 + 16 * tiles          # a, c, d read; keep words + slot counts written
+    scan = 8 * 4 * tiles                         # slot counts read, prefixes written
+    gather = n / 8 + 20 * kept + 12 * kept      # keep words; a, b, c of kept rows; outputs
+    return [pred, scan, gather]
+
+
+def traffic_from_profile(kernel):
+    """HBM bytes per launch of `kernel` measured with rocprofv3 counters (profile file)."""
     if not os.path.exists(PROFILE):
         return None
-    m = re.search(r"^traffic_bytes_per_launch:\s*([0-9.eE+]+)", open(PROFILE).read(), re.M)
-    return float(m.group(1)) if m else None
+    for line in open(PROFILE):
+        m = re.match(r"^traffic_bytes_per_launch\[(.+)\]:\s*([0-9.eE+]+)", line.strip())
+        if m and m.group(1) in kernel:
+            return float(m.group(2))
+    return None
 
 
 def cpu_baseline(sample_rows=10_000_000, min_seconds=10.0):
@@ -101,7 +120,7 @@ def main():
     frame = [B.DeviceColumn.from_numpy(x) for x in (a, b, c, d)]
     del a, b, c, d
     prog = B.Program(frame, FILTER, PROJECT)
-    print(f"[bench] program S={prog.s} R={prog.r} ops={prog.n_ops} slots={prog.n_slots} "
+    print(f"[bench] program kernel={prog.kernel} R={prog.r} tile={prog.tile_rows} ops={prog.n_ops} "
           f"grid={prog.grid}", file=sys.stderr, flush=True)
     for _ in range(args.warmup):
         prog.launch()
@@ -112,9 +131,15 @@ def main():
     kept, _ = prog.result()  # synchronises the stream
     dt = (time.perf_counter() - t0) / args.steps
     print(f"[bench] {args.steps} steps, {dt * 1e3:.3f} ms/step, kept {kept}", file=sys.stderr, flush=True)
-    kernel_ms = prog.time(iters=max(10, args.steps // 2))
+    stage_ms = prog.time_stages(iters=max(10, args.steps // 2))
+    names = STAGES if len(stage_ms) == 3 else ["program"]
+    sbytes = stage_bytes(n, kept, prog.tile_rows) if len(stage_ms) == 3 else [algorithmic_bytes(n, kept)]
+    dom = int(np.argmax(stage_ms))
+    kernel_name = (f"vm_kernel<{prog.r}, {1 if dom == 0 else 2}, false>" if names[dom] != "slot_scan"
+                   else "slot_scan_kernel") if len(stage_ms) == 3 else f"kernel {prog.kernel}"
+    achieved = sbytes[dom] / (stage_ms[dom] * 1e-3) / 1e9
     bytes_step = algorithmic_bytes(n, kept)
-    achieved = bytes_step / (kernel_ms * 1e-3) / 1e9
+    total_ms = float(sum(stage_ms))
     if args.calibrate:
         buf_a = B.DeviceBuffer(1 << 30)
         buf_b = B.DeviceBuffer(1 << 30)
@@ -139,10 +164,14 @@ def main():
                                "predicate, project b*(1-c), a+1",
                    "rows": n, "rows_kept": int(kept), "selectivity": kept / n,
                    "algorithmic_bytes_per_step": bytes_step,
-                   "register_file": f"S={prog.s} R={prog.r}"},
+                   "program": f"tile VM R={prog.r} tile={prog.tile_rows} rows"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_from_profile(),
-                     "kernel_ms": kernel_ms},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_from_profile(kernel_name),
+                     "kernel": kernel_name, "stage": names[dom], "kernel_ms": stage_ms[dom],
+                     "algorithmic_bytes": sbytes[dom]},
+        "pipeline": {"stages": dict(zip(names, stage_ms)), "kernel_ms": total_ms,
+                     "achieved": bytes_step / (total_ms * 1e-3) / 1e9,
+                     "frac": bytes_step / (total_ms * 1e-3) / 1e9 / HBM_PEAK_GBS},
         "cpu_baseline": cpu,
     }
     prog.close()

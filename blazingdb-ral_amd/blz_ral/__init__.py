@@ -103,6 +103,7 @@ def lib():
             "blz_program_launch": [P, P, ERR, SZ],
             "blz_program_result": [P, P, C.POINTER(C.c_int64), I32P, ERR, SZ],
             "blz_program_time": [P, P, I, C.POINTER(C.c_double), ERR, SZ],
+            "blz_program_time_stages": [P, P, I, C.POINTER(C.c_double), C.POINTER(C.c_int), ERR, SZ],
             "blz_program_info": [P] + [C.POINTER(C.c_int)] * 6,
             "blz_program_destroy": [P],
             "blz_device_malloc": [C.POINTER(P), SZ],
@@ -436,6 +437,14 @@ class Program:
         ms = C.c_double(0)
         _call(lib().blz_program_time, self.h, stream, iters, C.byref(ms))
         return ms.value
+
+    def time_stages(self, iters=10, stream=None):
+        """Mean milliseconds of each kernel launch of the program (HIP events on the
+        launch stream): [predicate, slot scan, gather] for a tile-VM filter."""
+        st = (C.c_double * 3)()
+        ns = C.c_int(0)
+        _call(lib().blz_program_time_stages, self.h, stream, iters, st, C.byref(ns))
+        return [st[i] for i in range(ns.value)]
 
     def close(self):
         if self.h.value:

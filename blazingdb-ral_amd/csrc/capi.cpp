@@ -168,23 +168,38 @@ void bind(LaunchBuffers &lb, int n_out) {
   lb.ws.wave_prefix = (uint32_t *)(p + lb.prefix_off);
 }
 
+// Kernel launches of one execution of a program: 3 for a tile-VM filter
+// (predicate, slot scan, gather), 1 otherwise.
+int program_stages(const Compiled &c, bool compact) { return c.vm && compact ? 3 : 1; }
+
+// Enqueues the program's kernels. evs (optional, program_stages + 1 events) are
+// recorded before each kernel and after the last.
 hipError_t launch_program(const Compiled &c, bool compact, int grid, hipStream_t st, const uint8_t *blob,
-                          const Workspace &ws) {
+                          const Workspace &ws, hipEvent_t *evs = nullptr) {
+  auto mark = [&](int i) { return evs ? hipEventRecord(evs[i], st) : hipSuccess; };
+  hipError_t e = mark(0);
+  if (e != hipSuccess) return e;
   if (c.vm) {
     const int64_t tiles = (c.n_rows + c.T - 1) / c.T;
     auto grid_for = [&](int mode, int lds) {
       const int64_t resident = (int64_t)device_cus() * vm_per_cu(c, mode, lds);
       return (int)std::max<int64_t>(1, std::min<int64_t>(tiles, resident));
     };
-    if (!compact) return launch_vm(c.R, 0, c.vm_slow, grid_for(0, c.lds_bytes), c.lds_bytes, st, blob, ws);
-    hipError_t e = launch_vm(c.R, 1, c.vm_slow, grid_for(1, c.lds_bytes), c.lds_bytes, st, blob, ws);
+    if (!compact) {
+      e = launch_vm(c.R, 0, c.vm_slow, grid_for(0, c.lds_bytes), c.lds_bytes, st, blob, ws);
+      return e == hipSuccess ? mark(1) : e;
+    }
+    e = launch_vm(c.R, 1, c.vm_slow, grid_for(1, c.lds_bytes), c.lds_bytes, st, blob, ws);
+    if (e == hipSuccess) e = mark(1);
     if (e == hipSuccess) e = launch_slot_scan(ws.wave_count, ws.wave_prefix, tiles * 4, st, ws);
+    if (e == hipSuccess) e = mark(2);
     if (e == hipSuccess)
       e = launch_vm(c.R, 2, c.vm_slow, grid_for(2, c.lds_gather), c.lds_gather, st, blob, ws);
-    return e;
+    return e == hipSuccess ? mark(3) : e;
   }
-  if (c.acc) return launch_acc(c.S, c.R, compact, grid, c.lds_bytes, st, blob, ws);
-  return launch_interp(c.S, c.R, compact, grid, c.lds_bytes, st, blob, ws);
+  if (c.acc) e = launch_acc(c.S, c.R, compact, grid, c.lds_bytes, st, blob, ws);
+  else e = launch_interp(c.S, c.R, compact, grid, c.lds_bytes, st, blob, ws);
+  return e == hipSuccess ? mark(1) : e;
 }
 
 struct RunResult {
@@ -634,6 +649,7 @@ struct blz_program_s {
   LaunchBuffers lb;
   std::vector<gdf_column> outs;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  hipEvent_t evs[4] = {nullptr, nullptr, nullptr, nullptr};
 };
 
 int blz_program_create(blz_program *prog, gdf_column *const *frame, int n_frame, const char *filter_qp,
@@ -700,6 +716,7 @@ int blz_program_create(blz_program *prog, gdf_column *const *frame, int n_frame,
       HIPCHECK(hipMemcpy(p->lb.blob, p->c.blob.data(), p->c.blob.size(), hipMemcpyHostToDevice));
       HIPCHECK(hipEventCreate(&p->ev0));
       HIPCHECK(hipEventCreate(&p->ev1));
+      for (auto &e : p->evs) HIPCHECK(hipEventCreate(&e));
       HIPCHECK(hipDeviceSynchronize());
       for (size_t i = 0; i < p->outs.size(); i++) out_columns[i] = p->outs[i];
       if (n_out) *n_out = (int)p->outs.size();
@@ -756,11 +773,12 @@ int blz_program_result(blz_program prog, blz_stream_t stream, int64_t *n_kept, i
   });
 }
 
-int blz_program_time(blz_program prog, blz_stream_t stream, int iters, double *mean_ms, char *err,
-                     size_t err_len) {
+int blz_program_time_stages(blz_program prog, blz_stream_t stream, int iters, double *stage_ms, int *n_stages,
+                            char *err, size_t err_len) {
   return guarded(err, err_len, [&] {
-    if (!prog || iters <= 0) invalid("bad arguments");
-    double total = 0;
+    if (!prog || iters <= 0 || !stage_ms) invalid("bad arguments");
+    const int ns = program_stages(prog->c, prog->compact);
+    std::vector<double> total(ns, 0.0);
     for (int i = 0; i < iters; i++) {
       // the zero-fill of counters is part of a launch but not of the kernel time
       HIPCHECK(launch_fill_zero(prog->lb.base + prog->lb.zero_off, (int64_t)prog->lb.zero_bytes,
@@ -771,16 +789,27 @@ int blz_program_time(blz_program prog, blz_stream_t stream, int iters, double *m
           HIPCHECK(launch_fill_zero(prog->outs[o].valid, (int64_t)vb, (int)std::min<size_t>(1024, vb / 2048 + 1),
                                     S(stream)));
         }
-      HIPCHECK(hipEventRecord(prog->ev0, S(stream)));
-      HIPCHECK(launch_program(prog->c, prog->compact, prog->grid, S(stream), prog->lb.blob, prog->lb.ws));
-      HIPCHECK(hipEventRecord(prog->ev1, S(stream)));
-      HIPCHECK(hipEventSynchronize(prog->ev1));
-      float ms = 0;
-      HIPCHECK(hipEventElapsedTime(&ms, prog->ev0, prog->ev1));
-      total += ms;
+      HIPCHECK(launch_program(prog->c, prog->compact, prog->grid, S(stream), prog->lb.blob, prog->lb.ws,
+                              prog->evs));
+      HIPCHECK(hipEventSynchronize(prog->evs[ns]));
+      for (int k = 0; k < ns; k++) {
+        float ms = 0;
+        HIPCHECK(hipEventElapsedTime(&ms, prog->evs[k], prog->evs[k + 1]));
+        total[k] += ms;
+      }
     }
-    *mean_ms = total / iters;
+    for (int k = 0; k < ns; k++) stage_ms[k] = total[k] / iters;
+    if (n_stages) *n_stages = ns;
   });
+}
+
+int blz_program_time(blz_program prog, blz_stream_t stream, int iters, double *mean_ms, char *err,
+                     size_t err_len) {
+  double st[3] = {0, 0, 0};
+  int ns = 0;
+  const int rc = blz_program_time_stages(prog, stream, iters, st, &ns, err, err_len);
+  if (rc == BLZ_OK && mean_ms) *mean_ms = st[0] + st[1] + st[2];
+  return rc;
 }
 
 int blz_program_info(blz_program prog, int *s, int *r, int *n_ops, int *n_vops, int *n_slots, int *grid) {
@@ -809,6 +838,8 @@ int blz_program_destroy(blz_program prog) {
   if (prog->lb.base) (void)hipFree(prog->lb.base);
   if (prog->ev0) (void)hipEventDestroy(prog->ev0);
   if (prog->ev1) (void)hipEventDestroy(prog->ev1);
+  for (auto &e : prog->evs)
+    if (e) (void)hipEventDestroy(e);
   delete prog;
   return BLZ_OK;
 }

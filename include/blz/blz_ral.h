@@ -153,6 +153,10 @@ int blz_program_result(blz_program prog, blz_stream_t stream, int64_t *n_kept,
  * returns the mean kernel time in milliseconds. */
 int blz_program_time(blz_program prog, blz_stream_t stream, int iters, double *mean_ms,
                      char *err, size_t err_len);
+/* As blz_program_time, per kernel launch of the program: stage_ms[0..*n_stages)
+ * (3 for a tile-VM filter: predicate, slot scan, gather of the kept rows; else 1). */
+int blz_program_time_stages(blz_program prog, blz_stream_t stream, int iters, double *stage_ms,
+                            int *n_stages, char *err, size_t err_len);
 /* Register-file shape and sizes chosen for the program. */
 int blz_program_info(blz_program prog, int *s, int *r, int *n_ops, int *n_vops, int *n_slots,
                      int *grid);
