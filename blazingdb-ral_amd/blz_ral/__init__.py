@@ -68,6 +68,12 @@ class GdfColumn(C.Structure):
                 ("col_name", C.c_char_p)]
 
 
+class BlzQuery(C.Structure):
+    _fields_ = [("frame", C.POINTER(C.POINTER(GdfColumn))), ("n_frame", C.c_int),
+                ("filter_query_part", C.c_char_p), ("project_query_part", C.c_char_p),
+                ("scalar_mode", C.c_int)]
+
+
 _lib = None
 
 
@@ -106,6 +112,12 @@ def lib():
             "blz_program_time_stages": [P, P, I, C.POINTER(C.c_double), C.POINTER(C.c_int), ERR, SZ],
             "blz_program_info": [P] + [C.POINTER(C.c_int)] * 6,
             "blz_program_destroy": [P],
+            "blz_batch_create": [C.POINTER(P), C.POINTER(BlzQuery), I, I, I, I, ERR, SZ],
+            "blz_batch_launch": [P, P, ERR, SZ],
+            "blz_batch_result": [P, P, I, C.POINTER(C.c_int64), I32P, colp, I, C.POINTER(C.c_int), ERR, SZ],
+            "blz_batch_time": [P, P, I, C.POINTER(C.c_double), ERR, SZ],
+            "blz_batch_info": [P] + [C.POINTER(C.c_int)] * 5,
+            "blz_batch_destroy": [P],
             "blz_device_malloc": [C.POINTER(P), SZ],
             "blz_device_free": [P],
             "blz_memcpy_htod": [P, P, SZ, P],
@@ -475,3 +487,57 @@ def plan_expression(frame_dtypes: Sequence[int], expression: str, cap=256):
                 binops=list(bops)[:k], unops=list(uops)[:k], lscalars=[ls[i] for i in range(k)],
                 rscalars=[rs[i] for i in range(k)], input_columns=list(in_cols)[:n_in.value],
                 final_position=final[0])
+
+
+class Batch:
+    """Many independent queries (frame, filter, projection) executed by one persistent
+    launch per kernel kind; r=0 lets the autotuner choose rows per lane and tile rows."""
+
+    def __init__(self, queries, r=0, t=0, use_graph=True, scalar_mode=SCALAR_INTENDED, max_out=64):
+        self.queries = [(list(f), fq, pq) for f, fq, pq in queries]
+        self.max_out = max_out
+        self._keep = []
+        arr = (BlzQuery * len(self.queries))()
+        for i, (frame, fq, pq) in enumerate(self.queries):
+            fpp = _colpp(frame)
+            self._keep.append(fpp)
+            arr[i].frame = C.cast(fpp, C.POINTER(C.POINTER(GdfColumn)))
+            arr[i].n_frame = len(frame)
+            arr[i].filter_query_part = fq.encode() if fq else None
+            arr[i].project_query_part = pq.encode() if pq else None
+            arr[i].scalar_mode = scalar_mode
+        self.h = C.c_void_p(0)
+        _call(lib().blz_batch_create, C.byref(self.h), arr, len(self.queries), r, t, 1 if use_graph else 0)
+        info = [C.c_int(0) for _ in range(5)]
+        lib().blz_batch_info(self.h, *[C.byref(x) for x in info])
+        self.r, self.t, self.n_queries, self.graph, self.tuned = [x.value for x in info]
+
+    def launch(self, stream=None):
+        _call(lib().blz_batch_launch, self.h, stream)
+
+    def result(self, q, stream=None):
+        """(rows kept, [DeviceColumn views of the outputs], [null counts])."""
+        kept = C.c_int64(0)
+        nulls = (C.c_int32 * self.max_out)()
+        out = (GdfColumn * self.max_out)()
+        n_out = C.c_int(0)
+        _call(lib().blz_batch_result, self.h, stream, q, C.byref(kept), nulls, out, self.max_out,
+              C.byref(n_out))
+        cols = [DeviceColumn.adopt(out[i], owned=False) for i in range(n_out.value)]
+        return kept.value, cols, [nulls[i] for i in range(n_out.value)]
+
+    def time(self, iters=5, stream=None):
+        ms = C.c_double(0)
+        _call(lib().blz_batch_time, self.h, stream, iters, C.byref(ms))
+        return ms.value
+
+    def close(self):
+        if self.h.value:
+            lib().blz_batch_destroy(self.h)
+            self.h = C.c_void_p(0)
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -8,6 +8,7 @@
  */
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -122,15 +123,18 @@ void check_column(const gdf_column *c, const char *what, int64_t n) {
 }
 
 // Device scratch of one launch:
-//   blob | null counts | counters | tile states  (zeroed before each launch)
-//   | keep words | slot counts | slot prefixes   (tile VM filter; written first)
+//   blob | VmBatch + VmPlanRef | null counts | counters | tile states  (zeroed part
+//   starts at the null counts) | keep words | slot counts | slot prefixes
 struct LaunchBuffers {
   uint8_t *base = nullptr;
   size_t bytes = 0;
+  size_t bd_off = 0;
   size_t zero_off = 0, zero_bytes = 0;
   size_t keep_off = 0, count_off = 0, prefix_off = 0;
   Workspace ws{};
   uint8_t *blob = nullptr;
+  VmBatch *bd = nullptr;  // device address of the launch descriptor
+  int64_t n_chunks = 0;   // slot-scan chunks
 };
 
 size_t round_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
@@ -138,14 +142,17 @@ size_t round_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 void layout(LaunchBuffers &lb, const Compiled &c, int n_out, bool compact, int64_t n) {
   const int64_t n_tiles = compact ? (n + tile_rows(c) - 1) / tile_rows(c) : 0;
   const size_t blob_sz = round_up(c.blob.size(), 256);
+  lb.bd_off = blob_sz;
+  const size_t bd_sz = round_up(sizeof(VmBatch) + sizeof(VmPlanRef), 256);
   const size_t nc = round_up(sizeof(unsigned long long) * std::max(n_out, 1), 64);
-  const size_t cnt = 64;  // tile counter + n_sel
+  const size_t cnt = 64;  // tile / chunk counter + n_sel
   const size_t ts = round_up(sizeof(unsigned long long) * std::max<int64_t>(n_tiles, 1), 64);
-  lb.zero_off = blob_sz;
+  lb.zero_off = blob_sz + bd_sz;
   lb.zero_bytes = nc + cnt + ts;
-  size_t off = blob_sz + lb.zero_bytes;
+  size_t off = lb.zero_off + lb.zero_bytes;
   const bool vmf = c.vm && compact;
   const int64_t slots = vmf ? n_tiles * 4 : 0;
+  lb.n_chunks = (slots + kScanChunk - 1) / kScanChunk;
   lb.keep_off = off;
   off += round_up(vmf ? sizeof(uint64_t) * n_tiles * (c.T / 64) : 0, 256);
   lb.count_off = off;
@@ -157,6 +164,7 @@ void layout(LaunchBuffers &lb, const Compiled &c, int n_out, bool compact, int64
 void bind(LaunchBuffers &lb, int n_out) {
   uint8_t *p = lb.base;
   lb.blob = p;
+  lb.bd = (VmBatch *)(p + lb.bd_off);
   uint8_t *z = p + lb.zero_off;
   lb.ws.null_counts = (unsigned long long *)z;
   const size_t nc = round_up(sizeof(unsigned long long) * std::max(n_out, 1), 64);
@@ -167,6 +175,35 @@ void bind(LaunchBuffers &lb, int n_out) {
   lb.ws.wave_count = (uint32_t *)(p + lb.count_off);
   lb.ws.wave_prefix = (uint32_t *)(p + lb.prefix_off);
 }
+// Host image of the blob and the single-program launch descriptor, copied to
+// lb.blob in one transfer.
+std::vector<uint8_t> device_image(const LaunchBuffers &lb, const Compiled &c) {
+  std::vector<uint8_t> img(lb.zero_off, 0);
+  std::memcpy(img.data(), c.blob.data(), c.blob.size());
+  if (c.vm) {
+    const int64_t tiles = (c.n_rows + c.T - 1) / c.T;
+    VmBatch b{};
+    b.n_plans = 1;
+    b.bufb = c.vm_bufb;
+    b.tmp_off = c.vm_tmp_off;
+    b.g_buf_off = c.vm_g_buf_off;
+    b.g_vb_off = c.vm_g_vb_off;
+    b.g_tmp_off = c.vm_g_tmp_off;
+    b.g_wave_bytes = c.vm_g_wave_bytes;
+    b.g_vb_wave_words = 66 * c.vm_nvout;
+    b.n_tiles = tiles;
+    b.n_spans = (tiles * (c.T / 64) + 63) / 64;
+    b.n_chunks = lb.n_chunks;
+    b.chunk_counter = lb.ws.tile_counter;
+    b.plans = (const VmPlanRef *)(lb.base + lb.bd_off + sizeof(VmBatch));
+    VmPlanRef r{};
+    r.blob = lb.blob;
+    r.ws = lb.ws;
+    std::memcpy(img.data() + lb.bd_off, &b, sizeof(b));
+    std::memcpy(img.data() + lb.bd_off + sizeof(VmBatch), &r, sizeof(r));
+  }
+  return img;
+}
 
 // Kernel launches of one execution of a program: 3 for a tile-VM filter
 // (predicate, slot scan, gather), 1 otherwise.
@@ -175,26 +212,27 @@ int program_stages(const Compiled &c, bool compact) { return c.vm && compact ? 3
 // Enqueues the program's kernels. evs (optional, program_stages + 1 events) are
 // recorded before each kernel and after the last.
 hipError_t launch_program(const Compiled &c, bool compact, int grid, hipStream_t st, const uint8_t *blob,
-                          const Workspace &ws, hipEvent_t *evs = nullptr) {
+                          const Workspace &ws, const VmBatch *bd, int64_t n_chunks, hipEvent_t *evs = nullptr) {
   auto mark = [&](int i) { return evs ? hipEventRecord(evs[i], st) : hipSuccess; };
   hipError_t e = mark(0);
   if (e != hipSuccess) return e;
   if (c.vm) {
     const int64_t tiles = (c.n_rows + c.T - 1) / c.T;
-    auto grid_for = [&](int mode, int lds) {
+    const int64_t spans = (tiles * (c.T / 64) + 63) / 64;
+    auto grid_for = [&](int mode, int lds, int64_t units) {
       const int64_t resident = (int64_t)device_cus() * vm_per_cu(c, mode, lds);
-      return (int)std::max<int64_t>(1, std::min<int64_t>(tiles, resident));
+      return (int)std::max<int64_t>(1, std::min<int64_t>(units, resident));
     };
     if (!compact) {
-      e = launch_vm(c.R, 0, c.vm_slow, grid_for(0, c.lds_bytes), c.lds_bytes, st, blob, ws);
+      e = launch_vm(c.R, 0, c.vm_slow, grid_for(0, c.lds_bytes, tiles), c.lds_bytes, st, bd);
       return e == hipSuccess ? mark(1) : e;
     }
-    e = launch_vm(c.R, 1, c.vm_slow, grid_for(1, c.lds_bytes), c.lds_bytes, st, blob, ws);
+    e = launch_vm(c.R, 1, c.vm_slow, grid_for(1, c.lds_bytes, tiles), c.lds_bytes, st, bd);
     if (e == hipSuccess) e = mark(1);
-    if (e == hipSuccess) e = launch_slot_scan(ws.wave_count, ws.wave_prefix, tiles * 4, st, ws);
+    if (e == hipSuccess) e = launch_slot_scan(bd, n_chunks, st);
     if (e == hipSuccess) e = mark(2);
     if (e == hipSuccess)
-      e = launch_vm(c.R, 2, c.vm_slow, grid_for(2, c.lds_gather), c.lds_gather, st, blob, ws);
+      e = launch_vm(c.R, 2, c.vm_slow, grid_for(2, c.lds_gather, (spans + 3) / 4), c.lds_gather, st, bd);
     return e == hipSuccess ? mark(3) : e;
   }
   if (c.acc) e = launch_acc(c.S, c.R, compact, grid, c.lds_bytes, st, blob, ws);
@@ -218,11 +256,12 @@ RunResult run_once(const Compiled &c, bool compact, int64_t n, int n_out, hipStr
   HIPCHECK(hipMallocAsync((void **)&lb.base, lb.bytes, st));
   bind(lb, n_out);
   std::vector<unsigned long long> host(n_out + 1, 0);
-  hipError_t e = hipMemcpyAsync(lb.blob, c.blob.data(), c.blob.size(), hipMemcpyHostToDevice, st);
+  const std::vector<uint8_t> img = device_image(lb, c);
+  hipError_t e = hipMemcpyAsync(lb.blob, img.data(), img.size(), hipMemcpyHostToDevice, st);
   if (e == hipSuccess) e = launch_fill_zero(lb.base + lb.zero_off, (int64_t)lb.zero_bytes,
                                             (int)std::min<size_t>(1024, lb.zero_bytes / 2048 + 1), st);
   if (e == hipSuccess)
-    e = launch_program(c, compact, interp_grid(c, compact, n), st, lb.blob, lb.ws);
+    e = launch_program(c, compact, interp_grid(c, compact, n), st, lb.blob, lb.ws, lb.bd, lb.n_chunks);
   if (e == hipSuccess && n_out > 0)
     e = hipMemcpyAsync(host.data(), lb.ws.null_counts, sizeof(unsigned long long) * n_out,
                        hipMemcpyDeviceToHost, st);
@@ -652,78 +691,91 @@ struct blz_program_s {
   hipEvent_t evs[4] = {nullptr, nullptr, nullptr, nullptr};
 };
 
+namespace {
+// Compiles and allocates one program (throws on error); blz_program_create and
+// the batch share it.
+blz_program make_program(gdf_column *const *frame, int n_frame, const char *filter_qp, const char *proj_qp,
+                         int scalar_mode, int kernel, int s, int r, int t) {
+  check_frame(frame, n_frame);
+  const int64_t n = frame[0]->size;
+  const auto dt = frame_dtypes(frame, n_frame);
+  InputMap im(n_frame);
+  planner::ExprPlan fp;
+  const bool has_filter = filter_qp != nullptr;
+  if (has_filter) {
+    fp = planner::plan_expression(dt, condition_of(filter_qp));
+    if (fp.bare) im.use(frame, fp.bare_col);
+    else im.remap(frame, fp.spec);
+  }
+  planner::ProjectPlan pp;
+  std::vector<planner::ProjOut> outs;
+  if (proj_qp) {
+    pp = planner::parse_project(dt, proj_qp);
+    if (pp.has_eval) im.remap(frame, pp.spec);
+    outs = pp.outputs;
+  } else {
+    for (int i = 0; i < n_frame; i++) outs.push_back({1, i, dt[i], ""});
+  }
+  auto *p = new blz_program_s();
+  try {
+    CompileRequest req;
+    req.n_rows = n;
+    req.scalar_mode = scalar_mode;
+    req.compact = has_filter;
+    req.force_s = s;
+    req.force_r = r;
+    req.force_t = t;
+    req.kernel = kernel;
+    for (size_t i = 0; i < outs.size(); i++) {
+      const auto &o = outs[i];
+      const bool with_valid = o.kind == 0 ? true : frame[o.index]->valid != nullptr;
+      gdf_column c = alloc_column(o.dtype, n, with_valid, nullptr);
+      p->outs.push_back(c);
+      if (o.kind == 0) req.outputs.push_back({c.data, c.valid, o.dtype, 0, o.index});
+      else req.outputs.push_back({c.data, c.valid, o.dtype, 1, im.use(frame, o.index)});
+    }
+    req.inputs = im.inputs;
+    if (has_filter) {
+      if (fp.bare) req.bare_stencil_input = im.frame_to_dev[fp.bare_col];
+      else req.pred = &fp.spec;
+    }
+    if (proj_qp && pp.has_eval) req.proj = &pp.spec;
+    if (req.inputs.empty()) invalid("nothing to evaluate");
+    p->c = compile_program(req);
+    p->compact = has_filter;
+    p->n = n;
+    p->n_out = (int)outs.size();
+    p->grid = interp_grid(p->c, has_filter, std::max<int64_t>(n, 1));
+    layout(p->lb, p->c, p->n_out, has_filter, n);
+    HIPCHECK(hipMalloc((void **)&p->lb.base, p->lb.bytes));
+    bind(p->lb, p->n_out);
+    const std::vector<uint8_t> img = device_image(p->lb, p->c);
+    HIPCHECK(hipMemcpy(p->lb.blob, img.data(), img.size(), hipMemcpyHostToDevice));
+    HIPCHECK(hipEventCreate(&p->ev0));
+    HIPCHECK(hipEventCreate(&p->ev1));
+    for (auto &e : p->evs) HIPCHECK(hipEventCreate(&e));
+    HIPCHECK(hipDeviceSynchronize());
+  } catch (...) {
+    blz_program_destroy(p);
+    throw;
+  }
+  return p;
+}
+}  // namespace
+
 int blz_program_create(blz_program *prog, gdf_column *const *frame, int n_frame, const char *filter_qp,
                        const char *proj_qp, gdf_column *out_columns, int n_out_cap, int *n_out,
                        int scalar_mode, int kernel, int s, int r, int t, char *err, size_t err_len) {
   return guarded(err, err_len, [&] {
     if (!prog) invalid("prog is NULL");
     *prog = nullptr;
-    check_frame(frame, n_frame);
-    const int64_t n = frame[0]->size;
-    const auto dt = frame_dtypes(frame, n_frame);
-    InputMap im(n_frame);
-    planner::ExprPlan fp;
-    const bool has_filter = filter_qp != nullptr;
-    if (has_filter) {
-      fp = planner::plan_expression(dt, condition_of(filter_qp));
-      if (fp.bare) im.use(frame, fp.bare_col);
-      else im.remap(frame, fp.spec);
-    }
-    planner::ProjectPlan pp;
-    std::vector<planner::ProjOut> outs;
-    if (proj_qp) {
-      pp = planner::parse_project(dt, proj_qp);
-      if (pp.has_eval) im.remap(frame, pp.spec);
-      outs = pp.outputs;
-    } else {
-      for (int i = 0; i < n_frame; i++) outs.push_back({1, i, dt[i], ""});
-    }
-    if ((int)outs.size() > n_out_cap) invalid("more outputs than out_columns can hold");
-    auto *p = new blz_program_s();
-    try {
-      CompileRequest req;
-      req.n_rows = n;
-      req.scalar_mode = scalar_mode;
-      req.compact = has_filter;
-      req.force_s = s;
-      req.force_r = r;
-      req.force_t = t;
-      req.kernel = kernel;
-      for (size_t i = 0; i < outs.size(); i++) {
-        const auto &o = outs[i];
-        const bool with_valid = o.kind == 0 ? true : frame[o.index]->valid != nullptr;
-        gdf_column c = alloc_column(o.dtype, n, with_valid, nullptr);
-        p->outs.push_back(c);
-        if (o.kind == 0) req.outputs.push_back({c.data, c.valid, o.dtype, 0, o.index});
-        else req.outputs.push_back({c.data, c.valid, o.dtype, 1, im.use(frame, o.index)});
-      }
-      req.inputs = im.inputs;
-      if (has_filter) {
-        if (fp.bare) req.bare_stencil_input = im.frame_to_dev[fp.bare_col];
-        else req.pred = &fp.spec;
-      }
-      if (proj_qp && pp.has_eval) req.proj = &pp.spec;
-      if (req.inputs.empty()) invalid("nothing to evaluate");
-      p->c = compile_program(req);
-      p->compact = has_filter;
-      p->n = n;
-      p->n_out = (int)outs.size();
-      p->grid = interp_grid(p->c, has_filter, std::max<int64_t>(n, 1));
-      const int64_t tiles = (n + tile_rows(p->c) - 1) / tile_rows(p->c);
-      layout(p->lb, p->c, p->n_out, has_filter, n);
-      HIPCHECK(hipMalloc((void **)&p->lb.base, p->lb.bytes));
-      bind(p->lb, p->n_out);
-      HIPCHECK(hipMemcpy(p->lb.blob, p->c.blob.data(), p->c.blob.size(), hipMemcpyHostToDevice));
-      HIPCHECK(hipEventCreate(&p->ev0));
-      HIPCHECK(hipEventCreate(&p->ev1));
-      for (auto &e : p->evs) HIPCHECK(hipEventCreate(&e));
-      HIPCHECK(hipDeviceSynchronize());
-      for (size_t i = 0; i < p->outs.size(); i++) out_columns[i] = p->outs[i];
-      if (n_out) *n_out = (int)p->outs.size();
-    } catch (...) {
+    blz_program p = make_program(frame, n_frame, filter_qp, proj_qp, scalar_mode, kernel, s, r, t);
+    if ((int)p->outs.size() > n_out_cap) {
       blz_program_destroy(p);
-      throw;
+      invalid("more outputs than out_columns can hold");
     }
+    for (size_t i = 0; i < p->outs.size(); i++) out_columns[i] = p->outs[i];
+    if (n_out) *n_out = (int)p->outs.size();
     *prog = p;
   });
 }
@@ -740,7 +792,7 @@ hipError_t program_enqueue(blz_program p, hipStream_t st) {
       e = launch_fill_zero(p->outs[i].valid, (int64_t)vb, (int)std::min<size_t>(1024, vb / 2048 + 1), st);
     }
   if (e == hipSuccess)
-    e = launch_program(p->c, p->compact, p->grid, st, p->lb.blob, p->lb.ws);
+    e = launch_program(p->c, p->compact, p->grid, st, p->lb.blob, p->lb.ws, p->lb.bd, p->lb.n_chunks);
   return e;
 }
 }  // namespace
@@ -790,7 +842,7 @@ int blz_program_time_stages(blz_program prog, blz_stream_t stream, int iters, do
                                     S(stream)));
         }
       HIPCHECK(launch_program(prog->c, prog->compact, prog->grid, S(stream), prog->lb.blob, prog->lb.ws,
-                              prog->evs));
+                              prog->lb.bd, prog->lb.n_chunks, prog->evs));
       HIPCHECK(hipEventSynchronize(prog->evs[ns]));
       for (int k = 0; k < ns; k++) {
         float ms = 0;
@@ -928,3 +980,326 @@ int blz_plan_expression(const int32_t *frame_dtypes_in, int n_frame, const char 
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Batches: many independent queries, one persistent launch per kernel kind.
+
+namespace {
+
+struct SubBatch {
+  std::vector<int> members;   // program indices
+  VmBatch *d_bd = nullptr;    // device descriptor (VmBatch + VmPlanRef[])
+  int lds = 0, lds_gather = 0;
+  bool slow = false;
+  int64_t n_tiles = 0, n_spans = 0, n_chunks = 0;
+  int grid = 1, grid_gather = 1;
+};
+
+}  // namespace
+
+struct blz_batch_s {
+  std::vector<blz_program> progs;
+  int r = 0, t = 0;
+  SubBatch proj, filt;
+  std::vector<int> others;           // queries the tile VM cannot run: launched one by one
+  uint8_t *d_mem = nullptr;          // descriptors, zero-range table, scan counter
+  void **d_range_ptrs = nullptr;
+  int64_t *d_range_bytes = nullptr;
+  int n_ranges = 0;
+  unsigned int *d_counter = nullptr;
+  bool use_graph = false;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  hipStream_t cap = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  int tuned = 0;
+};
+
+namespace {
+
+hipError_t batch_enqueue_direct(blz_batch b, hipStream_t st) {
+  hipError_t e = launch_fill_ranges(b->d_range_ptrs, b->d_range_bytes, b->n_ranges, 512, st);
+  for (size_t i = 0; e == hipSuccess && i < b->others.size(); i++) e = program_enqueue(b->progs[b->others[i]], st);
+  if (e == hipSuccess && !b->proj.members.empty())
+    e = launch_vm(b->r, 0, b->proj.slow, b->proj.grid, b->proj.lds, st, b->proj.d_bd);
+  if (e == hipSuccess && !b->filt.members.empty()) {
+    e = launch_vm(b->r, 1, b->filt.slow, b->filt.grid, b->filt.lds, st, b->filt.d_bd);
+    if (e == hipSuccess) e = launch_slot_scan(b->filt.d_bd, b->filt.n_chunks, st);
+    if (e == hipSuccess)
+      e = launch_vm(b->r, 2, b->filt.slow, b->filt.grid_gather, b->filt.lds_gather, st, b->filt.d_bd);
+  }
+  return e;
+}
+
+hipError_t batch_enqueue(blz_batch b, hipStream_t st) {
+  if (b->exec) return hipGraphLaunch(b->exec, st);
+  return batch_enqueue_direct(b, st);
+}
+
+void batch_free(blz_batch b) {
+  if (!b) return;
+  (void)hipDeviceSynchronize();
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  if (b->cap) (void)hipStreamDestroy(b->cap);
+  if (b->ev0) (void)hipEventDestroy(b->ev0);
+  if (b->ev1) (void)hipEventDestroy(b->ev1);
+  if (b->d_mem) (void)hipFree(b->d_mem);
+  for (auto p : b->progs) blz_program_destroy(p);
+  delete b;
+}
+
+// Builds one batch of fixed shape (r, t). Throws on error.
+blz_batch batch_build(const blz_query *qs, int n, int r, int t, bool use_graph) {
+  auto *b = new blz_batch_s();
+  try {
+    b->r = r;
+    b->t = t;
+    for (int i = 0; i < n; i++) {
+      const blz_query &q = qs[i];
+      blz_program p = nullptr;
+      try {
+        p = make_program(q.frame, q.n_frame, q.filter_query_part, q.project_query_part, q.scalar_mode, 4, 0, r, t);
+      } catch (const BlzError &e) {
+        throw;  // device or argument errors
+      } catch (const std::exception &e) {
+        if (std::string(e.what()).find("fit") == std::string::npos) throw;  // not a shape limit
+      }
+      if (p) {
+        b->progs.push_back(p);
+        (p->compact ? b->filt : b->proj).members.push_back(i);
+      } else {  // a plan outside the tile VM's limits: its own launch inside the batch
+        b->progs.push_back(make_program(q.frame, q.n_frame, q.filter_query_part, q.project_query_part,
+                                        q.scalar_mode, 0, 0, 0, 0));
+        b->others.push_back(i);
+      }
+    }
+    // Host images of the two descriptors and the zero-range table.
+    auto build_sub = [&](SubBatch &sb, std::vector<uint8_t> &img, size_t off) {
+      VmBatch bd{};
+      std::vector<VmPlanRef> refs;
+      int maxbuf = 0, maxtmp = 0, maxgw = 0, maxnv = 0, maxgtmp = 0;
+      int64_t tile0 = 0, span0 = 0, chunk0 = 0;
+      for (int i : sb.members) {
+        blz_program p = b->progs[i];
+        const Compiled &c = p->c;
+        VmPlanRef ref{};
+        ref.blob = p->lb.blob;
+        ref.ws = p->lb.ws;
+        ref.tile0 = tile0;
+        ref.span0 = span0;
+        ref.chunk0 = chunk0;
+        refs.push_back(ref);
+        const int64_t tiles = (c.n_rows + c.T - 1) / c.T;
+        tile0 += tiles;
+        span0 += (tiles * (c.T / 64) + 63) / 64;
+        chunk0 += (tiles * 4 + kScanChunk - 1) / kScanChunk;
+        maxbuf = std::max(maxbuf, c.vm_bufb);
+        maxtmp = std::max(maxtmp, c.lds_bytes - 2 * c.vm_bufb);
+        maxgw = std::max(maxgw, c.vm_g_wave_bytes);
+        maxnv = std::max(maxnv, c.vm_nvout);
+        maxgtmp = std::max(maxgtmp, c.lds_gather - c.vm_g_tmp_off);
+        sb.slow = sb.slow || c.vm_slow;
+      }
+      bd.n_plans = (int)refs.size();
+      bd.bufb = maxbuf;
+      bd.tmp_off = 2 * maxbuf;
+      bd.g_buf_off = 0;
+      bd.g_wave_bytes = maxgw;
+      bd.g_vb_wave_words = 66 * maxnv;
+      bd.g_vb_off = 4 * maxgw;
+      bd.g_tmp_off = bd.g_vb_off + 4 * 66 * 8 * maxnv;
+      bd.n_tiles = tile0;
+      bd.n_spans = span0;
+      bd.n_chunks = chunk0;
+      bd.chunk_counter = b->d_counter;
+      bd.plans = (const VmPlanRef *)(b->d_mem + off + sizeof(VmBatch));
+      sb.lds = 2 * maxbuf + maxtmp;
+      sb.lds_gather = bd.g_tmp_off + maxgtmp;
+      if (sb.lds > 64 * 1024 || sb.lds_gather > 64 * 1024) invalid("batch needs more than 64 KiB of LDS");
+      sb.n_tiles = tile0;
+      sb.n_spans = span0;
+      sb.n_chunks = chunk0;
+      sb.d_bd = (VmBatch *)(b->d_mem + off);
+      std::memcpy(img.data() + off, &bd, sizeof(bd));
+      std::memcpy(img.data() + off + sizeof(VmBatch), refs.data(), refs.size() * sizeof(VmPlanRef));
+      const int cus = device_cus();
+      if (!sb.members.empty()) {
+        const Compiled &c0 = b->progs[sb.members[0]]->c;
+        Compiled probe = c0;  // occupancy depends on (r, mode, slow, lds) only
+        probe.vm_slow = sb.slow;
+        const int mode = sb.members.empty() ? 0 : (b->progs[sb.members[0]]->compact ? 1 : 0);
+        sb.grid = (int)std::max<int64_t>(1, std::min<int64_t>(tile0, (int64_t)cus * vm_per_cu(probe, mode, sb.lds)));
+        sb.grid_gather = (int)std::max<int64_t>(
+            1, std::min<int64_t>((span0 + 3) / 4, (int64_t)cus * vm_per_cu(probe, 2, std::max(sb.lds_gather, 1))));
+      }
+    };
+    const size_t proj_sz = round_up(sizeof(VmBatch) + sizeof(VmPlanRef) * b->proj.members.size(), 256);
+    const size_t filt_sz = round_up(sizeof(VmBatch) + sizeof(VmPlanRef) * b->filt.members.size(), 256);
+    // zero ranges: each program's counters, the filtered outputs' validity, the scan counter
+    std::vector<void *> rp;
+    std::vector<int64_t> rb;
+    for (size_t i = 0; i < b->progs.size(); i++) {
+      blz_program p = b->progs[i];
+      if (std::find(b->others.begin(), b->others.end(), (int)i) != b->others.end()) continue;
+      rp.push_back(p->lb.base + p->lb.zero_off);
+      rb.push_back((int64_t)p->lb.zero_bytes);
+      if (p->compact)
+        for (auto &o : p->outs)
+          if (o.valid) {
+            rp.push_back(o.valid);
+            rb.push_back((int64_t)round_up(gdf_valid_allocation_size((gdf_size_type)p->n), 8));
+          }
+    }
+    const size_t ctr_off = proj_sz + filt_sz;
+    const size_t ptr_off = ctr_off + 256;
+    const size_t bytes_off = round_up(ptr_off + sizeof(void *) * (rp.size() + 1), 256);
+    const size_t total = round_up(bytes_off + sizeof(int64_t) * (rb.size() + 1), 256);
+    HIPCHECK(hipMalloc((void **)&b->d_mem, total));
+    b->d_counter = (unsigned int *)(b->d_mem + ctr_off);
+    rp.push_back(b->d_counter);
+    rb.push_back(8);
+    b->n_ranges = (int)rp.size();
+    b->d_range_ptrs = (void **)(b->d_mem + ptr_off);
+    b->d_range_bytes = (int64_t *)(b->d_mem + bytes_off);
+    std::vector<uint8_t> img(total, 0);
+    build_sub(b->proj, img, 0);
+    build_sub(b->filt, img, proj_sz);
+    std::memcpy(img.data() + ptr_off, rp.data(), sizeof(void *) * rp.size());
+    std::memcpy(img.data() + bytes_off, rb.data(), sizeof(int64_t) * rb.size());
+    HIPCHECK(hipMemcpy(b->d_mem, img.data(), total, hipMemcpyHostToDevice));
+    HIPCHECK(hipEventCreate(&b->ev0));
+    HIPCHECK(hipEventCreate(&b->ev1));
+    if (use_graph) {
+      // Capture the launch sequence once (kernels only: no memset nodes).
+      HIPCHECK(hipStreamCreateWithFlags(&b->cap, hipStreamNonBlocking));
+      HIPCHECK(hipStreamBeginCapture(b->cap, hipStreamCaptureModeThreadLocal));
+      const hipError_t e = batch_enqueue_direct(b, b->cap);
+      hipGraph_t g = nullptr;
+      const hipError_t e2 = hipStreamEndCapture(b->cap, &g);
+      HIPCHECK(e);
+      HIPCHECK(e2);
+      b->graph = g;
+      HIPCHECK(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+      b->use_graph = true;
+    }
+    HIPCHECK(hipDeviceSynchronize());
+  } catch (...) {
+    batch_free(b);
+    throw;
+  }
+  return b;
+}
+
+double batch_time_ms(blz_batch b, hipStream_t st, int iters) {
+  double total = 0;
+  for (int i = 0; i < iters; i++) {
+    HIPCHECK(hipEventRecord(b->ev0, st));
+    HIPCHECK(batch_enqueue(b, st));
+    HIPCHECK(hipEventRecord(b->ev1, st));
+    HIPCHECK(hipEventSynchronize(b->ev1));
+    float ms = 0;
+    HIPCHECK(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+    total += ms;
+  }
+  return total / iters;
+}
+
+}  // namespace
+
+int blz_batch_create(blz_batch *batch, const blz_query *queries, int n_queries, int r, int t, int use_graph,
+                     char *err, size_t err_len) {
+  return guarded(err, err_len, [&] {
+    if (!batch || !queries || n_queries <= 0) invalid("bad arguments");
+    *batch = nullptr;
+    if (r != 0) {
+      *batch = batch_build(queries, n_queries, r, t, use_graph != 0);
+      return;
+    }
+    // Autotuner: every (rows per lane, tile rows) candidate on this batch; the
+    // fastest of three timed executions after one warm-up wins.
+    static const int cand[][2] = {{4, 1024}, {4, 2048}, {8, 2048}, {2, 512}, {2, 1024}};
+    blz_batch best = nullptr;
+    double best_ms = 0;
+    int tried = 0;
+    for (auto &c : cand) {
+      blz_batch b = nullptr;
+      try {
+        b = batch_build(queries, n_queries, c[0], c[1], false);
+      } catch (const std::exception &) {
+        continue;  // a shape that does not fit some query
+      }
+      tried++;
+      HIPCHECK(batch_enqueue(b, nullptr));
+      const double ms = batch_time_ms(b, nullptr, 3);
+      if (!best || ms < best_ms) {
+        if (best) batch_free(best);
+        best = b;
+        best_ms = ms;
+      } else {
+        batch_free(b);
+      }
+    }
+    if (!best) invalid("no tile VM shape fits every query of the batch");
+    const int br = best->r, bt = best->t;
+    if (use_graph) {
+      batch_free(best);
+      best = batch_build(queries, n_queries, br, bt, true);
+    }
+    best->tuned = tried;
+    *batch = best;
+  });
+}
+
+int blz_batch_launch(blz_batch batch, blz_stream_t stream, char *err, size_t err_len) {
+  return guarded(err, err_len, [&] {
+    if (!batch) invalid("batch is NULL");
+    HIPCHECK(batch_enqueue(batch, S(stream)));
+  });
+}
+
+int blz_batch_result(blz_batch batch, blz_stream_t stream, int q, int64_t *n_kept, int32_t *null_counts,
+                     gdf_column *out_columns, int max_out, int *n_out, char *err, size_t err_len) {
+  return guarded(err, err_len, [&] {
+    if (!batch || q < 0 || q >= (int)batch->progs.size()) invalid("bad query index");
+    blz_program p = batch->progs[q];
+    if ((int)p->outs.size() > max_out) invalid("more outputs than out_columns can hold");
+    int64_t kept = 0;
+    std::vector<int32_t> nulls(std::max(p->n_out, 1), 0);
+    char e2[256];
+    const int rc = blz_program_result(p, stream, &kept, nulls.data(), e2, sizeof(e2));
+    if (rc != BLZ_OK) throw BlzError(rc, e2);
+    if (n_kept) *n_kept = kept;
+    for (int o = 0; o < p->n_out; o++) {
+      if (null_counts) null_counts[o] = nulls[o];
+      if (out_columns) {
+        out_columns[o] = p->outs[o];
+        out_columns[o].size = (gdf_size_type)kept;
+        out_columns[o].null_count = nulls[o];
+      }
+    }
+    if (n_out) *n_out = p->n_out;
+  });
+}
+
+int blz_batch_time(blz_batch batch, blz_stream_t stream, int iters, double *mean_ms, char *err, size_t err_len) {
+  return guarded(err, err_len, [&] {
+    if (!batch || iters <= 0 || !mean_ms) invalid("bad arguments");
+    *mean_ms = batch_time_ms(batch, S(stream), iters);
+  });
+}
+
+int blz_batch_info(blz_batch batch, int *r, int *t, int *n_queries, int *graph, int *tuned_candidates) {
+  if (!batch) return BLZ_ERR_INVALID;
+  if (r) *r = batch->r;
+  if (t) *t = batch->t;
+  if (n_queries) *n_queries = (int)batch->progs.size();
+  if (graph) *graph = batch->use_graph ? 1 : 0;
+  if (tuned_candidates) *tuned_candidates = batch->tuned;
+  return BLZ_OK;
+}
+
+int blz_batch_destroy(blz_batch batch) {
+  batch_free(batch);
+  return BLZ_OK;
+}

@@ -32,20 +32,45 @@ int interp_blocks_per_cu(int s, int r, bool compact, int lds_bytes);
 hipError_t launch_acc(int nt, int r, bool compact, int grid, int lds_bytes, hipStream_t st,
                       const uint8_t *blob, Workspace ws);
 int acc_blocks_per_cu(int nt, int r, bool compact, int lds_bytes);
+// Tile VM launch descriptor (device memory): one compiled program, or a batch of
+// independent programs processed by one persistent launch. Global tile / span /
+// scan-chunk ids run over the programs in order; each program keeps its own
+// workspace. The LDS offsets are the maxima over the programs.
+struct VmPlanRef {
+  const uint8_t *blob;   // ProgHeader | ... of the program (acc_mode 2)
+  Workspace ws;          // the program's workspace
+  int64_t tile0;         // first global tile of the program
+  int64_t span0;         // first global gather span (4096 rows)
+  int64_t chunk0;        // first global slot-scan chunk
+};
+struct VmBatch {
+  int32_t n_plans;
+  int32_t bufb;                      // stride of the two tile buffers
+  int32_t tmp_off;                   // LDS temporaries of the projection / predicate kernels
+  int32_t g_buf_off, g_vb_off, g_tmp_off;  // gather kernel: pass buffers, validity words, temporaries
+  int32_t g_wave_bytes;              // gather kernel: one wave's pass buffer (64 R rows)
+  int32_t g_vb_wave_words;           // gather kernel: one wave's validity words
+  int64_t n_tiles, n_spans, n_chunks;      // totals over the programs
+  unsigned int *chunk_counter;       // slot scan: dynamic chunk ids (zero before the scan)
+  const VmPlanRef *plans;            // [n_plans]
+};
+
 // Tile VM (vm_kernel.hip): r rows per lane in {8, 4, 2}; tile = 256 r P rows;
 // lds_bytes: two tile buffers + temporaries (modes 0, 1), or the gather kernel's
 // keep words, pass buffer, validity words and temporaries (mode 2). slow: the
 // program has ops of the out-of-line set (transcendentals, POWER, date parts).
 // mode: 0 projection, 1 predicate (keep words + slot counts), 2 gather of the kept
 // rows (after launch_slot_scan).
-hipError_t launch_vm(int r, int mode, bool slow, int grid, int lds_bytes, hipStream_t st, const uint8_t *blob,
-                     Workspace ws);
+hipError_t launch_vm(int r, int mode, bool slow, int grid, int lds_bytes, hipStream_t st, const VmBatch *bd);
 int vm_blocks_per_cu(int r, int mode, bool slow, int lds_bytes);
-// Exclusive prefix of n_slots counts (single-pass decoupled look-back over
-// ws.tile_state / ws.tile_counter, which must be zero); total to ws.n_sel.
-hipError_t launch_slot_scan(const uint32_t *counts, uint32_t *prefix, int64_t n_slots, hipStream_t st,
-                            Workspace ws);
-int64_t slot_scan_blocks(int64_t n_slots);
+// Exclusive prefix of every program's slot counts (ws.wave_count -> ws.wave_prefix,
+// total to ws.n_sel): chunks of 4096 slots, combined within a program by a
+// single-pass decoupled look-back over its ws.tile_state (zero before the scan).
+hipError_t launch_slot_scan(const VmBatch *bd, int64_t n_chunks, hipStream_t st);
+constexpr int64_t kScanChunk = 4096;  // slots per scan chunk
+// Zeroes `n` byte ranges (dst[i], bytes[i]; 8-byte multiples, 8-byte aligned) in
+// one launch.
+hipError_t launch_fill_ranges(void *const *dst, const int64_t *bytes, int n, int grid, hipStream_t st);
 // materialize_column: width in {1,2,4,8}.
 hipError_t launch_gather(int width, const void *in, const uint8_t *in_valid, int64_t n_in, const int32_t *idx,
                          int64_t n_idx, void *out, uint8_t *out_valid, unsigned long long *null_count,

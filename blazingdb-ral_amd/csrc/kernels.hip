@@ -1166,6 +1166,16 @@ fill_zero_kernel(uint64_t *p, int64_t n_words) {
     p[i] = 0;
 }
 
+// Zeroes several byte ranges (8-byte multiples) in one launch.
+__global__ void __launch_bounds__(kBlock)
+fill_ranges_kernel(void *const *dst, const int64_t *bytes, int n) {
+  for (int r = 0; r < n; r++) {
+    uint64_t *p = (uint64_t *)dst[r];
+    const int64_t w = bytes[r] / 8;
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < w; i += (int64_t)gridDim.x * kBlock) p[i] = 0;
+  }
+}
+
 // Plain 16-byte-wide copy (also the reference point for HBM byte counters).
 __global__ void __launch_bounds__(kBlock)
 copy_kernel(uint4 *__restrict__ dst, const uint4 *__restrict__ src, int64_t n16) {
@@ -1310,6 +1320,11 @@ hipError_t launch_sequence(int32_t *out, int64_t n, int32_t init, int32_t step, 
 
 hipError_t launch_fill_zero(void *p, int64_t bytes, int grid, hipStream_t st) {
   hipLaunchKernelGGL(fill_zero_kernel, dim3(grid), dim3(kBlock), 0, st, (uint64_t *)p, (bytes + 7) / 8);
+  return hipGetLastError();
+}
+
+hipError_t launch_fill_ranges(void *const *dst, const int64_t *bytes, int n, int grid, hipStream_t st) {
+  hipLaunchKernelGGL(fill_ranges_kernel, dim3(grid), dim3(kBlock), 0, st, dst, bytes, n);
   return hipGetLastError();
 }
 

@@ -667,9 +667,10 @@ bool compile_vm(const CompileRequest &req, Builder &B, const std::vector<AccOp> 
   for (int r : shapes) {
     if (req.force_r > 0 && req.force_r != r) continue;
     const int tb = acc_temps * 256 * r * 8;
-    int g = 0;  // gather pass buffer of a filter
+    int g = 0;  // gather pass buffers of a filter (four waves' regions of 64 r rows)
     for (int c = 0; c < n_dev && req.compact; c++)
-      if (used_q[c]) g = al16(g + 256 * r * std::max(1, wid(req.inputs[c].dtype)));
+      if (used_q[c]) g = al16(g + 64 * r * std::max(1, wid(req.inputs[c].dtype)));
+    g *= 4;
     for (int p = 64 / r; p >= 1; p /= 2) {
       const int t = 256 * r * p;
       if (req.force_t > 0 ? t != req.force_t : t > 256 * r) continue;
@@ -694,9 +695,9 @@ bool compile_vm(const CompileRequest &req, Builder &B, const std::vector<AccOp> 
     for (int c = 0; c < n_dev; c++) {
       if (!used_q[c]) continue;
       const int w = std::max(1, wid(req.inputs[c].dtype));
-      goff[c] = gbuf;
+      goff[c] = gbuf;  // within one wave's region of 64 R rows
       gcols.push_back(Piece{req.inputs[c].data, 0, 0, gbuf, 0, 0, w});
-      gbuf += 256 * R * w;
+      gbuf += 64 * R * w;
       gbuf = (gbuf + 15) & ~15;
     }
   }
@@ -757,7 +758,7 @@ bool compile_vm(const CompileRequest &req, Builder &B, const std::vector<AccOp> 
   // gather kernel LDS: pass buffer | validity words (per wave) | temporaries
   const int g_kw = 0;
   const int g_buf = 0;
-  const int g_vb = g_buf + gbuf;
+  const int g_vb = g_buf + 4 * gbuf;  // four waves' regions
   const int g_tmp = g_vb + 4 * n_valid_out * 66 * 8;  // 64 + 2 words per wave and output
   h.lds_vbits_off = g_vb;
   h.n_gcols = (int)gcols.size();
@@ -824,6 +825,13 @@ bool compile_vm(const CompileRequest &req, Builder &B, const std::vector<AccOp> 
   out.vm = true;
   out.vm_slow = slow;
   out.lds_gather = g_tmp + tmp_bytes;
+  out.vm_bufb = buf;
+  out.vm_tmp_off = 2 * buf;
+  out.vm_g_buf_off = g_buf;
+  out.vm_g_wave_bytes = gbuf;
+  out.vm_nvout = n_valid_out;
+  out.vm_g_vb_off = g_vb;
+  out.vm_g_tmp_off = g_tmp;
   (void)vcol_needed;
   return true;
 }

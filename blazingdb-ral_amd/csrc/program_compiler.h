@@ -65,6 +65,9 @@ struct Compiled {
   bool vm = false;    // tile VM program (S = temporaries, T = 256 R)
   bool vm_slow = false;  // the VM program has ops of the out-of-line set
   int lds_gather = 0;    // tile VM filter: LDS of the gather kernel
+  int vm_bufb = 0, vm_tmp_off = 0;                      // tile VM LDS layout (projection / predicate)
+  int vm_g_buf_off = 0, vm_g_vb_off = 0, vm_g_tmp_off = 0;  // tile VM LDS layout (gather)
+  int vm_g_wave_bytes = 0, vm_nvout = 0;  // gather pass buffer per wave, outputs with validity
 };
 
 // Throws std::runtime_error with a message on an unsupported or malformed plan.

@@ -503,7 +503,7 @@ DEV void vm_read_valid(const CONSTANT InCol *in, int n_in, clds_ptr tb, int grou
 //               kept-row count per wave slot (tile, wave);
 //   VM_GATHER   from the keep words and the exclusive prefix of the counts, evaluate
 //               the projection of the kept rows only, packed densely into lanes
-//               (output j of a tile is its j-th kept row), and store them, and their
+//               (output j of a span is its j-th kept row), and store them, and their
 //               validity bits, in row order.
 enum VmMode { VM_PROJECT = 0, VM_PRED = 1, VM_GATHER = 2 };
 
@@ -522,172 +522,260 @@ DEV int select_bit(uint64_t m, int r) {
   return p;
 }
 
+// Program of a global id: largest i with first(i) <= id (first(0) == 0).
+template <typename F>
+DEV int find_plan(int n_plans, int64_t id, F first) {
+  int lo = 0, hi = n_plans - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (first(mid) <= id) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+// A program's workspace pointers, read through the constant cache.
+DEV Workspace load_ws(const CONSTANT Workspace &w) {
+  Workspace r;
+  r.null_counts = w.null_counts;
+  r.tile_counter = w.tile_counter;
+  r.tile_state = w.tile_state;
+  r.n_sel = w.n_sel;
+  r.keep = w.keep;
+  r.wave_count = w.wave_count;
+  r.wave_prefix = w.wave_prefix;
+  return r;
+}
+
+// The fields of one program the kernels read (scalar loads through the constant
+// cache, once per program change).
+struct VmCtx {
+  const CONSTANT InCol *in;
+  const CONSTANT OutCol *out;
+  const CONSTANT VmOp *ops;
+  const CONSTANT VOp *vops;
+  const CONSTANT Piece *pieces;
+  const CONSTANT Piece *gcols;
+  int64_t n;
+  int n_in, n_out, n_ops_p, n_ops_q, n_vops_p, n_vops_q, any_valid_out, np, n_gcols, T, n_vout, st_width,
+      st_vsrc;
+};
+DEV VmCtx load_ctx(const uint8_t *blob_arg) {
+  const CONSTANT uint8_t *blob = (const CONSTANT uint8_t *)blob_arg;
+  const CONSTANT ProgHeader &h = *(const CONSTANT ProgHeader *)blob;
+  VmCtx c;
+  c.in = (const CONSTANT InCol *)(blob + h.off_in);
+  c.out = (const CONSTANT OutCol *)(blob + h.off_out);
+  c.ops = (const CONSTANT VmOp *)(blob + h.off_ops);
+  c.vops = (const CONSTANT VOp *)(blob + h.off_vops);
+  c.pieces = (const CONSTANT Piece *)(blob + h.off_pieces);
+  c.gcols = (const CONSTANT Piece *)(blob + h.off_gcols);
+  c.n = h.n_rows;
+  c.n_in = h.n_in;
+  c.n_out = h.n_out;
+  c.n_ops_p = h.n_ops_p;
+  c.n_ops_q = h.n_ops_q;
+  c.n_vops_p = h.n_vops_p;
+  c.n_vops_q = h.n_vops_q;
+  c.any_valid_out = h.any_valid_out;
+  c.np = h.n_pieces;
+  c.n_gcols = h.n_gcols;
+  c.T = h.tile_rows;
+  c.n_vout = h.n_valid_out;
+  c.st_width = h.stencil_width;
+  c.st_vsrc = h.stencil_vsrc;
+  return c;
+}
+
 template <int R, int MODE, bool SLOW>
 __global__ void __launch_bounds__(kBlock)
-vm_kernel(const uint8_t *__restrict__ blob_arg, Workspace ws) {
+vm_kernel(const VmBatch *__restrict__ bd_arg) {
   extern __shared__ __attribute__((aligned(16))) uint8_t lds_raw[];
   constexpr int VS = 8;  // validity registers
   constexpr bool STAGED = MODE != VM_GATHER;
 
   lds_ptr lds = (lds_ptr)lds_raw;
-  const CONSTANT uint8_t *blob = (const CONSTANT uint8_t *)blob_arg;
-  const CONSTANT ProgHeader &h = *(const CONSTANT ProgHeader *)blob;
-  const CONSTANT InCol *in = (const CONSTANT InCol *)(blob + h.off_in);
-  const CONSTANT OutCol *out = (const CONSTANT OutCol *)(blob + h.off_out);
-  const CONSTANT VmOp *ops = (const CONSTANT VmOp *)(blob + h.off_ops);
-  const CONSTANT VOp *vops = (const CONSTANT VOp *)(blob + h.off_vops);
-  const CONSTANT Piece *pieces = (const CONSTANT Piece *)(blob + h.off_pieces);
-  const CONSTANT Piece *gcols = (const CONSTANT Piece *)(blob + h.off_gcols);
-  const int n_gcols = h.n_gcols;
-  const int64_t n = h.n_rows;
-  const int n_in = h.n_in, n_out = h.n_out;
-  const int n_ops_p = h.n_ops_p, n_ops_q = h.n_ops_q;
-  const int n_vops_p = h.n_vops_p, n_vops_q = h.n_vops_q;
-  const int any_valid_out = h.any_valid_out;
-  const int np = h.n_pieces, bufb = h.buf_bytes;
-  const int T = h.tile_rows;            // 256 R P rows
-  const int P = T / (kBlock * R);       // passes per tile
-  const int GW = P * R;                 // 64-row groups per wave slot
+  const CONSTANT VmBatch &bd = *(const CONSTANT VmBatch *)bd_arg;
+  const CONSTANT VmPlanRef *plans = (const CONSTANT VmPlanRef *)bd.plans;
+  const int n_plans = bd.n_plans;
+  const int bufb = bd.bufb;
 
   const int wave = threadIdx.x >> 6;
   const int lane = threadIdx.x & 63;
-  const int lbw = wave * 64 * GW;       // first row of this wave's slot in the tile
-  const int64_t n_tiles = (n + T - 1) / T;
   // Temporaries: LDS rows of this lane (one pass of the workgroup: 256 R rows).
   const int tstride = 8 * kBlock * R;
-  lds_ptr tmpb = lds + (MODE == VM_GATHER ? h.g_tmp_off : h.lds_tmp_off) + 8 * (wave * 64 * R + lane);
+  lds_ptr tmpb = lds + (MODE == VM_GATHER ? bd.g_tmp_off : bd.tmp_off) + 8 * (wave * 64 * R + lane);
 
   // Every register below is written by the program before it is read (the host
   // compiler guarantees it); none is live across passes.
   uint64_t acc[R];
   uint64_t vrf[VS];
 
-  int64_t cur = MODE == VM_GATHER ? n_tiles : (int64_t)blockIdx.x;
-  int buf = 0;
-  if (STAGED && cur < n_tiles && (cur + 1) * T <= n) dma_tile(pieces, np, cur, lds);
-
-  for (; cur < n_tiles; cur += gridDim.x, buf ^= 1) {
-    const int64_t row0 = cur * T;
-    const int64_t rows = (n - row0) < T ? (n - row0) : T;
-    lds_ptr tb = lds + buf * bufb;
-    if (STAGED) {
+  if (STAGED) {
+    const int64_t n_tiles = bd.n_tiles;
+    auto tile_first = [&](int i) { return plans[i].tile0; };
+    int64_t cur = blockIdx.x;
+    int pi = -1;
+    VmCtx cx;
+    int buf = 0;
+    // staging of global tile g (full tiles only; the last tile of a program is
+    // staged with bounded loads when it is current)
+    auto issue = [&](int64_t g, lds_ptr dst) {
+      if (g >= n_tiles) return;
+      const int q = find_plan(n_plans, g, tile_first);
+      const CONSTANT uint8_t *qb = (const CONSTANT uint8_t *)plans[q].blob;
+      const CONSTANT ProgHeader &qh = *(const CONSTANT ProgHeader *)qb;
+      const int64_t t = g - plans[q].tile0;
+      if ((t + 1) * qh.tile_rows <= qh.n_rows)
+        dma_tile((const CONSTANT Piece *)(qb + qh.off_pieces), qh.n_pieces, t, dst);
+    };
+    issue(cur, lds);
+    for (; cur < n_tiles; cur += gridDim.x, buf ^= 1) {
+      const int p = find_plan(n_plans, cur, tile_first);
+      if (p != pi) {
+        pi = p;
+        cx = load_ctx(plans[p].blob);
+      }
+      const Workspace ws = load_ws(plans[p].ws);
+      const int64_t tile = cur - plans[p].tile0;
+      const int T = cx.T;
+      const int P = T / (kBlock * R);  // passes per tile
+      const int GW = P * R;            // 64-row groups per wave slot
+      const int lbw = wave * 64 * GW;  // first row of this wave's slot in the tile
+      const int64_t n = cx.n;
+      const int64_t row0 = tile * T;
+      const int64_t rows = (n - row0) < T ? (n - row0) : T;
+      lds_ptr tb = lds + buf * bufb;
       // This wave's staging loads of the tile are complete; the barrier makes every
       // wave's visible and ends all reads of the other buffer.
       __builtin_amdgcn_s_waitcnt(0);
       __syncthreads();
       if (rows < T) {
-        stage_partial(pieces, np, cur, rows, tb);
+        stage_partial(cx.pieces, cx.np, tile, rows, tb);
         __syncthreads();
       }
-      const int64_t nxt = cur + gridDim.x;
-      if (nxt < n_tiles && (nxt + 1) * T <= n) dma_tile(pieces, np, nxt, lds + (buf ^ 1) * bufb);
-    }
+      issue(cur + gridDim.x, lds + (buf ^ 1) * bufb);
 
-    if (MODE == VM_PROJECT) {
-      for (int p = 0; p < P; p++) {
-        const int lb = lbw + p * 64 * R;
-        if (lb >= rows) break;  // wave-uniform
-        const int lrow = lb + lane;
-        int orel[R];
-        bool put[R];
-        int64_t grow[R];
+      if (MODE == VM_PROJECT) {
+        for (int ps = 0; ps < P; ps++) {
+          const int lb = lbw + ps * 64 * R;
+          if (lb >= rows) break;  // wave-uniform
+          const int lrow = lb + lane;
+          int orel[R];
+          bool put[R];
+          int64_t grow[R];
 #pragma unroll
-        for (int u = 0; u < R; u++) {
-          orel[u] = lrow + 64 * u;
-          put[u] = orel[u] < rows;
-          grow[u] = row0 + orel[u];
-          acc[u] = 0;
-        }
-        run_vm<R, SLOW>(ops, n_ops_q, acc, tmpb, tstride, in, out, tb, lrow, grow, n, row0, orel, put);
-        if (any_valid_out) {
-          vm_read_valid<VS, R>(in, n_in, tb, lb >> 6, vrf);
-          run_vops<VS>(vops, n_vops_q, vrf);
-          for (int o = 0; o < n_out; o++) {
-            void *vp = out[o].valid;
-            if (vp == nullptr) continue;
-            const uint64_t word = vsrc_word<VS>(out[o].vsrc, vrf);
-            const int64_t wi = ((row0 + lb) >> 6) + lane;
-            uint64_t nulls = 0;
-            if (lane < R && wi * 64 < n) {
-              store_vword(vp, wi, n, word);
-              nulls = __builtin_popcountll(~word & low_mask(n - wi * 64));
+          for (int u = 0; u < R; u++) {
+            orel[u] = lrow + 64 * u;
+            put[u] = orel[u] < rows;
+            grow[u] = row0 + orel[u];
+            acc[u] = 0;
+          }
+          run_vm<R, SLOW>(cx.ops, cx.n_ops_q, acc, tmpb, tstride, cx.in, cx.out, tb, lrow, grow, n, row0, orel,
+                          put);
+          if (cx.any_valid_out) {
+            vm_read_valid<VS, R>(cx.in, cx.n_in, tb, lb >> 6, vrf);
+            run_vops<VS>(cx.vops, cx.n_vops_q, vrf);
+            for (int o = 0; o < cx.n_out; o++) {
+              void *vp = cx.out[o].valid;
+              if (vp == nullptr) continue;
+              const uint64_t word = vsrc_word<VS>(cx.out[o].vsrc, vrf);
+              const int64_t wi = ((row0 + lb) >> 6) + lane;
+              uint64_t nulls = 0;
+              if (lane < R && wi * 64 < n) {
+                store_vword(vp, wi, n, word);
+                nulls = __builtin_popcountll(~word & low_mask(n - wi * 64));
+              }
+              for (int s = 32; s > 0; s >>= 1) nulls += __shfl_xor(nulls, s);
+              if (lane == 0 && nulls) atomicAdd(ws.null_counts + o, (unsigned long long)nulls);
             }
-            for (int s = 32; s > 0; s >>= 1) nulls += __shfl_xor(nulls, s);
-            if (lane == 0 && nulls) atomicAdd(ws.null_counts + o, (unsigned long long)nulls);
           }
         }
-      }
-      continue;
-    }
-
-    if (MODE == VM_PRED) {
-      // ---- predicate -> keep word per 64-row group, kept rows per wave slot ----
-      const uint64_t st_mask = low_mask(8 * h.stencil_width);
-      uint64_t myword = 0, cnt = 0;
-      for (int p = 0; p < P; p++) {
-        const int lb = lbw + p * 64 * R;
-        if (lb >= rows) break;  // wave-uniform
-        const int lrow = lb + lane;
-        int orel[R];
-        bool none[R];
-        int64_t grow[R];
+      } else {
+        // ---- predicate -> keep word per 64-row group, kept rows per wave slot ----
+        const uint64_t st_mask = low_mask(8 * cx.st_width);
+        uint64_t myword = 0, cnt = 0;
+        for (int ps = 0; ps < P; ps++) {
+          const int lb = lbw + ps * 64 * R;
+          if (lb >= rows) break;  // wave-uniform
+          const int lrow = lb + lane;
+          int orel[R];
+          bool none[R];
+          int64_t grow[R];
 #pragma unroll
-        for (int u = 0; u < R; u++) {
-          orel[u] = 0;
-          none[u] = false;
-          grow[u] = row0 + lrow + 64 * u;
-          acc[u] = 0;
-        }
-        run_vm<R, SLOW>(ops, n_ops_p, acc, tmpb, tstride, in, out, tb, lrow, grow, n, 0, orel, none);
-        vm_read_valid<VS, R>(in, n_in, tb, lb >> 6, vrf);
-        run_vops<VS>(vops, n_vops_p, vrf);
-        const uint64_t vstencil = vsrc_word<VS>(h.stencil_vsrc, vrf);
+          for (int u = 0; u < R; u++) {
+            orel[u] = 0;
+            none[u] = false;
+            grow[u] = row0 + lrow + 64 * u;
+            acc[u] = 0;
+          }
+          run_vm<R, SLOW>(cx.ops, cx.n_ops_p, acc, tmpb, tstride, cx.in, cx.out, tb, lrow, grow, n, 0, orel, none);
+          vm_read_valid<VS, R>(cx.in, cx.n_in, tb, lb >> 6, vrf);
+          run_vops<VS>(cx.vops, cx.n_vops_p, vrf);
+          const uint64_t vstencil = vsrc_word<VS>(cx.st_vsrc, vrf);
 #pragma unroll
-        for (int u = 0; u < R; u++) {
-          const bool vbit = (readlane64(vstencil, u) >> lane) & 1;
-          const bool keep = lrow + 64 * u < rows && ((acc[u] & st_mask) != 0) && vbit;
-          const uint64_t km = __builtin_amdgcn_ballot_w64(keep);
-          if (lane == p * R + u) myword = km;
-          cnt += __builtin_popcountll(km);
+          for (int u = 0; u < R; u++) {
+            const bool vbit = (readlane64(vstencil, u) >> lane) & 1;
+            const bool keep = lrow + 64 * u < rows && ((acc[u] & st_mask) != 0) && vbit;
+            const uint64_t km = __builtin_amdgcn_ballot_w64(keep);
+            if (lane == ps * R + u) myword = km;
+            cnt += __builtin_popcountll(km);
+          }
         }
+        const int64_t g0 = tile * (T / 64) + wave * GW;  // this wave's first group
+        if (lane < GW) gstore<uint64_t>(ws.keep, g0 + lane, myword);
+        if (lane == 0) gstore<uint32_t>(ws.wave_count, tile * kWaves + wave, (uint32_t)cnt);
       }
-      const int64_t g0 = cur * (T / 64) + wave * GW;  // this wave's first group
-      if (lane < GW) gstore<uint64_t>(ws.keep, g0 + lane, myword);
-      if (lane == 0) gstore<uint32_t>(ws.wave_count, cur * kWaves + wave, (uint32_t)cnt);
-      continue;
     }
-
-    (void)row0;
+    return;
   }
-  if (MODE != VM_GATHER) return;
 
   // ---- VM_GATHER: every wave on its own spans of 64 keep words (4096 rows), no
   // barriers. Output j of a span is its j-th kept row: found by a binary search
   // over the span's per-word prefix counts (one word per lane, read with
   // cross-lane shuffles) and a bit select within the word. The next span's keep
   // words and output offset are loaded while this one is evaluated.
-  const int64_t n_words = n_tiles * (T / 64);
-  const int64_t n_spans = (n_words + 63) / 64;
-  const int n_vout = h.n_valid_out;
+  const int64_t n_spans = bd.n_spans;
+  auto span_first = [&](int i) { return plans[i].span0; };
   constexpr int VW = 66;  // validity words of a span's outputs (64 + 2)
-  LDSAS uint64_t *vbits = (LDSAS uint64_t *)(lds + h.lds_vbits_off) + wave * VW * n_vout;
-  lds_ptr gbuf = lds + h.g_buf_off;
-  const int plrow = wave * 64 * R + lane;  // this lane's row in the pass buffer
+  // this wave's pass buffer (64 R rows of each column the projection reads) and
+  // validity words: per-wave regions, so waves of one workgroup may serve different
+  // programs of a batch
+  lds_ptr gbuf = lds + bd.g_buf_off + wave * bd.g_wave_bytes;
+  const int plrow = lane;  // this lane's row in the pass buffer
+  LDSAS uint64_t *vbits = (LDSAS uint64_t *)(lds + bd.g_vb_off) + wave * bd.g_vb_wave_words;
   const int64_t stride = (int64_t)gridDim.x * kWaves;
-  int64_t span = (int64_t)blockIdx.x * kWaves + wave;
-  // wave_prefix[slot] of the span's first slot (a span holds 64 / GW whole slots)
-  auto span_words = [&](int64_t sp) -> uint64_t {
+  // keep words of global span g and the output offset of its first row
+  auto span_words = [&](int64_t g, uint64_t &w, uint64_t &base) {
+    w = 0;
+    base = 0;
+    if (g >= n_spans) return;
+    const int q = find_plan(n_plans, g, span_first);
+    const CONSTANT ProgHeader &qh = *(const CONSTANT ProgHeader *)plans[q].blob;
+    const Workspace qws = load_ws(plans[q].ws);
+    const int64_t sp = g - plans[q].span0;
+    const int64_t qtiles = (qh.n_rows + qh.tile_rows - 1) / qh.tile_rows;
+    const int64_t n_words = qtiles * (qh.tile_rows / 64);
     const int64_t wi = sp * 64 + lane;
-    return (sp < n_spans && wi < n_words) ? ws.keep[wi] : 0ull;
+    if (wi < n_words) w = qws.keep[wi];
+    base = qws.wave_prefix[sp * 64 / (qh.tile_rows / 256)];
   };
-  auto span_base = [&](int64_t sp) -> uint64_t {
-    return sp < n_spans ? (uint64_t)ws.wave_prefix[sp * 64 / GW] : 0ull;
-  };
-  uint64_t myw = span_words(span);
-  uint64_t obase = span_base(span);
+  int64_t span = (int64_t)blockIdx.x * kWaves + wave;
+  uint64_t myw, obase;
+  span_words(span, myw, obase);
+  int pi = -1;
+  VmCtx cx;
   for (; span < n_spans; span += stride) {
-    const uint64_t myw_next = span_words(span + stride);
-    const uint64_t obase_next = span_base(span + stride);
+    uint64_t myw_next, obase_next;
+    span_words(span + stride, myw_next, obase_next);
+    const int p = find_plan(n_plans, span, span_first);
+    if (p != pi) {
+      pi = p;
+      cx = load_ctx(plans[p].blob);
+    }
+    const Workspace ws = load_ws(plans[p].ws);
+    const int64_t n = cx.n;
+    const int n_vout = cx.n_vout;
     const uint32_t c = (uint32_t)__builtin_popcountll(myw);
     uint32_t inc = c;
     for (int off = 1; off < 64; off <<= 1) {
@@ -697,7 +785,7 @@ vm_kernel(const uint8_t *__restrict__ blob_arg, Workspace ws) {
     const uint32_t cnt = __shfl(inc, 63);
     const uint32_t mypre = inc - c;  // kept rows of the span before word `lane`
     if (cnt > 0) {
-      const int64_t srow0 = span * 4096;  // first row of the span
+      const int64_t srow0 = (span - plans[p].span0) * 4096;  // first row of the span
       const int lo = (int)(obase & 63);
       if (n_vout > 0) {
         for (int i = lane; i < VW * n_vout; i += 64) vbits[i] = 0;
@@ -726,25 +814,25 @@ vm_kernel(const uint8_t *__restrict__ blob_arg, Workspace ws) {
           grow[u] = srow0 + src;
           acc[u] = 0;
         }
-        stage_kept<R>(gcols, n_gcols, gbuf, plrow, grow, live);
-        run_vm<R, SLOW>(ops + n_ops_p, n_ops_q, acc, tmpb, tstride, in, out, gbuf, plrow, grow, n,
+        stage_kept<R>(cx.gcols, cx.n_gcols, gbuf, plrow, grow, live);
+        run_vm<R, SLOW>(cx.ops + cx.n_ops_p, cx.n_ops_q, acc, tmpb, tstride, cx.in, cx.out, gbuf, plrow, grow, n,
                         (int64_t)obase, orel, live);
-        if (any_valid_out) {
+        if (cx.any_valid_out) {
           // Validity of the 64 outputs of group u as one word: lane u holds it.
 #pragma unroll
           for (int u = 0; u < R; u++) {
-            for (int ci = 0; ci < n_in; ci++) {
-              const int vs = in[ci].vslot;
+            for (int ci = 0; ci < cx.n_in; ci++) {
+              const int vs = cx.in[ci].vslot;
               if (vs < 0) continue;
-              const uint64_t word = __builtin_amdgcn_ballot_w64(live[u] && valid_bit(in[ci].valid, grow[u]));
+              const uint64_t word = __builtin_amdgcn_ballot_w64(live[u] && valid_bit(cx.in[ci].valid, grow[u]));
               if (lane == u) vrf[vs] = word;
             }
           }
-          run_vops<VS>(vops + n_vops_p, n_vops_q, vrf);
-          for (int o = 0; o < n_out; o++) {
-            if (out[o].valid == nullptr) continue;
-            const uint64_t vword = vsrc_word<VS>(out[o].vsrc, vrf);
-            const int vb = out[o].vbuf;
+          run_vops<VS>(cx.vops + cx.n_vops_p, cx.n_vops_q, vrf);
+          for (int o = 0; o < cx.n_out; o++) {
+            if (cx.out[o].valid == nullptr) continue;
+            const uint64_t vword = vsrc_word<VS>(cx.out[o].vsrc, vrf);
+            const int vb = cx.out[o].vbuf;
 #pragma unroll
             for (int u = 0; u < R; u++) {
               const int j0 = g * 64 * R + 64 * u;
@@ -771,10 +859,10 @@ vm_kernel(const uint8_t *__restrict__ blob_arg, Workspace ws) {
         const int nw = (lo + (int)cnt + 63) >> 6;
         const int64_t w0 = (int64_t)(obase >> 6);
         const int end = lo + (int)cnt;
-        for (int o = 0; o < n_out; o++) {
-          void *vp = out[o].valid;
+        for (int o = 0; o < cx.n_out; o++) {
+          void *vp = cx.out[o].valid;
           if (vp == nullptr) continue;
-          const int vb = out[o].vbuf;
+          const int vb = cx.out[o].vbuf;
           for (int i = lane; i < nw; i += 64) {
             unsigned long long *gp = (unsigned long long *)((uint8_t *)vp + 8 * (w0 + i));
             const uint64_t word = vbits[vb * VW + i];
@@ -785,7 +873,7 @@ vm_kernel(const uint8_t *__restrict__ blob_arg, Workspace ws) {
         }
         __builtin_amdgcn_wave_barrier();
       }
-      for (int o = 0; o < n_out && o < 4; o++)
+      for (int o = 0; o < cx.n_out && o < 4; o++)
         if (lane == 0 && nl[o]) atomicAdd(ws.null_counts + o, nl[o]);
     }
     myw = myw_next;
@@ -793,31 +881,39 @@ vm_kernel(const uint8_t *__restrict__ blob_arg, Workspace ws) {
   }
 }
 
-// Exclusive prefix of the per-wave-slot kept counts (VM_PRED) in one pass:
-// each workgroup scans 4096 counts and combines its total with the workgroups
-// before it by a single-pass decoupled look-back. The last one writes the total.
+// Exclusive prefix of every program's per-wave-slot kept counts (VM_PRED) in one
+// pass: each workgroup scans a chunk of 4096 counts of one program and combines
+// its total with the chunks before it by a single-pass decoupled look-back over
+// that program's state words. Chunks are taken in the order workgroups start, so
+// every chunk a look-back waits for belongs to a workgroup that is running.
 constexpr int kScanItems = 16;
+static_assert(kScanItems * kBlock == kScanChunk, "scan chunk");
 __global__ void __launch_bounds__(kBlock)
-slot_scan_kernel(const uint32_t *__restrict__ counts, uint32_t *__restrict__ prefix, int64_t n_slots,
-                 Workspace ws) {
+slot_scan_kernel(const VmBatch *__restrict__ bd_arg) {
   __shared__ uint64_t s_wave[kWaves];
   __shared__ uint64_t s_prefix;
-  __shared__ int64_t s_block;
+  __shared__ int64_t s_chunk;
+  const CONSTANT VmBatch &bd = *(const CONSTANT VmBatch *)bd_arg;
+  const CONSTANT VmPlanRef *plans = (const CONSTANT VmPlanRef *)bd.plans;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  // Chunks are taken in the order workgroups start, so every chunk a look-back
-  // waits for belongs to a workgroup that is already running.
-  if (threadIdx.x == 0) s_block = (int64_t)atomicAdd(ws.tile_counter, 1u);
+  if (threadIdx.x == 0) s_chunk = (int64_t)atomicAdd(bd.chunk_counter, 1u);
   __syncthreads();
-  const int64_t b = s_block;
-  const int64_t base = b * kBlock * kScanItems + (int64_t)threadIdx.x * kScanItems;
+  const int64_t gch = s_chunk;
+  if (gch >= bd.n_chunks) return;
+  const int p = find_plan(bd.n_plans, gch, [&](int i) { return plans[i].chunk0; });
+  const Workspace ws = load_ws(plans[p].ws);
+  const CONSTANT ProgHeader &h = *(const CONSTANT ProgHeader *)plans[p].blob;
+  const int64_t n_slots = ((h.n_rows + h.tile_rows - 1) / h.tile_rows) * kWaves;
+  const int64_t b = gch - plans[p].chunk0;                 // chunk within the program
+  const int64_t n_blocks = (n_slots + kScanChunk - 1) / kScanChunk;
+  const int64_t base = b * kScanChunk + (int64_t)threadIdx.x * kScanItems;
   uint32_t v[kScanItems];
   uint64_t sum = 0;
 #pragma unroll
   for (int i = 0; i < kScanItems; i++) {
-    v[i] = base + i < n_slots ? counts[base + i] : 0u;
+    v[i] = base + i < n_slots ? ws.wave_count[base + i] : 0u;
     sum += v[i];
   }
-  // inclusive scan of the thread sums across the wave
   uint64_t inc = sum;
   for (int off = 1; off < 64; off <<= 1) {
     const uint64_t y = __shfl_up(inc, off);
@@ -864,14 +960,14 @@ slot_scan_kernel(const uint32_t *__restrict__ counts, uint32_t *__restrict__ pre
     }
     if (lane == 0) {
       s_prefix = pre;
-      if (b == (int64_t)gridDim.x - 1) *ws.n_sel = pre + total;
+      if (b == n_blocks - 1) *ws.n_sel = pre + total;
     }
   }
   __syncthreads();
   uint64_t run = s_prefix + wbase + inc - sum;
 #pragma unroll
   for (int i = 0; i < kScanItems; i++) {
-    if (base + i < n_slots) prefix[base + i] = (uint32_t)run;
+    if (base + i < n_slots) ws.wave_prefix[base + i] = (uint32_t)run;
     run += v[i];
   }
 }
@@ -882,9 +978,9 @@ slot_scan_kernel(const uint32_t *__restrict__ counts, uint32_t *__restrict__ pre
 #define BLZ_VM_SHAPES(X) X(8) X(4) X(2)
 
 template <int RR, int MODE>
-hipError_t launch_vm_mode(bool slow, int grid, int lds_bytes, hipStream_t st, const uint8_t *blob, Workspace ws) {
-  if (slow) hipLaunchKernelGGL((vm_kernel<RR, MODE, true>), dim3(grid), dim3(kBlock), lds_bytes, st, blob, ws);
-  else hipLaunchKernelGGL((vm_kernel<RR, MODE, false>), dim3(grid), dim3(kBlock), lds_bytes, st, blob, ws);
+hipError_t launch_vm_mode(bool slow, int grid, int lds_bytes, hipStream_t st, const VmBatch *bd) {
+  if (slow) hipLaunchKernelGGL((vm_kernel<RR, MODE, true>), dim3(grid), dim3(kBlock), lds_bytes, st, bd);
+  else hipLaunchKernelGGL((vm_kernel<RR, MODE, false>), dim3(grid), dim3(kBlock), lds_bytes, st, bd);
   return hipGetLastError();
 }
 
@@ -897,12 +993,11 @@ int vm_occupancy_mode(bool slow, int lds_bytes) {
 }
 
 template <int RR>
-hipError_t launch_vm_shape(int mode, bool slow, int grid, int lds_bytes, hipStream_t st, const uint8_t *blob,
-                           Workspace ws) {
+hipError_t launch_vm_shape(int mode, bool slow, int grid, int lds_bytes, hipStream_t st, const VmBatch *bd) {
   switch (mode) {
-    case VM_PROJECT: return launch_vm_mode<RR, VM_PROJECT>(slow, grid, lds_bytes, st, blob, ws);
-    case VM_PRED: return launch_vm_mode<RR, VM_PRED>(slow, grid, lds_bytes, st, blob, ws);
-    case VM_GATHER: return launch_vm_mode<RR, VM_GATHER>(slow, grid, lds_bytes, st, blob, ws);
+    case VM_PROJECT: return launch_vm_mode<RR, VM_PROJECT>(slow, grid, lds_bytes, st, bd);
+    case VM_PRED: return launch_vm_mode<RR, VM_PRED>(slow, grid, lds_bytes, st, bd);
+    case VM_GATHER: return launch_vm_mode<RR, VM_GATHER>(slow, grid, lds_bytes, st, bd);
     default: return hipErrorInvalidValue;
   }
 }
@@ -917,10 +1012,9 @@ int vm_occupancy(int mode, bool slow, int lds_bytes) {
   }
 }
 
-hipError_t launch_vm(int r, int mode, bool slow, int grid, int lds_bytes, hipStream_t st, const uint8_t *blob,
-                     Workspace ws) {
+hipError_t launch_vm(int r, int mode, bool slow, int grid, int lds_bytes, hipStream_t st, const VmBatch *bd) {
 #define BLZ_VM_LAUNCH(RR) \
-  if (r == RR) return launch_vm_shape<RR>(mode, slow, grid, lds_bytes, st, blob, ws);
+  if (r == RR) return launch_vm_shape<RR>(mode, slow, grid, lds_bytes, st, bd);
   BLZ_VM_SHAPES(BLZ_VM_LAUNCH)
 #undef BLZ_VM_LAUNCH
   return hipErrorInvalidValue;
@@ -934,16 +1028,9 @@ int vm_blocks_per_cu(int r, int mode, bool slow, int lds_bytes) {
   return 0;
 }
 
-hipError_t launch_slot_scan(const uint32_t *counts, uint32_t *prefix, int64_t n_slots, hipStream_t st,
-                            Workspace ws) {
-  const int64_t per = (int64_t)kBlock * kScanItems;
-  const int grid = (int)(n_slots > 0 ? (n_slots + per - 1) / per : 1);
-  hipLaunchKernelGGL(slot_scan_kernel, dim3(grid), dim3(kBlock), 0, st, counts, prefix, n_slots, ws);
+hipError_t launch_slot_scan(const VmBatch *bd, int64_t n_chunks, hipStream_t st) {
+  hipLaunchKernelGGL(slot_scan_kernel, dim3((unsigned)(n_chunks > 0 ? n_chunks : 1)), dim3(kBlock), 0, st, bd);
   return hipGetLastError();
-}
-int64_t slot_scan_blocks(int64_t n_slots) {
-  const int64_t per = (int64_t)kBlock * kScanItems;
-  return n_slots > 0 ? (n_slots + per - 1) / per : 1;
 }
 
 }  // namespace blz

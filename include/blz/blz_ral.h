@@ -165,6 +165,44 @@ int blz_program_info(blz_program prog, int *s, int *r, int *n_ops, int *n_vops, 
 int blz_program_tile(blz_program prog, int *tile_rows, int *lds_bytes, int *kernel);
 int blz_program_destroy(blz_program prog);
 
+/* ---- batches (no reference counterpart: many independent queries, one launch) ----
+ *
+ * A batch compiles n independent queries (each its own frame, optional filter and
+ * projection, the same semantics as blz_program_create) and executes all of them
+ * with one persistent launch per kernel kind: a projection kernel over every
+ * projection-only query's tiles, then the predicate, slot-scan and gather kernels
+ * over every filtered query's tiles, regardless of n. Results stay in
+ * batch-owned device columns until the batch is destroyed.
+ *
+ * r / t: rows per lane (2, 4 or 8) and tile rows of every query; r = 0 runs the
+ * autotuner, which times each candidate shape on the batch itself (a few
+ * executions each) and keeps the fastest. use_graph != 0 captures the launch
+ * sequence into a HIP graph once and replays it on every blz_batch_launch. */
+typedef struct blz_batch_s *blz_batch;
+typedef struct {
+  gdf_column *const *frame;
+  int n_frame;
+  const char *filter_query_part;   /* NULL: no filter */
+  const char *project_query_part;  /* NULL: every frame column */
+  int scalar_mode;
+} blz_query;
+
+int blz_batch_create(blz_batch *batch, const blz_query *queries, int n_queries, int r, int t, int use_graph,
+                     char *err, size_t err_len);
+/* Enqueues one execution of every query on `stream` (no allocation). */
+int blz_batch_launch(blz_batch batch, blz_stream_t stream, char *err, size_t err_len);
+/* Waits for `stream`, then describes query q's results: rows kept (rows for a
+ * projection), per-output null counts and the output columns (views owned by the
+ * batch, sized to the rows kept). */
+int blz_batch_result(blz_batch batch, blz_stream_t stream, int q, int64_t *n_kept, int32_t *null_counts,
+                     gdf_column *out_columns, int max_out, int *n_out, char *err, size_t err_len);
+/* Mean milliseconds of one batch execution over `iters` (HIP events on `stream`). */
+int blz_batch_time(blz_batch batch, blz_stream_t stream, int iters, double *mean_ms, char *err, size_t err_len);
+/* r, tile rows, number of queries, whether a graph is replayed, autotuner's
+ * candidates timed. */
+int blz_batch_info(blz_batch batch, int *r, int *t, int *n_queries, int *graph, int *tuned_candidates);
+int blz_batch_destroy(blz_batch batch);
+
 /* ---- memory / streams (HIP plumbing for hosts without a HIP toolchain) ------- */
 
 int blz_device_malloc(void **ptr, size_t bytes);

@@ -322,3 +322,91 @@ def test_vm_shapes(B, shape, mode):
                 g.col.null_count = nc
                 assert_column_equal(g, rc, what=f"vm r={r} T={T} n={n}: {filt} {proj}")
         prog.close()
+
+
+def _rand_query(rng, k_min=2, k_max=6, filt_p=0.6):
+    k = int(rng.integers(k_min, k_max))
+    dts = exprgen.rand_frame_dtypes(rng, k)
+    n = int(rng.choice([0, 1, 700, 5000, 70_001, 300_000]))
+    null_rates = [float(rng.choice([0.0, 0.1])) for _ in range(k)]
+    host = [gen_column(rng, dt, n, nr) for dt, nr in zip(dts, null_rates)]
+    nullable = [i for i, nr in enumerate(null_rates) if nr > 0]
+    exprs = [exprgen.rand_expr(rng, dts, 3, False, nullable) for _ in range(int(rng.integers(1, 4)))]
+    items = [f"E{i}=[{e}]" for i, e in enumerate(exprs)] + [f"A=[${int(rng.integers(0, k))}]"]
+    proj = "LogicalProject(" + ", ".join(items) + ")"
+    filt = None
+    if rng.random() < filt_p:
+        cond = f"<({exprgen.rand_expr(rng, dts, 2, False, nullable)}, {int(rng.integers(-5, 6))})"
+        filt = f"LogicalFilter(condition=[{cond}])"
+    return host, filt, proj
+
+
+def _oracle(host, filt, proj):
+    mid = O.process_filter(host, filt) if filt else host
+    return O.execute_project(mid, proj)
+
+
+def _check_batch_query(batch, q, ref, what):
+    kept, cols, nulls = batch.result(q)
+    assert kept == len(ref[0]), what
+    for g, r, nc in zip(cols, ref, nulls):
+        g.col.null_count = nc
+        assert_column_equal(g, r, what=what)
+
+
+@pytest.mark.parametrize("graph", [False, True])
+@pytest.mark.parametrize("r", [4, 2])
+def test_batch_matches_each_query_alone(B, r, graph):
+    """Each query's result in a batch of 24 uneven queries equals its result alone
+    (a Program of the same query) and the oracle's."""
+    rng = np.random.default_rng(40_000 + r + 10 * graph)
+    qs, refs, devs = [], [], []
+    while len(qs) < 24:
+        host, filt, proj = _rand_query(rng)
+        try:
+            ref = _oracle(host, filt, proj)
+        except (P.PlanError, ValueError):
+            continue
+        dev = [to_device(B, h) for h in host]
+        devs.append(dev)
+        qs.append((dev, filt, proj))
+        refs.append(ref)
+    _print_case(test="batch", r=r, graph=graph, queries=len(qs))
+    batch = B.Batch(qs, r=r, t=256 * r, use_graph=graph)
+    assert batch.r == r and batch.graph == int(graph)
+    for rep in range(3):  # replays, with eager work and synchronisation between them
+        batch.launch()
+        for q, ((dev, filt, proj), ref) in enumerate(zip(qs, refs)):
+            _check_batch_query(batch, q, ref, f"batch r={r} q={q} rep={rep}: {filt} {proj}")
+        if rep == 0:
+            for q in (0, 5, 11):
+                dev, filt, proj = qs[q]
+                prog = B.Program(dev, filt, proj)
+                prog.launch()
+                kept, _ = prog.result()
+                bk, bcols, _ = batch.result(q)
+                assert kept == bk
+                for a, b in zip(prog.outputs, bcols):
+                    assert a.data().tobytes() == b.data().tobytes()
+                prog.close()
+    batch.close()
+
+
+def test_batch_autotuner(B):
+    rng = np.random.default_rng(41_000)
+    qs, refs = [], []
+    while len(qs) < 12:
+        host, filt, proj = _rand_query(rng, filt_p=0.5)
+        try:
+            ref = _oracle(host, filt, proj)
+        except (P.PlanError, ValueError):
+            continue
+        qs.append(([to_device(B, h) for h in host], filt, proj))
+        refs.append(ref)
+    _print_case(test="batch_autotuner", n=len(qs))
+    batch = B.Batch(qs, r=0, use_graph=True)
+    assert batch.tuned >= 1 and batch.r in (2, 4, 8) and batch.graph == 1
+    batch.launch()
+    for q, ref in enumerate(refs):
+        _check_batch_query(batch, q, ref, f"autotuned batch q={q}")
+    batch.close()
