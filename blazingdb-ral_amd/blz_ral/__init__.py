@@ -118,6 +118,16 @@ def lib():
             "blz_batch_time": [P, P, I, C.POINTER(C.c_double), ERR, SZ],
             "blz_batch_info": [P] + [C.POINTER(C.c_int)] * 5,
             "blz_batch_destroy": [P],
+            "blz_server_create": [C.POINTER(P), SZ, I, I, I, ERR, SZ],
+            "blz_server_submit": [P, C.POINTER(BlzQuery), C.POINTER(C.c_int64), ERR, SZ],
+            "blz_server_wait": [P, C.c_int64, C.POINTER(C.c_int64), I32P, colp, I, C.POINTER(C.c_int),
+                                C.POINTER(C.c_double), ERR, SZ],
+            "blz_server_release": [P, C.c_int64],
+            "blz_server_replay": [P, C.POINTER(BlzQuery), C.POINTER(C.c_double), I, C.POINTER(C.c_double),
+                                  C.POINTER(C.c_int64), C.POINTER(C.c_double), ERR, SZ],
+            "blz_server_stats": [P, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(SZ)],
+            "blz_server_destroy": [P],
+            "blz_synth_column": [colp, C.c_uint64, I, C.c_double, C.c_double, C.c_double, P, ERR, SZ],
             "blz_device_malloc": [C.POINTER(P), SZ],
             "blz_device_free": [P],
             "blz_memcpy_htod": [P, P, SZ, P],
@@ -541,3 +551,83 @@ class Batch:
             self.close()
         except Exception:
             pass
+
+
+def _queries(queries, scalar_mode, keep):
+    arr = (BlzQuery * len(queries))()
+    for i, (frame, fq, pq) in enumerate(queries):
+        fpp = _colpp(frame)
+        keep.append(fpp)
+        arr[i].frame = C.cast(fpp, C.POINTER(C.POINTER(GdfColumn)))
+        arr[i].n_frame = len(frame)
+        arr[i].filter_query_part = fq.encode() if fq else None
+        arr[i].project_query_part = pq.encode() if pq else None
+        arr[i].scalar_mode = scalar_mode
+    return arr
+
+
+class Server:
+    """Request loop: HBM-budget admission, continuous batching over n_streams."""
+
+    def __init__(self, hbm_budget_bytes, max_batch=64, n_streams=4, r=4):
+        self.h = C.c_void_p(0)
+        self._keep = {}
+        _call(lib().blz_server_create, C.byref(self.h), int(hbm_budget_bytes), max_batch, n_streams, r)
+
+    def submit(self, frame, filter_query_part, project_query_part, scalar_mode=SCALAR_INTENDED):
+        keep = []
+        arr = _queries([(frame, filter_query_part, project_query_part)], scalar_mode, keep)
+        tk = C.c_int64(0)
+        _call(lib().blz_server_submit, self.h, arr, C.byref(tk))
+        self._keep[tk.value] = (keep, list(frame))
+        return tk.value
+
+    def wait(self, ticket, max_out=64):
+        """(rows kept, [DeviceColumn views], [null counts], latency ms)."""
+        kept = C.c_int64(0)
+        nulls = (C.c_int32 * max_out)()
+        out = (GdfColumn * max_out)()
+        n_out = C.c_int(0)
+        lat = C.c_double(0)
+        _call(lib().blz_server_wait, self.h, ticket, C.byref(kept), nulls, out, max_out, C.byref(n_out),
+              C.byref(lat))
+        cols = [DeviceColumn.adopt(out[i], owned=False) for i in range(n_out.value)]
+        return kept.value, cols, [nulls[i] for i in range(n_out.value)], lat.value
+
+    def release(self, ticket):
+        lib().blz_server_release(self.h, ticket)
+        self._keep.pop(ticket, None)
+
+    def replay(self, queries, arrival_s, scalar_mode=SCALAR_INTENDED):
+        """Submits queries at the given offsets (seconds); returns (latency ms array,
+        rows kept array, wall seconds)."""
+        n = len(queries)
+        keep = []
+        arr = _queries(queries, scalar_mode, keep)
+        arr_t = (C.c_double * n)(*[float(x) for x in arrival_s])
+        lat = (C.c_double * n)()
+        kept = (C.c_int64 * n)()
+        wall = C.c_double(0)
+        _call(lib().blz_server_replay, self.h, arr, arr_t, n, lat, kept, C.byref(wall))
+        return np.array(lat[:]), np.array(kept[:]), wall.value
+
+    def stats(self):
+        nb, pk, used = C.c_int64(0), C.c_int64(0), C.c_size_t(0)
+        lib().blz_server_stats(self.h, C.byref(nb), C.byref(pk), C.byref(used))
+        return {"batches": nb.value, "peak_inflight": pk.value, "used_bytes": used.value}
+
+    def close(self):
+        if self.h.value:
+            lib().blz_server_destroy(self.h)
+            self.h = C.c_void_p(0)
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def synth_column(col: "DeviceColumn", seed: int, dist: int, a: float, b: float, null_rate=0.0, stream=None):
+    """Fills an allocated device column with seeded synthetic values in place."""
+    _call(lib().blz_synth_column, C.byref(col.col), seed, dist, a, b, null_rate, stream)

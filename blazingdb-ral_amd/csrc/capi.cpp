@@ -10,6 +10,11 @@
 
 #include <algorithm>
 #include <cstring>
+#include <unordered_map>
+#include <thread>
+#include <deque>
+#include <condition_variable>
+#include <chrono>
 #include <map>
 #include <mutex>
 #include <tuple>
@@ -106,8 +111,7 @@ int vm_per_cu(const Compiled &c, int mode, int lds) {
 int interp_grid(const Compiled &c, bool compact, int64_t n) {
   const int64_t tiles = (n + tile_rows(c) - 1) / tile_rows(c);
   const int per_cu = c.vm ? vm_per_cu(c, compact ? 1 : 0, c.lds_bytes)
-                     : c.acc ? acc_blocks_per_cu(c.S, c.R, compact, c.lds_bytes)
-                             : interp_blocks_per_cu(c.S, c.R, compact, c.lds_bytes);
+                     : interp_blocks_per_cu(c.S, c.R, compact, c.lds_bytes);
   const int64_t resident = (int64_t)device_cus() * per_cu;
   return (int)std::max<int64_t>(1, std::min<int64_t>(tiles, resident));
 }
@@ -235,8 +239,7 @@ hipError_t launch_program(const Compiled &c, bool compact, int grid, hipStream_t
       e = launch_vm(c.R, 2, c.vm_slow, grid_for(2, c.lds_gather, (spans + 3) / 4), c.lds_gather, st, bd);
     return e == hipSuccess ? mark(3) : e;
   }
-  if (c.acc) e = launch_acc(c.S, c.R, compact, grid, c.lds_bytes, st, blob, ws);
-  else e = launch_interp(c.S, c.R, compact, grid, c.lds_bytes, st, blob, ws);
+  e = launch_interp(c.S, c.R, compact, grid, c.lds_bytes, st, blob, ws);
   return e == hipSuccess ? mark(1) : e;
 }
 
@@ -310,23 +313,26 @@ void check_frame(gdf_column *const *frame, int n_frame) {
   for (int i = 0; i < n_frame; i++) check_column(frame[i], "frame column", n);
 }
 
-void *dalloc(size_t bytes) {
+// Device allocation; stream-ordered (no implicit device synchronisation) when
+// `async`, for callers that must not wait for other streams' work.
+void *dalloc(size_t bytes, hipStream_t st = nullptr, bool async = false) {
   if (bytes == 0) return nullptr;
   void *p = nullptr;
-  HIPCHECK(hipMalloc(&p, bytes));
+  if (async) HIPCHECK(hipMallocAsync(&p, bytes, st));
+  else HIPCHECK(hipMalloc(&p, bytes));
   return p;
 }
 
 // Allocates a result column of capacity n (validity zeroed: compaction ORs words).
-gdf_column alloc_column(int dtype, int64_t n, bool with_valid, hipStream_t st) {
+gdf_column alloc_column(int dtype, int64_t n, bool with_valid, hipStream_t st, bool async = false) {
   gdf_column c{};
   c.dtype = (gdf_dtype)dtype;
   const int w = blz_dtype_width((gdf_dtype)dtype);
   if (w == 0) throw BlzError(BLZ_ERR_UNSUPPORTED, "dtype without a fixed width is not supported");
-  c.data = dalloc(round_up((size_t)n * w, 64));
+  c.data = dalloc(round_up((size_t)n * w, 64), st, async);
   if (with_valid && n > 0) {
     const size_t vb = gdf_valid_allocation_size((gdf_size_type)n);
-    c.valid = (gdf_valid_type *)dalloc(vb);
+    c.valid = (gdf_valid_type *)dalloc(vb, st, async);
     HIPCHECK(launch_fill_zero(c.valid, (int64_t)vb, (int)std::min<size_t>(1024, vb / 2048 + 1), st));
   }
   c.size = (gdf_size_type)n;
@@ -689,13 +695,17 @@ struct blz_program_s {
   std::vector<gdf_column> outs;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   hipEvent_t evs[4] = {nullptr, nullptr, nullptr, nullptr};
+  std::vector<uint8_t> img;  // host image of the blob + launch descriptor
+  bool async_mem = false;    // device memory is stream-ordered on mem_st
+  hipStream_t mem_st = nullptr;
 };
 
 namespace {
 // Compiles and allocates one program (throws on error); blz_program_create and
 // the batch share it.
 blz_program make_program(gdf_column *const *frame, int n_frame, const char *filter_qp, const char *proj_qp,
-                         int scalar_mode, int kernel, int s, int r, int t) {
+                         int scalar_mode, int kernel, int s, int r, int t, hipStream_t st = nullptr,
+                         bool sync = true) {
   check_frame(frame, n_frame);
   const int64_t n = frame[0]->size;
   const auto dt = frame_dtypes(frame, n_frame);
@@ -729,11 +739,13 @@ blz_program make_program(gdf_column *const *frame, int n_frame, const char *filt
     for (size_t i = 0; i < outs.size(); i++) {
       const auto &o = outs[i];
       const bool with_valid = o.kind == 0 ? true : frame[o.index]->valid != nullptr;
-      gdf_column c = alloc_column(o.dtype, n, with_valid, nullptr);
+      gdf_column c = alloc_column(o.dtype, n, with_valid, st, !sync);
       p->outs.push_back(c);
       if (o.kind == 0) req.outputs.push_back({c.data, c.valid, o.dtype, 0, o.index});
       else req.outputs.push_back({c.data, c.valid, o.dtype, 1, im.use(frame, o.index)});
     }
+    p->async_mem = !sync;
+    p->mem_st = st;
     req.inputs = im.inputs;
     if (has_filter) {
       if (fp.bare) req.bare_stencil_input = im.frame_to_dev[fp.bare_col];
@@ -747,14 +759,15 @@ blz_program make_program(gdf_column *const *frame, int n_frame, const char *filt
     p->n_out = (int)outs.size();
     p->grid = interp_grid(p->c, has_filter, std::max<int64_t>(n, 1));
     layout(p->lb, p->c, p->n_out, has_filter, n);
-    HIPCHECK(hipMalloc((void **)&p->lb.base, p->lb.bytes));
+    p->lb.base = (uint8_t *)dalloc(p->lb.bytes, st, !sync);
     bind(p->lb, p->n_out);
-    const std::vector<uint8_t> img = device_image(p->lb, p->c);
-    HIPCHECK(hipMemcpy(p->lb.blob, img.data(), img.size(), hipMemcpyHostToDevice));
+    p->img = device_image(p->lb, p->c);
+    if (sync) HIPCHECK(hipMemcpy(p->lb.blob, p->img.data(), p->img.size(), hipMemcpyHostToDevice));
+    else HIPCHECK(hipMemcpyAsync(p->lb.blob, p->img.data(), p->img.size(), hipMemcpyHostToDevice, st));
     HIPCHECK(hipEventCreate(&p->ev0));
     HIPCHECK(hipEventCreate(&p->ev1));
     for (auto &e : p->evs) HIPCHECK(hipEventCreate(&e));
-    HIPCHECK(hipDeviceSynchronize());
+    if (sync) HIPCHECK(hipDeviceSynchronize());
   } catch (...) {
     blz_program_destroy(p);
     throw;
@@ -879,13 +892,23 @@ int blz_program_tile(blz_program prog, int *tile_rows, int *lds_bytes, int *kern
   if (!prog) return BLZ_ERR_INVALID;
   if (tile_rows) *tile_rows = prog->c.T;
   if (lds_bytes) *lds_bytes = prog->c.lds_bytes;
-  if (kernel) *kernel = prog->c.vm ? 4 : prog->c.acc ? 1 : prog->c.T > 0 ? 2 : 3;
+  if (kernel) *kernel = prog->c.vm ? 4 : 3;
   return BLZ_OK;
 }
 
 int blz_program_destroy(blz_program prog) {
   if (!prog) return BLZ_OK;
-  (void)hipDeviceSynchronize();
+  if (prog->async_mem) {  // stream-ordered: freed after the stream's earlier work, no device-wide wait
+    for (auto &c : prog->outs) {
+      if (c.data) (void)hipFreeAsync(c.data, prog->mem_st);
+      if (c.valid) (void)hipFreeAsync(c.valid, prog->mem_st);
+    }
+    if (prog->lb.base) (void)hipFreeAsync(prog->lb.base, prog->mem_st);
+    prog->outs.clear();
+    prog->lb.base = nullptr;
+  } else {
+    (void)hipDeviceSynchronize();
+  }
   for (auto &c : prog->outs) { (void)hipFree(c.data); (void)hipFree(c.valid); }
   if (prog->lb.base) (void)hipFree(prog->lb.base);
   if (prog->ev0) (void)hipEventDestroy(prog->ev0);
@@ -1013,6 +1036,9 @@ struct blz_batch_s {
   hipStream_t cap = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   int tuned = 0;
+  std::vector<uint8_t> img;  // host image of the descriptors
+  bool async_mem = false;
+  hipStream_t mem_st = nullptr;
 };
 
 namespace {
@@ -1038,7 +1064,12 @@ hipError_t batch_enqueue(blz_batch b, hipStream_t st) {
 
 void batch_free(blz_batch b) {
   if (!b) return;
-  (void)hipDeviceSynchronize();
+  if (b->async_mem) {
+    if (b->d_mem) (void)hipFreeAsync(b->d_mem, b->mem_st);
+    b->d_mem = nullptr;
+  } else {
+    (void)hipDeviceSynchronize();
+  }
   if (b->exec) (void)hipGraphExecDestroy(b->exec);
   if (b->graph) (void)hipGraphDestroy(b->graph);
   if (b->cap) (void)hipStreamDestroy(b->cap);
@@ -1050,7 +1081,8 @@ void batch_free(blz_batch b) {
 }
 
 // Builds one batch of fixed shape (r, t). Throws on error.
-blz_batch batch_build(const blz_query *qs, int n, int r, int t, bool use_graph) {
+blz_batch batch_build(const blz_query *qs, int n, int r, int t, bool use_graph, hipStream_t st = nullptr,
+                      bool sync = true) {
   auto *b = new blz_batch_s();
   try {
     b->r = r;
@@ -1059,7 +1091,8 @@ blz_batch batch_build(const blz_query *qs, int n, int r, int t, bool use_graph) 
       const blz_query &q = qs[i];
       blz_program p = nullptr;
       try {
-        p = make_program(q.frame, q.n_frame, q.filter_query_part, q.project_query_part, q.scalar_mode, 4, 0, r, t);
+        p = make_program(q.frame, q.n_frame, q.filter_query_part, q.project_query_part, q.scalar_mode, 4, 0, r, t,
+                         st, sync);
       } catch (const BlzError &e) {
         throw;  // device or argument errors
       } catch (const std::exception &e) {
@@ -1070,7 +1103,7 @@ blz_batch batch_build(const blz_query *qs, int n, int r, int t, bool use_graph) 
         (p->compact ? b->filt : b->proj).members.push_back(i);
       } else {  // a plan outside the tile VM's limits: its own launch inside the batch
         b->progs.push_back(make_program(q.frame, q.n_frame, q.filter_query_part, q.project_query_part,
-                                        q.scalar_mode, 0, 0, 0, 0));
+                                        q.scalar_mode, 0, 0, 0, 0, st, sync));
         b->others.push_back(i);
       }
     }
@@ -1155,19 +1188,23 @@ blz_batch batch_build(const blz_query *qs, int n, int r, int t, bool use_graph) 
     const size_t ptr_off = ctr_off + 256;
     const size_t bytes_off = round_up(ptr_off + sizeof(void *) * (rp.size() + 1), 256);
     const size_t total = round_up(bytes_off + sizeof(int64_t) * (rb.size() + 1), 256);
-    HIPCHECK(hipMalloc((void **)&b->d_mem, total));
+    b->d_mem = (uint8_t *)dalloc(total, st, !sync);
+    b->async_mem = !sync;
+    b->mem_st = st;
     b->d_counter = (unsigned int *)(b->d_mem + ctr_off);
     rp.push_back(b->d_counter);
     rb.push_back(8);
     b->n_ranges = (int)rp.size();
     b->d_range_ptrs = (void **)(b->d_mem + ptr_off);
     b->d_range_bytes = (int64_t *)(b->d_mem + bytes_off);
-    std::vector<uint8_t> img(total, 0);
+    std::vector<uint8_t> &img = b->img;
+    img.assign(total, 0);
     build_sub(b->proj, img, 0);
     build_sub(b->filt, img, proj_sz);
     std::memcpy(img.data() + ptr_off, rp.data(), sizeof(void *) * rp.size());
     std::memcpy(img.data() + bytes_off, rb.data(), sizeof(int64_t) * rb.size());
-    HIPCHECK(hipMemcpy(b->d_mem, img.data(), total, hipMemcpyHostToDevice));
+    if (sync) HIPCHECK(hipMemcpy(b->d_mem, img.data(), total, hipMemcpyHostToDevice));
+    else HIPCHECK(hipMemcpyAsync(b->d_mem, img.data(), total, hipMemcpyHostToDevice, st));
     HIPCHECK(hipEventCreate(&b->ev0));
     HIPCHECK(hipEventCreate(&b->ev1));
     if (use_graph) {
@@ -1183,7 +1220,7 @@ blz_batch batch_build(const blz_query *qs, int n, int r, int t, bool use_graph) 
       HIPCHECK(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
       b->use_graph = true;
     }
-    HIPCHECK(hipDeviceSynchronize());
+    if (sync) HIPCHECK(hipDeviceSynchronize());
   } catch (...) {
     batch_free(b);
     throw;
@@ -1301,5 +1338,371 @@ int blz_batch_info(blz_batch batch, int *r, int *t, int *n_queries, int *graph, 
 
 int blz_batch_destroy(blz_batch batch) {
   batch_free(batch);
+  return BLZ_OK;
+}
+
+int blz_synth_column(gdf_column *column, uint64_t seed, int dist, double a, double b, double null_rate,
+                     blz_stream_t stream, char *err, size_t err_len) {
+  return guarded(err, err_len, [&] {
+    if (!column || column->size < 0) invalid("bad column");
+    if (column->size == 0) return;
+    if (blz_dtype_width(column->dtype) == 0 || column->dtype == GDF_BOOL8 || column->dtype > GDF_FLOAT64)
+      invalid("unsupported dtype");
+    if (column->valid == nullptr && null_rate > 0) invalid("null_rate > 0 needs a validity buffer");
+    HIPCHECK(launch_synth(column->data, null_rate > 0 ? column->valid : nullptr, column->size, column->dtype, seed,
+                          dist, a, b, null_rate, S(stream)));
+  });
+}
+
+// ---------------------------------------------------------------------------
+// Serving: a request loop over device-resident tables. Queries are admitted
+// against an HBM budget (their outputs and workspace), batched as they arrive and
+// launched on one of several streams, so batches admitted while others run
+// overlap on the device (continuous batching). A query's outputs stay resident
+// until it is released.
+
+namespace {
+
+double now_s() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+struct SrvBatch {
+  blz_batch b = nullptr;
+  hipEvent_t done = nullptr;
+  hipStream_t st = nullptr;
+  std::vector<int64_t> tickets;
+  int live = 0;          // queries not yet released
+  size_t ws_bytes = 0;   // budget held for workspace until the batch is freed
+};
+
+struct SrvQuery {
+  std::vector<gdf_column *> frame;
+  std::string filt, proj;
+  bool has_filt = false, has_proj = false;
+  int scalar_mode = 0;
+  double t_submit = 0, t_done = 0;
+  int state = 0;  // 0 pending, 1 running, 2 done, 3 failed
+  size_t out_bytes = 0, ws_bytes = 0;
+  std::string err;
+  SrvBatch *batch = nullptr;
+  int index = -1;
+};
+
+}  // namespace
+
+struct blz_server_s {
+  std::mutex mu;
+  std::condition_variable cv_work, cv_done;
+  std::deque<int64_t> pending;
+  std::unordered_map<int64_t, SrvQuery> qs;
+  std::vector<SrvBatch *> inflight, held;
+  size_t budget = 0, used = 0;
+  int max_batch = 64, n_streams = 4, r = 4, next_stream = 0, device = 0;
+  std::vector<hipStream_t> streams;
+  std::thread worker;
+  bool stop = false;
+  int64_t next_ticket = 1;
+  int64_t n_batches = 0, peak_inflight = 0;
+  std::string fatal;
+};
+
+namespace {
+
+void srv_free_batch(blz_server s, SrvBatch *sb) {
+  batch_free(sb->b);
+  if (sb->done) (void)hipEventDestroy(sb->done);
+  s->used -= std::min(s->used, sb->ws_bytes);
+  s->held.erase(std::remove(s->held.begin(), s->held.end(), sb), s->held.end());
+  delete sb;
+}
+
+void srv_loop(blz_server s) {
+  (void)hipSetDevice(s->device);
+  std::unique_lock<std::mutex> lk(s->mu);
+  while (!s->stop) {
+    // completions
+    bool any_done = false;
+    for (size_t i = 0; i < s->inflight.size();) {
+      SrvBatch *sb = s->inflight[i];
+      const hipError_t q = hipEventQuery(sb->done);
+      if (q == hipErrorNotReady) { i++; continue; }
+      const double t = now_s();
+      for (int64_t tk : sb->tickets) {
+        SrvQuery &sq = s->qs[tk];
+        sq.t_done = t;
+        sq.state = q == hipSuccess ? 2 : 3;
+        if (q != hipSuccess) sq.err = hipGetErrorString(q);
+      }
+      s->held.push_back(sb);
+      s->inflight.erase(s->inflight.begin() + i);
+      any_done = true;
+    }
+    if (any_done) s->cv_done.notify_all();
+    // admission: FIFO while the budget and a stream slot allow
+    if (!s->pending.empty() && (int)s->inflight.size() < s->n_streams) {
+      std::vector<int64_t> take;
+      size_t add = 0;
+      while (!s->pending.empty() && (int)take.size() < s->max_batch) {
+        const SrvQuery &sq = s->qs[s->pending.front()];
+        const size_t need = sq.out_bytes + sq.ws_bytes;
+        const bool alone = s->inflight.empty() && s->held.empty() && take.empty();
+        if (s->used + add + need > s->budget && !alone) break;
+        add += need;
+        take.push_back(s->pending.front());
+        s->pending.pop_front();
+      }
+      if (!take.empty()) {
+        s->used += add;
+        std::vector<blz_query> bq;
+        std::vector<std::string *> keep;
+        for (int64_t tk : take) {
+          SrvQuery &sq = s->qs[tk];
+          sq.state = 1;
+          blz_query q{};
+          q.frame = sq.frame.data();
+          q.n_frame = (int)sq.frame.size();
+          q.filter_query_part = sq.has_filt ? sq.filt.c_str() : nullptr;
+          q.project_query_part = sq.has_proj ? sq.proj.c_str() : nullptr;
+          q.scalar_mode = sq.scalar_mode;
+          bq.push_back(q);
+        }
+        hipStream_t st = s->streams[s->next_stream++ % s->streams.size()];
+        lk.unlock();
+        auto *sb = new SrvBatch();
+        sb->st = st;
+        sb->tickets = take;
+        sb->live = (int)take.size();
+        std::string err;
+        try {
+          sb->b = batch_build(bq.data(), (int)bq.size(), s->r, 256 * s->r, false, st, false);
+          HIPCHECK(batch_enqueue(sb->b, st));
+          HIPCHECK(hipEventCreateWithFlags(&sb->done, hipEventDisableTiming));
+          HIPCHECK(hipEventRecord(sb->done, st));
+        } catch (const std::exception &e) {
+          err = e.what();
+        }
+        lk.lock();
+        if (!err.empty()) {
+          for (int64_t tk : take) {
+            s->qs[tk].state = 3;
+            s->qs[tk].err = err;
+            s->qs[tk].t_done = now_s();
+          }
+          s->used -= std::min(s->used, add);
+          if (sb->b) batch_free(sb->b);
+          if (sb->done) (void)hipEventDestroy(sb->done);
+          delete sb;
+          s->cv_done.notify_all();
+        } else {
+          for (size_t i = 0; i < take.size(); i++) {
+            SrvQuery &sq = s->qs[take[i]];
+            sq.batch = sb;
+            sq.index = (int)i;
+            sb->ws_bytes += sq.ws_bytes;
+          }
+          s->inflight.push_back(sb);
+          s->n_batches++;
+          s->peak_inflight = std::max<int64_t>(s->peak_inflight, (int64_t)s->inflight.size());
+        }
+        continue;
+      }
+    }
+    // nothing admitted: sleep until a submission or a release (budget freed), or
+    // poll the running batches
+    if (s->inflight.empty()) s->cv_work.wait_for(lk, std::chrono::milliseconds(2));
+    else s->cv_work.wait_for(lk, std::chrono::microseconds(20));
+  }
+}
+
+size_t srv_estimate(gdf_column *const *frame, int n_frame, const char *filt, const char *proj, size_t *ws) {
+  const int64_t n = frame[0]->size;
+  const auto dt = frame_dtypes(frame, n_frame);
+  size_t out = 0;
+  if (proj) {
+    const auto pp = planner::parse_project(dt, proj);
+    for (auto &o : pp.outputs) out += (size_t)n * std::max(1, blz_dtype_width((gdf_dtype)o.dtype)) + n / 8 + 64;
+  } else {
+    for (int i = 0; i < n_frame; i++) out += (size_t)n * std::max(1, blz_dtype_width((gdf_dtype)dt[i])) + n / 8 + 64;
+  }
+  if (filt) (void)planner::plan_expression(dt, condition_of(filt));
+  *ws = (size_t)n / 8 + (size_t)n / 64 + 64 * 1024;
+  return out;
+}
+
+}  // namespace
+
+int blz_server_create(blz_server *server, size_t hbm_budget_bytes, int max_batch, int n_streams, int r, char *err,
+                      size_t err_len) {
+  return guarded(err, err_len, [&] {
+    if (!server || max_batch <= 0 || n_streams <= 0) invalid("bad arguments");
+    if (r != 2 && r != 4 && r != 8) invalid("r must be 2, 4 or 8");
+    auto *s = new blz_server_s();
+    s->budget = hbm_budget_bytes;
+    s->max_batch = max_batch;
+    s->n_streams = n_streams;
+    s->r = r;
+    HIPCHECK(hipGetDevice(&s->device));
+    for (int i = 0; i < n_streams; i++) {
+      hipStream_t st;
+      HIPCHECK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+      s->streams.push_back(st);
+    }
+    s->worker = std::thread(srv_loop, s);
+    *server = s;
+  });
+}
+
+int blz_server_submit(blz_server s, const blz_query *q, int64_t *ticket, char *err, size_t err_len) {
+  return guarded(err, err_len, [&] {
+    if (!s || !q || !ticket) invalid("bad arguments");
+    check_frame(q->frame, q->n_frame);
+    SrvQuery sq;
+    sq.frame.assign(q->frame, q->frame + q->n_frame);
+    sq.has_filt = q->filter_query_part != nullptr;
+    sq.has_proj = q->project_query_part != nullptr;
+    if (sq.has_filt) sq.filt = q->filter_query_part;
+    if (sq.has_proj) sq.proj = q->project_query_part;
+    sq.scalar_mode = q->scalar_mode;
+    sq.out_bytes = srv_estimate(q->frame, q->n_frame, q->filter_query_part, q->project_query_part, &sq.ws_bytes);
+    std::lock_guard<std::mutex> g(s->mu);
+    sq.t_submit = now_s();
+    const int64_t tk = s->next_ticket++;
+    s->qs.emplace(tk, std::move(sq));
+    s->pending.push_back(tk);
+    *ticket = tk;
+    s->cv_work.notify_one();
+  });
+}
+
+int blz_server_wait(blz_server s, int64_t ticket, int64_t *n_kept, int32_t *null_counts, gdf_column *out_columns,
+                    int max_out, int *n_out, double *latency_ms, char *err, size_t err_len) {
+  return guarded(err, err_len, [&] {
+    if (!s) invalid("server is NULL");
+    SrvBatch *sb = nullptr;
+    int idx = -1;
+    {
+      std::unique_lock<std::mutex> lk(s->mu);
+      auto it = s->qs.find(ticket);
+      if (it == s->qs.end()) invalid("unknown ticket");
+      s->cv_done.wait(lk, [&] { return it->second.state >= 2; });
+      if (it->second.state == 3) throw BlzError(BLZ_ERR_HIP, "query failed: " + it->second.err);
+      if (latency_ms) *latency_ms = (it->second.t_done - it->second.t_submit) * 1e3;
+      sb = it->second.batch;
+      idx = it->second.index;
+    }
+    char e2[256];
+    const int rc = blz_batch_result(sb->b, sb->st, idx, n_kept, null_counts, out_columns, max_out, n_out, e2,
+                                    sizeof(e2));
+    if (rc != BLZ_OK) throw BlzError(rc, e2);
+  });
+}
+
+int blz_server_release(blz_server s, int64_t ticket) {
+  if (!s) return BLZ_ERR_INVALID;
+  std::lock_guard<std::mutex> g(s->mu);
+  auto it = s->qs.find(ticket);
+  if (it == s->qs.end() || it->second.state < 2) return BLZ_ERR_INVALID;
+  SrvBatch *sb = it->second.batch;
+  s->used -= std::min(s->used, it->second.out_bytes);
+  if (sb && it->second.index >= 0) {
+    blz_program p = sb->b->progs[it->second.index];
+    for (auto &c : p->outs) {
+      if (c.data) (void)hipFreeAsync(c.data, sb->st);
+      if (c.valid) (void)hipFreeAsync(c.valid, sb->st);
+      c.data = nullptr;
+      c.valid = nullptr;
+    }
+    if (--sb->live == 0) srv_free_batch(s, sb);
+  }
+  s->qs.erase(it);
+  s->cv_work.notify_one();
+  return BLZ_OK;
+}
+
+int blz_server_replay(blz_server s, const blz_query *queries, const double *arrival_s, int n, double *latency_ms,
+                      int64_t *n_kept, double *wall_s, char *err, size_t err_len) {
+  return guarded(err, err_len, [&] {
+    if (!s || !queries || !arrival_s || n <= 0) invalid("bad arguments");
+    std::vector<int64_t> tickets(n, 0);
+    std::thread waiter;
+    std::string werr;
+    std::mutex tm;
+    std::condition_variable tcv;
+    int submitted = 0;
+    const double t0 = now_s();
+    waiter = std::thread([&] {
+      for (int i = 0; i < n; i++) {
+        int64_t tk;
+        {
+          std::unique_lock<std::mutex> lk(tm);
+          tcv.wait(lk, [&] { return submitted > i; });
+          tk = tickets[i];
+        }
+        if (tk == 0) continue;
+        char e2[256];
+        int64_t kept = 0;
+        double lat = 0;
+        if (blz_server_wait(s, tk, &kept, nullptr, nullptr, 64, nullptr, &lat, e2, sizeof(e2)) != BLZ_OK) {
+          if (werr.empty()) werr = e2;
+        }
+        if (latency_ms) latency_ms[i] = lat;
+        if (n_kept) n_kept[i] = kept;
+        blz_server_release(s, tk);
+      }
+    });
+    for (int i = 0; i < n; i++) {
+      const double due = t0 + arrival_s[i];
+      while (now_s() < due) {
+        const double left = due - now_s();
+        if (left > 200e-6) std::this_thread::sleep_for(std::chrono::microseconds((int64_t)((left - 100e-6) * 1e6)));
+      }
+      int64_t tk = 0;
+      char e2[256];
+      const int rc = blz_server_submit(s, &queries[i], &tk, e2, sizeof(e2));
+      {
+        std::lock_guard<std::mutex> lk(tm);
+        tickets[i] = rc == BLZ_OK ? tk : 0;
+        if (rc != BLZ_OK && werr.empty()) werr = e2;
+        submitted = i + 1;
+      }
+      tcv.notify_all();
+    }
+    waiter.join();
+    if (wall_s) *wall_s = now_s() - t0;
+    if (!werr.empty()) invalid("replay: " + werr);
+  });
+}
+
+int blz_server_stats(blz_server s, int64_t *n_batches, int64_t *peak_inflight, size_t *used_bytes) {
+  if (!s) return BLZ_ERR_INVALID;
+  std::lock_guard<std::mutex> g(s->mu);
+  if (n_batches) *n_batches = s->n_batches;
+  if (peak_inflight) *peak_inflight = s->peak_inflight;
+  if (used_bytes) *used_bytes = s->used;
+  return BLZ_OK;
+}
+
+int blz_server_destroy(blz_server s) {
+  if (!s) return BLZ_OK;
+  {
+    std::lock_guard<std::mutex> g(s->mu);
+    s->stop = true;
+    s->cv_work.notify_all();
+  }
+  if (s->worker.joinable()) s->worker.join();
+  (void)hipDeviceSynchronize();
+  for (auto sb : s->inflight) {
+    batch_free(sb->b);
+    if (sb->done) (void)hipEventDestroy(sb->done);
+    delete sb;
+  }
+  for (auto sb : s->held) {
+    batch_free(sb->b);
+    if (sb->done) (void)hipEventDestroy(sb->done);
+    delete sb;
+  }
+  for (auto st : s->streams) (void)hipStreamDestroy(st);
+  delete s;
   return BLZ_OK;
 }

@@ -21,17 +21,13 @@ struct Workspace {
   uint32_t *wave_prefix;            // exclusive prefix of wave_count
 };
 
-// Register-file interpreter (project or fused filter+project/gather).
-// (s, r) in {(8,1), (8,2), (16,1), (32,1), (64,1)}: s slots, r rows per lane.
-// lds_bytes > 0: the LDS-staged kernel (shapes (8,1), (8,2), (16,1)) with that much
-// dynamic LDS per workgroup; 0: the direct kernel.
+// Register-file interpreter (project or fused filter+project/gather), the general
+// fallback for plans outside the tile VM's limits.
+// (s, r) in {(8,1), (8,2), (16,1), (32,1), (64,1)}: s slots, r rows per lane;
+// lds_bytes must be 0.
 hipError_t launch_interp(int s, int r, bool compact, int grid, int lds_bytes, hipStream_t st,
                          const uint8_t *blob, Workspace ws);
 int interp_blocks_per_cu(int s, int r, bool compact, int lds_bytes);
-// Accumulator-form staged kernel: (nt, r) in {(2,8), (4,4), (8,2)} temporaries x rows per lane.
-hipError_t launch_acc(int nt, int r, bool compact, int grid, int lds_bytes, hipStream_t st,
-                      const uint8_t *blob, Workspace ws);
-int acc_blocks_per_cu(int nt, int r, bool compact, int lds_bytes);
 // Tile VM launch descriptor (device memory): one compiled program, or a batch of
 // independent programs processed by one persistent launch. Global tile / span /
 // scan-chunk ids run over the programs in order; each program keeps its own
@@ -68,6 +64,10 @@ int vm_blocks_per_cu(int r, int mode, bool slow, int lds_bytes);
 // single-pass decoupled look-back over its ws.tile_state (zero before the scan).
 hipError_t launch_slot_scan(const VmBatch *bd, int64_t n_chunks, hipStream_t st);
 constexpr int64_t kScanChunk = 4096;  // slots per scan chunk
+// Seeded synthetic column in place (dist: 0 int uniform [a,b], 1 real uniform [a,b),
+// 2 normal(a,b), 3 lognormal(a,b), 4 k/100 for k in [a,b], 5 a + row).
+hipError_t launch_synth(void *data, uint8_t *valid, int64_t n, int dtype, uint64_t seed, int dist, double a,
+                        double b, double null_rate, hipStream_t st);
 // Zeroes `n` byte ranges (dst[i], bytes[i]; 8-byte multiples, 8-byte aligned) in
 // one launch.
 hipError_t launch_fill_ranges(void *const *dst, const int64_t *bytes, int n, int grid, hipStream_t st);

@@ -1112,6 +1112,7 @@ Compiled compile_program(const CompileRequest &req) {
     }
   }
 
+  if (req.kernel == 1 || req.kernel == 2) fail("kernel kind not available (1 and 2 are superseded by 4)");
   // ---- Tile VM: the accumulator form, pre-decoded, over double-buffered tiles ----
   if (req.kernel == 4 && !acc_ok) fail("plan does not fit the tile VM");
   if (acc_ok && (req.kernel == 0 || req.kernel == 4)) {
@@ -1126,8 +1127,7 @@ Compiled compile_program(const CompileRequest &req) {
   const int need = std::max(n_slots, n_vslots);
   int S = need <= 8 ? 8 : need <= 16 ? 16 : need <= 32 ? 32 : 64;
   int R = S == 8 ? 2 : 1;
-  if (req.force_s > 0 && req.kernel != 1) { S = req.force_s; R = req.force_r > 0 ? req.force_r : 1; }
-  if (req.kernel == 1 && !acc_ok) fail("plan does not fit the accumulator kernel");
+  if (req.force_s > 0) { S = req.force_s; R = req.force_r > 0 ? req.force_r : 1; }
 
   // Device ops.
   auto emit = [&](const std::vector<int> &ord, std::vector<DevOp> &dst) {
@@ -1150,94 +1150,11 @@ Compiled compile_program(const CompileRequest &req) {
   emit(order_p, ops_p);
   emit(order_q, ops_q);
 
-  // LDS staging layout (staged kernel): per tile of T rows, each input's values
-  // and each needed bitmap, 16-byte aligned, then the compacted-validity words.
+  // The direct kernel reads inputs straight from HBM: no staging layout.
   const int n_out = (int)req.outputs.size();
   int n_valid_out = 0;
   for (auto &o : req.outputs) n_valid_out += o.valid != nullptr;
-  std::vector<int> lds_off(n_dev, 0), lds_voff(n_dev, -1);
-  std::vector<Chunk> chunks;
-  int T = 0, lds_bytes = 0, vbits_off = 0;
-  bool layout_ok = req.force_t >= 0 && req.kernel != 3;
-  for (int c = 0; c < n_dev && layout_ok; c++) {
-    const int w = wid(req.inputs[c].dtype);
-    if (w == 0 && load_kind(req.inputs[c].dtype) != LD_NONE) layout_ok = false;
-    if ((uintptr_t)req.inputs[c].data % 16) layout_ok = false;
-    if (B.vnodes[c].needed && (uintptr_t)req.inputs[c].valid % 16) layout_ok = false;
-  }
-  auto layout_for = [&](int t) {
-    size_t off = 0;
-    for (int c = 0; c < n_dev; c++) {
-      lds_off[c] = (int)off;
-      off += (size_t)t * std::max(1, wid(req.inputs[c].dtype));
-      off = (off + 15) & ~size_t(15);
-    }
-    for (int c = 0; c < n_dev; c++) {
-      lds_voff[c] = -1;
-      if (B.vnodes[c].needed) {
-        lds_voff[c] = (int)off;
-        off += (size_t)t / 8;
-        off = (off + 15) & ~size_t(15);
-      }
-    }
-    vbits_off = (int)off;
-    off += (size_t)n_valid_out * (t / 64 + 2) * 8;
-    return off;
-  };
-  // Tile rows for rows-per-lane `r`: the forced size, or the largest multiple of
-  // 256 r (<= 4096) whose staging stays within ~52 KiB (three workgroups per CU).
-  auto pick_tile = [&](int r) {
-    const int unit = 256 * r;
-    if (req.force_t > 0) {
-      const int t = req.force_t;
-      return (t % unit || t > 4096 || (int)layout_for(t) > req.lds_budget) ? 0 : t;
-    }
-    for (int t = 4096; t >= unit; t /= 2)
-      if (t % unit == 0 && layout_for(t) <= (size_t)std::min(req.lds_budget, 52 * 1024)) return t;
-    return 0;
-  };
-  bool use_acc = false;
-  if (!layout_ok) acc_ok = false;
-  if (acc_ok) {
-    // Most rows per lane whose temporaries and tile fit: (NT, R) = (2,8), (4,4), (8,2).
-    const int shapes[3][2] = {{2, 8}, {4, 4}, {8, 2}};
-    for (auto &sh : shapes) {
-      if (acc_temps > sh[0]) continue;
-      if (req.kernel == 1 && req.force_s > 0 && (req.force_s != sh[0] || req.force_r != sh[1])) continue;
-      const int t = pick_tile(sh[1]);
-      if (t == 0) continue;
-      S = sh[0];
-      R = sh[1];
-      T = t;
-      use_acc = true;
-      break;
-    }
-    if (!use_acc && req.kernel == 1) fail("requested accumulator shape or tile does not fit the plan");
-  }
-  if (!use_acc && S < need) fail("register file too small for the plan");
-  bool staged_ok = layout_ok && (use_acc || ((S == 8 || S == 16) && S * R <= 16));
-  if (staged_ok) {
-    if (!use_acc) {
-      T = pick_tile(R);
-      if (T == 0 && req.force_t > 0) fail("bad tile size for this plan");
-    }
-    if (T > 0) {
-      lds_bytes = (int)layout_for(T);
-      for (int c = 0; c < n_dev; c++) {
-        const int w = std::max(1, wid(req.inputs[c].dtype));
-        const int sh = w == 1 ? 0 : w == 2 ? 1 : w == 4 ? 2 : 3;
-        const int slice = T * w;
-        for (int off = 0; off < slice; off += kChunkBytes)
-          chunks.push_back(Chunk{req.inputs[c].data, slice, off, lds_off[c] + off, sh});
-      }
-      for (int c = 0; c < n_dev; c++) {
-        if (lds_voff[c] < 0) continue;
-        const int slice = T / 8;
-        for (int off = 0; off < slice; off += kChunkBytes)
-          chunks.push_back(Chunk{req.inputs[c].valid, slice, off, lds_voff[c] + off, -3});
-      }
-    }
-  }
+  if (S < need) fail("register file too small for the plan");
 
   // Assemble the blob.
   size_t off_in = (sizeof(ProgHeader) + 15) & ~size_t(15);
@@ -1245,11 +1162,9 @@ Compiled compile_program(const CompileRequest &req) {
   off_out = (off_out + 15) & ~size_t(15);
   size_t off_ops = off_out + sizeof(OutCol) * n_out;
   off_ops = (off_ops + 15) & ~size_t(15);
-  if (use_acc && T == 0) use_acc = false;
-  const size_t n_ops_total = use_acc ? acc_p.size() + acc_q.size() : ops_p.size() + ops_q.size();
-  size_t off_chunks = off_ops + sizeof(DevOp) * n_ops_total;
-  off_chunks = (off_chunks + 15) & ~size_t(15);
-  size_t off_vops = off_chunks + sizeof(Chunk) * chunks.size();
+  const size_t n_ops_total = ops_p.size() + ops_q.size();
+  size_t off_vops = off_ops + sizeof(DevOp) * n_ops_total;
+  off_vops = (off_vops + 15) & ~size_t(15);
   size_t total = off_vops + sizeof(VOp) * (vops_p.size() + vops_q.size());
   total = (total + 15) & ~size_t(15);
   if ((int)total > blob_capacity()) fail("plan too large for the on-chip program buffer");
@@ -1258,10 +1173,10 @@ Compiled compile_program(const CompileRequest &req) {
   h.n_rows = req.n_rows;
   h.n_in = n_dev;
   h.n_out = n_out;
-  h.n_ops_p = (int)(use_acc ? acc_p.size() : ops_p.size());
-  h.n_ops_q = (int)(use_acc ? acc_q.size() : ops_q.size());
-  h.acc_mode = use_acc ? 1 : 0;
-  h.n_temps = acc_temps;
+  h.n_ops_p = (int)ops_p.size();
+  h.n_ops_q = (int)ops_q.size();
+  h.acc_mode = 0;
+  h.n_temps = 0;
   h.n_vops_p = (int)vops_p.size();
   h.n_vops_q = (int)vops_q.size();
   h.n_vin = n_dev;
@@ -1273,12 +1188,12 @@ Compiled compile_program(const CompileRequest &req) {
   int any_valid = 0;
   for (auto &o : req.outputs) any_valid |= o.valid != nullptr;
   h.any_valid_out = any_valid;
-  h.tile_rows = T;
-  h.n_chunks = (int)chunks.size();
-  h.lds_vbits_off = vbits_off;
+  h.tile_rows = 0;
+  h.n_chunks = 0;
+  h.lds_vbits_off = 0;
   h.n_valid_out = n_valid_out;
-  h.lds_bytes = lds_bytes;
-  h.off_chunks = (int)off_chunks;
+  h.lds_bytes = 0;
+  h.off_chunks = (int)off_vops;
   h.off_in = (int)off_in;
   h.off_out = (int)off_out;
   h.off_ops = (int)off_ops;
@@ -1291,11 +1206,10 @@ Compiled compile_program(const CompileRequest &req) {
     ic.valid = req.inputs[c].valid;
     ic.load = load_kind(req.inputs[c].dtype);
     ic.vslot = B.vnodes[c].needed ? c : -1;
-    ic.lds_off = lds_off[c];
-    ic.lds_voff = lds_voff[c];
+    ic.lds_off = 0;
+    ic.lds_voff = -1;
     put(out.blob, off_in + sizeof(InCol) * c, ic);
   }
-  for (size_t k = 0; k < chunks.size(); k++) put(out.blob, off_chunks + sizeof(Chunk) * k, chunks[k]);
   int vbuf_next = 0;
   for (int o = 0; o < n_out; o++) {
     const OutSpec &s = req.outputs[o];
@@ -1310,13 +1224,8 @@ Compiled compile_program(const CompileRequest &req) {
     put(out.blob, off_out + sizeof(OutCol) * o, oc);
   }
   size_t p = off_ops;
-  if (use_acc) {
-    for (auto &op : acc_p) { put(out.blob, p, op); p += sizeof(AccOp); }
-    for (auto &op : acc_q) { put(out.blob, p, op); p += sizeof(AccOp); }
-  } else {
-    for (auto &op : ops_p) { put(out.blob, p, op); p += sizeof(DevOp); }
-    for (auto &op : ops_q) { put(out.blob, p, op); p += sizeof(DevOp); }
-  }
+  for (auto &op : ops_p) { put(out.blob, p, op); p += sizeof(DevOp); }
+  for (auto &op : ops_q) { put(out.blob, p, op); p += sizeof(DevOp); }
   p = off_vops;
   for (auto &op : vops_p) { put(out.blob, p, op); p += sizeof(VOp); }
   for (auto &op : vops_q) { put(out.blob, p, op); p += sizeof(VOp); }
@@ -1326,10 +1235,9 @@ Compiled compile_program(const CompileRequest &req) {
   out.n_vslots = n_vslots;
   out.n_ops = (int)(ops_p.size() + ops_q.size());
   out.n_vops = (int)(vops_p.size() + vops_q.size());
-  out.T = T;
-  out.lds_bytes = lds_bytes;
-  out.acc = use_acc;
-  if (use_acc) out.n_ops = (int)n_ops_total;
+  out.T = 0;
+  out.lds_bytes = 0;
+  out.acc = false;
   return out;
 }
 

@@ -78,9 +78,9 @@ static_assert(sizeof(DevOp) == 24, "DevOp layout");
 
 constexpr uint8_t F_A_IMM = 1, F_B_IMM = 2;
 
-// Accumulator form (staged kernel): each op combines the accumulator with at most
-// one operand, so results need no write-back and inputs are read straight from the
-// LDS tile. Same 24-byte footprint as DevOp.
+// Accumulator form (the compiler's intermediate, lowered to VmOps): each op
+// combines the accumulator with at most one operand, so results need no
+// write-back and inputs are read straight from the LDS tile.
 enum AccCode : uint8_t {
   A_LOAD = 0,   // acc = operand
   A_STORE,      // temp[oidx] = acc
@@ -182,16 +182,6 @@ struct InCol {
   int32_t lds_voff;      // staged kernel: LDS offset of the tile's bitmap words (-1: none)
 };
 
-// Staged kernel: one 4 KiB piece of a tile's column (or bitmap) slice, copied
-// HBM -> LDS by the 256 threads of a workgroup, 16 bytes each.
-struct Chunk {
-  const void *base;      // column data / bitmap base
-  int32_t tile_bytes;    // bytes of the slice per full tile (T * width, or T / 8)
-  int32_t chunk_off;     // offset of this piece within the slice
-  int32_t lds_off;       // LDS destination of the piece
-  int32_t shift;         // log2(width) of a data slice; -3 for a bitmap slice
-};
-constexpr int kChunkBytes = 4096;
 
 // Output descriptor. `src_kind`: 0 = program slot (of the program that owns the
 // output), 1 = raw input column (a gathered alias).

@@ -828,7 +828,9 @@ vm_kernel(const VmBatch *__restrict__ bd_arg) {
               if (lane == u) vrf[vs] = word;
             }
           }
-          run_vops<VS>(cx.vops + cx.n_vops_p, cx.n_vops_q, vrf);
+          // The projection's validity program may reuse words the predicate's
+          // computed (shared subexpressions): run both.
+          run_vops<VS>(cx.vops, cx.n_vops_p + cx.n_vops_q, vrf);
           for (int o = 0; o < cx.n_out; o++) {
             if (cx.out[o].valid == nullptr) continue;
             const uint64_t vword = vsrc_word<VS>(cx.out[o].vsrc, vrf);

@@ -133,13 +133,11 @@ typedef struct blz_program_s *blz_program;
  * projection; project_query_part NULL keeps every frame column) against fixed
  * device buffers the program allocates and owns: out_columns[0..*n_out) receive
  * them (capacity: every frame row; sizes valid after blz_program_result).
- * kernel: 0 automatic, 1 accumulator form (LDS-staged), 2 LDS-staged register file,
- * 3 unstaged register file, 4 tile VM (double-buffered direct-to-LDS staging).
- * s / r (0 = automatic) force the shape: temporaries x rows per lane (2,8) (4,4)
- * (8,2) for kernel 1, (2,8) (4,8) (2,4) (4,4) (8,4) (2,2) ... as listed in
- * vm_kernel.hip for kernel 4, slots x rows (8,1) (8,2) (16,1) (32,1) (64,1)
- * otherwise; t (0 = automatic) forces the tile rows (a multiple of 256*r, <= 4096;
- * exactly 256*r for kernel 4). */
+ * kernel: 0 automatic (the tile VM when the plan fits it, else the direct kernel),
+ * 3 direct register-file kernel, 4 tile VM (double-buffered direct-to-LDS staging;
+ * a filter runs as predicate, slot-scan and gather launches). r / t (0 =
+ * automatic) force the tile VM's rows per lane (2, 4, 8) and tile rows (256 r P);
+ * s / r force the direct kernel's slots x rows (8,1) (8,2) (16,1) (32,1) (64,1). */
 int blz_program_create(blz_program *prog, gdf_column *const *frame, int n_frame,
                        const char *filter_query_part, const char *project_query_part,
                        gdf_column *out_columns, int max_out, int *n_out, int scalar_mode,
@@ -202,6 +200,41 @@ int blz_batch_time(blz_batch batch, blz_stream_t stream, int iters, double *mean
  * candidates timed. */
 int blz_batch_info(blz_batch batch, int *r, int *t, int *n_queries, int *graph, int *tuned_candidates);
 int blz_batch_destroy(blz_batch batch);
+
+/* ---- serving (no reference counterpart; the reference runs one detached thread
+ * per query, CalciteInterpreter.cpp:929) ----
+ *
+ * A request loop over device-resident tables: submitted queries (the frame's
+ * columns must stay valid until the query is released) are admitted in arrival
+ * order while their estimated outputs + workspace fit hbm_budget_bytes and one of
+ * n_streams stream slots is free, batched (up to max_batch per launch) and
+ * launched, so batches admitted while others run overlap on the device. Results
+ * are identical to running each query alone. r: rows per lane of every batch. */
+typedef struct blz_server_s *blz_server;
+int blz_server_create(blz_server *server, size_t hbm_budget_bytes, int max_batch, int n_streams, int r,
+                      char *err, size_t err_len);
+int blz_server_submit(blz_server server, const blz_query *query, int64_t *ticket, char *err, size_t err_len);
+/* Blocks until the query has run; describes its results like blz_batch_result and
+ * its latency (submission to completion observed by the loop). */
+int blz_server_wait(blz_server server, int64_t ticket, int64_t *n_kept, int32_t *null_counts,
+                    gdf_column *out_columns, int max_out, int *n_out, double *latency_ms, char *err,
+                    size_t err_len);
+/* Frees the query's outputs and returns their budget. */
+int blz_server_release(blz_server server, int64_t ticket);
+/* Submits queries[i] at arrival_s[i] seconds after the call (a client thread),
+ * waits for and releases each, and reports per-query latency and rows kept. */
+int blz_server_replay(blz_server server, const blz_query *queries, const double *arrival_s, int n,
+                      double *latency_ms, int64_t *n_kept, double *wall_s, char *err, size_t err_len);
+int blz_server_stats(blz_server server, int64_t *n_batches, int64_t *peak_inflight, size_t *used_bytes);
+int blz_server_destroy(blz_server server);
+
+/* ---- benchmark inputs (no reference counterpart) ----
+ * Fills an allocated column in place with seeded synthetic values: row i depends
+ * only on (seed, i). dist: 0 integer uniform [a, b]; 1 real uniform [a, b);
+ * 2 normal(mean a, sd b); 3 lognormal(a, b); 4 k / 100 for integer k uniform in
+ * [a, b]; 5 a + i. null_rate > 0 also writes the validity bitmap. */
+int blz_synth_column(gdf_column *column, uint64_t seed, int dist, double a, double b, double null_rate,
+                     blz_stream_t stream, char *err, size_t err_len);
 
 /* ---- memory / streams (HIP plumbing for hosts without a HIP toolchain) ------- */
 
