@@ -76,6 +76,119 @@ def traffic_from_profile(kernel):
     return None
 
 
+def _query_bytes(frame, proj, outs, kept=None):
+    """Algorithmic bytes of one query: the input columns its expressions reference
+    (data + bitmaps) and its outputs (data + bitmaps) for the rows kept."""
+    used = set(int(m) for m in re.findall(r"\$(\d+)", proj))
+    n = frame[0].col.size
+    k = n if kept is None else kept
+    b = 0
+    for i in used:
+        c = frame[i].col
+        b += n * np.dtype(__import__("blz_ral").NP_OF[c.dtype]).itemsize + ((n + 7) // 8 if c.valid else 0)
+    for o in outs:
+        b += k * np.dtype(__import__("blz_ral").NP_OF[o.col.dtype]).itemsize + ((k + 7) // 8 if o.col.valid else 0)
+    return b
+
+
+def run_batch(B, n_plans, rows_lo, rows_hi, seed=404):
+    """Batched execution of n_plans independent projections (4-40 ops each) over
+    their own 4-8 column tables of log-uniform row counts, one persistent launch
+    per kernel kind, shapes chosen by the autotuner, replayed as a HIP graph;
+    against the same queries' prepared per-query launches, one after another."""
+    from blz_ral import workloads as W
+    rng = np.random.default_rng(seed)
+    specs = [W.table_spec(rng, W.log_uniform_rows(rng, rows_lo, rows_hi)) for _ in range(n_plans)]
+    tables = [W.make_table(sp) for sp in specs]
+    queries = [(t, None, W.rand_query(rng, sp, False)[1]) for t, sp in zip(tables, specs)]
+    print(f"[bench] batch: {n_plans} plans, {sum(sp['rows'] for sp in specs)} rows", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    batch = B.Batch(queries, r=0, use_graph=True)
+    t_build = time.perf_counter() - t0
+    batch.launch()
+    batch_ms = batch.time(iters=5)
+    total_bytes = 0
+    for q, (frame, _, proj) in enumerate(queries):
+        kept, outs, _ = batch.result(q)
+        total_bytes += _query_bytes(frame, proj, outs, kept)
+    batch_r, batch_t, tuned = batch.r, batch.t, batch.tuned
+    batch.close()
+    # per-query: each query's prepared program, launched one after another
+    progs = [B.Program(f, fq, pq) for f, fq, pq in queries]
+    for p in progs:
+        p.launch()
+    progs[-1].result()
+    each = [p.time(iters=3) for p in progs]
+    t0 = time.perf_counter()
+    for p in progs:
+        p.launch()
+    for p in progs:
+        p.result()
+    seq_wall_ms = (time.perf_counter() - t0) * 1e3
+    for p in progs:
+        p.close()
+    done = np.cumsum(each)  # completion time of each query when all arrive at once
+    for t in tables:
+        for c in t:
+            c.free()
+    return {"workload": f"{n_plans} projection plans, 4-40 ops, 4-8 column tables (int32/int64/f32/f64, "
+                        f"0-10% nulls), rows log-uniform [{rows_lo:.0e}, {rows_hi:.0e}]",
+            "rows": int(sum(sp["rows"] for sp in specs)), "algorithmic_bytes": int(total_bytes),
+            "batched": {"ms": batch_ms, "GBps": total_bytes / (batch_ms * 1e-3) / 1e9, "r": batch_r,
+                        "tile": batch_t, "autotuner_candidates": tuned, "graph": True,
+                        "p50_ms": batch_ms, "p99_ms": batch_ms, "build_s": t_build},
+            "per_query": {"ms": float(done[-1]), "wall_ms": seq_wall_ms,
+                          "GBps": total_bytes / (done[-1] * 1e-3) / 1e9,
+                          "p50_ms": float(np.percentile(done, 50)), "p99_ms": float(np.percentile(done, 99))},
+            "speedup": float(done[-1] / batch_ms)}
+
+
+def run_serve(B, total_gb, rates, seconds, seed=505):
+    """Served query loop over device-resident tables (wide-expression column mix,
+    5% nulls) totalling total_gb: Poisson arrivals at each rate, filter + project
+    queries of 4-40 ops, admitted by a 200 GB HBM budget and continuously batched
+    over 4 streams; against per-query execution (one query per launch, one
+    stream). p50 / p99 latency from submission to completion."""
+    from blz_ral import workloads as W
+    rng = np.random.default_rng(seed)
+    mix = [B.GDF_INT32, B.GDF_INT32, B.GDF_INT64, B.GDF_INT64, B.GDF_FLOAT32, B.GDF_FLOAT32,
+           B.GDF_FLOAT64, B.GDF_FLOAT64]
+    specs, tables, have = [], [], 0
+    while have < total_gb * 1e9:
+        sp = W.table_spec(rng, W.log_uniform_rows(rng, 1e4, 5e7), dtypes=mix, null_max=0.0)
+        sp["nulls"] = [0.05] * len(mix)
+        specs.append(sp)
+        tables.append(W.make_table(sp))
+        have += W.table_bytes(sp)
+    print(f"[bench] serve: {len(tables)} resident tables, {have / 1e9:.1f} GB", file=sys.stderr, flush=True)
+    out = {"workload": f"{len(tables)} resident tables, {have / 1e9:.1f} GB (2 i32, 2 i64, 2 f32, 2 f64, "
+                       f"5% nulls), filter + project queries of 4-40 ops, HBM budget 200 GB", "rates": {}}
+    for qps in rates:
+        n = max(20, int(qps * seconds))
+        arrivals = np.cumsum(rng.exponential(1.0 / qps, n))
+        qs = []
+        for _ in range(n):
+            i = int(rng.integers(0, len(tables)))
+            f, p = W.rand_query(rng, specs[i], True)
+            qs.append((tables[i], f, p))
+        row = {}
+        for mode, mb, ns in (("served", 64, 4), ("per_query", 1, 1)):
+            srv = B.Server(200 << 30, max_batch=mb, n_streams=ns, r=4)
+            lat, kept, wall = srv.replay(qs, arrivals)
+            st = srv.stats()
+            srv.close()
+            rows = sum(specs_rows for specs_rows in (t[0].col.size for t, _, _ in qs))
+            row[mode] = {"p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+                         "rows_per_s": rows / wall, "queries": n, "wall_s": wall, "batches": st["batches"],
+                         "peak_inflight": st["peak_inflight"]}
+        out["rates"][str(qps)] = row
+        print(f"[bench] serve {qps} QPS: {row}", file=sys.stderr, flush=True)
+    for t in tables:
+        for c in t:
+            c.free()
+    return out
+
+
 def cpu_baseline(sample_rows=10_000_000, min_seconds=10.0):
     from oracle import pyoracle as O
     from oracle import planner as P
@@ -105,6 +218,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--calibrate", action="store_true",
                     help="also run a 1 GiB plain copy kernel (counter calibration)")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-plans measurement")
+    ap.add_argument("--no-serve", action="store_true", help="skip the served-queries measurement")
+    ap.add_argument("--batch-plans", type=int, default=256)
+    ap.add_argument("--serve-gb", type=float, default=100.0)
+    ap.add_argument("--serve-seconds", type=float, default=1.0)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; multi-GPU runs are not part of it",
@@ -146,6 +264,11 @@ def main():
         for _ in range(3):
             B._check(B.lib().blz_device_copy(buf_b.ptr, buf_a.ptr, 1 << 30, None), "copy")
         print("[bench] calibration copy: 3 x 1 GiB", file=sys.stderr, flush=True)
+    prog.close()
+    for c in frame:
+        c.free()
+    batch = None if args.no_batch else run_batch(B, args.batch_plans, 1e4, 5e7)
+    serve = None if args.no_serve else run_serve(B, args.serve_gb, [200, 1000, 5000], args.serve_seconds)
     cpu = None if args.no_cpu else cpu_baseline()
     line = {
         "metric": METRIC,
@@ -173,8 +296,9 @@ def main():
                      "achieved": bytes_step / (total_ms * 1e-3) / 1e9,
                      "frac": bytes_step / (total_ms * 1e-3) / 1e9 / HBM_PEAK_GBS},
         "cpu_baseline": cpu,
+        "batch": batch,
+        "serve": serve,
     }
-    prog.close()
     print(json.dumps(line))
 
 

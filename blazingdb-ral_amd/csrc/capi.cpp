@@ -97,7 +97,7 @@ int64_t tile_rows(const Compiled &c) { return c.T > 0 ? c.T : rows_per_tile(c.R)
 // Workgroups per CU of a tile-VM kernel, cached per (shape, mode, LDS).
 int vm_per_cu(const Compiled &c, int mode, int lds) {
   static std::mutex mu;
-  static std::map<std::tuple<int, int, int, bool, int>, int> cache;
+  static std::map<std::tuple<int, int, int, int, int>, int> cache;
   const auto key = std::make_tuple(c.R, c.S, mode, c.vm_slow, lds);
   std::lock_guard<std::mutex> g(mu);
   auto it = cache.find(key);
@@ -1013,7 +1013,7 @@ struct SubBatch {
   std::vector<int> members;   // program indices
   VmBatch *d_bd = nullptr;    // device descriptor (VmBatch + VmPlanRef[])
   int lds = 0, lds_gather = 0;
-  bool slow = false;
+  int slow = 0;  // op set level (max over the programs)
   int64_t n_tiles = 0, n_spans = 0, n_chunks = 0;
   int grid = 1, grid_gather = 1;
 };
@@ -1132,7 +1132,7 @@ blz_batch batch_build(const blz_query *qs, int n, int r, int t, bool use_graph, 
         maxgw = std::max(maxgw, c.vm_g_wave_bytes);
         maxnv = std::max(maxnv, c.vm_nvout);
         maxgtmp = std::max(maxgtmp, c.lds_gather - c.vm_g_tmp_off);
-        sb.slow = sb.slow || c.vm_slow;
+        sb.slow = std::max(sb.slow, c.vm_slow);
       }
       bd.n_plans = (int)refs.size();
       bd.bufb = maxbuf;

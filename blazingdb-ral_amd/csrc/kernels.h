@@ -53,12 +53,13 @@ struct VmBatch {
 
 // Tile VM (vm_kernel.hip): r rows per lane in {8, 4, 2}; tile = 256 r P rows;
 // lds_bytes: two tile buffers + temporaries (modes 0, 1), or the gather kernel's
-// keep words, pass buffer, validity words and temporaries (mode 2). slow: the
-// program has ops of the out-of-line set (transcendentals, POWER, date parts).
+// keep words, pass buffer, validity words and temporaries (mode 2). ops: the op set
+// compiled in (0 common, 1 hot, 2 hot + out-of-line: transcendentals, POWER, date
+// parts).
 // mode: 0 projection, 1 predicate (keep words + slot counts), 2 gather of the kept
 // rows (after launch_slot_scan).
-hipError_t launch_vm(int r, int mode, bool slow, int grid, int lds_bytes, hipStream_t st, const VmBatch *bd);
-int vm_blocks_per_cu(int r, int mode, bool slow, int lds_bytes);
+hipError_t launch_vm(int r, int mode, int ops, int grid, int lds_bytes, hipStream_t st, const VmBatch *bd);
+int vm_blocks_per_cu(int r, int mode, int ops, int lds_bytes);
 // Exclusive prefix of every program's slot counts (ws.wave_count -> ws.wave_prefix,
 // total to ws.n_sel): chunks of 4096 slots, combined within a program by a
 // single-pass decoupled look-back over its ws.tile_state (zero before the scan).

@@ -530,7 +530,8 @@ bool compile_vm(const CompileRequest &req, Builder &B, const std::vector<AccOp> 
           const OutSpec &s = req.outputs[a.oidx];
           VmOp v = new_vm(VM_EMIT);
           v.arg = a.oidx;
-          v.cls = (uint8_t)(s.kind == 0 ? store_kind_program(s.dtype) : store_kind_alias(s.dtype));
+          v.cls = VC_NONE;
+          v.gen = (uint8_t)(s.kind == 0 ? store_kind_program(s.dtype) : store_kind_alias(s.dtype));
           dst.push_back(v);
           break;
         }
@@ -625,11 +626,6 @@ bool compile_vm(const CompileRequest &req, Builder &B, const std::vector<AccOp> 
   // The gather pass of a filter stages the kept rows of the columns its ops read
   // into a pass buffer (256 R rows x each width) before evaluating them.
   std::vector<int> goff(n_dev, -1);
-  // A VM_GENERIC with no operand still runs `fetch` on VC_NONE: give it an
-  // immediate so no memory is touched.
-  for (auto *v : {&vm_p, &vm_q})
-    for (auto &op : *v)
-      if (op.cls == VC_NONE && op.code != VM_EMIT) { op.cls = VC_IMM; op.imm = 0; }
 
   // Tile buffers: staged column slices, then needed bitmaps, each 16-byte aligned.
   int n_valid_out = 0;
@@ -823,7 +819,12 @@ bool compile_vm(const CompileRequest &req, Builder &B, const std::vector<AccOp> 
   out.lds_bytes = (int)lds_total;
   out.acc = false;
   out.vm = true;
-  out.vm_slow = slow;
+  // op set level: 2 with out-of-line ops, 1 with hot ops outside the common set
+  int level = slow ? 2 : 0;
+  for (auto *v : {&vm_p, &vm_q})
+    for (auto &op : *v)
+      if (level == 0 && op.code > VM_I2F && op.code != VM_GENERIC) level = 1;
+  out.vm_slow = level;
   out.lds_gather = g_tmp + tmp_bytes;
   out.vm_bufb = buf;
   out.vm_tmp_off = 2 * buf;

@@ -114,7 +114,7 @@ static_assert(sizeof(AccOp) == 24, "AccOp layout");
 enum VmCode : uint8_t {
   VM_LOAD = 0,  // acc = B
   VM_STORE,     // tmp[arg] = acc
-  VM_EMIT,      // output arg <- acc; cls = StoreKind
+  VM_EMIT,      // output arg <- acc; gen = StoreKind
   VM_ADD_I, VM_ADD_F, VM_SUB_I, VM_SUB_F, VM_RSUB_I, VM_RSUB_F, VM_MUL_I, VM_MUL_F,
   VM_AND_B,     // acc &= B (both 0/1)
   VM_CMP,       // acc = acc CMP B; arg = comparison DevOpCode
@@ -139,9 +139,9 @@ constexpr uint8_t VG_REV = 0x40, VG_UNARY = 0x80;
 struct VmOp {
   uint8_t code;   // VmCode
   uint8_t arg;    // temp / output / comparison / DevOpCode
-  uint8_t cls;    // VmCls of the operand (StoreKind for VM_EMIT)
+  uint8_t cls;    // VmCls of the operand (VC_NONE for ops without one)
   uint8_t cv;     // operand conversion (Conv) applied after the fetch
-  uint8_t gen;    // VM_GENERIC flags
+  uint8_t gen;    // VM_GENERIC flags; VM_EMIT: StoreKind
   uint8_t vcol;   // input whose raw validity COALESCE / IS_[NOT_]NULL read; 0xFF none
   uint8_t arg2;   // VC_G* / VC_L*: input column; VC_TMP: temporary
   uint8_t pad;

@@ -63,7 +63,7 @@ struct Compiled {
   int lds_bytes = 0;  // staged kernel dynamic LDS per workgroup
   bool acc = false;   // accumulator-form program (S = temporaries)
   bool vm = false;    // tile VM program (S = temporaries, T = 256 R)
-  bool vm_slow = false;  // the VM program has ops of the out-of-line set
+  int vm_slow = 0;  // op set the VM program needs: 0 common, 1 hot, 2 with out-of-line ops
   int lds_gather = 0;    // tile VM filter: LDS of the gather kernel
   int vm_bufb = 0, vm_tmp_off = 0;                      // tile VM LDS layout (projection / predicate)
   int vm_g_buf_off = 0, vm_g_vb_off = 0, vm_g_tmp_off = 0;  // tile VM LDS layout (gather)

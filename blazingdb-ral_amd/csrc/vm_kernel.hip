@@ -133,6 +133,10 @@ DEV void fetch(int cls, int off, int arg, int arg2, const CONSTANT VmOp &op, cld
                clds_ptr tmpb, int tstride, const CONSTANT InCol *in, const int64_t (&grow)[R],
                const bool (&live)[R], uint64_t (&B)[R]) {
   switch (cls) {
+    case VC_NONE:
+#pragma unroll
+      for (int u = 0; u < R; u++) B[u] = 0;
+      break;
     case VC_IMM: {
       const uint64_t im = op.imm;
 #pragma unroll
@@ -173,27 +177,10 @@ DEV void fetch(int cls, int off, int arg, int arg2, const CONSTANT VmOp &op, cld
 #pragma unroll
       for (int u = 0; u < R; u++) B[u] = (uint64_t)*(const LDSAS uint8_t *)(p + 64 * u);
     } break;
-    default: {
-      // VC_G*: the column in HBM, read only for live rows.
-      const void *base = in[arg2].data;
+    default:
 #pragma unroll
-      for (int u = 0; u < R; u++) {
-        uint64_t v = 0;
-        if (live[u]) {
-          const int64_t g = grow[u];
-          switch (cls) {
-            case VC_G64: v = gload<uint64_t>(base, g); break;
-            case VC_G32I: v = (uint64_t)(int64_t)gload<int32_t>(base, g); break;
-            case VC_G32F: v = d2b((double)gload<float>(base, g)); break;
-            case VC_G16I: v = (uint64_t)(int64_t)gload<int16_t>(base, g); break;
-            case VC_G8I: v = (uint64_t)(int64_t)gload<int8_t>(base, g); break;
-            case VC_G8U: v = (uint64_t)gload<uint8_t>(base, g); break;
-            default: break;
-          }
-        }
-        B[u] = v;
-      }
-    } break;
+      for (int u = 0; u < R; u++) B[u] = 0;
+      break;
   }
 }
 
@@ -285,12 +272,111 @@ DEV void vcol_bits(const CONSTANT InCol *in, int vcol, const int64_t (&grow)[R],
   }
 }
 
+// The ops outside the common set (see run_vm).
+template <int R, int OPS>
+DEV void run_rare(int code, int arg, const u32x4 &w, uint64_t (&acc)[R], const uint64_t (&B)[R],
+                  const CONSTANT InCol *in, const int64_t (&grow)[R], int64_t n) {
+    switch (code) {
+      case VM_F2I:
+#pragma unroll
+        for (int u = 0; u < R; u++) acc[u] = (uint64_t)f2i(b2d(acc[u]));
+        break;
+      case VM_FLOOR:
+#pragma unroll
+        for (int u = 0; u < R; u++) acc[u] = d2b(__builtin_floor(b2d(acc[u])));
+        break;
+      case VM_CEIL:
+#pragma unroll
+        for (int u = 0; u < R; u++) acc[u] = d2b(__builtin_ceil(b2d(acc[u])));
+        break;
+      case VM_ABS_F:
+#pragma unroll
+        for (int u = 0; u < R; u++) acc[u] = acc[u] & 0x7FFFFFFFFFFFFFFFull;
+        break;
+      case VM_NOT_I:
+#pragma unroll
+        for (int u = 0; u < R; u++) acc[u] = (uint64_t)(acc[u] == 0);
+        break;
+      case VM_NOT_F:
+#pragma unroll
+        for (int u = 0; u < R; u++) acc[u] = (uint64_t)(!(b2d(acc[u]) != 0.0));
+        break;
+      case VM_DIV_F:
+#pragma unroll
+        for (int u = 0; u < R; u++) acc[u] = d2b(b2d(acc[u]) / b2d(B[u]));
+        break;
+      case VM_RDIV_F:
+#pragma unroll
+        for (int u = 0; u < R; u++) acc[u] = d2b(b2d(B[u]) / b2d(acc[u]));
+        break;
+      case VM_DIV_I:
+#pragma nounroll
+        for (int u = 0; u < R; u++) acc[u] = vm_idiv(acc[u], B[u]);
+        break;
+      case VM_RDIV_I:
+#pragma nounroll
+        for (int u = 0; u < R; u++) acc[u] = vm_idiv(B[u], acc[u]);
+        break;
+      case VM_MOD_I:
+#pragma nounroll
+        for (int u = 0; u < R; u++) acc[u] = vm_imod(acc[u], B[u]);
+        break;
+      case VM_RMOD_I:
+#pragma nounroll
+        for (int u = 0; u < R; u++) acc[u] = vm_imod(B[u], acc[u]);
+        break;
+      case VM_COALESCE:
+      case VM_RCOALESCE: {
+        bool vb[R];
+        vcol_bits<R>(in, (int)((w.y >> 8) & 0xFF), grow, n, vb);
+        const bool rev = code == VM_RCOALESCE;
+#pragma unroll
+        for (int u = 0; u < R; u++) {
+          const uint64_t x = rev ? B[u] : acc[u], y = rev ? acc[u] : B[u];
+          acc[u] = vb[u] ? x : y;
+        }
+      } break;
+      case VM_IS_NULL:
+      case VM_IS_NOT_NULL: {
+        bool vb[R];
+        vcol_bits<R>(in, (int)((w.y >> 8) & 0xFF), grow, n, vb);
+        const bool want = code == VM_IS_NOT_NULL;
+#pragma unroll
+        for (int u = 0; u < R; u++) acc[u] = (uint64_t)(vb[u] == want);
+      } break;
+      default:
+        if constexpr (OPS >= 2) {  // VM_GENERIC
+          const int gen = (int)(w.y & 0xFF), vcol = (int)((w.y >> 8) & 0xFF);
+          const void *vp = vcol != 0xFF ? in[vcol].valid : nullptr;
+#pragma nounroll
+          for (int u = 0; u < R; u++) {
+            const int64_t g = grow[u];
+            const bool vb = valid_bit(vp, g < n ? g : n - 1);
+            acc[u] = vm_generic_row(arg, gen, acc[u], B[u], vb);
+          }
+        }
+        break;
+    }
+}
+
 // Runs `nops` VmOps over rows lrow + 64 u of the tile buffer `tb` (global rows
 // grow[u]). Temporaries live in LDS: temporary t of row u at
 // tmpb + t * tstride + 512 u. VM_EMIT stores row u at output position
 // obase + orel[u] when live[u]. SLOW: the program has ops of the out-of-line set
 // (VM_GENERIC).
-template <int R, bool SLOW>
+//
+// OPS selects the op set compiled into the loop: 0 the common set (loads,
+// + - * of int64 / binary64, comparisons, boolean AND, temporaries, stores,
+// int64 -> binary64); 1 adds the rest of the hot set (division, modulo, floor /
+// ceil / abs, NOT, binary64 -> int64, COALESCE, IS [NOT] NULL); 2 adds the
+// out-of-line set. The host picks the smallest set covering the program: fewer
+// cases mean fewer registers and a shorter dispatch.
+//
+// The loop body is two uniform switches with a single merge point each: the
+// operand fetch by class (VC_NONE for ops without an operand), then the op. Early
+// exits or a separate switch for operand-less ops make the compiler thread the
+// accumulator through extra copies and flag branches (measured: 2.3x slower).
+template <int R, int OPS>
 DEV void run_vm(const CONSTANT VmOp *ops, int nops, uint64_t (&acc)[R], lds_ptr tmpb, int tstride,
                 const CONSTANT InCol *in, const CONSTANT OutCol *out, clds_ptr tb, int lrow,
                 const int64_t (&grow)[R], int64_t n, int64_t obase, const int (&orel)[R],
@@ -300,80 +386,6 @@ DEV void run_vm(const CONSTANT VmOp *ops, int nops, uint64_t (&acc)[R], lds_ptr 
     const u32x4 w = *(const CONSTANT u32x4 *)&op;
     const int code = (int)(w.x & 0xFF), arg = (int)((w.x >> 8) & 0xFF);
     const int cls = (int)((w.x >> 16) & 0xFF), cv = (int)(w.x >> 24);
-    switch (code) {
-      case VM_STORE: {
-        lds_ptr p = tmpb + arg * tstride;
-#pragma unroll
-        for (int u = 0; u < R; u++) *(LDSAS uint64_t *)(p + 512 * u) = acc[u];
-      }
-        continue;
-      case VM_EMIT: {
-        void *base = out[arg].data;
-        switch (cls) {
-          case ST_I64: case ST_F64:
-#pragma unroll
-            for (int u = 0; u < R; u++) if (live[u]) gstore<uint64_t>(base, obase + orel[u], acc[u]);
-            break;
-          case ST_I32:
-#pragma unroll
-            for (int u = 0; u < R; u++) if (live[u]) gstore<int32_t>(base, obase + orel[u], (int32_t)acc[u]);
-            break;
-          case ST_F32:
-#pragma unroll
-            for (int u = 0; u < R; u++) if (live[u]) gstore<float>(base, obase + orel[u], (float)b2d(acc[u]));
-            break;
-          case ST_I16:
-#pragma unroll
-            for (int u = 0; u < R; u++) if (live[u]) gstore<int16_t>(base, obase + orel[u], (int16_t)acc[u]);
-            break;
-          case ST_I8:
-#pragma unroll
-            for (int u = 0; u < R; u++) if (live[u]) gstore<int8_t>(base, obase + orel[u], (int8_t)acc[u]);
-            break;
-          default: break;
-        }
-      }
-        continue;
-      case VM_I2F:
-#pragma unroll
-        for (int u = 0; u < R; u++) acc[u] = d2b((double)(int64_t)acc[u]);
-        continue;
-      case VM_F2I:
-#pragma unroll
-        for (int u = 0; u < R; u++) acc[u] = (uint64_t)f2i(b2d(acc[u]));
-        continue;
-      case VM_FLOOR:
-#pragma unroll
-        for (int u = 0; u < R; u++) acc[u] = d2b(__builtin_floor(b2d(acc[u])));
-        continue;
-      case VM_CEIL:
-#pragma unroll
-        for (int u = 0; u < R; u++) acc[u] = d2b(__builtin_ceil(b2d(acc[u])));
-        continue;
-      case VM_ABS_F:
-#pragma unroll
-        for (int u = 0; u < R; u++) acc[u] = acc[u] & 0x7FFFFFFFFFFFFFFFull;
-        continue;
-      case VM_NOT_I:
-#pragma unroll
-        for (int u = 0; u < R; u++) acc[u] = (uint64_t)(acc[u] == 0);
-        continue;
-      case VM_NOT_F:
-#pragma unroll
-        for (int u = 0; u < R; u++) acc[u] = (uint64_t)(!(b2d(acc[u]) != 0.0));
-        continue;
-      case VM_IS_NULL:
-      case VM_IS_NOT_NULL: {
-        bool vb[R];
-        vcol_bits<R>(in, (int)((w.y >> 8) & 0xFF), grow, n, vb);
-        const bool want = code == VM_IS_NOT_NULL;
-#pragma unroll
-        for (int u = 0; u < R; u++) acc[u] = (uint64_t)(vb[u] == want);
-      }
-        continue;
-      default:
-        break;
-    }
     const int off = (int)w.z;
     const int arg2 = (int)((w.y >> 16) & 0xFF);
     uint64_t B[R];
@@ -432,52 +444,44 @@ DEV void run_vm(const CONSTANT VmOp *ops, int nops, uint64_t (&acc)[R], lds_ptr 
       case VM_CMPAND:
         compare_and<R>(arg, acc, B, op.imm2);
         break;
-      case VM_DIV_F:
+      case VM_STORE: {
+        lds_ptr p = tmpb + arg * tstride;
 #pragma unroll
-        for (int u = 0; u < R; u++) acc[u] = d2b(b2d(acc[u]) / b2d(B[u]));
-        break;
-      case VM_RDIV_F:
+        for (int u = 0; u < R; u++) *(LDSAS uint64_t *)(p + 512 * u) = acc[u];
+      } break;
+      case VM_EMIT: {
+        void *base = out[arg].data;
+        switch ((int)(w.y & 0xFF)) {  // StoreKind
+          case ST_I64: case ST_F64:
 #pragma unroll
-        for (int u = 0; u < R; u++) acc[u] = d2b(b2d(B[u]) / b2d(acc[u]));
-        break;
-      case VM_DIV_I:
-#pragma nounroll
-        for (int u = 0; u < R; u++) acc[u] = vm_idiv(acc[u], B[u]);
-        break;
-      case VM_RDIV_I:
-#pragma nounroll
-        for (int u = 0; u < R; u++) acc[u] = vm_idiv(B[u], acc[u]);
-        break;
-      case VM_MOD_I:
-#pragma nounroll
-        for (int u = 0; u < R; u++) acc[u] = vm_imod(acc[u], B[u]);
-        break;
-      case VM_RMOD_I:
-#pragma nounroll
-        for (int u = 0; u < R; u++) acc[u] = vm_imod(B[u], acc[u]);
-        break;
-      case VM_COALESCE:
-      case VM_RCOALESCE: {
-        bool vb[R];
-        vcol_bits<R>(in, (int)((w.y >> 8) & 0xFF), grow, n, vb);
-        const bool rev = code == VM_RCOALESCE;
+            for (int u = 0; u < R; u++) if (live[u]) gstore<uint64_t>(base, obase + orel[u], acc[u]);
+            break;
+          case ST_I32:
 #pragma unroll
-        for (int u = 0; u < R; u++) {
-          const uint64_t x = rev ? B[u] : acc[u], y = rev ? acc[u] : B[u];
-          acc[u] = vb[u] ? x : y;
+            for (int u = 0; u < R; u++) if (live[u]) gstore<int32_t>(base, obase + orel[u], (int32_t)acc[u]);
+            break;
+          case ST_F32:
+#pragma unroll
+            for (int u = 0; u < R; u++) if (live[u]) gstore<float>(base, obase + orel[u], (float)b2d(acc[u]));
+            break;
+          case ST_I16:
+#pragma unroll
+            for (int u = 0; u < R; u++) if (live[u]) gstore<int16_t>(base, obase + orel[u], (int16_t)acc[u]);
+            break;
+          case ST_I8:
+#pragma unroll
+            for (int u = 0; u < R; u++) if (live[u]) gstore<int8_t>(base, obase + orel[u], (int8_t)acc[u]);
+            break;
+          default: break;
         }
       } break;
+      case VM_I2F:
+#pragma unroll
+        for (int u = 0; u < R; u++) acc[u] = d2b((double)(int64_t)acc[u]);
+        break;
       default:
-        if (SLOW) {  // VM_GENERIC
-          const int gen = (int)(w.y & 0xFF), vcol = (int)((w.y >> 8) & 0xFF);
-          const void *vp = vcol != 0xFF ? in[vcol].valid : nullptr;
-#pragma nounroll
-          for (int u = 0; u < R; u++) {
-            const int64_t g = grow[u];
-            const bool vb = valid_bit(vp, g < n ? g : n - 1);
-            acc[u] = vm_generic_row(arg, gen, acc[u], B[u], vb);
-          }
-        }
+        if constexpr (OPS >= 1)
+          run_rare<R, OPS>(code, arg, w, acc, B, in, grow, n);
         break;
     }
   }
@@ -587,7 +591,7 @@ DEV VmCtx load_ctx(const uint8_t *blob_arg) {
   return c;
 }
 
-template <int R, int MODE, bool SLOW>
+template <int R, int MODE, int OPS>
 __global__ void __launch_bounds__(kBlock)
 vm_kernel(const VmBatch *__restrict__ bd_arg) {
   extern __shared__ __attribute__((aligned(16))) uint8_t lds_raw[];
@@ -671,7 +675,7 @@ vm_kernel(const VmBatch *__restrict__ bd_arg) {
             grow[u] = row0 + orel[u];
             acc[u] = 0;
           }
-          run_vm<R, SLOW>(cx.ops, cx.n_ops_q, acc, tmpb, tstride, cx.in, cx.out, tb, lrow, grow, n, row0, orel,
+          run_vm<R, OPS>(cx.ops, cx.n_ops_q, acc, tmpb, tstride, cx.in, cx.out, tb, lrow, grow, n, row0, orel,
                           put);
           if (cx.any_valid_out) {
             vm_read_valid<VS, R>(cx.in, cx.n_in, tb, lb >> 6, vrf);
@@ -709,7 +713,7 @@ vm_kernel(const VmBatch *__restrict__ bd_arg) {
             grow[u] = row0 + lrow + 64 * u;
             acc[u] = 0;
           }
-          run_vm<R, SLOW>(cx.ops, cx.n_ops_p, acc, tmpb, tstride, cx.in, cx.out, tb, lrow, grow, n, 0, orel, none);
+          run_vm<R, OPS>(cx.ops, cx.n_ops_p, acc, tmpb, tstride, cx.in, cx.out, tb, lrow, grow, n, 0, orel, none);
           vm_read_valid<VS, R>(cx.in, cx.n_in, tb, lb >> 6, vrf);
           run_vops<VS>(cx.vops, cx.n_vops_p, vrf);
           const uint64_t vstencil = vsrc_word<VS>(cx.st_vsrc, vrf);
@@ -815,7 +819,7 @@ vm_kernel(const VmBatch *__restrict__ bd_arg) {
           acc[u] = 0;
         }
         stage_kept<R>(cx.gcols, cx.n_gcols, gbuf, plrow, grow, live);
-        run_vm<R, SLOW>(cx.ops + cx.n_ops_p, cx.n_ops_q, acc, tmpb, tstride, cx.in, cx.out, gbuf, plrow, grow, n,
+        run_vm<R, OPS>(cx.ops + cx.n_ops_p, cx.n_ops_q, acc, tmpb, tstride, cx.in, cx.out, gbuf, plrow, grow, n,
                         (int64_t)obase, orel, live);
         if (cx.any_valid_out) {
           // Validity of the 64 outputs of group u as one word: lane u holds it.
@@ -980,22 +984,24 @@ slot_scan_kernel(const VmBatch *__restrict__ bd_arg) {
 #define BLZ_VM_SHAPES(X) X(8) X(4) X(2)
 
 template <int RR, int MODE>
-hipError_t launch_vm_mode(bool slow, int grid, int lds_bytes, hipStream_t st, const VmBatch *bd) {
-  if (slow) hipLaunchKernelGGL((vm_kernel<RR, MODE, true>), dim3(grid), dim3(kBlock), lds_bytes, st, bd);
-  else hipLaunchKernelGGL((vm_kernel<RR, MODE, false>), dim3(grid), dim3(kBlock), lds_bytes, st, bd);
+hipError_t launch_vm_mode(int ops, int grid, int lds_bytes, hipStream_t st, const VmBatch *bd) {
+  if (ops >= 2) hipLaunchKernelGGL((vm_kernel<RR, MODE, 2>), dim3(grid), dim3(kBlock), lds_bytes, st, bd);
+  else if (ops == 1) hipLaunchKernelGGL((vm_kernel<RR, MODE, 1>), dim3(grid), dim3(kBlock), lds_bytes, st, bd);
+  else hipLaunchKernelGGL((vm_kernel<RR, MODE, 0>), dim3(grid), dim3(kBlock), lds_bytes, st, bd);
   return hipGetLastError();
 }
 
 template <int RR, int MODE>
-int vm_occupancy_mode(bool slow, int lds_bytes) {
+int vm_occupancy_mode(int ops, int lds_bytes) {
   int b = 0;
-  if (slow) (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, vm_kernel<RR, MODE, true>, kBlock, lds_bytes);
-  else (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, vm_kernel<RR, MODE, false>, kBlock, lds_bytes);
+  if (ops >= 2) (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, vm_kernel<RR, MODE, 2>, kBlock, lds_bytes);
+  else if (ops == 1) (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, vm_kernel<RR, MODE, 1>, kBlock, lds_bytes);
+  else (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, vm_kernel<RR, MODE, 0>, kBlock, lds_bytes);
   return b;
 }
 
 template <int RR>
-hipError_t launch_vm_shape(int mode, bool slow, int grid, int lds_bytes, hipStream_t st, const VmBatch *bd) {
+hipError_t launch_vm_shape(int mode, int slow, int grid, int lds_bytes, hipStream_t st, const VmBatch *bd) {
   switch (mode) {
     case VM_PROJECT: return launch_vm_mode<RR, VM_PROJECT>(slow, grid, lds_bytes, st, bd);
     case VM_PRED: return launch_vm_mode<RR, VM_PRED>(slow, grid, lds_bytes, st, bd);
@@ -1005,7 +1011,7 @@ hipError_t launch_vm_shape(int mode, bool slow, int grid, int lds_bytes, hipStre
 }
 
 template <int RR>
-int vm_occupancy(int mode, bool slow, int lds_bytes) {
+int vm_occupancy(int mode, int slow, int lds_bytes) {
   switch (mode) {
     case VM_PROJECT: return vm_occupancy_mode<RR, VM_PROJECT>(slow, lds_bytes);
     case VM_PRED: return vm_occupancy_mode<RR, VM_PRED>(slow, lds_bytes);
@@ -1014,7 +1020,7 @@ int vm_occupancy(int mode, bool slow, int lds_bytes) {
   }
 }
 
-hipError_t launch_vm(int r, int mode, bool slow, int grid, int lds_bytes, hipStream_t st, const VmBatch *bd) {
+hipError_t launch_vm(int r, int mode, int slow, int grid, int lds_bytes, hipStream_t st, const VmBatch *bd) {
 #define BLZ_VM_LAUNCH(RR) \
   if (r == RR) return launch_vm_shape<RR>(mode, slow, grid, lds_bytes, st, bd);
   BLZ_VM_SHAPES(BLZ_VM_LAUNCH)
@@ -1022,7 +1028,7 @@ hipError_t launch_vm(int r, int mode, bool slow, int grid, int lds_bytes, hipStr
   return hipErrorInvalidValue;
 }
 
-int vm_blocks_per_cu(int r, int mode, bool slow, int lds_bytes) {
+int vm_blocks_per_cu(int r, int mode, int slow, int lds_bytes) {
 #define BLZ_VM_OCC(RR) \
   if (r == RR) return vm_occupancy<RR>(mode, slow, lds_bytes);
   BLZ_VM_SHAPES(BLZ_VM_OCC)
