@@ -195,6 +195,9 @@ std::vector<uint8_t> device_image(const LaunchBuffers &lb, const Compiled &c) {
     b.g_tmp_off = c.vm_g_tmp_off;
     b.g_wave_bytes = c.vm_g_wave_bytes;
     b.g_vb_wave_words = 66 * c.vm_nvout;
+    b.vr_off = c.vm_vr_off;
+    b.g_vr_off = c.vm_g_vr_off;
+    b.vr_wave_words = c.n_vslots * c.R;
     b.n_tiles = tiles;
     b.n_spans = (tiles * (c.T / 64) + 63) / 64;
     b.n_chunks = lb.n_chunks;
@@ -1111,7 +1114,7 @@ blz_batch batch_build(const blz_query *qs, int n, int r, int t, bool use_graph, 
     auto build_sub = [&](SubBatch &sb, std::vector<uint8_t> &img, size_t off) {
       VmBatch bd{};
       std::vector<VmPlanRef> refs;
-      int maxbuf = 0, maxtmp = 0, maxgw = 0, maxnv = 0, maxgtmp = 0;
+      int maxbuf = 0, maxtmp = 0, maxgw = 0, maxnv = 0, maxgtmp = 0, maxvs = 0;
       int64_t tile0 = 0, span0 = 0, chunk0 = 0;
       for (int i : sb.members) {
         blz_program p = b->progs[i];
@@ -1128,7 +1131,8 @@ blz_batch batch_build(const blz_query *qs, int n, int r, int t, bool use_graph, 
         span0 += (tiles * (c.T / 64) + 63) / 64;
         chunk0 += (tiles * 4 + kScanChunk - 1) / kScanChunk;
         maxbuf = std::max(maxbuf, c.vm_bufb);
-        maxtmp = std::max(maxtmp, c.lds_bytes - 2 * c.vm_bufb);
+        maxtmp = std::max(maxtmp, c.lds_bytes - c.vm_tmp_off);
+        maxvs = std::max(maxvs, c.n_vslots);
         maxgw = std::max(maxgw, c.vm_g_wave_bytes);
         maxnv = std::max(maxnv, c.vm_nvout);
         maxgtmp = std::max(maxgtmp, c.lds_gather - c.vm_g_tmp_off);
@@ -1136,18 +1140,21 @@ blz_batch batch_build(const blz_query *qs, int n, int r, int t, bool use_graph, 
       }
       bd.n_plans = (int)refs.size();
       bd.bufb = maxbuf;
-      bd.tmp_off = 2 * maxbuf;
+      bd.vr_wave_words = maxvs * b->r;
+      bd.vr_off = 2 * maxbuf;
+      bd.tmp_off = bd.vr_off + 4 * 8 * bd.vr_wave_words;
       bd.g_buf_off = 0;
       bd.g_wave_bytes = maxgw;
       bd.g_vb_wave_words = 66 * maxnv;
       bd.g_vb_off = 4 * maxgw;
-      bd.g_tmp_off = bd.g_vb_off + 4 * 66 * 8 * maxnv;
+      bd.g_vr_off = bd.g_vb_off + 4 * 66 * 8 * maxnv;
+      bd.g_tmp_off = bd.g_vr_off + 4 * 8 * bd.vr_wave_words;
       bd.n_tiles = tile0;
       bd.n_spans = span0;
       bd.n_chunks = chunk0;
       bd.chunk_counter = b->d_counter;
       bd.plans = (const VmPlanRef *)(b->d_mem + off + sizeof(VmBatch));
-      sb.lds = 2 * maxbuf + maxtmp;
+      sb.lds = bd.tmp_off + maxtmp;
       sb.lds_gather = bd.g_tmp_off + maxgtmp;
       if (sb.lds > 64 * 1024 || sb.lds_gather > 64 * 1024) invalid("batch needs more than 64 KiB of LDS");
       sb.n_tiles = tile0;

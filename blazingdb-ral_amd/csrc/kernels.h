@@ -46,6 +46,8 @@ struct VmBatch {
   int32_t g_buf_off, g_vb_off, g_tmp_off;  // gather kernel: pass buffers, validity words, temporaries
   int32_t g_wave_bytes;              // gather kernel: one wave's pass buffer (64 R rows)
   int32_t g_vb_wave_words;           // gather kernel: one wave's validity words
+  int32_t vr_off, g_vr_off;          // validity registers (projection / predicate, gather kernel)
+  int32_t vr_wave_words;             // one wave's validity registers (registers x R words)
   int64_t n_tiles, n_spans, n_chunks;      // totals over the programs
   unsigned int *chunk_counter;       // slot scan: dynamic chunk ids (zero before the scan)
   const VmPlanRef *plans;            // [n_plans]

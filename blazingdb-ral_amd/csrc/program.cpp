@@ -474,7 +474,7 @@ bool compile_vm(const CompileRequest &req, Builder &B, const std::vector<AccOp> 
                 int stencil_v, int n_vslots, Compiled &out, std::string &why) {
   const int n_dev = (int)req.inputs.size();
   const int n_out = (int)req.outputs.size();
-  if (n_vslots > 8) { why = "plan needs more than 8 validity registers"; return false; }
+  if (n_vslots > kVmVSlots) { why = "plan needs more than 32 validity registers"; return false; }
   if (req.n_rows > (int64_t)INT32_MAX * 256) { why = "table too large"; return false; }
 
   // Which inputs the predicate reads (staged) and which only the projection of a
@@ -651,7 +651,8 @@ bool compile_vm(const CompileRequest &req, Builder &B, const std::vector<AccOp> 
     return 2 * (size_t)buf;
   };
   // Shape: rows per lane R and tile rows T = 256 R P (P passes of the four waves).
-  // Temporaries live in LDS (256 R rows each). The default is R = 4 and one pass
+  // Temporaries live in LDS (256 R rows each), after the validity registers (four
+  // waves x registers x R words). The default is R = 4 and one pass
   // per tile (T = 1024): small double buffers keep four workgroups per CU, which
   // hides more latency than longer tiles do; the autotuner may choose otherwise.
   static const int shapes[] = {4, 8, 2};
@@ -662,7 +663,7 @@ bool compile_vm(const CompileRequest &req, Builder &B, const std::vector<AccOp> 
   auto al16 = [](int x) { return (x + 15) & ~15; };
   for (int r : shapes) {
     if (req.force_r > 0 && req.force_r != r) continue;
-    const int tb = acc_temps * 256 * r * 8;
+    const int tb = acc_temps * 256 * r * 8 + 4 * n_vslots * r * 8;
     int g = 0;  // gather pass buffers of a filter (four waves' regions of 64 r rows)
     for (int c = 0; c < n_dev && req.compact; c++)
       if (used_q[c]) g = al16(g + 64 * r * std::max(1, wid(req.inputs[c].dtype)));
@@ -755,7 +756,9 @@ bool compile_vm(const CompileRequest &req, Builder &B, const std::vector<AccOp> 
   const int g_kw = 0;
   const int g_buf = 0;
   const int g_vb = g_buf + 4 * gbuf;  // four waves' regions
-  const int g_tmp = g_vb + 4 * n_valid_out * 66 * 8;  // 64 + 2 words per wave and output
+  const int vr_bytes = 4 * n_vslots * R * 8;
+  const int g_vr = g_vb + 4 * n_valid_out * 66 * 8;  // 64 + 2 words per wave and output
+  const int g_tmp = g_vr + vr_bytes;
   h.lds_vbits_off = g_vb;
   h.n_gcols = (int)gcols.size();
   h.off_gcols = (int)off_gcols;
@@ -763,7 +766,7 @@ bool compile_vm(const CompileRequest &req, Builder &B, const std::vector<AccOp> 
   h.g_kw_off = g_kw;
   h.g_buf_off = g_buf;
   h.g_tmp_off = g_tmp;
-  h.lds_tmp_off = 2 * buf;
+  h.lds_tmp_off = 2 * buf + vr_bytes;
   h.n_valid_out = n_valid_out;
   h.lds_bytes = (int)lds_total;
   h.acc_mode = 2;
@@ -783,7 +786,7 @@ bool compile_vm(const CompileRequest &req, Builder &B, const std::vector<AccOp> 
     ic.data = req.inputs[c].data;
     ic.valid = req.inputs[c].valid;
     ic.load = load_kind(req.inputs[c].dtype);
-    ic.vslot = B.vnodes[c].needed ? c : -1;
+    ic.vslot = B.vnodes[c].needed ? B.vnodes[c].vslot : -1;
     ic.lds_off = lds_off[c];
     ic.lds_voff = lds_voff[c];
     put(out.blob, off_in + sizeof(InCol) * c, ic);
@@ -825,9 +828,11 @@ bool compile_vm(const CompileRequest &req, Builder &B, const std::vector<AccOp> 
     for (auto &op : *v)
       if (level == 0 && op.code > VM_I2F && op.code != VM_GENERIC) level = 1;
   out.vm_slow = level;
-  out.lds_gather = g_tmp + tmp_bytes;
+  out.lds_gather = g_vr + tmp_bytes;  // tmp_bytes: validity registers + temporaries
   out.vm_bufb = buf;
-  out.vm_tmp_off = 2 * buf;
+  out.vm_vr_off = 2 * buf;
+  out.vm_g_vr_off = g_vr;
+  out.vm_tmp_off = 2 * buf + vr_bytes;
   out.vm_g_buf_off = g_buf;
   out.vm_g_wave_bytes = gbuf;
   out.vm_nvout = n_valid_out;
@@ -938,28 +943,52 @@ Compiled compile_program(const CompileRequest &req) {
   }
   if (n_slots > kMaxSlots) fail("plan needs more than 64 device registers");
 
-  // Validity program: needed vnodes, in creation order (already topological).
+  // Validity program: needed vnodes in creation order (topological), the
+  // predicate's ops before the projection's (the projection may read words the
+  // predicate computed, never the reverse). Registers are reused after a word's
+  // last use (linear scan); input words are loaded before the first op and the
+  // roots (stencil and output validity) live to the end.
   mark_vneeded(B, stencil_v);
   for (int v : out_v) mark_vneeded(B, v);
-  int vs_next = n_dev, n_vslots = n_dev;
-  std::vector<VOp> vops_p, vops_q;
+  std::vector<int> vseq;
+  for (int prog = 0; prog < 2; prog++)
+    for (size_t v = n_dev; v < B.vnodes.size(); v++)
+      if (B.vnodes[v].needed && B.vnodes[v].program == prog) vseq.push_back((int)v);
+  const int vend = (int)vseq.size();
+  std::vector<int> vlast(B.vnodes.size(), -1);
+  for (int i = 0; i < vend; i++) {
+    const VNode &n = B.vnodes[vseq[i]];
+    vlast[n.a] = std::max(vlast[n.a], i);
+    vlast[n.b] = std::max(vlast[n.b], i);
+  }
+  if (stencil_v >= 0) vlast[stencil_v] = vend;
+  for (int v : out_v)
+    if (v >= 0) vlast[v] = vend;
+  std::vector<bool> vbusy(kMaxVSlots, false);
+  int n_vslots = 0;
+  auto valloc = [&]() {
+    for (int k = 0; k < kMaxVSlots; k++)
+      if (!vbusy[k]) { vbusy[k] = true; n_vslots = std::max(n_vslots, k + 1); return k; }
+    fail("plan needs more than 64 validity registers");
+    return -1;
+  };
   for (int c = 0; c < n_dev; c++)
-    if (B.vnodes[c].needed) B.vnodes[c].vslot = c;
-  for (size_t v = n_dev; v < B.vnodes.size(); v++) {
-    VNode &n = B.vnodes[v];
-    if (!n.needed) continue;
-    n.vslot = vs_next++;
+    if (B.vnodes[c].needed) B.vnodes[c].vslot = valloc();
+  std::vector<VOp> vops_p, vops_q;
+  for (int i = 0; i < vend; i++) {
+    VNode &n = B.vnodes[vseq[i]];
+    for (int o : {n.a, n.b})
+      if (vlast[o] == i) vbusy[B.vnodes[o].vslot] = false;
+    n.vslot = valloc();
     VOp op{(uint8_t)n.code, (uint8_t)n.vslot, (uint8_t)B.vnodes[n.a].vslot, (uint8_t)B.vnodes[n.b].vslot};
     (n.program == 0 ? vops_p : vops_q).push_back(op);
   }
-  n_vslots = vs_next;
-  if (n_vslots > kMaxVSlots) fail("plan needs more than 64 validity registers");
   auto vsrc = [&](int v) { return v == V_ALL ? -1 : v == V_NONE ? -2 : B.vnodes[v].vslot; };
 
   // Accumulator form: expression trees lowered to acc = acc OP operand code.
   std::vector<AccOp> acc_p, acc_q;
   int acc_temps = 0;
-  bool acc_ok = n_vslots <= 8 && req.kernel != 2 && req.kernel != 3;
+  bool acc_ok = n_vslots <= kVmVSlots && req.kernel != 2 && req.kernel != 3;
   if (acc_ok) {
     std::vector<int> uses(B.nodes.size(), 0);
     for (auto &n : B.nodes)
@@ -1206,7 +1235,7 @@ Compiled compile_program(const CompileRequest &req) {
     ic.data = req.inputs[c].data;
     ic.valid = req.inputs[c].valid;
     ic.load = load_kind(req.inputs[c].dtype);
-    ic.vslot = B.vnodes[c].needed ? c : -1;
+    ic.vslot = B.vnodes[c].needed ? B.vnodes[c].vslot : -1;
     ic.lds_off = 0;
     ic.lds_voff = -1;
     put(out.blob, off_in + sizeof(InCol) * c, ic);

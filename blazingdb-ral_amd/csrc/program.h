@@ -24,6 +24,7 @@ namespace blz {
 // Device register file limits. Slots are 8-byte (int64 / binary64 bits).
 constexpr int kMaxSlots = 64;   // reference limit (interpreter_cpp.cu:86-217 handles <= 64)
 constexpr int kMaxVSlots = 64;
+constexpr int kVmVSlots = 32;  // tile VM: validity registers (LDS words per 64-row group)
 constexpr int kMaxOps = 256;    // per program
 
 // Input load kinds: how a column widens into an 8-byte slot (interpreter_ops.cuh:386-452).

@@ -68,6 +68,7 @@ struct Compiled {
   int vm_bufb = 0, vm_tmp_off = 0;                      // tile VM LDS layout (projection / predicate)
   int vm_g_buf_off = 0, vm_g_vb_off = 0, vm_g_tmp_off = 0;  // tile VM LDS layout (gather)
   int vm_g_wave_bytes = 0, vm_nvout = 0;  // gather pass buffer per wave, outputs with validity
+  int vm_vr_off = 0, vm_g_vr_off = 0;     // validity registers (four waves' regions)
 };
 
 // Throws std::runtime_error with a message on an unsupported or malformed plan.

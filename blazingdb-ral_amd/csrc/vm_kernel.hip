@@ -487,16 +487,42 @@ DEV void run_vm(const CONSTANT VmOp *ops, int nops, uint64_t (&acc)[R], lds_ptr 
   }
 }
 
-// Validity words of a pass from the tile buffer: lane u < R loads group u's word
-// of every staged bitmap.
-template <int VS, int R>
-DEV void vm_read_valid(const CONSTANT InCol *in, int n_in, clds_ptr tb, int group0, uint64_t (&vrf)[VS]) {
+// Validity registers live in LDS: word u (64-row group u of the pass) of register
+// s of a wave is vr[s * R + u]. Lane u < R computes group u; a register is read
+// back by any lane at a wave-uniform address. LDS operations of one wave complete
+// in order, so a lane reads what another lane of its wave wrote before it.
+
+// Input validity words of a pass from the tile buffer (groups group0 .. + R).
+template <int R>
+DEV void vm_load_valid(const CONSTANT InCol *in, int n_in, clds_ptr tb, int group0, LDSAS uint64_t *vr) {
   const int lane = threadIdx.x & 63;
-  const int g = group0 + (lane < R ? lane : 0);
   for (int c = 0; c < n_in; c++) {
     const int vs = in[c].vslot;
-    if (vs >= 0) vrf[vs] = *(const LDSAS uint64_t *)(tb + in[c].lds_voff + 8 * g);
+    if (vs >= 0 && lane < R) vr[vs * R + lane] = *(const LDSAS uint64_t *)(tb + in[c].lds_voff + 8 * (group0 + lane));
   }
+  __builtin_amdgcn_wave_barrier();
+}
+
+// The validity program (AND / OR of 64-bit words).
+template <int R>
+DEV void vm_run_vops(const CONSTANT VOp *vops, int nvops, LDSAS uint64_t *vr) {
+  const int lane = threadIdx.x & 63;
+  for (int k = 0; k < nvops; k++) {
+    const uint32_t w = *(const CONSTANT uint32_t *)(vops + k);
+    const int code = w & 0xFF, d = (w >> 8) & 0xFF, a = (w >> 16) & 0xFF, b = (w >> 24) & 0xFF;
+    if (lane < R) {
+      const uint64_t x = vr[a * R + lane], y = vr[b * R + lane];
+      vr[d * R + lane] = code == VOP_OR ? (x | y) : (x & y);
+    }
+  }
+  __builtin_amdgcn_wave_barrier();
+}
+
+// Word of group u of a validity source: a register, all valid (-1) or none (-2).
+template <int R>
+DEV uint64_t vm_vword(int vsrc, const LDSAS uint64_t *vr, int u) {
+  if (vsrc >= 0) return vr[vsrc * R + u];
+  return vsrc == -2 ? 0ull : ~0ull;
 }
 
 }  // namespace
@@ -595,7 +621,6 @@ template <int R, int MODE, int OPS>
 __global__ void __launch_bounds__(kBlock)
 vm_kernel(const VmBatch *__restrict__ bd_arg) {
   extern __shared__ __attribute__((aligned(16))) uint8_t lds_raw[];
-  constexpr int VS = 8;  // validity registers
   constexpr bool STAGED = MODE != VM_GATHER;
 
   lds_ptr lds = (lds_ptr)lds_raw;
@@ -613,7 +638,8 @@ vm_kernel(const VmBatch *__restrict__ bd_arg) {
   // Every register below is written by the program before it is read (the host
   // compiler guarantees it); none is live across passes.
   uint64_t acc[R];
-  uint64_t vrf[VS];
+  // this wave's validity registers
+  LDSAS uint64_t *vr = (LDSAS uint64_t *)(lds + (MODE == VM_GATHER ? bd.g_vr_off : bd.vr_off)) + wave * bd.vr_wave_words;
 
   if (STAGED) {
     const int64_t n_tiles = bd.n_tiles;
@@ -678,12 +704,12 @@ vm_kernel(const VmBatch *__restrict__ bd_arg) {
           run_vm<R, OPS>(cx.ops, cx.n_ops_q, acc, tmpb, tstride, cx.in, cx.out, tb, lrow, grow, n, row0, orel,
                           put);
           if (cx.any_valid_out) {
-            vm_read_valid<VS, R>(cx.in, cx.n_in, tb, lb >> 6, vrf);
-            run_vops<VS>(cx.vops, cx.n_vops_q, vrf);
+            vm_load_valid<R>(cx.in, cx.n_in, tb, lb >> 6, vr);
+            vm_run_vops<R>(cx.vops, cx.n_vops_q, vr);
             for (int o = 0; o < cx.n_out; o++) {
               void *vp = cx.out[o].valid;
               if (vp == nullptr) continue;
-              const uint64_t word = vsrc_word<VS>(cx.out[o].vsrc, vrf);
+              const uint64_t word = vm_vword<R>(cx.out[o].vsrc, vr, lane < R ? lane : 0);
               const int64_t wi = ((row0 + lb) >> 6) + lane;
               uint64_t nulls = 0;
               if (lane < R && wi * 64 < n) {
@@ -714,12 +740,11 @@ vm_kernel(const VmBatch *__restrict__ bd_arg) {
             acc[u] = 0;
           }
           run_vm<R, OPS>(cx.ops, cx.n_ops_p, acc, tmpb, tstride, cx.in, cx.out, tb, lrow, grow, n, 0, orel, none);
-          vm_read_valid<VS, R>(cx.in, cx.n_in, tb, lb >> 6, vrf);
-          run_vops<VS>(cx.vops, cx.n_vops_p, vrf);
-          const uint64_t vstencil = vsrc_word<VS>(cx.st_vsrc, vrf);
+          vm_load_valid<R>(cx.in, cx.n_in, tb, lb >> 6, vr);
+          vm_run_vops<R>(cx.vops, cx.n_vops_p, vr);
 #pragma unroll
           for (int u = 0; u < R; u++) {
-            const bool vbit = (readlane64(vstencil, u) >> lane) & 1;
+            const bool vbit = (vm_vword<R>(cx.st_vsrc, vr, u) >> lane) & 1;
             const bool keep = lrow + 64 * u < rows && ((acc[u] & st_mask) != 0) && vbit;
             const uint64_t km = __builtin_amdgcn_ballot_w64(keep);
             if (lane == ps * R + u) myword = km;
@@ -829,22 +854,23 @@ vm_kernel(const VmBatch *__restrict__ bd_arg) {
               const int vs = cx.in[ci].vslot;
               if (vs < 0) continue;
               const uint64_t word = __builtin_amdgcn_ballot_w64(live[u] && valid_bit(cx.in[ci].valid, grow[u]));
-              if (lane == u) vrf[vs] = word;
+              if (lane == 0) vr[vs * R + u] = word;
             }
           }
+          __builtin_amdgcn_wave_barrier();
           // The projection's validity program may reuse words the predicate's
           // computed (shared subexpressions): run both.
-          run_vops<VS>(cx.vops, cx.n_vops_p + cx.n_vops_q, vrf);
+          vm_run_vops<R>(cx.vops, cx.n_vops_p + cx.n_vops_q, vr);
           for (int o = 0; o < cx.n_out; o++) {
             if (cx.out[o].valid == nullptr) continue;
-            const uint64_t vword = vsrc_word<VS>(cx.out[o].vsrc, vrf);
+            const int vsrc = cx.out[o].vsrc;
             const int vb = cx.out[o].vbuf;
 #pragma unroll
             for (int u = 0; u < R; u++) {
               const int j0 = g * 64 * R + 64 * u;
               if (j0 >= (int)cnt) break;  // wave-uniform
               const uint64_t lm = low_mask((int64_t)cnt - j0);
-              const uint64_t bits = readlane64(vword, u) & lm;
+              const uint64_t bits = vm_vword<R>(vsrc, vr, u) & lm;
               if (o < 4) nl[o] += __builtin_popcountll(~bits & lm);
               else if (lane == 0) atomicAdd(ws.null_counts + o, (unsigned long long)__builtin_popcountll(~bits & lm));
               const int L = lo + j0;  // span-local output position of the group's first row

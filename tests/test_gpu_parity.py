@@ -259,6 +259,36 @@ def test_kernel_kinds_random_filter_project(B, kernel, seed):
     prog.close()
 
 
+WIDE_MIX = [P.GDF_INT32, P.GDF_INT32, P.GDF_INT64, P.GDF_INT64, P.GDF_FLOAT32, P.GDF_FLOAT32,
+            P.GDF_FLOAT64, P.GDF_FLOAT64]
+
+
+@pytest.mark.parametrize("kernel", [0, 3, 4])
+@pytest.mark.parametrize("seed", range(10))
+def test_wide_workload_queries(B, kernel, seed):
+    """Queries of the batched / served workloads (4-40 ops over eight nullable
+    columns of the wide-expression mix, filtered or not): both kernel kinds and
+    the default choice against the oracle. Every one of them fits the tile VM."""
+    from blz_ral import workloads as W
+    rng = np.random.default_rng(6000 + seed)
+    n = int(rng.choice([1000, 70_001]))
+    spec = {"rows": n, "dtypes": WIDE_MIX, "nulls": [0.05] * 8, "seeds": [0] * 8}
+    host = [gen_column(rng, dt, n, 0.05) for dt in WIDE_MIX]
+    filt, proj = W.rand_query(rng, spec, with_filter=bool(seed % 2))
+    _print_case(test="wide_workload", kernel=kernel, seed=seed, n=n, filt=filt, proj=proj)
+    ref = _oracle(host, filt, proj)
+    dev = [to_device(B, h) for h in host]
+    prog = B.Program(dev, filt, proj, kernel=kernel)
+    assert prog.kernel == (4 if kernel == 0 else kernel)
+    prog.launch()
+    kept, nulls = prog.result()
+    assert kept == len(ref[0])
+    for g, r, nc in zip(prog.outputs, ref, nulls):
+        g.col.null_count = nc
+        assert_column_equal(g, r, what=f"kernel {kernel}: {filt} {proj}")
+    prog.close()
+
+
 @pytest.mark.parametrize("kernel", [3, 4])
 def test_kernel_kinds_c1(B, kernel):
     n = 1_000_003
