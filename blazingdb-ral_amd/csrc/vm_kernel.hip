@@ -42,16 +42,41 @@ typedef const LDSAS uint8_t *clds_ptr;
 
 namespace {
 
-// Direct HBM -> LDS staging of full tile `tile` into the buffer at `tb`: wave w
-// moves pieces w, w + 4, ...; lane l carries bytes [16 l, 16 l + 16) of a piece.
-DEV void dma_tile(const CONSTANT Piece *pieces, int np, int64_t tile, lds_ptr tb) {
+// This wave's share of a program's tile staging (pieces w, w + 4, ...), read from
+// the piece table once per program into vector registers (lane i holds the i-th
+// piece of the wave), so that staging a tile costs no scalar loads: each piece's
+// fields are read across lanes.
+struct WaveDma {
+  int np;           // pieces of this wave (< 64)
+  uint32_t src_lo;  // lane i: slice base + offset of the piece in the slice
+  uint32_t src_hi;
+  int slice;        // bytes per full tile
+  int dst;          // LDS offset (low 20 bits) | bytes << 20
+};
+DEV void wave_dma_load(WaveDma &d, const CONSTANT Piece *pieces, int np) {
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  for (int j = wave; j < np; j += kWaves) {
-    const CONSTANT Piece &p = pieces[j];
-    const int slice = p.slice, off = p.off, dst = p.lds, bytes = p.bytes;
-    const uint8_t *src = (const uint8_t *)p.base + tile * (int64_t)slice + off + lane * 16;
-    if (lane * 16 < bytes)
-      __builtin_amdgcn_global_load_lds((const GLOBAL void *)src, (LDSAS void *)(tb + dst), 16, 0, 0);
+  d.np = np > wave ? (np - wave + kWaves - 1) / kWaves : 0;
+  d.src_lo = d.src_hi = 0;
+  d.slice = d.dst = 0;
+  if (lane < d.np) {
+    const CONSTANT Piece &pc = pieces[wave + kWaves * lane];
+    const uint64_t src = (uint64_t)pc.base + (uint64_t)(int64_t)pc.off;
+    d.src_lo = (uint32_t)src;
+    d.src_hi = (uint32_t)(src >> 32);
+    d.slice = pc.slice;
+    d.dst = pc.lds | (pc.bytes << 20);
+  }
+}
+DEV void wave_dma_issue(const WaveDma &d, int64_t tile, lds_ptr tb) {
+  const int lane = threadIdx.x & 63;
+  for (int i = 0; i < d.np; i++) {
+    const uint64_t base = (uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)d.src_lo, i) |
+                          ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)d.src_hi, i) << 32);
+    const int slice = __builtin_amdgcn_readlane(d.slice, i);
+    const int dst = __builtin_amdgcn_readlane(d.dst, i);
+    const uint8_t *src = (const uint8_t *)(base + (uint64_t)(tile * (int64_t)slice)) + lane * 16;
+    if (lane * 16 < (dst >> 20))
+      __builtin_amdgcn_global_load_lds((const GLOBAL void *)src, (LDSAS void *)(tb + (dst & 0xFFFFF)), 16, 0, 0);
   }
 }
 
@@ -129,7 +154,7 @@ DEV void stage_kept(const CONSTANT Piece *gcols, int ng, lds_ptr gb, int plrow, 
 
 // Operand rows (lane + 64 u) of the op, widened to 8-byte slots.
 template <int R>
-DEV void fetch(int cls, int off, int arg, int arg2, const CONSTANT VmOp &op, clds_ptr tb, int lrow,
+DEV void fetch(int cls, int off, int arg, int arg2, uint64_t im, clds_ptr tb, int lrow,
                clds_ptr tmpb, int tstride, const CONSTANT InCol *in, const int64_t (&grow)[R],
                const bool (&live)[R], uint64_t (&B)[R]) {
   switch (cls) {
@@ -137,11 +162,10 @@ DEV void fetch(int cls, int off, int arg, int arg2, const CONSTANT VmOp &op, cld
 #pragma unroll
       for (int u = 0; u < R; u++) B[u] = 0;
       break;
-    case VC_IMM: {
-      const uint64_t im = op.imm;
+    case VC_IMM:
 #pragma unroll
       for (int u = 0; u < R; u++) B[u] = im;
-    } break;
+      break;
     case VC_TMP: {
       clds_ptr p = tmpb + arg2 * tstride;
 #pragma unroll
@@ -381,15 +405,24 @@ DEV void run_vm(const CONSTANT VmOp *ops, int nops, uint64_t (&acc)[R], lds_ptr 
                 const CONSTANT InCol *in, const CONSTANT OutCol *out, clds_ptr tb, int lrow,
                 const int64_t (&grow)[R], int64_t n, int64_t obase, const int (&orel)[R],
                 const bool (&live)[R]) {
+  // The next op's descriptor is loaded while this one executes.
+  const CONSTANT u32x4 *opw = (const CONSTANT u32x4 *)ops;
+  u32x4 w = u32x4{0u, 0u, 0u, 0u}, wi = w;
+  if (nops > 0) {
+    w = opw[0];
+    wi = opw[1];
+  }
   for (int k = 0; k < nops; k++) {
-    const CONSTANT VmOp &op = ops[k];
-    const u32x4 w = *(const CONSTANT u32x4 *)&op;
+    const int kn = k + 1 < nops ? k + 1 : k;
+    const u32x4 w_next = opw[2 * kn], wi_next = opw[2 * kn + 1];
+    const uint64_t imm = (uint64_t)wi.x | ((uint64_t)wi.y << 32);
+    const uint64_t imm2 = (uint64_t)wi.z | ((uint64_t)wi.w << 32);
     const int code = (int)(w.x & 0xFF), arg = (int)((w.x >> 8) & 0xFF);
     const int cls = (int)((w.x >> 16) & 0xFF), cv = (int)(w.x >> 24);
     const int off = (int)w.z;
     const int arg2 = (int)((w.y >> 16) & 0xFF);
     uint64_t B[R];
-    fetch<R>(cls, off, arg, arg2, op, tb, lrow, tmpb, tstride, in, grow, live, B);
+    fetch<R>(cls, off, arg, arg2, imm, tb, lrow, tmpb, tstride, in, grow, live, B);
     if (cv == CV_I2F) {
 #pragma unroll
       for (int u = 0; u < R; u++) B[u] = d2b((double)(int64_t)B[u]);
@@ -442,7 +475,7 @@ DEV void run_vm(const CONSTANT VmOp *ops, int nops, uint64_t (&acc)[R], lds_ptr 
         compare<R>(arg, acc, B);
         break;
       case VM_CMPAND:
-        compare_and<R>(arg, acc, B, op.imm2);
+        compare_and<R>(arg, acc, B, imm2);
         break;
       case VM_STORE: {
         lds_ptr p = tmpb + arg * tstride;
@@ -484,6 +517,8 @@ DEV void run_vm(const CONSTANT VmOp *ops, int nops, uint64_t (&acc)[R], lds_ptr 
           run_rare<R, OPS>(code, arg, w, acc, B, in, grow, n);
         break;
     }
+    w = w_next;
+    wi = wi_next;
   }
 }
 
@@ -644,30 +679,52 @@ vm_kernel(const VmBatch *__restrict__ bd_arg) {
   if (STAGED) {
     const int64_t n_tiles = bd.n_tiles;
     auto tile_first = [&](int i) { return plans[i].tile0; };
+    auto tile_end = [&](int i) { return i + 1 < n_plans ? plans[i + 1].tile0 : n_tiles; };
     int64_t cur = blockIdx.x;
-    int pi = -1;
+    // The tiles a workgroup visits increase, so the program of its current tile,
+    // and of the tile being staged, change only at program boundaries.
+    int64_t p_first = 0, p_end = -1;  // global tiles of the current program
     VmCtx cx;
+    unsigned long long *null_counts = nullptr;  // the current program's workspace
+    // nulls of output `lane` (< 64) counted by this wave since the program began,
+    // added to the program's counts once, when the wave leaves it
+    unsigned long long nacc = 0;
+    uint64_t *keep = nullptr;
+    uint32_t *wave_count = nullptr;
+    int64_t d_first = 0, d_end = -1, d_full = 0;  // the staged program: its tiles, full tiles
+    WaveDma dma;
     int buf = 0;
     // staging of global tile g (full tiles only; the last tile of a program is
     // staged with bounded loads when it is current)
     auto issue = [&](int64_t g, lds_ptr dst) {
       if (g >= n_tiles) return;
-      const int q = find_plan(n_plans, g, tile_first);
-      const CONSTANT uint8_t *qb = (const CONSTANT uint8_t *)plans[q].blob;
-      const CONSTANT ProgHeader &qh = *(const CONSTANT ProgHeader *)qb;
-      const int64_t t = g - plans[q].tile0;
-      if ((t + 1) * qh.tile_rows <= qh.n_rows)
-        dma_tile((const CONSTANT Piece *)(qb + qh.off_pieces), qh.n_pieces, t, dst);
+      if (g >= d_end) {
+        const int q = find_plan(n_plans, g, tile_first);
+        const CONSTANT uint8_t *qb = (const CONSTANT uint8_t *)plans[q].blob;
+        const CONSTANT ProgHeader &qh = *(const CONSTANT ProgHeader *)qb;
+        d_first = plans[q].tile0;
+        d_end = tile_end(q);
+        d_full = qh.n_rows / qh.tile_rows;
+        wave_dma_load(dma, (const CONSTANT Piece *)(qb + qh.off_pieces), qh.n_pieces);
+      }
+      const int64_t t = g - d_first;
+      if (t < d_full) wave_dma_issue(dma, t, dst);
     };
     issue(cur, lds);
     for (; cur < n_tiles; cur += gridDim.x, buf ^= 1) {
-      const int p = find_plan(n_plans, cur, tile_first);
-      if (p != pi) {
-        pi = p;
+      if (cur >= p_end) {
+        if (MODE == VM_PROJECT && null_counts != nullptr && nacc) atomicAdd(null_counts + lane, nacc);
+        nacc = 0;
+        const int p = find_plan(n_plans, cur, tile_first);
+        p_first = plans[p].tile0;
+        p_end = tile_end(p);
         cx = load_ctx(plans[p].blob);
+        const Workspace ws = load_ws(plans[p].ws);
+        null_counts = ws.null_counts;
+        keep = ws.keep;
+        wave_count = ws.wave_count;
       }
-      const Workspace ws = load_ws(plans[p].ws);
-      const int64_t tile = cur - plans[p].tile0;
+      const int64_t tile = cur - p_first;
       const int T = cx.T;
       const int P = T / (kBlock * R);  // passes per tile
       const int GW = P * R;            // 64-row groups per wave slot
@@ -717,7 +774,8 @@ vm_kernel(const VmBatch *__restrict__ bd_arg) {
                 nulls = __builtin_popcountll(~word & low_mask(n - wi * 64));
               }
               for (int s = 32; s > 0; s >>= 1) nulls += __shfl_xor(nulls, s);
-              if (lane == 0 && nulls) atomicAdd(ws.null_counts + o, (unsigned long long)nulls);
+              if (o < 64) nacc += lane == o ? nulls : 0;
+              else if (lane == 0 && nulls) atomicAdd(null_counts + o, (unsigned long long)nulls);
             }
           }
         }
@@ -740,22 +798,25 @@ vm_kernel(const VmBatch *__restrict__ bd_arg) {
             acc[u] = 0;
           }
           run_vm<R, OPS>(cx.ops, cx.n_ops_p, acc, tmpb, tstride, cx.in, cx.out, tb, lrow, grow, n, 0, orel, none);
-          vm_load_valid<R>(cx.in, cx.n_in, tb, lb >> 6, vr);
-          vm_run_vops<R>(cx.vops, cx.n_vops_p, vr);
+          if (cx.st_vsrc >= 0) {  // the predicate's validity is computed
+            vm_load_valid<R>(cx.in, cx.n_in, tb, lb >> 6, vr);
+            vm_run_vops<R>(cx.vops, cx.n_vops_p, vr);
+          }
 #pragma unroll
           for (int u = 0; u < R; u++) {
             const bool vbit = (vm_vword<R>(cx.st_vsrc, vr, u) >> lane) & 1;
-            const bool keep = lrow + 64 * u < rows && ((acc[u] & st_mask) != 0) && vbit;
-            const uint64_t km = __builtin_amdgcn_ballot_w64(keep);
+            const bool kp = lrow + 64 * u < rows && ((acc[u] & st_mask) != 0) && vbit;
+            const uint64_t km = __builtin_amdgcn_ballot_w64(kp);
             if (lane == ps * R + u) myword = km;
             cnt += __builtin_popcountll(km);
           }
         }
         const int64_t g0 = tile * (T / 64) + wave * GW;  // this wave's first group
-        if (lane < GW) gstore<uint64_t>(ws.keep, g0 + lane, myword);
-        if (lane == 0) gstore<uint32_t>(ws.wave_count, tile * kWaves + wave, (uint32_t)cnt);
+        if (lane < GW) gstore<uint64_t>(keep, g0 + lane, myword);
+        if (lane == 0) gstore<uint32_t>(wave_count, tile * kWaves + wave, (uint32_t)cnt);
       }
     }
+    if (MODE == VM_PROJECT && null_counts != nullptr && nacc) atomicAdd(null_counts + lane, nacc);
     return;
   }
 
@@ -794,15 +855,21 @@ vm_kernel(const VmBatch *__restrict__ bd_arg) {
   span_words(span, myw, obase);
   int pi = -1;
   VmCtx cx;
+  // nulls of output `lane` (< 64) counted by this wave since it entered the
+  // current program, added to the program's counts when the wave leaves it
+  unsigned long long *null_counts = nullptr;
+  unsigned long long nacc = 0;
   for (; span < n_spans; span += stride) {
     uint64_t myw_next, obase_next;
     span_words(span + stride, myw_next, obase_next);
     const int p = find_plan(n_plans, span, span_first);
     if (p != pi) {
+      if (null_counts != nullptr && nacc) atomicAdd(null_counts + lane, nacc);
+      nacc = 0;
       pi = p;
       cx = load_ctx(plans[p].blob);
+      null_counts = load_ws(plans[p].ws).null_counts;
     }
-    const Workspace ws = load_ws(plans[p].ws);
     const int64_t n = cx.n;
     const int n_vout = cx.n_vout;
     const uint32_t c = (uint32_t)__builtin_popcountll(myw);
@@ -820,7 +887,6 @@ vm_kernel(const VmBatch *__restrict__ bd_arg) {
         for (int i = lane; i < VW * n_vout; i += 64) vbits[i] = 0;
         __builtin_amdgcn_wave_barrier();
       }
-      unsigned long long nl[4] = {0, 0, 0, 0};
       for (int g = 0; g * 64 * R < (int)cnt; g++) {
         int orel[R];
         bool live[R];
@@ -871,8 +937,9 @@ vm_kernel(const VmBatch *__restrict__ bd_arg) {
               if (j0 >= (int)cnt) break;  // wave-uniform
               const uint64_t lm = low_mask((int64_t)cnt - j0);
               const uint64_t bits = vm_vword<R>(vsrc, vr, u) & lm;
-              if (o < 4) nl[o] += __builtin_popcountll(~bits & lm);
-              else if (lane == 0) atomicAdd(ws.null_counts + o, (unsigned long long)__builtin_popcountll(~bits & lm));
+              const unsigned long long nn = __builtin_popcountll(~bits & lm);
+              if (o < 64) nacc += lane == o ? nn : 0;
+              else if (lane == 0) atomicAdd(null_counts + o, nn);
               const int L = lo + j0;  // span-local output position of the group's first row
               if (lane == 0 && bits) {
                 LDSAS uint64_t *wp = vbits + vb * VW + (L >> 6);
@@ -905,12 +972,11 @@ vm_kernel(const VmBatch *__restrict__ bd_arg) {
         }
         __builtin_amdgcn_wave_barrier();
       }
-      for (int o = 0; o < cx.n_out && o < 4; o++)
-        if (lane == 0 && nl[o]) atomicAdd(ws.null_counts + o, nl[o]);
     }
     myw = myw_next;
     obase = obase_next;
   }
+  if (null_counts != nullptr && nacc) atomicAdd(null_counts + lane, nacc);
 }
 
 // Exclusive prefix of every program's per-wave-slot kept counts (VM_PRED) in one
