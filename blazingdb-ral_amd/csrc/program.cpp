@@ -466,6 +466,47 @@ int flip_cmp(int c) {  // a OP b  ==  b FLIP(OP) a
 }
 bool is_cmp_code(int c) { return c >= OP_EQ_I && c <= OP_GE_F; }
 
+// Dispatch leaf of a lowered op, once the rows per lane R (and so the byte offsets
+// of temporaries, 8 * 256 R apart) are known.
+void assign_leaf(VmOp &op, int R) {
+  const int tstride = 8 * 256 * R;
+  if (op.cls == VC_TMP) op.off = op.arg2 * tstride;
+  switch (op.code) {
+    case VM_STORE: op.off = op.arg * tstride; op.leaf = kLeafStore; return;
+    case VM_EMIT: op.leaf = kLeafEmit; return;
+    case VM_I2F: op.leaf = kLeafI2F; return;
+    default: break;
+  }
+  int hc = -1;
+  switch (op.cls) {
+    case VC_IMM: hc = HC_IMM; break;  // converted on the host
+    case VC_TMP: hc = op.cv == CV_NONE ? HC_TMP : -1; break;
+    case VC_L64: hc = op.cv == CV_NONE ? HC_L64 : op.cv == CV_I2F ? HC_L64_F : -1; break;
+    case VC_L32I: hc = op.cv == CV_NONE ? HC_L32I : op.cv == CV_I2F ? HC_L32I_F : -1; break;
+    case VC_L32F: hc = op.cv == CV_NONE ? HC_L32F : -1; break;
+    default: break;
+  }
+  int ho = -1;
+  switch (op.code) {
+    case VM_LOAD: ho = HO_LOAD; break;
+    case VM_ADD_I: ho = HO_ADD_I; break;
+    case VM_ADD_F: ho = HO_ADD_F; break;
+    case VM_SUB_I: ho = HO_SUB_I; break;
+    case VM_SUB_F: ho = HO_SUB_F; break;
+    case VM_RSUB_I: ho = HO_RSUB_I; break;
+    case VM_RSUB_F: ho = HO_RSUB_F; break;
+    case VM_MUL_I: ho = HO_MUL_I; break;
+    case VM_MUL_F: ho = HO_MUL_F; break;
+    case VM_AND_B: ho = HO_AND_B; break;
+    case VM_CMP: case VM_CMPAND:
+      for (int i = 0; i < 12; i++)
+        if (kHotCmp[i] == op.arg) ho = (op.code == VM_CMP ? HO_CMP : HO_CMPAND) + i;
+      break;
+    default: break;
+  }
+  op.leaf = hc >= 0 && ho >= 0 ? ho * HC_COUNT_ + hc : kLeafGeneral;
+}
+
 // Lowers the accumulator form to VmOps and lays out the double-buffered tile.
 // Returns false (with the reason) when the plan does not fit the tile VM.
 bool compile_vm(const CompileRequest &req, Builder &B, const std::vector<AccOp> &acc_p,
@@ -700,6 +741,8 @@ bool compile_vm(const CompileRequest &req, Builder &B, const std::vector<AccOp> 
   }
   for (auto &op : vm_q)
     if (op.cls >= VC_L64 && op.cls <= VC_L8U) op.off = req.compact ? goff[op.arg2] : lds_off[op.arg2];
+  for (auto *v : {&vm_p, &vm_q})
+    for (auto &op : *v) assign_leaf(op, R);
 
   std::vector<Piece> pieces;
   for (int c = 0; c < n_dev; c++) {

@@ -137,7 +137,34 @@ enum VmCls : uint8_t {
 // VM_GENERIC flag bits (`gen`): 0-1 operand conversion, 2-3 accumulator conversion,
 // 4-5 result conversion, 6 operand is the left-hand side, 7 unary.
 constexpr uint8_t VG_REV = 0x40, VG_UNARY = 0x80;
+// Hot leaves: the common ops with an operand of a common class, each a
+// straight-line case of the kernel's single dispatch switch (leaf = op * kHotCls
+// + class). Everything else runs through the general path (kLeafGeneral), which
+// decodes code / cls / cv.
+enum VmHotOp : uint8_t {
+  HO_LOAD = 0, HO_ADD_I, HO_ADD_F, HO_SUB_I, HO_SUB_F, HO_RSUB_I, HO_RSUB_F, HO_MUL_I, HO_MUL_F, HO_AND_B,
+  HO_CMP,         // 12 comparisons, acc CMP B, in the order of kHotCmp
+  HO_CMPAND = HO_CMP + 12,  // 12 comparisons, acc &= (B CMP imm2)
+  HO_COUNT_ = HO_CMPAND + 12
+};
+enum VmHotCls : uint8_t {
+  HC_IMM = 0, HC_TMP, HC_L64, HC_L32I, HC_L32F,
+  HC_L32I_F,  // int32 column widened to binary64
+  HC_L64_F,   // int64 column converted to binary64
+  HC_COUNT_
+};
+// Comparison order of the HO_CMP / HO_CMPAND leaves (DevOpCode values).
+constexpr int kHotCmp[12] = {OP_LT_I, OP_LE_I, OP_GT_I, OP_GE_I, OP_EQ_I, OP_NE_I,
+                             OP_LT_F, OP_LE_F, OP_GT_F, OP_GE_F, OP_EQ_F, OP_NE_F};
+constexpr int kHotLeaves = HO_COUNT_ * HC_COUNT_;
+constexpr int kLeafStore = kHotLeaves;      // tmp = acc (off: byte offset of the temporary)
+constexpr int kLeafEmit = kHotLeaves + 1;   // output arg <- acc
+constexpr int kLeafI2F = kHotLeaves + 2;    // acc int64 -> binary64
+constexpr int kLeafGeneral = kHotLeaves + 3;
 struct VmOp {
+  int32_t leaf;   // dispatch id (set once the tile shape is known)
+  int32_t off;    // VC_L*: byte offset of the column slice in the tile buffer;
+                  // VC_TMP operand / VM_STORE: byte offset of the temporary (t * 8 * 256 R)
   uint8_t code;   // VmCode
   uint8_t arg;    // temp / output / comparison / DevOpCode
   uint8_t cls;    // VmCls of the operand (VC_NONE for ops without one)
@@ -146,8 +173,6 @@ struct VmOp {
   uint8_t vcol;   // input whose raw validity COALESCE / IS_[NOT_]NULL read; 0xFF none
   uint8_t arg2;   // VC_G* / VC_L*: input column; VC_TMP: temporary
   uint8_t pad;
-  int32_t off;    // VC_L*: byte offset of the column slice in the tile buffer
-  int32_t pad2;
   uint64_t imm;   // VC_IMM operand
   uint64_t imm2;  // VM_CMPAND immediate
 };

@@ -167,7 +167,7 @@ DEV void fetch(int cls, int off, int arg, int arg2, uint64_t im, clds_ptr tb, in
       for (int u = 0; u < R; u++) B[u] = im;
       break;
     case VC_TMP: {
-      clds_ptr p = tmpb + arg2 * tstride;
+      clds_ptr p = tmpb + off;
 #pragma unroll
       for (int u = 0; u < R; u++) B[u] = *(const LDSAS uint64_t *)(p + 512 * u);
     } break;
@@ -298,7 +298,7 @@ DEV void vcol_bits(const CONSTANT InCol *in, int vcol, const int64_t (&grow)[R],
 
 // The ops outside the common set (see run_vm).
 template <int R, int OPS>
-DEV void run_rare(int code, int arg, const u32x4 &w, uint64_t (&acc)[R], const uint64_t (&B)[R],
+DEV void run_rare(int code, int arg, uint32_t w3, uint64_t (&acc)[R], const uint64_t (&B)[R],
                   const CONSTANT InCol *in, const int64_t (&grow)[R], int64_t n) {
     switch (code) {
       case VM_F2I:
@@ -352,7 +352,7 @@ DEV void run_rare(int code, int arg, const u32x4 &w, uint64_t (&acc)[R], const u
       case VM_COALESCE:
       case VM_RCOALESCE: {
         bool vb[R];
-        vcol_bits<R>(in, (int)((w.y >> 8) & 0xFF), grow, n, vb);
+        vcol_bits<R>(in, (int)((w3 >> 8) & 0xFF), grow, n, vb);
         const bool rev = code == VM_RCOALESCE;
 #pragma unroll
         for (int u = 0; u < R; u++) {
@@ -363,14 +363,14 @@ DEV void run_rare(int code, int arg, const u32x4 &w, uint64_t (&acc)[R], const u
       case VM_IS_NULL:
       case VM_IS_NOT_NULL: {
         bool vb[R];
-        vcol_bits<R>(in, (int)((w.y >> 8) & 0xFF), grow, n, vb);
+        vcol_bits<R>(in, (int)((w3 >> 8) & 0xFF), grow, n, vb);
         const bool want = code == VM_IS_NOT_NULL;
 #pragma unroll
         for (int u = 0; u < R; u++) acc[u] = (uint64_t)(vb[u] == want);
       } break;
       default:
         if constexpr (OPS >= 2) {  // VM_GENERIC
-          const int gen = (int)(w.y & 0xFF), vcol = (int)((w.y >> 8) & 0xFF);
+          const int gen = (int)(w3 & 0xFF), vcol = (int)((w3 >> 8) & 0xFF);
           const void *vp = vcol != 0xFF ? in[vcol].valid : nullptr;
 #pragma nounroll
           for (int u = 0; u < R; u++) {
@@ -384,107 +384,112 @@ DEV void run_rare(int code, int arg, const u32x4 &w, uint64_t (&acc)[R], const u
 }
 
 // Runs `nops` VmOps over rows lrow + 64 u of the tile buffer `tb` (global rows
-// grow[u]). Temporaries live in LDS: temporary t of row u at
-// tmpb + t * tstride + 512 u. VM_EMIT stores row u at output position
-// obase + orel[u] when live[u]. SLOW: the program has ops of the out-of-line set
-// (VM_GENERIC).
+// grow[u]). Temporaries live in LDS: the temporary at byte offset t of row u is at
+// tmpb + t + 512 u. VM_EMIT stores row u at output position obase + orel[u] when
+// live[u].
 //
-// OPS selects the op set compiled into the loop: 0 the common set (loads,
-// + - * of int64 / binary64, comparisons, boolean AND, temporaries, stores,
-// int64 -> binary64); 1 adds the rest of the hot set (division, modulo, floor /
-// ceil / abs, NOT, binary64 -> int64, COALESCE, IS [NOT] NULL); 2 adds the
-// out-of-line set. The host picks the smallest set covering the program: fewer
-// cases mean fewer registers and a shorter dispatch.
+// Dispatch: one uniform switch over the op's leaf id. A hot leaf (a common op with
+// an operand of a common class) is straight-line code: the operand is read where
+// it is used, so no operand value crosses a join and the accumulator is updated in
+// place. The rest decode their fields in the general leaf. Scalar work per op is
+// the switch's compare tree; the next op's descriptor is loaded while this one
+// executes.
 //
-// The loop body is two uniform switches with a single merge point each: the
-// operand fetch by class (VC_NONE for ops without an operand), then the op. Early
-// exits or a separate switch for operand-less ops make the compiler thread the
-// accumulator through extra copies and flag branches (measured: 2.3x slower).
+// OPS selects what the general leaf can run: 0 the common set (any operand class
+// and conversion of the common ops); 1 adds the rest of the hot set (division,
+// modulo, floor / ceil / abs, NOT, binary64 -> int64, COALESCE, IS [NOT] NULL); 2
+// adds the out-of-line set (VM_GENERIC: transcendentals, POWER, date parts).
+#define BLZ_HOT_OPS(X)                                                                         \
+  X(HO_LOAD, acc[u] = b)                                                                       \
+  X(HO_ADD_I, acc[u] = acc[u] + b)                                                             \
+  X(HO_ADD_F, acc[u] = d2b(b2d(acc[u]) + b2d(b)))                                              \
+  X(HO_SUB_I, acc[u] = acc[u] - b)                                                             \
+  X(HO_SUB_F, acc[u] = d2b(b2d(acc[u]) - b2d(b)))                                              \
+  X(HO_RSUB_I, acc[u] = b - acc[u])                                                            \
+  X(HO_RSUB_F, acc[u] = d2b(b2d(b) - b2d(acc[u])))                                             \
+  X(HO_MUL_I, acc[u] = acc[u] * b)                                                             \
+  X(HO_MUL_F, acc[u] = d2b(b2d(acc[u]) * b2d(b)))                                              \
+  X(HO_AND_B, acc[u] = acc[u] & b)                                                             \
+  X(HO_CMP + 0, acc[u] = (uint64_t)((int64_t)acc[u] < (int64_t)b))                             \
+  X(HO_CMP + 1, acc[u] = (uint64_t)((int64_t)acc[u] <= (int64_t)b))                            \
+  X(HO_CMP + 2, acc[u] = (uint64_t)((int64_t)acc[u] > (int64_t)b))                             \
+  X(HO_CMP + 3, acc[u] = (uint64_t)((int64_t)acc[u] >= (int64_t)b))                            \
+  X(HO_CMP + 4, acc[u] = (uint64_t)(acc[u] == b))                                              \
+  X(HO_CMP + 5, acc[u] = (uint64_t)(acc[u] != b))                                              \
+  X(HO_CMP + 6, acc[u] = (uint64_t)(b2d(acc[u]) < b2d(b)))                                     \
+  X(HO_CMP + 7, acc[u] = (uint64_t)(b2d(acc[u]) <= b2d(b)))                                    \
+  X(HO_CMP + 8, acc[u] = (uint64_t)(b2d(acc[u]) > b2d(b)))                                     \
+  X(HO_CMP + 9, acc[u] = (uint64_t)(b2d(acc[u]) >= b2d(b)))                                    \
+  X(HO_CMP + 10, acc[u] = (uint64_t)(b2d(acc[u]) == b2d(b)))                                   \
+  X(HO_CMP + 11, acc[u] = (uint64_t)(b2d(acc[u]) != b2d(b)))                                   \
+  X(HO_CMPAND + 0, acc[u] = (int64_t)b < ii ? acc[u] : 0ull)                                   \
+  X(HO_CMPAND + 1, acc[u] = (int64_t)b <= ii ? acc[u] : 0ull)                                  \
+  X(HO_CMPAND + 2, acc[u] = (int64_t)b > ii ? acc[u] : 0ull)                                   \
+  X(HO_CMPAND + 3, acc[u] = (int64_t)b >= ii ? acc[u] : 0ull)                                  \
+  X(HO_CMPAND + 4, acc[u] = b == im2 ? acc[u] : 0ull)                                          \
+  X(HO_CMPAND + 5, acc[u] = b != im2 ? acc[u] : 0ull)                                          \
+  X(HO_CMPAND + 6, acc[u] = b2d(b) < fi ? acc[u] : 0ull)                                       \
+  X(HO_CMPAND + 7, acc[u] = b2d(b) <= fi ? acc[u] : 0ull)                                      \
+  X(HO_CMPAND + 8, acc[u] = b2d(b) > fi ? acc[u] : 0ull)                                       \
+  X(HO_CMPAND + 9, acc[u] = b2d(b) >= fi ? acc[u] : 0ull)                                      \
+  X(HO_CMPAND + 10, acc[u] = b2d(b) == fi ? acc[u] : 0ull)                                     \
+  X(HO_CMPAND + 11, acc[u] = b2d(b) != fi ? acc[u] : 0ull)
+// Operand of row u by hot class.
+#define BLZ_B_IMM im
+#define BLZ_B_TMP (*(const LDSAS uint64_t *)(tp + 512 * u))
+#define BLZ_B_L64 (*(const LDSAS uint64_t *)(lp8 + 512 * u))
+#define BLZ_B_L32I ((uint64_t)(int64_t)*(const LDSAS int32_t *)(lp4 + 256 * u))
+#define BLZ_B_L32F d2b((double)*(const LDSAS float *)(lp4 + 256 * u))
+#define BLZ_B_L32I_F d2b((double)*(const LDSAS int32_t *)(lp4 + 256 * u))
+#define BLZ_B_L64_F d2b((double)*(const LDSAS int64_t *)(lp8 + 512 * u))
+#define BLZ_LEAF(OP, CLS, FETCH, STMT)                  \
+  case (OP) * HC_COUNT_ + (CLS): {                      \
+    _Pragma("unroll") for (int u = 0; u < R; u++) {     \
+      const uint64_t b = FETCH;                         \
+      STMT;                                             \
+    }                                                   \
+  } break;
+#define BLZ_LEAVES(OP, STMT)                            \
+  BLZ_LEAF(OP, HC_IMM, BLZ_B_IMM, STMT)                 \
+  BLZ_LEAF(OP, HC_TMP, BLZ_B_TMP, STMT)                 \
+  BLZ_LEAF(OP, HC_L64, BLZ_B_L64, STMT)                 \
+  BLZ_LEAF(OP, HC_L32I, BLZ_B_L32I, STMT)               \
+  BLZ_LEAF(OP, HC_L32F, BLZ_B_L32F, STMT)               \
+  BLZ_LEAF(OP, HC_L32I_F, BLZ_B_L32I_F, STMT)           \
+  BLZ_LEAF(OP, HC_L64_F, BLZ_B_L64_F, STMT)
+
 template <int R, int OPS>
 DEV void run_vm(const CONSTANT VmOp *ops, int nops, uint64_t (&acc)[R], lds_ptr tmpb, int tstride,
                 const CONSTANT InCol *in, const CONSTANT OutCol *out, clds_ptr tb, int lrow,
                 const int64_t (&grow)[R], int64_t n, int64_t obase, const int (&orel)[R],
                 const bool (&live)[R]) {
-  // The next op's descriptor is loaded while this one executes.
-  const CONSTANT u32x4 *opw = (const CONSTANT u32x4 *)ops;
-  u32x4 w = u32x4{0u, 0u, 0u, 0u}, wi = w;
-  if (nops > 0) {
-    w = opw[0];
-    wi = opw[1];
-  }
+  typedef uint32_t u32x8 __attribute__((ext_vector_type(8)));
+  static_assert(sizeof(VmOp) == sizeof(u32x8), "VmOp is one 32-byte descriptor");
+  const CONSTANT u32x8 *opw = (const CONSTANT u32x8 *)ops;
+  u32x8 w = nops > 0 ? opw[0] : u32x8{0u, 0u, 0u, 0u, 0u, 0u, 0u, 0u};
+  clds_ptr b4 = tb + 4 * lrow;  // this lane's row in a 4- / 8-byte column slice
+  clds_ptr b8 = tb + 8 * lrow;
   for (int k = 0; k < nops; k++) {
-    const int kn = k + 1 < nops ? k + 1 : k;
-    const u32x4 w_next = opw[2 * kn], wi_next = opw[2 * kn + 1];
-    const uint64_t imm = (uint64_t)wi.x | ((uint64_t)wi.y << 32);
-    const uint64_t imm2 = (uint64_t)wi.z | ((uint64_t)wi.w << 32);
-    const int code = (int)(w.x & 0xFF), arg = (int)((w.x >> 8) & 0xFF);
-    const int cls = (int)((w.x >> 16) & 0xFF), cv = (int)(w.x >> 24);
-    const int off = (int)w.z;
-    const int arg2 = (int)((w.y >> 16) & 0xFF);
-    uint64_t B[R];
-    fetch<R>(cls, off, arg, arg2, imm, tb, lrow, tmpb, tstride, in, grow, live, B);
-    if (cv == CV_I2F) {
-#pragma unroll
-      for (int u = 0; u < R; u++) B[u] = d2b((double)(int64_t)B[u]);
-    } else if (cv == CV_F2I) {
-#pragma unroll
-      for (int u = 0; u < R; u++) B[u] = (uint64_t)f2i(b2d(B[u]));
-    }
-    switch (code) {
-      case VM_LOAD:
-#pragma unroll
-        for (int u = 0; u < R; u++) acc[u] = B[u];
-        break;
-      case VM_ADD_I:
-#pragma unroll
-        for (int u = 0; u < R; u++) acc[u] = acc[u] + B[u];
-        break;
-      case VM_ADD_F:
-#pragma unroll
-        for (int u = 0; u < R; u++) acc[u] = d2b(b2d(acc[u]) + b2d(B[u]));
-        break;
-      case VM_SUB_I:
-#pragma unroll
-        for (int u = 0; u < R; u++) acc[u] = acc[u] - B[u];
-        break;
-      case VM_SUB_F:
-#pragma unroll
-        for (int u = 0; u < R; u++) acc[u] = d2b(b2d(acc[u]) - b2d(B[u]));
-        break;
-      case VM_RSUB_I:
-#pragma unroll
-        for (int u = 0; u < R; u++) acc[u] = B[u] - acc[u];
-        break;
-      case VM_RSUB_F:
-#pragma unroll
-        for (int u = 0; u < R; u++) acc[u] = d2b(b2d(B[u]) - b2d(acc[u]));
-        break;
-      case VM_MUL_I:
-#pragma unroll
-        for (int u = 0; u < R; u++) acc[u] = acc[u] * B[u];
-        break;
-      case VM_MUL_F:
-#pragma unroll
-        for (int u = 0; u < R; u++) acc[u] = d2b(b2d(acc[u]) * b2d(B[u]));
-        break;
-      case VM_AND_B:
-#pragma unroll
-        for (int u = 0; u < R; u++) acc[u] = acc[u] & B[u];
-        break;
-      case VM_CMP:
-        compare<R>(arg, acc, B);
-        break;
-      case VM_CMPAND:
-        compare_and<R>(arg, acc, B, imm2);
-        break;
-      case VM_STORE: {
-        lds_ptr p = tmpb + arg * tstride;
+    const u32x8 w_next = opw[k + 1 < nops ? k + 1 : k];
+    const int leaf = (int)w.s0;
+    const int off = (int)w.s1;
+    const uint64_t im = (uint64_t)w.s4 | ((uint64_t)w.s5 << 32);
+    const uint64_t im2 = (uint64_t)w.s6 | ((uint64_t)w.s7 << 32);
+    const double fi = b2d(im2);
+    const int64_t ii = (int64_t)im2;
+    clds_ptr tp = (clds_ptr)tmpb + off;
+    clds_ptr lp4 = b4 + off, lp8 = b8 + off;
+    switch (leaf) {
+      BLZ_HOT_OPS(BLZ_LEAVES)
+      case kLeafStore: {
+        lds_ptr p = tmpb + off;
 #pragma unroll
         for (int u = 0; u < R; u++) *(LDSAS uint64_t *)(p + 512 * u) = acc[u];
       } break;
-      case VM_EMIT: {
+      case kLeafEmit: {
+        const int arg = (int)((w.s2 >> 8) & 0xFF);
         void *base = out[arg].data;
-        switch ((int)(w.y & 0xFF)) {  // StoreKind
+        switch ((int)(w.s3 & 0xFF)) {  // StoreKind
           case ST_I64: case ST_F64:
 #pragma unroll
             for (int u = 0; u < R; u++) if (live[u]) gstore<uint64_t>(base, obase + orel[u], acc[u]);
@@ -508,19 +513,82 @@ DEV void run_vm(const CONSTANT VmOp *ops, int nops, uint64_t (&acc)[R], lds_ptr 
           default: break;
         }
       } break;
-      case VM_I2F:
+      case kLeafI2F:
 #pragma unroll
         for (int u = 0; u < R; u++) acc[u] = d2b((double)(int64_t)acc[u]);
         break;
-      default:
-        if constexpr (OPS >= 1)
-          run_rare<R, OPS>(code, arg, w, acc, B, in, grow, n);
-        break;
+      default: {  // general leaf
+        const int code = (int)(w.s2 & 0xFF), arg = (int)((w.s2 >> 8) & 0xFF);
+        const int cls = (int)((w.s2 >> 16) & 0xFF), cv = (int)(w.s2 >> 24);
+        const int arg2 = (int)((w.s3 >> 16) & 0xFF);
+        uint64_t B[R];
+        fetch<R>(cls, off, arg, arg2, im, tb, lrow, tmpb, tstride, in, grow, live, B);
+        if (cv == CV_I2F) {
+#pragma unroll
+          for (int u = 0; u < R; u++) B[u] = d2b((double)(int64_t)B[u]);
+        } else if (cv == CV_F2I) {
+#pragma unroll
+          for (int u = 0; u < R; u++) B[u] = (uint64_t)f2i(b2d(B[u]));
+        }
+        switch (code) {
+          case VM_LOAD:
+#pragma unroll
+            for (int u = 0; u < R; u++) acc[u] = B[u];
+            break;
+          case VM_ADD_I:
+#pragma unroll
+            for (int u = 0; u < R; u++) acc[u] = acc[u] + B[u];
+            break;
+          case VM_ADD_F:
+#pragma unroll
+            for (int u = 0; u < R; u++) acc[u] = d2b(b2d(acc[u]) + b2d(B[u]));
+            break;
+          case VM_SUB_I:
+#pragma unroll
+            for (int u = 0; u < R; u++) acc[u] = acc[u] - B[u];
+            break;
+          case VM_SUB_F:
+#pragma unroll
+            for (int u = 0; u < R; u++) acc[u] = d2b(b2d(acc[u]) - b2d(B[u]));
+            break;
+          case VM_RSUB_I:
+#pragma unroll
+            for (int u = 0; u < R; u++) acc[u] = B[u] - acc[u];
+            break;
+          case VM_RSUB_F:
+#pragma unroll
+            for (int u = 0; u < R; u++) acc[u] = d2b(b2d(B[u]) - b2d(acc[u]));
+            break;
+          case VM_MUL_I:
+#pragma unroll
+            for (int u = 0; u < R; u++) acc[u] = acc[u] * B[u];
+            break;
+          case VM_MUL_F:
+#pragma unroll
+            for (int u = 0; u < R; u++) acc[u] = d2b(b2d(acc[u]) * b2d(B[u]));
+            break;
+          case VM_AND_B:
+#pragma unroll
+            for (int u = 0; u < R; u++) acc[u] = acc[u] & B[u];
+            break;
+          case VM_CMP:
+            compare<R>(arg, acc, B);
+            break;
+          case VM_CMPAND:
+            compare_and<R>(arg, acc, B, im2);
+            break;
+          default:
+            if constexpr (OPS >= 1) run_rare<R, OPS>(code, arg, w.s3, acc, B, in, grow, n);
+            break;
+        }
+      } break;
     }
     w = w_next;
-    wi = wi_next;
   }
 }
+#undef BLZ_LEAVES
+#undef BLZ_LEAF
+#undef BLZ_HOT_OPS
 
 // Validity registers live in LDS: word u (64-row group u of the pass) of register
 // s of a wave is vr[s * R + u]. Lane u < R computes group u; a register is read
