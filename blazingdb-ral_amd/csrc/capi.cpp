@@ -128,13 +128,13 @@ void check_column(const gdf_column *c, const char *what, int64_t n) {
 
 // Device scratch of one launch:
 //   blob | VmBatch + VmPlanRef | null counts | counters | tile states  (zeroed part
-//   starts at the null counts) | keep words | slot counts | slot prefixes
+//   starts at the null counts) | keep words | slot counts | kept-row indices
 struct LaunchBuffers {
   uint8_t *base = nullptr;
   size_t bytes = 0;
   size_t bd_off = 0;
   size_t zero_off = 0, zero_bytes = 0;
-  size_t keep_off = 0, count_off = 0, prefix_off = 0;
+  size_t keep_off = 0, count_off = 0, idx_off = 0;
   Workspace ws{};
   uint8_t *blob = nullptr;
   VmBatch *bd = nullptr;  // device address of the launch descriptor
@@ -161,8 +161,8 @@ void layout(LaunchBuffers &lb, const Compiled &c, int n_out, bool compact, int64
   off += round_up(vmf ? sizeof(uint64_t) * n_tiles * (c.T / 64) : 0, 256);
   lb.count_off = off;
   off += round_up(sizeof(uint32_t) * slots, 256);
-  lb.prefix_off = off;
-  off += round_up(sizeof(uint32_t) * slots, 256);
+  lb.idx_off = off;
+  off += round_up(vmf ? sizeof(uint32_t) * n : 0, 256);
   lb.bytes = std::max<size_t>(off, 256);
 }
 void bind(LaunchBuffers &lb, int n_out) {
@@ -177,7 +177,7 @@ void bind(LaunchBuffers &lb, int n_out) {
   lb.ws.tile_state = (unsigned long long *)(z + nc + 64);
   lb.ws.keep = (uint64_t *)(p + lb.keep_off);
   lb.ws.wave_count = (uint32_t *)(p + lb.count_off);
-  lb.ws.wave_prefix = (uint32_t *)(p + lb.prefix_off);
+  lb.ws.idx = (uint32_t *)(p + lb.idx_off);
 }
 // Host image of the blob and the single-program launch descriptor, copied to
 // lb.blob in one transfer.
@@ -191,15 +191,13 @@ std::vector<uint8_t> device_image(const LaunchBuffers &lb, const Compiled &c) {
     b.bufb = c.vm_bufb;
     b.tmp_off = c.vm_tmp_off;
     b.g_buf_off = c.vm_g_buf_off;
-    b.g_vb_off = c.vm_g_vb_off;
     b.g_tmp_off = c.vm_g_tmp_off;
+    b.g_cpre_off = c.vm_g_cpre_off;
     b.g_wave_bytes = c.vm_g_wave_bytes;
-    b.g_vb_wave_words = 66 * c.vm_nvout;
     b.vr_off = c.vm_vr_off;
     b.g_vr_off = c.vm_g_vr_off;
     b.vr_wave_words = c.n_vslots * c.R;
     b.n_tiles = tiles;
-    b.n_spans = (tiles * (c.T / 64) + 63) / 64;
     b.n_chunks = lb.n_chunks;
     b.chunk_counter = lb.ws.tile_counter;
     b.plans = (const VmPlanRef *)(lb.base + lb.bd_off + sizeof(VmBatch));
@@ -225,7 +223,7 @@ hipError_t launch_program(const Compiled &c, bool compact, int grid, hipStream_t
   if (e != hipSuccess) return e;
   if (c.vm) {
     const int64_t tiles = (c.n_rows + c.T - 1) / c.T;
-    const int64_t spans = (tiles * (c.T / 64) + 63) / 64;
+    const int64_t chunks = (c.n_rows + 64 * c.R - 1) / (64 * c.R);  // most gather chunks (all rows kept)
     auto grid_for = [&](int mode, int lds, int64_t units) {
       const int64_t resident = (int64_t)device_cus() * vm_per_cu(c, mode, lds);
       return (int)std::max<int64_t>(1, std::min<int64_t>(units, resident));
@@ -239,7 +237,7 @@ hipError_t launch_program(const Compiled &c, bool compact, int grid, hipStream_t
     if (e == hipSuccess) e = launch_slot_scan(bd, n_chunks, st);
     if (e == hipSuccess) e = mark(2);
     if (e == hipSuccess)
-      e = launch_vm(c.R, 2, c.vm_slow, grid_for(2, c.lds_gather, (spans + 3) / 4), c.lds_gather, st, bd);
+      e = launch_vm(c.R, 2, c.vm_slow, grid_for(2, c.lds_gather, (chunks + 3) / 4), c.lds_gather, st, bd);
     return e == hipSuccess ? mark(3) : e;
   }
   e = launch_interp(c.S, c.R, compact, grid, c.lds_bytes, st, blob, ws);
@@ -801,8 +799,9 @@ hipError_t program_enqueue(blz_program p, hipStream_t st) {
   if (p->n == 0) return hipSuccess;
   hipError_t e = launch_fill_zero(p->lb.base + p->lb.zero_off, (int64_t)p->lb.zero_bytes,
                                   (int)std::min<size_t>(1024, p->lb.zero_bytes / 2048 + 1), st);
-  // compaction ORs boundary validity words: clear the outputs' validity first
-  for (size_t i = 0; e == hipSuccess && p->compact && i < p->outs.size(); i++)
+  // the direct kernel's compaction ORs boundary validity words: clear the outputs'
+  // validity first (the tile VM writes every word of its outputs whole)
+  for (size_t i = 0; e == hipSuccess && p->compact && !p->c.vm && i < p->outs.size(); i++)
     if (p->outs[i].valid) {
       const size_t vb = gdf_valid_allocation_size((gdf_size_type)p->n);
       e = launch_fill_zero(p->outs[i].valid, (int64_t)vb, (int)std::min<size_t>(1024, vb / 2048 + 1), st);
@@ -851,7 +850,7 @@ int blz_program_time_stages(blz_program prog, blz_stream_t stream, int iters, do
       // the zero-fill of counters is part of a launch but not of the kernel time
       HIPCHECK(launch_fill_zero(prog->lb.base + prog->lb.zero_off, (int64_t)prog->lb.zero_bytes,
                                 (int)std::min<size_t>(1024, prog->lb.zero_bytes / 2048 + 1), S(stream)));
-      for (size_t o = 0; prog->compact && o < prog->outs.size(); o++)
+      for (size_t o = 0; prog->compact && !prog->c.vm && o < prog->outs.size(); o++)
         if (prog->outs[o].valid) {
           const size_t vb = gdf_valid_allocation_size((gdf_size_type)prog->n);
           HIPCHECK(launch_fill_zero(prog->outs[o].valid, (int64_t)vb, (int)std::min<size_t>(1024, vb / 2048 + 1),
@@ -1017,7 +1016,7 @@ struct SubBatch {
   VmBatch *d_bd = nullptr;    // device descriptor (VmBatch + VmPlanRef[])
   int lds = 0, lds_gather = 0;
   int slow = 0;  // op set level (max over the programs)
-  int64_t n_tiles = 0, n_spans = 0, n_chunks = 0;
+  int64_t n_tiles = 0, n_chunks = 0;
   int grid = 1, grid_gather = 1;
 };
 
@@ -1114,8 +1113,8 @@ blz_batch batch_build(const blz_query *qs, int n, int r, int t, bool use_graph, 
     auto build_sub = [&](SubBatch &sb, std::vector<uint8_t> &img, size_t off) {
       VmBatch bd{};
       std::vector<VmPlanRef> refs;
-      int maxbuf = 0, maxtmp = 0, maxgw = 0, maxnv = 0, maxgtmp = 0, maxvs = 0;
-      int64_t tile0 = 0, span0 = 0, chunk0 = 0;
+      int maxbuf = 0, maxtmp = 0, maxgw = 0, maxgtmp = 0, maxvs = 0;
+      int64_t tile0 = 0, chunk0 = 0, gchunks = 0;
       for (int i : sb.members) {
         blz_program p = b->progs[i];
         const Compiled &c = p->c;
@@ -1123,18 +1122,16 @@ blz_batch batch_build(const blz_query *qs, int n, int r, int t, bool use_graph, 
         ref.blob = p->lb.blob;
         ref.ws = p->lb.ws;
         ref.tile0 = tile0;
-        ref.span0 = span0;
         ref.chunk0 = chunk0;
         refs.push_back(ref);
         const int64_t tiles = (c.n_rows + c.T - 1) / c.T;
         tile0 += tiles;
-        span0 += (tiles * (c.T / 64) + 63) / 64;
+        gchunks += (c.n_rows + 64 * b->r - 1) / (64 * b->r);
         chunk0 += (tiles * 4 + kScanChunk - 1) / kScanChunk;
         maxbuf = std::max(maxbuf, c.vm_bufb);
         maxtmp = std::max(maxtmp, c.lds_bytes - c.vm_tmp_off);
         maxvs = std::max(maxvs, c.n_vslots);
         maxgw = std::max(maxgw, c.vm_g_wave_bytes);
-        maxnv = std::max(maxnv, c.vm_nvout);
         maxgtmp = std::max(maxgtmp, c.lds_gather - c.vm_g_tmp_off);
         sb.slow = std::max(sb.slow, c.vm_slow);
       }
@@ -1145,20 +1142,17 @@ blz_batch batch_build(const blz_query *qs, int n, int r, int t, bool use_graph, 
       bd.tmp_off = bd.vr_off + 4 * 8 * bd.vr_wave_words;
       bd.g_buf_off = 0;
       bd.g_wave_bytes = maxgw;
-      bd.g_vb_wave_words = 66 * maxnv;
-      bd.g_vb_off = 4 * maxgw;
-      bd.g_vr_off = bd.g_vb_off + 4 * 66 * 8 * maxnv;
+      bd.g_vr_off = 4 * maxgw;
       bd.g_tmp_off = bd.g_vr_off + 4 * 8 * bd.vr_wave_words;
+      bd.g_cpre_off = (int32_t)round_up((size_t)(bd.g_tmp_off + maxgtmp), 8);
       bd.n_tiles = tile0;
-      bd.n_spans = span0;
       bd.n_chunks = chunk0;
       bd.chunk_counter = b->d_counter;
       bd.plans = (const VmPlanRef *)(b->d_mem + off + sizeof(VmBatch));
       sb.lds = bd.tmp_off + maxtmp;
-      sb.lds_gather = bd.g_tmp_off + maxgtmp;
+      sb.lds_gather = bd.g_cpre_off + 8 * (bd.n_plans + 1);
       if (sb.lds > 64 * 1024 || sb.lds_gather > 64 * 1024) invalid("batch needs more than 64 KiB of LDS");
       sb.n_tiles = tile0;
-      sb.n_spans = span0;
       sb.n_chunks = chunk0;
       sb.d_bd = (VmBatch *)(b->d_mem + off);
       std::memcpy(img.data() + off, &bd, sizeof(bd));
@@ -1171,12 +1165,12 @@ blz_batch batch_build(const blz_query *qs, int n, int r, int t, bool use_graph, 
         const int mode = sb.members.empty() ? 0 : (b->progs[sb.members[0]]->compact ? 1 : 0);
         sb.grid = (int)std::max<int64_t>(1, std::min<int64_t>(tile0, (int64_t)cus * vm_per_cu(probe, mode, sb.lds)));
         sb.grid_gather = (int)std::max<int64_t>(
-            1, std::min<int64_t>((span0 + 3) / 4, (int64_t)cus * vm_per_cu(probe, 2, std::max(sb.lds_gather, 1))));
+            1, std::min<int64_t>((gchunks + 3) / 4, (int64_t)cus * vm_per_cu(probe, 2, std::max(sb.lds_gather, 1))));
       }
     };
     const size_t proj_sz = round_up(sizeof(VmBatch) + sizeof(VmPlanRef) * b->proj.members.size(), 256);
     const size_t filt_sz = round_up(sizeof(VmBatch) + sizeof(VmPlanRef) * b->filt.members.size(), 256);
-    // zero ranges: each program's counters, the filtered outputs' validity, the scan counter
+    // zero ranges: each program's counters, the scan counter
     std::vector<void *> rp;
     std::vector<int64_t> rb;
     for (size_t i = 0; i < b->progs.size(); i++) {
@@ -1184,12 +1178,6 @@ blz_batch batch_build(const blz_query *qs, int n, int r, int t, bool use_graph, 
       if (std::find(b->others.begin(), b->others.end(), (int)i) != b->others.end()) continue;
       rp.push_back(p->lb.base + p->lb.zero_off);
       rb.push_back((int64_t)p->lb.zero_bytes);
-      if (p->compact)
-        for (auto &o : p->outs)
-          if (o.valid) {
-            rp.push_back(o.valid);
-            rb.push_back((int64_t)round_up(gdf_valid_allocation_size((gdf_size_type)p->n), 8));
-          }
     }
     const size_t ctr_off = proj_sz + filt_sz;
     const size_t ptr_off = ctr_off + 256;
@@ -1533,7 +1521,8 @@ size_t srv_estimate(gdf_column *const *frame, int n_frame, const char *filt, con
     for (int i = 0; i < n_frame; i++) out += (size_t)n * std::max(1, blz_dtype_width((gdf_dtype)dt[i])) + n / 8 + 64;
   }
   if (filt) (void)planner::plan_expression(dt, condition_of(filt));
-  *ws = (size_t)n / 8 + (size_t)n / 64 + 64 * 1024;
+  // keep words, slot counts and (a filter) the kept-row indices, plus descriptors
+  *ws = (size_t)n / 8 + (size_t)n / 64 + (filt ? (size_t)n * 4 : 0) + 64 * 1024;
   return out;
 }
 

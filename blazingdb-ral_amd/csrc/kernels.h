@@ -18,7 +18,7 @@ struct Workspace {
   // tile VM filter (written before they are read; no zeroing needed)
   uint64_t *keep;                   // keep word per 64-row group
   uint32_t *wave_count;             // kept rows per (tile, wave) slot
-  uint32_t *wave_prefix;            // exclusive prefix of wave_count
+  uint32_t *idx;                    // row of each kept row, in output order (n_sel entries)
 };
 
 // Register-file interpreter (project or fused filter+project/gather), the general
@@ -29,44 +29,45 @@ hipError_t launch_interp(int s, int r, bool compact, int grid, int lds_bytes, hi
                          const uint8_t *blob, Workspace ws);
 int interp_blocks_per_cu(int s, int r, bool compact, int lds_bytes);
 // Tile VM launch descriptor (device memory): one compiled program, or a batch of
-// independent programs processed by one persistent launch. Global tile / span /
-// scan-chunk ids run over the programs in order; each program keeps its own
-// workspace. The LDS offsets are the maxima over the programs.
+// independent programs processed by one persistent launch. Global tile and
+// scan-chunk ids run over the programs in order (gather chunks likewise, from the
+// programs' kept-row counts); each program keeps its own workspace. The LDS
+// offsets are the maxima over the programs.
 struct VmPlanRef {
   const uint8_t *blob;   // ProgHeader | ... of the program (acc_mode 2)
   Workspace ws;          // the program's workspace
   int64_t tile0;         // first global tile of the program
-  int64_t span0;         // first global gather span (4096 rows)
   int64_t chunk0;        // first global slot-scan chunk
 };
 struct VmBatch {
   int32_t n_plans;
   int32_t bufb;                      // stride of the two tile buffers
   int32_t tmp_off;                   // LDS temporaries of the projection / predicate kernels
-  int32_t g_buf_off, g_vb_off, g_tmp_off;  // gather kernel: pass buffers, validity words, temporaries
+  int32_t g_buf_off, g_tmp_off;     // gather kernel: pass buffers, temporaries
+  int32_t g_cpre_off;                // gather kernel: per-program chunk prefix (n_plans + 1 int64)
   int32_t g_wave_bytes;              // gather kernel: one wave's pass buffer (64 R rows)
-  int32_t g_vb_wave_words;           // gather kernel: one wave's validity words
   int32_t vr_off, g_vr_off;          // validity registers (projection / predicate, gather kernel)
   int32_t vr_wave_words;             // one wave's validity registers (registers x R words)
-  int64_t n_tiles, n_spans, n_chunks;      // totals over the programs
+  int64_t n_tiles, n_chunks;        // totals over the programs
   unsigned int *chunk_counter;       // slot scan: dynamic chunk ids (zero before the scan)
   const VmPlanRef *plans;            // [n_plans]
 };
 
-// Tile VM (vm_kernel.hip): r rows per lane in {8, 4, 2}; tile = 256 r P rows;
-// lds_bytes: two tile buffers + temporaries (modes 0, 1), or the gather kernel's
-// keep words, pass buffer, validity words and temporaries (mode 2). ops: the op set
-// compiled in (0 common, 1 hot, 2 hot + out-of-line: transcendentals, POWER, date
-// parts).
+// Tile VM (vm_device.h): r rows per lane in {8, 4, 2}; tile = 256 r P rows;
+// lds_bytes: two tile buffers, validity registers and temporaries (modes 0, 1), or
+// the gather kernel's pass buffers, validity registers, temporaries and chunk
+// prefix (mode 2). ops: the op set compiled in (0 common, 1 hot, 2 hot +
+// out-of-line: transcendentals, POWER, date parts).
 // mode: 0 projection, 1 predicate (keep words + slot counts), 2 gather of the kept
 // rows (after launch_slot_scan).
 hipError_t launch_vm(int r, int mode, int ops, int grid, int lds_bytes, hipStream_t st, const VmBatch *bd);
 int vm_blocks_per_cu(int r, int mode, int ops, int lds_bytes);
-// Exclusive prefix of every program's slot counts (ws.wave_count -> ws.wave_prefix,
-// total to ws.n_sel): chunks of 4096 slots, combined within a program by a
-// single-pass decoupled look-back over its ws.tile_state (zero before the scan).
+// Exclusive prefix of every program's slot counts (ws.wave_count; total to
+// ws.n_sel) and, from it and the keep words, the row of every kept row in output
+// order (ws.idx): chunks of 4096 slots, combined within a program by a single-pass
+// decoupled look-back over its ws.tile_state (zero before the scan).
 hipError_t launch_slot_scan(const VmBatch *bd, int64_t n_chunks, hipStream_t st);
-constexpr int64_t kScanChunk = 4096;  // slots per scan chunk
+constexpr int64_t kScanChunk = 1024;  // slots per scan chunk
 // Seeded synthetic column in place (dist: 0 int uniform [a,b], 1 real uniform [a,b),
 // 2 normal(a,b), 3 lognormal(a,b), 4 k/100 for k in [a,b], 5 a + row).
 hipError_t launch_synth(void *data, uint8_t *valid, int64_t n, int dtype, uint64_t seed, int dist, double a,

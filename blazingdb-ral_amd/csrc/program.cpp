@@ -715,7 +715,7 @@ bool compile_vm(const CompileRequest &req, Builder &B, const std::vector<AccOp> 
       int b = 0;
       const size_t tot = layout(t, b) + tb;
       if (tot > (size_t)budget) continue;
-      const int gl = g + 4 * n_valid_out * 66 * 8 + tb;
+      const int gl = g + tb + 16;
       if (req.compact && gl > budget) continue;
       R = r; T = t; buf = b; lds_total = tot; tmp_bytes = tb;
       break;
@@ -795,14 +795,15 @@ bool compile_vm(const CompileRequest &req, Builder &B, const std::vector<AccOp> 
   h.any_valid_out = any_valid;
   h.tile_rows = T;
   h.n_chunks = 0;
-  // gather kernel LDS: pass buffer | validity words (per wave) | temporaries
+  // gather kernel LDS: pass buffers (four waves') | validity registers |
+  // temporaries | chunk prefix (one program: two int64)
   const int g_kw = 0;
   const int g_buf = 0;
-  const int g_vb = g_buf + 4 * gbuf;  // four waves' regions
   const int vr_bytes = 4 * n_vslots * R * 8;
-  const int g_vr = g_vb + 4 * n_valid_out * 66 * 8;  // 64 + 2 words per wave and output
+  const int g_vr = g_buf + 4 * gbuf;
   const int g_tmp = g_vr + vr_bytes;
-  h.lds_vbits_off = g_vb;
+  const int g_cpre = (g_vr + tmp_bytes + 7) & ~7;  // tmp_bytes: validity registers + temporaries
+  h.lds_vbits_off = 0;
   h.n_gcols = (int)gcols.size();
   h.off_gcols = (int)off_gcols;
   h.gbuf_bytes = gbuf;
@@ -871,7 +872,8 @@ bool compile_vm(const CompileRequest &req, Builder &B, const std::vector<AccOp> 
     for (auto &op : *v)
       if (level == 0 && op.code > VM_I2F && op.code != VM_GENERIC) level = 1;
   out.vm_slow = level;
-  out.lds_gather = g_vr + tmp_bytes;  // tmp_bytes: validity registers + temporaries
+  out.lds_gather = g_cpre + 16;
+  out.vm_g_cpre_off = g_cpre;
   out.vm_bufb = buf;
   out.vm_vr_off = 2 * buf;
   out.vm_g_vr_off = g_vr;
@@ -879,7 +881,6 @@ bool compile_vm(const CompileRequest &req, Builder &B, const std::vector<AccOp> 
   out.vm_g_buf_off = g_buf;
   out.vm_g_wave_bytes = gbuf;
   out.vm_nvout = n_valid_out;
-  out.vm_g_vb_off = g_vb;
   out.vm_g_tmp_off = g_tmp;
   (void)vcol_needed;
   return true;

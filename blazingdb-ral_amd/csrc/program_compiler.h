@@ -66,7 +66,7 @@ struct Compiled {
   int vm_slow = 0;  // op set the VM program needs: 0 common, 1 hot, 2 with out-of-line ops
   int lds_gather = 0;    // tile VM filter: LDS of the gather kernel
   int vm_bufb = 0, vm_tmp_off = 0;                      // tile VM LDS layout (projection / predicate)
-  int vm_g_buf_off = 0, vm_g_vb_off = 0, vm_g_tmp_off = 0;  // tile VM LDS layout (gather)
+  int vm_g_buf_off = 0, vm_g_tmp_off = 0, vm_g_cpre_off = 0;  // tile VM LDS layout (gather)
   int vm_g_wave_bytes = 0, vm_nvout = 0;  // gather pass buffer per wave, outputs with validity
   int vm_vr_off = 0, vm_g_vr_off = 0;     // validity registers (four waves' regions)
 };
