@@ -60,8 +60,8 @@ def stage_bytes(n, kept, tile_rows):
     """Algorithmic bytes of each kernel of the filter+project program."""
     tiles = (n + tile_rows - 1) // tile_rows
     pred = 16 * n + n / 8 + 16 * tiles          # a, c, d read; keep words + slot counts written
-    scan = 8 * 4 * tiles                         # slot counts read, prefixes written
-    gather = n / 8 + 20 * kept + 12 * kept      # keep words; a, b, c of kept rows; outputs
+    scan = 16 * tiles + n / 8 + 4 * kept        # slot counts, keep words read; kept-row indices written
+    gather = 4 * kept + 20 * kept + 12 * kept   # indices; a, b, c of kept rows; outputs
     return [pred, scan, gather]
 
 
@@ -71,7 +71,7 @@ def traffic_from_profile(kernel):
         return None
     for line in open(PROFILE):
         m = re.match(r"^traffic_bytes_per_launch\[(.+)\]:\s*([0-9.eE+]+)", line.strip())
-        if m and m.group(1) in kernel:
+        if m and m.group(1).startswith(kernel):
             return float(m.group(2))
     return None
 
@@ -253,7 +253,8 @@ def main():
     names = STAGES if len(stage_ms) == 3 else ["program"]
     sbytes = stage_bytes(n, kept, prog.tile_rows) if len(stage_ms) == 3 else [algorithmic_bytes(n, kept)]
     dom = int(np.argmax(stage_ms))
-    kernel_name = (f"vm_kernel<{prog.r}, {1 if dom == 0 else 2}, false>" if names[dom] != "slot_scan"
+    # demangled kernel name prefix: vm_kernel<rows per lane, mode (1 predicate, 2 gather), op set>
+    kernel_name = (f"vm_kernel<{prog.r}, {1 if dom == 0 else 2}," if names[dom] != "slot_scan"
                    else "slot_scan_kernel") if len(stage_ms) == 3 else f"kernel {prog.kernel}"
     achieved = sbytes[dom] / (stage_ms[dom] * 1e-3) / 1e9
     bytes_step = algorithmic_bytes(n, kept)

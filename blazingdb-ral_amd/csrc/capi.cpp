@@ -1109,6 +1109,35 @@ blz_batch batch_build(const blz_query *qs, int n, int r, int t, bool use_graph, 
         b->others.push_back(i);
       }
     }
+    // One persistent launch sizes its LDS by the maxima over its programs (tile
+    // buffers, validity registers, temporaries): while that exceeds 64 KiB, the
+    // program with the largest own LDS leaves the shared launch for one of its own.
+    auto shared_lds = [&](const std::vector<int> &m) {
+      int maxbuf = 0, maxtmp = 0, maxgw = 0, maxgtmp = 0, maxvs = 0;
+      for (int i : m) {
+        const Compiled &c = b->progs[i]->c;
+        maxbuf = std::max(maxbuf, c.vm_bufb);
+        maxtmp = std::max(maxtmp, c.lds_bytes - c.vm_tmp_off);
+        maxvs = std::max(maxvs, c.n_vslots);
+        maxgw = std::max(maxgw, c.vm_g_wave_bytes);
+        maxgtmp = std::max(maxgtmp, c.lds_gather - c.vm_g_tmp_off);
+      }
+      const int vr = 4 * 8 * maxvs * r;
+      const int staged = 2 * maxbuf + vr + maxtmp;
+      const int gather = (int)round_up((size_t)(4 * maxgw + vr + maxgtmp), 8) + 8 * ((int)m.size() + 1);
+      return std::max(staged, gather);
+    };
+    for (SubBatch *sb : {&b->proj, &b->filt}) {
+      while (sb->members.size() > 1 && shared_lds(sb->members) > 64 * 1024) {
+        auto big = std::max_element(sb->members.begin(), sb->members.end(), [&](int x, int y) {
+          const Compiled &cx = b->progs[x]->c, &cy = b->progs[y]->c;
+          return std::max(cx.lds_bytes, cx.lds_gather) < std::max(cy.lds_bytes, cy.lds_gather);
+        });
+        b->others.push_back(*big);
+        sb->members.erase(big);
+      }
+    }
+    std::sort(b->others.begin(), b->others.end());
     // Host images of the two descriptors and the zero-range table.
     auto build_sub = [&](SubBatch &sb, std::vector<uint8_t> &img, size_t off) {
       VmBatch bd{};
