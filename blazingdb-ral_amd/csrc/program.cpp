@@ -502,9 +502,79 @@ void assign_leaf(VmOp &op, int R) {
       for (int i = 0; i < 12; i++)
         if (kHotCmp[i] == op.arg) ho = (op.code == VM_CMP ? HO_CMP : HO_CMPAND) + i;
       break;
+    case VM_RANGE_I: ho = HO_RANGE_I; break;
+    case VM_RANGE_F: ho = HO_RANGE_F; break;
     default: break;
   }
   op.leaf = hc >= 0 && ho >= 0 ? ho * HC_COUNT_ + hc : kLeafGeneral;
+}
+
+// Conjunctions of comparisons (filters such as a date range, a discount range and
+// a quantity bound): a predicate opening with LOAD x; CMP imm opens with
+// CMPAND(x CMP imm) instead (the predicate kernel starts with acc = 1), the
+// CMPANDs of a run are grouped by operand (AND of 0/1 values commutes), and a
+// lower and an upper bound on one operand become one range op.
+bool cmp_is_float(int sub) {
+  return sub == OP_LT_F || sub == OP_LE_F || sub == OP_GT_F || sub == OP_GE_F || sub == OP_EQ_F || sub == OP_NE_F;
+}
+bool cmp_lower(int sub) { return sub == OP_GT_I || sub == OP_GE_I || sub == OP_GT_F || sub == OP_GE_F; }
+bool cmp_upper(int sub) { return sub == OP_LT_I || sub == OP_LE_I || sub == OP_LT_F || sub == OP_LE_F; }
+void fuse_conjunctions(std::vector<VmOp> &v, bool predicate) {
+  auto has_operand = [](const VmOp &o) { return o.cls != VC_NONE && o.cls != VC_IMM; };
+  if (predicate && v.size() >= 2 && v[0].code == VM_LOAD && has_operand(v[0]) && v[1].code == VM_CMP &&
+      v[1].cls == VC_IMM && is_cmp_code(v[1].arg)) {
+    VmOp c = v[0];
+    c.code = VM_CMPAND;
+    c.arg = v[1].arg;
+    c.imm2 = v[1].imm;
+    c.imm = 0;
+    v.erase(v.begin());
+    v[0] = c;
+  }
+  auto key = [](const VmOp &o) { return (o.cls << 16) | (o.arg2 << 8) | o.cv; };
+  for (size_t i = 0; i < v.size();) {
+    size_t j = i;
+    while (j < v.size() && v[j].code == VM_CMPAND) j++;
+    if (j - i >= 2) {
+      std::stable_sort(v.begin() + i, v.begin() + j, [&](const VmOp &a, const VmOp &b) { return key(a) < key(b); });
+      for (size_t k = i; k + 1 < j; k++) {
+        VmOp &a = v[k], &b = v[k + 1];
+        if (a.code != VM_CMPAND || b.code != VM_CMPAND || key(a) != key(b) || !has_operand(a)) continue;
+        if (cmp_is_float(a.arg) != cmp_is_float(b.arg)) continue;
+        const VmOp *lo = cmp_lower(a.arg) && cmp_upper(b.arg) ? &a : cmp_lower(b.arg) && cmp_upper(a.arg) ? &b : nullptr;
+        if (!lo) continue;
+        const VmOp *hi = lo == &a ? &b : &a;
+        VmOp r = a;
+        if (cmp_is_float(a.arg)) {
+          r.code = VM_RANGE_F;
+          r.imm2 = lo->imm2;
+          r.imm = hi->imm2;
+          r.gen = (uint8_t)((lo->arg == OP_GE_F ? 1 : 0) | (hi->arg == OP_LE_F ? 2 : 0));
+        } else {
+          int64_t l = (int64_t)lo->imm2, h = (int64_t)hi->imm2;
+          if (lo->arg == OP_GE_I) {
+            if (l == INT64_MIN) continue;
+            l -= 1;
+          }
+          if (hi->arg == OP_LE_I) {
+            if (h == INT64_MAX) continue;
+            h += 1;
+          }
+          r.code = VM_RANGE_I;
+          r.imm2 = (uint64_t)l;
+          r.imm = (uint64_t)h;
+        }
+        a = r;
+        b.code = VM_COUNT_;  // removed below
+        k++;
+      }
+      v.erase(std::remove_if(v.begin() + i, v.begin() + j, [](const VmOp &o) { return o.code == VM_COUNT_; }),
+              v.begin() + j);
+      j = i;
+      while (j < v.size() && (v[j].code == VM_CMPAND || v[j].code == VM_RANGE_I || v[j].code == VM_RANGE_F)) j++;
+    }
+    i = j > i ? j : i + 1;
+  }
 }
 
 // Lowers the accumulator form to VmOps and lays out the double-buffered tile.
@@ -664,6 +734,8 @@ bool compile_vm(const CompileRequest &req, Builder &B, const std::vector<AccOp> 
   if (!lower(acc_p, vm_p)) return false;
   gather_pass = req.compact;
   if (!lower(acc_q, vm_q)) return false;
+  fuse_conjunctions(vm_p, true);
+  fuse_conjunctions(vm_q, false);
   // The gather pass of a filter stages the kept rows of the columns its ops read
   // into a pass buffer (256 R rows x each width) before evaluating them.
   std::vector<int> goff(n_dev, -1);

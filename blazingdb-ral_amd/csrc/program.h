@@ -120,6 +120,9 @@ enum VmCode : uint8_t {
   VM_AND_B,     // acc &= B (both 0/1)
   VM_CMP,       // acc = acc CMP B; arg = comparison DevOpCode
   VM_CMPAND,    // acc &= (B CMP imm2); arg = comparison DevOpCode
+  VM_RANGE_I,   // acc &= (imm2 < B < imm) as int64 (bounds made exclusive on the host)
+  VM_RANGE_F,   // acc &= (imm2 <(=) B <(=) imm) as binary64; gen bit 0: lower bound
+                // inclusive, bit 1: upper bound inclusive
   VM_I2F, VM_F2I,                   // acc conversion
   VM_FLOOR, VM_CEIL, VM_ABS_F, VM_NOT_I, VM_NOT_F,
   VM_DIV_F, VM_RDIV_F, VM_DIV_I, VM_RDIV_I, VM_MOD_I, VM_RMOD_I,
@@ -145,7 +148,8 @@ enum VmHotOp : uint8_t {
   HO_LOAD = 0, HO_ADD_I, HO_ADD_F, HO_SUB_I, HO_SUB_F, HO_RSUB_I, HO_RSUB_F, HO_MUL_I, HO_MUL_F, HO_AND_B,
   HO_CMP,         // 12 comparisons, acc CMP B, in the order of kHotCmp
   HO_CMPAND = HO_CMP + 12,  // 12 comparisons, acc &= (B CMP imm2)
-  HO_COUNT_ = HO_CMPAND + 12
+  HO_RANGE_I = HO_CMPAND + 12, HO_RANGE_F,
+  HO_COUNT_
 };
 enum VmHotCls : uint8_t {
   HC_IMM = 0, HC_TMP, HC_L64, HC_L32I, HC_L32F,

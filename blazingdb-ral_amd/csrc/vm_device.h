@@ -435,7 +435,10 @@ DEV void run_rare(int code, int arg, uint32_t w3, uint64_t (&acc)[R], const uint
   X(HO_CMPAND + 8, acc[u] = b2d(b) > fi ? acc[u] : 0ull)                                       \
   X(HO_CMPAND + 9, acc[u] = b2d(b) >= fi ? acc[u] : 0ull)                                      \
   X(HO_CMPAND + 10, acc[u] = b2d(b) == fi ? acc[u] : 0ull)                                     \
-  X(HO_CMPAND + 11, acc[u] = b2d(b) != fi ? acc[u] : 0ull)
+  X(HO_CMPAND + 11, acc[u] = b2d(b) != fi ? acc[u] : 0ull)                                     \
+  X(HO_RANGE_I, acc[u] = ((int64_t)b > ii && (int64_t)b < (int64_t)im) ? acc[u] : 0ull)         \
+  X(HO_RANGE_F, acc[u] = ((b2d(b) > fi || ((rf & 1) && b2d(b) == fi)) &&                       \
+                          (b2d(b) < hf || ((rf & 2) && b2d(b) == hf))) ? acc[u] : 0ull)
 // Operand of row u by hot class.
 #define BLZ_B_IMM im
 #define BLZ_B_TMP (*(const LDSAS uint64_t *)(tp + 512 * u))
@@ -477,8 +480,9 @@ DEV void run_vm(const CONSTANT VmOp *ops, int nops, uint64_t (&acc)[R], lds_ptr 
     const int off = (int)w.s1;
     const uint64_t im = (uint64_t)w.s4 | ((uint64_t)w.s5 << 32);
     const uint64_t im2 = (uint64_t)w.s6 | ((uint64_t)w.s7 << 32);
-    const double fi = b2d(im2);
+    const double fi = b2d(im2), hf = b2d(im);
     const int64_t ii = (int64_t)im2;
+    const int rf = (int)(w.s3 & 0xFF);  // VM_RANGE_F: bound inclusivity
     clds_ptr tp = (clds_ptr)tmpb + off;
     clds_ptr lp4 = b4 + off, lp8 = b8 + off;
     switch (leaf) {
@@ -578,6 +582,17 @@ DEV void run_vm(const CONSTANT VmOp *ops, int nops, uint64_t (&acc)[R], lds_ptr 
             break;
           case VM_CMPAND:
             compare_and<R>(arg, acc, B, im2);
+            break;
+          case VM_RANGE_I:
+#pragma unroll
+            for (int u = 0; u < R; u++) acc[u] = ((int64_t)B[u] > ii && (int64_t)B[u] < (int64_t)im) ? acc[u] : 0ull;
+            break;
+          case VM_RANGE_F:
+#pragma unroll
+            for (int u = 0; u < R; u++) {
+              const double x = b2d(B[u]);
+              acc[u] = ((x > fi || ((rf & 1) && x == fi)) && (x < hf || ((rf & 2) && x == hf))) ? acc[u] : 0ull;
+            }
             break;
           default:
             if constexpr (OPS >= 1) run_rare<R, OPS>(code, arg, w.s3, acc, B, in, grow, n);
@@ -864,7 +879,7 @@ vm_kernel(const VmBatch *__restrict__ bd_arg) {
             orel[u] = 0;
             none[u] = false;
             grow[u] = row0 + lrow + 64 * u;
-            acc[u] = 0;
+            acc[u] = 1;  // a predicate may open with VM_CMPAND (conjunction of comparisons)
           }
           run_vm<R, OPS>(cx.ops, cx.n_ops_p, acc, tmpb, tstride, cx.in, cx.out, tb, lrow, grow, n, 0, orel, none);
           if (cx.st_vsrc >= 0) {  // the predicate's validity is computed

@@ -259,6 +259,48 @@ def test_kernel_kinds_random_filter_project(B, kernel, seed):
     prog.close()
 
 
+RANGE_FILTERS = [
+    "AND(>=($0, -10), <($0, 10))",                     # int range, inclusive / exclusive
+    "AND(<=($0, 10), >($0, -10), <($1, 500))",          # bounds in either order, a third term
+    "AND(>=($0, -1000), <=($0, 1000))",                 # the whole domain
+    "AND(>=($0, 5), >=($0, 7))",                        # two lower bounds: not a range
+    "AND(>=($2, -2.5), <=($2, 2.5))",                   # f64 inclusive, with NaN rows
+    "AND(>($2, -2.5), <($2, 2.5), >=($3, 0.0), <($3, 40.0))",  # f64 exclusive, f32 range
+    "AND(<($3, 40.0), >($1, 100), >=($3, -40.0), <=($1, 900))",  # interleaved columns
+    "AND(>=($1, -9223372036854775808), <=($1, 9223372036854775807))",  # int64 extremes
+    "AND(>=($4, 0), <($4, 2))",                         # int8 column (general leaf)
+    "AND(=($0, 3), <>($2, 1.0))",
+]
+
+
+@pytest.mark.parametrize("cond", RANGE_FILTERS)
+def test_range_conjunctions(B, cond):
+    """Conjunctions of comparisons, the shape the predicate compiler fuses into
+    range ops (lower and upper bound on one operand), against the oracle; float
+    columns hold NaNs (every comparison with NaN is false)."""
+    rng = np.random.default_rng(7000 + RANGE_FILTERS.index(cond))
+    n = 70_001
+    host = [gen_column(rng, P.GDF_INT32, n, 0.05), gen_column(rng, P.GDF_INT64, n),
+            gen_column(rng, P.GDF_FLOAT64, n, 0.05, -5.0, 5.0), gen_column(rng, P.GDF_FLOAT32, n, 0.0, -60.0, 60.0),
+            gen_column(rng, P.GDF_INT8, n, 0.0, -3, 3)]
+    host[2].data[rng.integers(0, n, 500)] = np.nan
+    host[3].data[rng.integers(0, n, 500)] = np.nan
+    filt = f"LogicalFilter(condition=[{cond}])"
+    proj = "LogicalProject(A=[$0], B=[$1], C=[+($2, $3)], D=[*($1, 2)])"
+    _print_case(test="range_conjunctions", cond=cond)
+    ref = _oracle(host, filt, proj)
+    dev = [to_device(B, h) for h in host]
+    for kernel in (0, 3):
+        prog = B.Program(dev, filt, proj, kernel=kernel)
+        prog.launch()
+        kept, nulls = prog.result()
+        assert kept == len(ref[0]), f"kernel {kernel}: {cond}"
+        for g, r, nc in zip(prog.outputs, ref, nulls):
+            g.col.null_count = nc
+            assert_column_equal(g, r, what=f"kernel {kernel}: {cond}")
+        prog.close()
+
+
 WIDE_MIX = [P.GDF_INT32, P.GDF_INT32, P.GDF_INT64, P.GDF_INT64, P.GDF_FLOAT32, P.GDF_FLOAT32,
             P.GDF_FLOAT64, P.GDF_FLOAT64]
 
