@@ -1123,7 +1123,7 @@ blz_batch batch_build(const blz_query *qs, int n, int r, int t, bool use_graph, 
         maxgtmp = std::max(maxgtmp, c.lds_gather - c.vm_g_tmp_off);
       }
       const int vr = 4 * 8 * maxvs * r;
-      const int staged = 2 * maxbuf + vr + maxtmp;
+      const int staged = kTileBuffers * maxbuf + vr + maxtmp;
       const int gather = (int)round_up((size_t)(4 * maxgw + vr + maxgtmp), 8) + 8 * ((int)m.size() + 1);
       return std::max(staged, gather);
     };
@@ -1167,7 +1167,7 @@ blz_batch batch_build(const blz_query *qs, int n, int r, int t, bool use_graph, 
       bd.n_plans = (int)refs.size();
       bd.bufb = maxbuf;
       bd.vr_wave_words = maxvs * b->r;
-      bd.vr_off = 2 * maxbuf;
+      bd.vr_off = kTileBuffers * maxbuf;
       bd.tmp_off = bd.vr_off + 4 * 8 * bd.vr_wave_words;
       bd.g_buf_off = 0;
       bd.g_wave_bytes = maxgw;

@@ -761,7 +761,7 @@ bool compile_vm(const CompileRequest &req, Builder &B, const std::vector<AccOp> 
       off = (off + 15) & ~size_t(15);
     }
     buf = (int)std::max<size_t>(off, 16);
-    return 2 * (size_t)buf;
+    return (size_t)kTileBuffers * buf;
   };
   // Shape: rows per lane R and tile rows T = 256 R P (P passes of the four waves).
   // Temporaries live in LDS (256 R rows each), after the validity registers (four
@@ -882,7 +882,7 @@ bool compile_vm(const CompileRequest &req, Builder &B, const std::vector<AccOp> 
   h.g_kw_off = g_kw;
   h.g_buf_off = g_buf;
   h.g_tmp_off = g_tmp;
-  h.lds_tmp_off = 2 * buf + vr_bytes;
+  h.lds_tmp_off = kTileBuffers * buf + vr_bytes;
   h.n_valid_out = n_valid_out;
   h.lds_bytes = (int)lds_total;
   h.acc_mode = 2;
@@ -947,9 +947,9 @@ bool compile_vm(const CompileRequest &req, Builder &B, const std::vector<AccOp> 
   out.lds_gather = g_cpre + 16;
   out.vm_g_cpre_off = g_cpre;
   out.vm_bufb = buf;
-  out.vm_vr_off = 2 * buf;
+  out.vm_vr_off = kTileBuffers * buf;
   out.vm_g_vr_off = g_vr;
-  out.vm_tmp_off = 2 * buf + vr_bytes;
+  out.vm_tmp_off = kTileBuffers * buf + vr_bytes;
   out.vm_g_buf_off = g_buf;
   out.vm_g_wave_bytes = gbuf;
   out.vm_nvout = n_valid_out;

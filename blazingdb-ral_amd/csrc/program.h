@@ -25,6 +25,10 @@ namespace blz {
 constexpr int kMaxSlots = 64;   // reference limit (interpreter_cpp.cu:86-217 handles <= 64)
 constexpr int kMaxVSlots = 64;
 constexpr int kVmVSlots = 32;  // tile VM: validity registers (LDS words per 64-row group)
+// Tile VM staging buffers per workgroup: 2 overlaps a tile's loads with the
+// evaluation of the one before; 1 halves the LDS per workgroup (more workgroups
+// per CU, whose loads and evaluations overlap instead).
+constexpr int kTileBuffers = 2;
 constexpr int kMaxOps = 256;    // per program
 
 // Input load kinds: how a column widens into an 8-byte slot (interpreter_ops.cuh:386-452).

@@ -794,8 +794,8 @@ vm_kernel(const VmBatch *__restrict__ bd_arg) {
       const int64_t t = g - d_first;
       if (t < d_full) wave_dma_issue(dma, t, dst);
     };
-    issue(cur, lds);
-    for (; cur < n_tiles; cur += gridDim.x, buf ^= 1) {
+    if (kTileBuffers == 2) issue(cur, lds);
+    for (; cur < n_tiles; cur += gridDim.x, buf ^= (kTileBuffers == 2)) {
       if (cur >= p_end) {
         if (MODE == VM_PROJECT && null_counts != nullptr && nacc) atomicAdd(null_counts + lane, nacc);
         nacc = 0;
@@ -817,6 +817,7 @@ vm_kernel(const VmBatch *__restrict__ bd_arg) {
       const int64_t row0 = tile * T;
       const int64_t rows = (n - row0) < T ? (n - row0) : T;
       lds_ptr tb = lds + buf * bufb;
+      if (kTileBuffers == 1) issue(cur, tb);
       // This wave's staging loads of the tile are complete; the barrier makes every
       // wave's visible and ends all reads of the other buffer.
       __builtin_amdgcn_s_waitcnt(0);
@@ -825,7 +826,7 @@ vm_kernel(const VmBatch *__restrict__ bd_arg) {
         stage_partial(cx.pieces, cx.np, tile, rows, tb);
         __syncthreads();
       }
-      issue(cur + gridDim.x, lds + (buf ^ 1) * bufb);
+      if (kTileBuffers == 2) issue(cur + gridDim.x, lds + (buf ^ 1) * bufb);
 
       if (MODE == VM_PROJECT) {
         for (int ps = 0; ps < P; ps++) {
@@ -899,6 +900,7 @@ vm_kernel(const VmBatch *__restrict__ bd_arg) {
         if (lane < GW) gstore<uint64_t>(keep, g0 + lane, myword);
         if (lane == 0) gstore<uint32_t>(wave_count, tile * kWaves + wave, (uint32_t)cnt);
       }
+      if (kTileBuffers == 1) __syncthreads();  // every wave is done with the buffer
     }
     if (MODE == VM_PROJECT && null_counts != nullptr && nacc) atomicAdd(null_counts + lane, nacc);
     return;
