@@ -1,0 +1,266 @@
+/*
+ * ral.hpp — the reference's C++ operator API for the expression / filter / gather
+ * path, over the C ABI of blz_ral.h (header only, C++17).
+ *
+ * Same names, argument meaning and error behaviour as the reference:
+ *   perform_operation      src/Interpreter/interpreter_cpp.h:25-37
+ *   evaluate_expression    src/LogicalFilter.h:20-23
+ *   process_filter         src/CalciteInterpreter.cpp:509 (no header in the reference)
+ *   execute_project_plan   src/CalciteInterpreter.h:61-63
+ *   materialize_column     src/ColumnManipulation.cuh:16-18
+ *   gdf_sequence           src/Utils.cuh:187-190
+ * with the frame and column types they take (blazing_frame, DataFrame.h:16-181;
+ * gdf_column_cpp, GDFColumn.cuh:17-95) reduced to the members this path uses.
+ *
+ * Errors: every failure of the C ABI is rethrown as std::runtime_error carrying the
+ * library's message (the reference throws std::runtime_error from CheckCudaErrors
+ * / CUDF_CALL, Utils.cuh:25-55). Calls block until their work is complete, as the
+ * reference's do (interpreter_cpp.cu:282-285); each runs on the default stream.
+ */
+#ifndef BLZ_RAL_HPP_
+#define BLZ_RAL_HPP_
+
+#include <cstddef>
+#include <cstdint>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "blz_ral.h"
+
+namespace blz_detail {
+
+inline void check(int rc, const char *err, const char *what) {
+  if (rc != BLZ_OK) throw std::runtime_error(std::string(what) + ": " + (err[0] ? err : "error " + std::to_string(rc)));
+}
+
+// Device buffers of one column; freed with the last gdf_column_cpp sharing them.
+struct ColumnHolder {
+  gdf_column col{};
+  bool owned = true;  // false: an alias of another column's buffers
+  std::shared_ptr<ColumnHolder> base;  // keeps the aliased column alive
+  ~ColumnHolder() {
+    if (owned) blz_free_column(&col);
+  }
+};
+
+}  // namespace blz_detail
+
+// A device column with shared ownership (GDFColumn.cuh:17-95).
+class gdf_column_cpp {
+ public:
+  gdf_column_cpp() = default;
+
+  // n values of `type`; input_data (host, may be NULL) is copied in; a validity
+  // buffer of gdf_valid_allocation_size(n) bytes is allocated all valid
+  // (GDFColumn.cu:302-338, allocate_valid :163-173); host_valid (may be NULL)
+  // is copied over it.
+  void create_gdf_column(gdf_dtype type, size_t num_values, void *input_data, size_t width_per_value,
+                         const std::string &column_name = "") {
+    create_gdf_column(type, num_values, input_data, nullptr, width_per_value, column_name);
+  }
+  void create_gdf_column(gdf_dtype type, size_t num_values, void *input_data, gdf_valid_type *host_valid,
+                         size_t width_per_value, const std::string &column_name = "") {
+    auto h = std::make_shared<blz_detail::ColumnHolder>();
+    h->col.dtype = type;
+    h->col.size = (gdf_size_type)num_values;
+    const size_t bytes = num_values * width_per_value;
+    if (bytes > 0) {
+      if (blz_device_malloc(&h->col.data, (bytes + 63) / 64 * 64) != BLZ_OK)
+        throw std::runtime_error("create_gdf_column: device allocation failed");
+      if (input_data && blz_memcpy_htod(h->col.data, input_data, bytes, nullptr) != BLZ_OK)
+        throw std::runtime_error("create_gdf_column: copy to the device failed");
+    }
+    const size_t vb = gdf_valid_allocation_size((gdf_size_type)num_values);
+    if (num_values > 0) {
+      void *v = nullptr;
+      if (blz_device_malloc(&v, vb) != BLZ_OK) throw std::runtime_error("create_gdf_column: device allocation failed");
+      h->col.valid = (gdf_valid_type *)v;
+      if (blz_memset_device(v, 0xFF, vb, nullptr) != BLZ_OK)
+        throw std::runtime_error("create_gdf_column: validity initialisation failed");
+      if (host_valid && blz_memcpy_htod(v, host_valid, (num_values + 7) / 8, nullptr) != BLZ_OK)
+        throw std::runtime_error("create_gdf_column: copy to the device failed");
+    }
+    h_ = h;
+    name_ = column_name;
+    if (host_valid) update_null_count();
+  }
+  // Takes ownership of a column whose buffers this library allocated.
+  void create_gdf_column(gdf_column *column) {
+    auto h = std::make_shared<blz_detail::ColumnHolder>();
+    h->col = *column;
+    h_ = h;
+  }
+
+  gdf_column *get_gdf_column() const { return h_ ? &h_->col : nullptr; }
+  void *data() const { return h_ ? h_->col.data : nullptr; }
+  gdf_valid_type *valid() const { return h_ ? h_->col.valid : nullptr; }
+  gdf_size_type size() const { return h_ ? h_->col.size : 0; }
+  gdf_dtype dtype() const { return h_ ? h_->col.dtype : GDF_invalid; }
+  gdf_size_type null_count() { return h_ ? h_->col.null_count : 0; }
+  std::string name() const { return name_; }
+  void set_name(std::string name) { name_ = std::move(name); }
+
+  // null_count = size - popcount(valid bits < size) (GDFColumn.cu:519-528).
+  void update_null_count() {
+    if (!h_) return;
+    char err[256] = {0};
+    blz_detail::check(blz_update_null_count(&h_->col, nullptr, err, sizeof(err)), err, "update_null_count");
+  }
+
+  // A column sharing `other`'s device buffers (a projection's "$k" output).
+  static gdf_column_cpp alias_of(const gdf_column_cpp &other, const gdf_column &view) {
+    gdf_column_cpp c;
+    auto h = std::make_shared<blz_detail::ColumnHolder>();
+    h->col = view;
+    h->owned = false;
+    h->base = other.h_;
+    c.h_ = h;
+    c.name_ = other.name_;
+    return c;
+  }
+
+ private:
+  std::shared_ptr<blz_detail::ColumnHolder> h_;
+  std::string name_;
+};
+
+// Tables of columns addressed by one flat column index (DataFrame.h:16-181).
+class blazing_frame {
+ public:
+  gdf_column_cpp &get_column(int column_index) {
+    int base = 0;
+    for (auto &t : columns_) {
+      if (column_index < base + (int)t.size()) return t[column_index - base];
+      base += (int)t.size();
+    }
+    throw std::runtime_error("get_column: index " + std::to_string(column_index) + " out of range");
+  }
+  std::vector<std::vector<gdf_column_cpp>> &get_columns() { return columns_; }
+  void add_table(std::vector<gdf_column_cpp> columns_to_add) { columns_.push_back(std::move(columns_to_add)); }
+  void add_column(gdf_column_cpp column_to_add, int table_index = 0) {
+    if ((int)columns_.size() <= table_index) columns_.resize(table_index + 1);
+    columns_[table_index].push_back(std::move(column_to_add));
+  }
+  void set_column(size_t column_index, gdf_column_cpp column) { get_column((int)column_index) = std::move(column); }
+  size_t get_width() {
+    size_t w = 0;
+    for (auto &t : columns_) w += t.size();
+    return w;
+  }
+  size_t get_size_column(int table_index = 0) { return columns_[table_index].size(); }
+  gdf_size_type get_num_rows_in_table(int table_index) {
+    return columns_[table_index].empty() ? 0 : columns_[table_index][0].size();
+  }
+  void clear() { columns_.clear(); }
+
+  // Device column pointers of every column, in flat-index order.
+  std::vector<gdf_column *> column_pointers() {
+    std::vector<gdf_column *> p;
+    for (auto &t : columns_)
+      for (auto &c : t) p.push_back(c.get_gdf_column());
+    return p;
+  }
+
+ private:
+  std::vector<std::vector<gdf_column_cpp>> columns_;
+};
+
+// Evaluates a register-file plan (interpreter_cpp.h:25-37). output_columns'
+// null_count is not written (interpreter_cpp.cu:280); new_input_indices is unused,
+// as in the reference.
+inline void perform_operation(std::vector<gdf_column *> output_columns, std::vector<gdf_column *> input_columns,
+                              std::vector<column_index_type> &left_inputs,
+                              std::vector<column_index_type> &right_inputs, std::vector<column_index_type> &outputs,
+                              std::vector<column_index_type> &final_output_positions,
+                              std::vector<gdf_binary_operator> &operators,
+                              std::vector<gdf_unary_operator> &unary_operators, std::vector<gdf_scalar> &left_scalars,
+                              std::vector<gdf_scalar> &right_scalars,
+                              std::vector<column_index_type> new_input_indices) {
+  (void)new_input_indices;
+  char err[512] = {0};
+  blz_detail::check(
+      blz_perform_operation(output_columns.data(), (int)output_columns.size(), input_columns.data(),
+                            (int)input_columns.size(), left_inputs.data(), right_inputs.data(), outputs.data(),
+                            (int)operators.size(), final_output_positions.data(), operators.data(),
+                            unary_operators.data(), left_scalars.data(), right_scalars.data(), BLZ_SCALAR_INTENDED,
+                            nullptr, nullptr, err, sizeof(err)),
+      err, "perform_operation");
+}
+
+// Plans and evaluates one Calcite expression over the frame into `output`
+// (LogicalFilter.cpp:271-371); output's null_count is not written (:368).
+inline void evaluate_expression(blazing_frame &inputs, const std::string &expression, gdf_column_cpp &output) {
+  auto frame = inputs.column_pointers();
+  char err[512] = {0};
+  blz_detail::check(blz_evaluate_expression(frame.data(), (int)frame.size(), expression.c_str(),
+                                            output.get_gdf_column(), BLZ_SCALAR_INTENDED, nullptr, nullptr, err,
+                                            sizeof(err)),
+                    err, "evaluate_expression");
+}
+
+// Keeps the rows whose predicate is non-zero and valid, in row order, for every
+// column of the frame, which then holds the filtered columns as one table
+// (CalciteInterpreter.cpp:509-573).
+inline void process_filter(blazing_frame &input, std::string query_part) {
+  auto frame = input.column_pointers();
+  std::vector<gdf_column> outs(frame.size());
+  int64_t kept = 0;
+  char err[512] = {0};
+  blz_detail::check(blz_process_filter(frame.data(), (int)frame.size(), query_part.c_str(), outs.data(), &kept,
+                                       BLZ_SCALAR_INTENDED, nullptr, err, sizeof(err)),
+                    err, "process_filter");
+  std::vector<gdf_column_cpp> table(outs.size());
+  for (size_t i = 0; i < outs.size(); i++) {
+    table[i].create_gdf_column(&outs[i]);
+    table[i].set_name(input.get_column((int)i).name());
+  }
+  input.clear();
+  input.add_table(std::move(table));
+}
+
+// Evaluates a LogicalProject over the frame, which then holds the projected
+// columns; "$k" outputs alias frame column k (CalciteInterpreter.cpp:123-336).
+inline void execute_project_plan(blazing_frame &input, std::string query_part) {
+  auto frame = input.column_pointers();
+  const int max_out = 256;
+  std::vector<gdf_column> outs(max_out);
+  std::vector<int32_t> alias(max_out, 0);
+  int n_out = 0;
+  char err[512] = {0};
+  blz_detail::check(blz_execute_project(frame.data(), (int)frame.size(), query_part.c_str(), outs.data(),
+                                        alias.data(), max_out, &n_out, BLZ_SCALAR_INTENDED, nullptr, err,
+                                        sizeof(err)),
+                    err, "execute_project_plan");
+  std::vector<gdf_column_cpp> table(n_out);
+  for (int i = 0; i < n_out; i++) {
+    if (alias[i]) {
+      int k = -1;  // the frame column whose buffers the output shares
+      for (size_t c = 0; c < frame.size(); c++)
+        if (frame[c]->data == outs[i].data) k = (int)c;
+      if (k < 0) throw std::runtime_error("execute_project_plan: alias of an unknown column");
+      table[i] = gdf_column_cpp::alias_of(input.get_column(k), outs[i]);
+    } else {
+      table[i].create_gdf_column(&outs[i]);
+    }
+  }
+  input.clear();
+  input.add_table(std::move(table));
+}
+
+// output[i] = input[row_indices[i]] (and its validity bit); sets
+// output->null_count (ColumnManipulation.cu:83-150).
+inline void materialize_column(gdf_column *input, gdf_column *output, gdf_column *row_indices) {
+  char err[512] = {0};
+  blz_detail::check(blz_materialize_column(input, output, row_indices, nullptr, err, sizeof(err)), err,
+                    "materialize_column");
+}
+
+// data[i] = init_val + i * step (Utils.cu:4-12).
+inline void gdf_sequence(int32_t *data, size_t size, int32_t init_val, int32_t step = 1) {
+  char err[256] = {0};
+  blz_detail::check(blz_gdf_sequence(data, size, init_val, step, nullptr, err, sizeof(err)), err, "gdf_sequence");
+}
+
+#endif  // BLZ_RAL_HPP_

@@ -558,3 +558,38 @@ def test_synth_column_deterministic(B):
     c = B.DeviceColumn(B.GDF_INT32, n)
     B.synth_column(c, 9, 5, 10, 0)
     assert np.array_equal(c.data(), np.arange(10, 10 + n, dtype=np.int32))
+
+
+def test_cpp_operator_api_filter_project(B):
+    """The reference's call sequence (evaluate_expression, process_filter,
+    execute_project_plan) from C++ through include/blz/ral.hpp, against the oracle
+    on the same generated frame, bit-exact."""
+    import os
+    import subprocess
+    shim = os.path.join(os.path.dirname(os.path.abspath(__file__)), "cpp", "ral_shim_test")
+    n = 20_000
+    _print_case(test="cpp_operator_api", n=n)
+    r = subprocess.run([shim, str(n)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    lines = r.stdout.splitlines()
+    i = np.arange(n, dtype=np.int64)
+    a = (1 + (i * 7919) % 50).astype(np.int32)
+    b = 900.0 + ((i * 104729) % 104100).astype(np.float64)
+    c = ((i * 31) % 11).astype(np.float64) / 100.0
+    d = ((i * 40503) % 2556).astype(np.int32)
+    host = [O.HostColumn(a, P.GDF_INT32), O.HostColumn(b, P.GDF_FLOAT64), O.HostColumn(c, P.GDF_FLOAT64),
+            O.HostColumn(d, P.GDF_INT32)]
+    st = O.evaluate_expression(host, ">=($3, 730)", P.GDF_INT8, True)
+    assert lines[0] == f"stencil {int(np.count_nonzero(st.data))}"
+    filt = "LogicalFilter(condition=[AND(>=($3, 730), <($3, 1095), >=($2, 0.03), <=($2, 0.05), <($0, 30))])"
+    proj = "LogicalProject(EXPR$0=[*($1, -(1, $2))], EXPR$1=[+($0, 1)], D=[$3])"
+    ref = O.execute_project(O.process_filter(host, filt), proj)
+    assert lines[1] == f"kept {len(ref[0])}"
+    rows = [ln.split() for ln in lines[2:]]
+    assert len(rows) == len(ref[0])
+    got0 = np.array([float(x[1]) for x in rows])
+    got1 = np.array([int(x[2]) for x in rows], dtype=np.int32)
+    got2 = np.array([int(x[3]) for x in rows], dtype=np.int32)
+    assert got0.tobytes() == ref[0].data.tobytes()
+    assert got1.tobytes() == ref[1].data.tobytes()
+    assert got2.tobytes() == ref[2].data.tobytes()

@@ -140,3 +140,21 @@ def test_c1_predicate_plan_by_hand(B):
     assert got["outputs"][-1] == 3
     assert all(o == 3 or o >= 4 for o in got["outputs"])
     assert compare_plans(B, dtypes, expr) == "ok"
+
+
+SHIM = os.path.join(os.path.dirname(os.path.abspath(__file__)), "cpp", "ral_shim_test")
+
+
+def _shim():
+    if not os.path.exists(SHIM):
+        subprocess.run(["make", "-C", os.path.dirname(SHIM)], check=True, capture_output=True)
+    return SHIM
+
+
+def test_cpp_operator_api_compiles_and_reports_errors():
+    """include/blz/ral.hpp (the reference's C++ operator API over the C ABI) builds
+    into a program, and a failing call surfaces as std::runtime_error with the
+    library's message (the reference's error behaviour, Utils.cuh:25-55)."""
+    r = subprocess.run([_shim(), "errors"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert r.stdout.startswith("runtime_error: evaluate_expression:"), r.stdout
