@@ -1,6 +1,6 @@
 /*
  * device_common.h — device helpers shared by the kernels of kernels.hip and
- * vm_kernel.hip: slot reinterpretation, conversions, date parts, wide global
+ * vm_device.h: slot reinterpretation, conversions, date parts, wide global
  * loads/stores through explicit address spaces, validity words, the out-of-line
  * slow op set and the validity program.
  *
