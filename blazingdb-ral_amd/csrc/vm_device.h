@@ -736,8 +736,15 @@ DEV VmCtx load_ctx(const uint8_t *blob_arg) {
   return c;
 }
 
+// Minimum waves per SIMD the register allocator must allow (launch bounds): with
+// the common op set, R <= 4 programs fit 8 waves (projection, predicate) or 4
+// (gather) without spills, which hides more of the staging and gather latency.
+constexpr int vm_min_waves(int r, int mode, int ops) {
+  return ops != 0 || r > 4 ? 1 : mode == 2 ? 4 : 8;
+}
+
 template <int R, int MODE, int OPS>
-__global__ void __launch_bounds__(kBlock)
+__global__ void __launch_bounds__(kBlock, vm_min_waves(R, MODE, OPS))
 vm_kernel(const VmBatch *__restrict__ bd_arg) {
   extern __shared__ __attribute__((aligned(16))) uint8_t lds_raw[];
   constexpr bool STAGED = MODE != VM_GATHER;
