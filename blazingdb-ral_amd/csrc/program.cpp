@@ -466,6 +466,34 @@ int flip_cmp(int c) {  // a OP b  ==  b FLIP(OP) a
 }
 bool is_cmp_code(int c) { return c >= OP_EQ_I && c <= OP_GE_F; }
 
+// Multiplier and shift for truncating int64 division by a constant d (|d| >= 2):
+// q = (mulhi(M, n) [+ n if d > 0 and M < 0] [- n if d < 0 and M > 0]) >> s, plus
+// one if negative. The signed magic-number algorithm of Granlund & Montgomery
+// (PLDI 1994) in the formulation of Warren, "Hacker's Delight", 2nd ed., 10-4.
+void magic_signed(int64_t d, int64_t &M, int &s) {
+  const uint64_t two63 = 1ull << 63;
+  const uint64_t ad = d < 0 ? (uint64_t)0 - (uint64_t)d : (uint64_t)d;
+  const uint64_t t = two63 + ((uint64_t)d >> 63);
+  const uint64_t anc = t - 1 - t % ad;  // |nc|
+  int p = 63;
+  uint64_t q1 = two63 / anc, r1 = two63 - q1 * anc;  // 2^p / |nc|
+  uint64_t q2 = two63 / ad, r2 = two63 - q2 * ad;    // 2^p / |d|
+  uint64_t delta;
+  do {
+    p++;
+    q1 *= 2;
+    r1 *= 2;
+    if (r1 >= anc) { q1++; r1 -= anc; }
+    q2 *= 2;
+    r2 *= 2;
+    if (r2 >= ad) { q2++; r2 -= ad; }
+    delta = ad - r2;
+  } while (q1 < delta || (q1 == delta && r1 == 0));
+  M = (int64_t)(q2 + 1);
+  if (d < 0) M = -M;
+  s = p - 64;
+}
+
 // Dispatch leaf of a lowered op, once the rows per lane R (and so the byte offsets
 // of temporaries, 8 * 256 R apart) are known.
 void assign_leaf(VmOp &op, int R) {
@@ -475,6 +503,22 @@ void assign_leaf(VmOp &op, int R) {
     case VM_STORE: op.off = op.arg * tstride; op.leaf = kLeafStore; return;
     case VM_EMIT: op.leaf = kLeafEmit; return;
     case VM_I2F: op.leaf = kLeafI2F; return;
+    case VM_NOT_I: op.leaf = kLeafNotI; return;
+    case VM_NOT_F: op.leaf = kLeafNotF; return;
+    case VM_DIV_I: case VM_MOD_I: {
+      const int64_t d = (int64_t)op.imm;
+      if (op.cls == VC_IMM && op.cv == CV_NONE && d != 0 && d != 1 && d != -1) {
+        int64_t M;
+        int sh;
+        magic_signed(d, M, sh);
+        op.imm2 = (uint64_t)M;
+        op.gen = (uint8_t)sh;
+        op.leaf = op.code == VM_DIV_I ? kLeafDivK : kLeafModK;
+      } else {
+        op.leaf = kLeafGeneral;
+      }
+      return;
+    }
     default: break;
   }
   int hc = -1;
@@ -942,7 +986,7 @@ bool compile_vm(const CompileRequest &req, Builder &B, const std::vector<AccOp> 
   int level = slow ? 2 : 0;
   for (auto *v : {&vm_p, &vm_q})
     for (auto &op : *v)
-      if (level == 0 && op.code > VM_I2F && op.code != VM_GENERIC) level = 1;
+      if (level == 0 && op.code > VM_I2F && op.code != VM_GENERIC && op.leaf == kLeafGeneral) level = 1;
   out.vm_slow = level;
   out.lds_gather = g_cpre + 16;
   out.vm_g_cpre_off = g_cpre;

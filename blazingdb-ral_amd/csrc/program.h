@@ -170,7 +170,11 @@ constexpr int kHotLeaves = HO_COUNT_ * HC_COUNT_;
 constexpr int kLeafStore = kHotLeaves;      // tmp = acc (off: byte offset of the temporary)
 constexpr int kLeafEmit = kHotLeaves + 1;   // output arg <- acc
 constexpr int kLeafI2F = kHotLeaves + 2;    // acc int64 -> binary64
-constexpr int kLeafGeneral = kHotLeaves + 3;
+constexpr int kLeafNotI = kHotLeaves + 3;   // acc = (acc == 0)
+constexpr int kLeafNotF = kHotLeaves + 4;   // acc = !(binary64 acc != 0)
+constexpr int kLeafDivK = kHotLeaves + 5;   // acc = acc / imm (int64, imm not in {-1, 0, 1}): by
+constexpr int kLeafModK = kHotLeaves + 6;   // acc = acc % imm   the magic multiplier imm2, shift gen
+constexpr int kLeafGeneral = kHotLeaves + 7;
 struct VmOp {
   int32_t leaf;   // dispatch id (set once the tile shape is known)
   int32_t off;    // VC_L*: byte offset of the column slice in the tile buffer;

@@ -523,6 +523,32 @@ DEV void run_vm(const CONSTANT VmOp *ops, int nops, uint64_t (&acc)[R], lds_ptr 
 #pragma unroll
         for (int u = 0; u < R; u++) acc[u] = d2b((double)(int64_t)acc[u]);
         break;
+      case kLeafNotI:
+#pragma unroll
+        for (int u = 0; u < R; u++) acc[u] = (uint64_t)(acc[u] == 0);
+        break;
+      case kLeafNotF:
+#pragma unroll
+        for (int u = 0; u < R; u++) acc[u] = (uint64_t)(!(b2d(acc[u]) != 0.0));
+        break;
+      case kLeafDivK:
+      case kLeafModK: {
+        // truncating division by the constant im (magic multiplier im2, shift in gen)
+        const int64_t d = (int64_t)im, M = (int64_t)im2;
+        const int sh = (int)(w.s3 & 0xFF);
+        const bool add_n = d > 0 && M < 0, sub_n = d < 0 && M > 0;
+        const bool div = leaf == kLeafDivK;
+#pragma unroll
+        for (int u = 0; u < R; u++) {
+          const int64_t x = (int64_t)acc[u];
+          uint64_t q = (uint64_t)__mul64hi(M, x);
+          if (add_n) q += (uint64_t)x;
+          if (sub_n) q -= (uint64_t)x;
+          int64_t qs = (int64_t)q >> sh;
+          qs += (int64_t)((uint64_t)qs >> 63);
+          acc[u] = div ? (uint64_t)qs : (uint64_t)x - (uint64_t)qs * (uint64_t)d;
+        }
+      } break;
       default: {  // general leaf
         const int code = (int)(w.s2 & 0xFF), arg = (int)((w.s2 >> 8) & 0xFF);
         const int cls = (int)((w.s2 >> 16) & 0xFF), cv = (int)(w.s2 >> 24);

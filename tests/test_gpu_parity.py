@@ -593,3 +593,39 @@ def test_cpp_operator_api_filter_project(B):
     assert got0.tobytes() == ref[0].data.tobytes()
     assert got1.tobytes() == ref[1].data.tobytes()
     assert got2.tobytes() == ref[2].data.tobytes()
+
+
+DIVISORS = [2, 3, 7, 10, 87, 1000, 65536, -2, -7, -1000, 6700417, 1099511627781, 9223372036854775807,
+            -9223372036854775807, 0, 1, -1]
+
+
+@pytest.mark.parametrize("kernel", [0, 3])
+def test_integer_division_by_constants(B, kernel):
+    """Integer / and MOD by literal divisors (the tile VM divides by a magic
+    multiplier; 0, 1 and -1 take the general path) against the oracle, with
+    int64 extremes among the dividends."""
+    rng = np.random.default_rng(8100 + kernel)
+    n = 20_000
+    x = rng.integers(-2**63, 2**63 - 1, n, dtype=np.int64)
+    x[:6] = [0, 1, -1, 2**63 - 1, -2**63, -2**63 + 1]
+    x[6:2000] >>= rng.integers(0, 63, 1994)
+    y = rng.integers(-2**31, 2**31 - 1, n).astype(np.int32)
+    host = [O.HostColumn(x, P.GDF_INT64, rng.random(n) > 0.05), O.HostColumn(y, P.GDF_INT32)]
+    dev = [to_device(B, h) for h in host]
+    items = []
+    for i, d in enumerate(DIVISORS):
+        items.append(f"Q{i}=[/($0, {d})]")
+        items.append(f"M{i}=[MOD($0, {d})]")
+        items.append(f"R{i}=[MOD($1, {d})]")
+    items.append("X=[+(MOD($0, 7), /($1, -3))]")
+    for k in range(0, len(items), 16):
+        proj = "LogicalProject(" + ", ".join(items[k:k + 16]) + ")"
+        _print_case(test="division_by_constants", kernel=kernel, part=k)
+        ref = O.execute_project(host, proj)
+        prog = B.Program(dev, None, proj, kernel=kernel)
+        prog.launch()
+        kept, nulls = prog.result()
+        for g, r, nc in zip(prog.outputs, ref, nulls):
+            g.col.null_count = nc
+            assert_column_equal(g, r, what=f"kernel {kernel}: {proj}")
+        prog.close()
