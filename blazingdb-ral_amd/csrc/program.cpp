@@ -825,9 +825,15 @@ bool compile_vm(const CompileRequest &req, Builder &B, const std::vector<AccOp> 
     for (int c = 0; c < n_dev && req.compact; c++)
       if (used_q[c]) g = al16(g + 64 * r * std::max(1, wid(req.inputs[c].dtype)));
     g *= 4;
+    // default passes per tile: the fewest (P >= 1) that stage at least 16 KiB per
+    // tile, so narrow programs do not pay a tile's fixed costs for a few KiB
+    int b1 = 0;
+    const size_t one = layout(256 * r, b1);  // one pass
+    int pmax = 1;
+    while (pmax < 64 / r && one / kTileBuffers * pmax < 16384) pmax *= 2;
     for (int p = 64 / r; p >= 1; p /= 2) {
       const int t = 256 * r * p;
-      if (req.force_t > 0 ? t != req.force_t : t > 256 * r) continue;
+      if (req.force_t > 0 ? t != req.force_t : p > pmax) continue;
       int b = 0;
       const size_t tot = layout(t, b) + tb;
       if (tot > (size_t)budget) continue;

@@ -979,8 +979,14 @@ vm_kernel(const VmBatch *__restrict__ bd_arg) {
   // current program, added to the program's counts when the wave leaves it
   unsigned long long *null_counts = nullptr;
   unsigned long long nacc = 0;
-  for (int64_t ch = (int64_t)blockIdx.x * kWaves + wave; ch < total_chunks; ch += (int64_t)gridDim.x * kWaves) {
+  // the next chunk's source rows are loaded while this one is gathered (when it
+  // belongs to the same program)
+  const int64_t cstride = (int64_t)gridDim.x * kWaves;
+  uint32_t nidx[R];
+  bool have_next = false;
+  for (int64_t ch = (int64_t)blockIdx.x * kWaves + wave; ch < total_chunks; ch += cstride) {
     if (ch >= p_end) {
+      have_next = false;
       if (null_counts != nullptr && nacc) atomicAdd(null_counts + lane, nacc);
       nacc = 0;
       int lo = 0, hi = n_plans - 1;  // the program with cpre[p] <= ch < cpre[p + 1]
@@ -1008,7 +1014,13 @@ vm_kernel(const VmBatch *__restrict__ bd_arg) {
       live[u] = j0 + orel[u] < n_sel;
     }
 #pragma unroll
-    for (int u = 0; u < R; u++) grow[u] = live[u] ? (int64_t)idx[j0 + orel[u]] : 0;
+    for (int u = 0; u < R; u++) grow[u] = !live[u] ? 0 : have_next ? (int64_t)nidx[u] : (int64_t)idx[j0 + orel[u]];
+    have_next = ch + cstride < p_end;
+    if (have_next) {
+      const int64_t jn = j0 + cstride * CR;
+#pragma unroll
+      for (int u = 0; u < R; u++) nidx[u] = jn + orel[u] < n_sel ? idx[jn + orel[u]] : 0u;
+    }
 #pragma unroll
     for (int u = 0; u < R; u++) acc[u] = 0;
     stage_kept<R>(cx.gcols, cx.n_gcols, gbuf, plrow, grow, live);
