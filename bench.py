@@ -186,10 +186,10 @@ def run_serve(B, total_gb, rates, seconds, seed=505):
     print(f"[bench] serve: {len(tables)} resident tables, {have / 1e9:.1f} GB", file=sys.stderr, flush=True)
     out = {"workload": f"{len(tables)} resident tables, {have / 1e9:.1f} GB (2 i32, 2 i64, 2 f32, 2 f64, "
                        f"5% nulls), filter + project queries of 4-40 ops, HBM budget 200 GB", "rates": {}}
-    # untimed warm-up of both modes (kernel modules, the stream-ordered memory pool)
+    # untimed warm-up of both modes (kernel modules, and the stream-ordered memory
+    # pool grown to the outputs and workspaces of a query on every table)
     warm = []
-    for _ in range(64):
-        i = int(rng.integers(0, len(tables)))
+    for i in range(len(tables)):
         f, p = W.rand_query(rng, specs[i], True)
         warm.append((tables[i], f, p))
     for mb, ns in ((64, 4), (1, 1)):

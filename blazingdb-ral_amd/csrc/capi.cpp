@@ -1568,6 +1568,13 @@ int blz_server_create(blz_server *server, size_t hbm_budget_bytes, int max_batch
     s->n_streams = n_streams;
     s->r = r;
     HIPCHECK(hipGetDevice(&s->device));
+    // Freed memory stays in the device's stream-ordered pool up to the budget (by
+    // default the pool hands it back at every synchronisation, and the next
+    // queries' outputs and workspaces are then mapped again).
+    hipMemPool_t pool = nullptr;
+    HIPCHECK(hipDeviceGetDefaultMemPool(&pool, s->device));
+    uint64_t keep = hbm_budget_bytes;
+    HIPCHECK(hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep));
     for (int i = 0; i < n_streams; i++) {
       hipStream_t st;
       HIPCHECK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));

@@ -66,35 +66,7 @@ slot_scan_kernel(const VmBatch *__restrict__ bd_arg) {
     total += c;
   }
   if (wave == 0) {
-    constexpr uint64_t FA = 1ull << 62, FP = 2ull << 62, VM = (1ull << 62) - 1;
-    uint64_t pre = 0;
-    if (b == 0) {
-      if (lane == 0) __hip_atomic_store(ws.tile_state, FP | total, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    } else {
-      if (lane == 0)
-        __hip_atomic_store(ws.tile_state + b, FA | total, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      int64_t look = b - 1;
-      while (true) {
-        const int64_t idx = look - lane;
-        uint64_t st = FP;
-        if (idx >= 0) st = __hip_atomic_load(ws.tile_state + idx, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        const uint64_t flag = st >> 62;
-        const uint64_t is_x = __builtin_amdgcn_ballot_w64(flag == 0);
-        const uint64_t is_p = __builtin_amdgcn_ballot_w64(flag == 2);
-        const uint64_t upto = is_p ? (((is_p & (~is_p + 1)) << 1) - 1) : ~0ull;
-        if (is_x & upto) {
-          __builtin_amdgcn_s_sleep(1);
-          continue;
-        }
-        uint64_t x = ((upto >> lane) & 1) ? (st & VM) : 0;
-        for (int s = 32; s > 0; s >>= 1) x += __shfl_xor(x, s);
-        pre += x;
-        if (is_p) break;
-        look -= 64;
-      }
-      if (lane == 0)
-        __hip_atomic_store(ws.tile_state + b, FP | (pre + total), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
+    const uint64_t pre = lookback_prefix(ws.tile_state, b, total, lane);
     if (lane == 0) {
       s_prefix = pre;
       if (b == n_blocks - 1) *ws.n_sel = pre + total;
