@@ -889,8 +889,16 @@ bool compile_vm(const CompileRequest &req, Builder &B, const std::vector<AccOp> 
   const size_t off_out = al(off_in + sizeof(InCol) * n_dev);
   const size_t off_ops = al(off_out + sizeof(OutCol) * n_out);
   const size_t off_pieces = al(off_ops + sizeof(VmOp) * (vm_p.size() + vm_q.size()));
+  // a filter's bare-column outputs (no ops: the gather kernel copies them)
+  std::vector<CopyCol> copies;
+  for (int o = 0; o < n_out && req.compact; o++) {
+    const OutSpec &s = req.outputs[o];
+    if (s.kind != 1 || store_kind_alias(s.dtype) == ST_NONE) continue;
+    copies.push_back(CopyCol{req.inputs[s.index].data, s.data, std::max(1, wid(s.dtype)), 0, 0});
+  }
   const size_t off_gcols = al(off_pieces + sizeof(Piece) * pieces.size());
-  const size_t off_vops = al(off_gcols + sizeof(Piece) * gcols.size());
+  const size_t off_copy = al(off_gcols + sizeof(Piece) * gcols.size());
+  const size_t off_vops = al(off_copy + sizeof(CopyCol) * copies.size());
   const size_t total = al(off_vops + sizeof(VOp) * (vops_p.size() + vops_q.size()));
   if ((int)total > blob_capacity()) { why = "plan too large for the program buffer"; return false; }
   auto vsrc = [&](int v) { return v == V_ALL ? -1 : v == V_NONE ? -2 : B.vnodes[v].vslot; };
@@ -928,6 +936,8 @@ bool compile_vm(const CompileRequest &req, Builder &B, const std::vector<AccOp> 
   h.lds_vbits_off = 0;
   h.n_gcols = (int)gcols.size();
   h.off_gcols = (int)off_gcols;
+  h.n_copy = (int)copies.size();
+  h.off_copy = (int)off_copy;
   h.gbuf_bytes = gbuf;
   h.g_kw_off = g_kw;
   h.g_buf_off = g_buf;
@@ -975,6 +985,7 @@ bool compile_vm(const CompileRequest &req, Builder &B, const std::vector<AccOp> 
   for (auto &op : vm_q) { put(out.blob, p, op); p += sizeof(VmOp); }
   for (size_t k = 0; k < pieces.size(); k++) put(out.blob, off_pieces + sizeof(Piece) * k, pieces[k]);
   for (size_t k = 0; k < gcols.size(); k++) put(out.blob, off_gcols + sizeof(Piece) * k, gcols[k]);
+  for (size_t k = 0; k < copies.size(); k++) put(out.blob, off_copy + sizeof(CopyCol) * k, copies[k]);
   p = off_vops;
   for (auto &op : vops_p) { put(out.blob, p, op); p += sizeof(VOp); }
   for (auto &op : vops_q) { put(out.blob, p, op); p += sizeof(VOp); }
@@ -1297,6 +1308,8 @@ Compiled compile_program(const CompileRequest &req) {
       std::fill(mat.begin(), mat.end(), -1);
       code = &acc_q;
       for (size_t o = 0; o < req.outputs.size(); o++) {
+        // a filter's bare-column outputs are copied by the gather kernel, raw
+        if (req.compact && req.outputs[o].kind == 1) continue;
         gen(out_node[o]);
         AccOp e = new_op(A_EMIT);
         e.oidx = (uint8_t)o;

@@ -154,6 +154,49 @@ DEV void stage_kept(const CONSTANT Piece *gcols, int ng, lds_ptr gb, int plrow, 
   }
 }
 
+// A filter's bare-column outputs: the kept rows grow[u] copied, raw, to outputs
+// obase + orel[u]; four outputs' loads in flight at a time.
+template <int R>
+DEV void copy_kept(const CONSTANT CopyCol *cc, int nc, const int64_t (&grow)[R], int64_t obase,
+                   const int (&orel)[R], const bool (&live)[R]) {
+  for (int g0 = 0; g0 < nc; g0 += 4) {
+    uint64_t v[4][R];
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      if (g0 + j < nc) {
+        const int w = cc[g0 + j].width;
+        const void *src = cc[g0 + j].src;
+#pragma unroll
+        for (int u = 0; u < R; u++) v[j][u] = live[u] ? load_raw(w, src, grow[u]) : 0;
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      if (g0 + j < nc) {
+        void *dst = cc[g0 + j].dst;
+        switch (cc[g0 + j].width) {
+          case 8:
+#pragma unroll
+            for (int u = 0; u < R; u++) if (live[u]) gstore<uint64_t>(dst, obase + orel[u], v[j][u]);
+            break;
+          case 4:
+#pragma unroll
+            for (int u = 0; u < R; u++) if (live[u]) gstore<uint32_t>(dst, obase + orel[u], (uint32_t)v[j][u]);
+            break;
+          case 2:
+#pragma unroll
+            for (int u = 0; u < R; u++) if (live[u]) gstore<uint16_t>(dst, obase + orel[u], (uint16_t)v[j][u]);
+            break;
+          default:
+#pragma unroll
+            for (int u = 0; u < R; u++) if (live[u]) gstore<uint8_t>(dst, obase + orel[u], (uint8_t)v[j][u]);
+            break;
+        }
+      }
+    }
+  }
+}
+
 // Operand rows (lane + 64 u) of the op, widened to 8-byte slots.
 template <int R>
 DEV void fetch(int cls, int off, int arg, int arg2, uint64_t im, clds_ptr tb, int lrow,
@@ -768,9 +811,10 @@ struct VmCtx {
   const CONSTANT VOp *vops;
   const CONSTANT Piece *pieces;
   const CONSTANT Piece *gcols;
+  const CONSTANT CopyCol *copies;
   int64_t n;
-  int n_in, n_out, n_ops_p, n_ops_q, n_vops_p, n_vops_q, any_valid_out, np, n_gcols, T, n_vout, st_width,
-      st_vsrc;
+  int n_in, n_out, n_ops_p, n_ops_q, n_vops_p, n_vops_q, any_valid_out, np, n_gcols, n_copy, T, n_vout,
+      st_width, st_vsrc;
 };
 DEV VmCtx load_ctx(const uint8_t *blob_arg) {
   const CONSTANT uint8_t *blob = (const CONSTANT uint8_t *)blob_arg;
@@ -782,6 +826,7 @@ DEV VmCtx load_ctx(const uint8_t *blob_arg) {
   c.vops = (const CONSTANT VOp *)(blob + h.off_vops);
   c.pieces = (const CONSTANT Piece *)(blob + h.off_pieces);
   c.gcols = (const CONSTANT Piece *)(blob + h.off_gcols);
+  c.copies = (const CONSTANT CopyCol *)(blob + h.off_copy);
   c.n = h.n_rows;
   c.n_in = h.n_in;
   c.n_out = h.n_out;
@@ -792,6 +837,7 @@ DEV VmCtx load_ctx(const uint8_t *blob_arg) {
   c.any_valid_out = h.any_valid_out;
   c.np = h.n_pieces;
   c.n_gcols = h.n_gcols;
+  c.n_copy = h.n_copy;
   c.T = h.tile_rows;
   c.n_vout = h.n_valid_out;
   c.st_width = h.stencil_width;
@@ -1063,6 +1109,7 @@ vm_kernel(const VmBatch *__restrict__ bd_arg) {
     stage_kept<R>(cx.gcols, cx.n_gcols, gbuf, plrow, grow, live);
     run_vm<R, OPS>(cx.ops + cx.n_ops_p, cx.n_ops_q, acc, tmpb, tstride, cx.in, cx.out, gbuf, plrow, grow, n, j0,
                     orel, live);
+    if (cx.n_copy) copy_kept<R>(cx.copies, cx.n_copy, grow, j0, orel, live);
     if (cx.any_valid_out) {
       // input validity of the chunk's rows: word u (outputs j0 + 64 u ..) to lane 0
 #pragma unroll
