@@ -20,6 +20,9 @@ rocprofv3 FETCH_SIZE / WRITE_SIZE summary in profiles/. cpu_baseline times the
 scalar oracle on a 10M-row sample of the same workload on this host.
 
 python bench.py [--gpus 1] [--steps K] [--warmup W] [--rows N] [--calibrate]
+python bench.py --workload filter-gather [--selectivity S]   (500M-row filter + gather:
+    int32 key U[0, 2^31), 3 int64 sequential and 3 float64 U[0, 1) payload columns,
+    key < S * 2^31, the payload compacted in row order; its own JSON line)
 """
 import argparse
 import json
@@ -40,6 +43,7 @@ FILTER = ("LogicalFilter(condition=[AND(>=($3, 730), <($3, 1095), >=($2, 0.03), 
           "<=($2, 0.05), <($0, 30))])")
 PROJECT = "LogicalProject(EXPR$0=[*($1, -(1, $2))], EXPR$1=[+($0, 1)])"
 PROFILE = os.path.join(ROOT, "profiles", "vm_kernel_c1_filter_project_100M.txt")
+PROFILE_FG = os.path.join(ROOT, "profiles", "vm_kernel_filter_gather_500M_s0.9.txt")
 STAGES = ["predicate", "slot_scan", "gather"]
 
 
@@ -65,11 +69,11 @@ def stage_bytes(n, kept, tile_rows):
     return [pred, scan, gather]
 
 
-def traffic_from_profile(kernel):
+def traffic_from_profile(kernel, profile=PROFILE):
     """HBM bytes per launch of `kernel` measured with rocprofv3 counters (profile file)."""
-    if not os.path.exists(PROFILE):
+    if not os.path.exists(profile):
         return None
-    for line in open(PROFILE):
+    for line in open(profile):
         m = re.match(r"^traffic_bytes_per_launch\[(.+)\]:\s*([0-9.eE+]+)", line.strip())
         if m and m.group(1).startswith(kernel):
             return float(m.group(2))
@@ -242,6 +246,67 @@ def cpu_baseline(sample_rows=10_000_000, min_seconds=10.0):
                       f"-ffp-contract=off), median of {len(times)} passes ({t:.2f} s each)"}
 
 
+def run_filter_gather(B, args):
+    """500M-row filter + gather: one JSON line, value = (52 N + 48 kept) bytes per step
+    over the step's wall time (inputs generated on the device, resident); roofline
+    for the dominant kernel (the gather of the kept rows at high selectivity)."""
+    n = args.rows if args.rows != 100_000_000 else 500_000_000
+    s = args.selectivity
+
+    def synth(dtype, seed, dist, a, b):
+        c = B.DeviceColumn(dtype, n, with_valid=False)
+        B.synth_column(c, seed, dist, a, b, 0.0)
+        return c
+
+    print(f"[bench] generating {n} rows on the device", file=sys.stderr, flush=True)
+    frame = [synth(B.GDF_INT32, 7201, 0, 0, 2.0**31 - 1), synth(B.GDF_INT64, 0, 5, 0, 0),
+             synth(B.GDF_INT64, 0, 5, 1e12, 0), synth(B.GDF_INT64, 0, 5, -5, 0),
+             synth(B.GDF_FLOAT64, 7202, 1, 0.0, 1.0), synth(B.GDF_FLOAT64, 7203, 1, 0.0, 1.0),
+             synth(B.GDF_FLOAT64, 7204, 1, 0.0, 1.0)]
+    filt = f"LogicalFilter(condition=[<($0, {int(s * 2**31)})])"
+    proj = "LogicalProject(p1=[$1], p2=[$2], p3=[$3], f1=[$4], f2=[$5], f3=[$6])"
+    prog = B.Program(frame, filt, proj)
+    for _ in range(args.warmup):
+        prog.launch()
+    kept, _ = prog.result()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        prog.launch()
+    kept, _ = prog.result()
+    dt = (time.perf_counter() - t0) / args.steps
+    stage_ms = prog.time_stages(iters=max(10, args.steps // 2))
+    tiles = (n + prog.tile_rows - 1) // prog.tile_rows
+    sbytes = [4 * n + n / 8 + 16 * tiles, 16 * tiles + n / 8 + 4 * kept, 4 * kept + 48 * kept + 48 * kept]
+    dom = int(np.argmax(stage_ms))
+    kernel_name = (f"vm_kernel<{prog.r}, {1 if dom == 0 else 2}," if dom != 1 else "slot_scan_kernel")
+    achieved = sbytes[dom] / (stage_ms[dom] * 1e-3) / 1e9
+    bytes_step = 52 * n + 48 * kept
+    prog.close()
+    for c in frame:
+        c.free()
+    if args.calibrate:
+        buf_a = B.DeviceBuffer(1 << 30)
+        buf_b = B.DeviceBuffer(1 << 30)
+        for _ in range(3):
+            B._check(B.lib().blz_device_copy(buf_b.ptr, buf_a.ptr, 1 << 30, None), "copy")
+        print("[bench] calibration copy: 3 x 1 GiB", file=sys.stderr, flush=True)
+    print(json.dumps({
+        "metric": "filter + gather GB/s at 500M rows", "value": bytes_step / dt / 1e9, "unit": "GB/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": dt * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "int64/f64",
+        "data": "synthetic",
+        "config": {"workload": "filter + gather: int32 key, 3 int64 + 3 f64 payload, key < s*2^31",
+                   "rows": n, "selectivity": s, "rows_kept": int(kept),
+                   "min_traffic_GBps": (4 * n + 96 * kept) / dt / 1e9,
+                   "program": f"tile VM R={prog.r} tile={prog.tile_rows} rows"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_from_profile(kernel_name, PROFILE_FG),
+                     "kernel": kernel_name, "stage": STAGES[dom], "kernel_ms": stage_ms[dom],
+                     "algorithmic_bytes": sbytes[dom]},
+        "pipeline": {"stages": dict(zip(STAGES, stage_ms)), "kernel_ms": float(sum(stage_ms))},
+    }))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -256,6 +321,8 @@ def main():
     ap.add_argument("--batch-plans", type=int, default=256)
     ap.add_argument("--serve-gb", type=float, default=100.0)
     ap.add_argument("--serve-seconds", type=float, default=1.0)
+    ap.add_argument("--workload", choices=["filter-project", "filter-gather"], default="filter-project")
+    ap.add_argument("--selectivity", type=float, default=0.9, help="filter-gather: fraction of rows kept")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; multi-GPU runs are not part of it",
@@ -264,6 +331,9 @@ def main():
     import blz_ral as B
     if B.device_count() < 1:
         raise SystemExit("no HIP device visible")
+    if args.workload == "filter-gather":
+        run_filter_gather(B, args)
+        return
 
     n = args.rows
     print(f"[bench] generating {n} rows", file=sys.stderr, flush=True)
