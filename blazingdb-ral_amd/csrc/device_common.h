@@ -155,6 +155,19 @@ DEV uint32_t load_narrow(int kind, const void *base, int64_t row) {
   }
 }
 
+// A bare-column output keeps its bytes (materialize_column copies by width): row
+// irow of the input column to row orow of the output, raw (a float32 is not
+// widened, so its NaN payloads stay as they are).
+DEV void copy_alias(int kind, void *dst, int64_t orow, const void *src, int64_t irow) {
+  switch (kind) {
+    case ST_I8: gstore<uint8_t>(dst, orow, gload<uint8_t>(src, irow)); break;
+    case ST_I16: gstore<uint16_t>(dst, orow, gload<uint16_t>(src, irow)); break;
+    case ST_I32: case ST_F32: gstore<uint32_t>(dst, orow, gload<uint32_t>(src, irow)); break;
+    case ST_I64: case ST_F64: gstore<uint64_t>(dst, orow, gload<uint64_t>(src, irow)); break;
+    default: break;
+  }
+}
+
 DEV void store_narrow(int kind, void *base, int64_t row, uint64_t v) {
   switch (kind) {
     case ST_I8: gstore<int8_t>(base, row, (int8_t)v); break;

@@ -200,9 +200,15 @@ interp_kernel(const uint8_t *__restrict__ blob_arg, Workspace ws) {
         const int kind = out[o].store;
         const int slot = out[o].slot;
         void *base = out[o].data;
+        if (out[o].src_kind == 1) {  // a bare column (its slot is the input's)
 #pragma unroll
-        for (int u = 0; u < R; u++)
-          if (row[u] < n) store_narrow(kind, base, row[u], rf[slot * R + u]);
+          for (int u = 0; u < R; u++)
+            if (row[u] < n) copy_alias(kind, base, row[u], in[slot].data, row[u]);
+        } else {
+#pragma unroll
+          for (int u = 0; u < R; u++)
+            if (row[u] < n) store_narrow(kind, base, row[u], rf[slot * R + u]);
+        }
         void *vp = out[o].valid;
         if (vp != nullptr) {
           const uint64_t word = vsrc_word<S>(out[o].vsrc, vrf);
@@ -310,7 +316,11 @@ interp_kernel(const uint8_t *__restrict__ blob_arg, Workspace ws) {
         const int kind = out[o].store;
         const int slot = out[o].slot;
         void *base = out[o].data;
-        if (wave_cnt > 0) {
+        if (wave_cnt > 0 && out[o].src_kind == 1) {  // a bare column (its slot is the input's)
+#pragma unroll
+          for (int u = 0; u < R; u++)
+            if (keep[u]) copy_alias(kind, base, (int64_t)pos[u], in[slot].data, row[u]);
+        } else if (wave_cnt > 0) {
 #pragma unroll
           for (int u = 0; u < R; u++)
             if (keep[u]) store_narrow(kind, base, (int64_t)pos[u], rf[slot * R + u]);

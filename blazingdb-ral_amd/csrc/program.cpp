@@ -889,9 +889,9 @@ bool compile_vm(const CompileRequest &req, Builder &B, const std::vector<AccOp> 
   const size_t off_out = al(off_in + sizeof(InCol) * n_dev);
   const size_t off_ops = al(off_out + sizeof(OutCol) * n_out);
   const size_t off_pieces = al(off_ops + sizeof(VmOp) * (vm_p.size() + vm_q.size()));
-  // a filter's bare-column outputs (no ops: the gather kernel copies them)
+  // bare-column outputs (no ops: the projection / gather kernel copies them)
   std::vector<CopyCol> copies;
-  for (int o = 0; o < n_out && req.compact; o++) {
+  for (int o = 0; o < n_out; o++) {
     const OutSpec &s = req.outputs[o];
     if (s.kind != 1 || store_kind_alias(s.dtype) == ST_NONE) continue;
     copies.push_back(CopyCol{req.inputs[s.index].data, s.data, std::max(1, wid(s.dtype)), 0, 0});
@@ -1308,8 +1308,8 @@ Compiled compile_program(const CompileRequest &req) {
       std::fill(mat.begin(), mat.end(), -1);
       code = &acc_q;
       for (size_t o = 0; o < req.outputs.size(); o++) {
-        // a filter's bare-column outputs are copied by the gather kernel, raw
-        if (req.compact && req.outputs[o].kind == 1) continue;
+        // bare-column outputs are copied by the kernels, raw
+        if (req.outputs[o].kind == 1) continue;
         gen(out_node[o]);
         AccOp e = new_op(A_EMIT);
         e.oidx = (uint8_t)o;

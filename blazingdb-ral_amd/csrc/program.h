@@ -206,7 +206,7 @@ struct Piece {
 static_assert(sizeof(Piece) == 32, "Piece layout");
 constexpr int kPieceBytes = 1024;
 
-// A filter's output that is a bare input column: the gather kernel copies the kept
+// An output that is a bare input column: the projection / gather kernel copies the
 // rows' bytes straight from the column (as materialize_column does).
 struct CopyCol {
   const void *src;  // input column data
@@ -268,7 +268,7 @@ struct ProgHeader {
   int32_t n_pieces;              // tile VM: staging pieces per tile
   int32_t off_pieces;            // tile VM: offset of the Piece table
   int32_t n_gcols;               // tile VM filter: columns the gather pass stages (Piece: base, width, lds)
-  int32_t n_copy, off_copy;      // tile VM filter: bare-column outputs (CopyCol table)
+  int32_t n_copy, off_copy;      // tile VM: bare-column outputs (CopyCol table)
   int32_t off_gcols;             // tile VM filter: offset of that table
   int32_t gbuf_bytes;            // tile VM filter: bytes of the gather pass buffer (256 R rows)
   int32_t lds_tmp_off;           // tile VM: LDS temporaries (projection / predicate kernels)

@@ -154,8 +154,9 @@ DEV void stage_kept(const CONSTANT Piece *gcols, int ng, lds_ptr gb, int plrow, 
   }
 }
 
-// A filter's bare-column outputs: the kept rows grow[u] copied, raw, to outputs
-// obase + orel[u]; four outputs' loads in flight at a time.
+// Bare-column outputs: rows grow[u] copied, raw, to outputs obase + orel[u] (the
+// kept rows of a filter, or every row of a projection); four outputs' loads in
+// flight at a time.
 template <int R>
 DEV void copy_kept(const CONSTANT CopyCol *cc, int nc, const int64_t (&grow)[R], int64_t obase,
                    const int (&orel)[R], const bool (&live)[R]) {
@@ -961,6 +962,7 @@ vm_kernel(const VmBatch *__restrict__ bd_arg) {
           }
           run_vm<R, OPS>(cx.ops, cx.n_ops_q, acc, tmpb, tstride, cx.in, cx.out, tb, lrow, grow, n, row0, orel,
                           put);
+          if (cx.n_copy) copy_kept<R>(cx.copies, cx.n_copy, grow, row0, orel, put);
           if (cx.any_valid_out) {
             vm_load_valid<R>(cx.in, cx.n_in, tb, lb >> 6, vr);
             vm_run_vops<R>(cx.vops, cx.n_vops_q, vr);

@@ -629,3 +629,46 @@ def test_integer_division_by_constants(B, kernel):
             g.col.null_count = nc
             assert_column_equal(g, r, what=f"kernel {kernel}: {proj}")
         prog.close()
+
+
+BARE_DTYPES = [P.GDF_INT8, P.GDF_BOOL8, P.GDF_INT16, P.GDF_INT32, P.GDF_INT64, P.GDF_FLOAT32,
+               P.GDF_FLOAT64, P.GDF_DATE32, P.GDF_DATE64, P.GDF_TIMESTAMP]
+
+
+@pytest.mark.parametrize("kernel", [0, 3])
+@pytest.mark.parametrize("mode", ["filter", "project"])
+def test_bare_column_outputs_keep_their_bytes(B, kernel, mode):
+    """`$k` outputs are the (kept) rows' bytes, unchanged (the reference gathers them
+    by width, ColumnManipulation.cu:83-118, or aliases them): every dtype, with
+    nulls, float NaNs with payloads (quiet and signalling) and signed zeros, next to
+    an evaluated output; row counts around whole tiles and chunks."""
+    for n in (1, 255, 4096, 70_001):
+        rng = np.random.default_rng(9100 + n)
+        host = [gen_column(rng, P.GDF_INT32, n, 0.1, -50, 50)]
+        for i, dt in enumerate(BARE_DTYPES):
+            h = gen_column(rng, dt if dt not in (P.GDF_BOOL8,) else P.GDF_INT8, n, 0.1 * (i % 2))
+            if dt == P.GDF_BOOL8:
+                h = O.HostColumn((h.data != 0).astype(np.int8), dt, h.valid)
+            host.append(h)
+        f32 = host[1 + BARE_DTYPES.index(P.GDF_FLOAT32)].data.view(np.uint32)
+        f64 = host[1 + BARE_DTYPES.index(P.GDF_FLOAT64)].data.view(np.uint64)
+        k = min(n, 40)
+        f32[:k:4] = np.array([0x7FC00001, 0x7F800001, 0xFFA5A5A5, 0x80000000] * 10, np.uint32)[:len(f32[:k:4])]
+        f64[:k:4] = np.array([0x7FF8000000000123, 0x7FF0000000000001, 0xFFF4000000000000,
+                              0x8000000000000000] * 10, np.uint64)[:len(f64[:k:4])]
+        filt = "LogicalFilter(condition=[<($0, 20)])" if mode == "filter" else None
+        items = [f"C{i}=[${i + 1}]" for i in range(len(BARE_DTYPES))] + ["E=[*($0, 3)]"]
+        proj = "LogicalProject(" + ", ".join(items) + ")"
+        _print_case(test="bare_outputs", kernel=kernel, mode=mode, n=n)
+        ref = _oracle(host, filt, proj)
+        dev = [to_device(B, h) for h in host]
+        prog = B.Program(dev, filt, proj, kernel=kernel)
+        prog.launch()
+        kept, nulls = prog.result()
+        assert kept == len(ref[0])
+        for g, r, nc in zip(prog.outputs, ref, nulls):
+            g.col.null_count = nc
+            assert_column_equal(g, r, what=f"kernel {kernel} n={n}: bare-column output {r.dtype}")
+        prog.close()
+        for d in dev:
+            d.free()
