@@ -246,6 +246,29 @@ def cpu_baseline(sample_rows=10_000_000, min_seconds=10.0):
                       f"-ffp-contract=off), median of {len(times)} passes ({t:.2f} s each)"}
 
 
+def run_aggregate(B, frame, prog, kept, n, iters=20):
+    """Aggregation without GROUP BY through the drop-in call (process_aggregate: parse,
+    reduce, one-row results, blocking): SUM over the 100M-row float64 column b (one
+    reduction, 8 B per row read), and TPC-H Q6's SUM(b*(1-c)) over the filter+project
+    program's kept rows (with COUNT and MAX). Wall milliseconds per call."""
+    def timed(cols, q):
+        for _ in range(3):
+            for c in B.process_aggregate(cols, q):
+                c.free()
+        t0 = time.perf_counter()
+        for _ in range(iters):
+            for c in B.process_aggregate(cols, q):
+                c.free()
+        return (time.perf_counter() - t0) / iters * 1e3
+    col_ms = timed(frame, "LogicalAggregate(group=[{}], s=[SUM($1)])")
+    prog.launch()
+    prog.result()
+    q6_ms = timed(prog.outputs, "LogicalAggregate(group=[{}], revenue=[SUM($0)], n=[COUNT()], m=[MAX($1)])")
+    return {"sum_100M_f64": {"ms": col_ms, "GBps": 8 * n / (col_ms * 1e-3) / 1e9, "rows": n},
+            "q6_sum_of_projection": {"ms": q6_ms, "rows": int(kept),
+                                     "path": "process_aggregate over the filter+project outputs"}}
+
+
 def run_filter_gather(B, args):
     """500M-row filter + gather: one JSON line, value = (52 N + 48 kept) bytes per step
     over the step's wall time (inputs generated on the device, resident); roofline
@@ -368,6 +391,7 @@ def main():
         for _ in range(3):
             B._check(B.lib().blz_device_copy(buf_b.ptr, buf_a.ptr, 1 << 30, None), "copy")
         print("[bench] calibration copy: 3 x 1 GiB", file=sys.stderr, flush=True)
+    aggregate = run_aggregate(B, frame, prog, kept, n)
     prog.close()
     for c in frame:
         c.free()
@@ -400,6 +424,7 @@ def main():
                      "achieved": bytes_step / (total_ms * 1e-3) / 1e9,
                      "frac": bytes_step / (total_ms * 1e-3) / 1e9 / HBM_PEAK_GBS},
         "cpu_baseline": cpu,
+        "aggregate": aggregate,
         "batch": batch,
         "serve": serve,
     }

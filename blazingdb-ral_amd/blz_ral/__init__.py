@@ -99,6 +99,7 @@ def lib():
                                     ERR, SZ],
             "blz_filter_project": [colpp, I, C.c_char_p, C.c_char_p, colp, I, C.POINTER(C.c_int),
                                    C.POINTER(C.c_int64), I, P, ERR, SZ],
+            "blz_process_aggregate": [colpp, I, C.c_char_p, colp, I, C.POINTER(C.c_int), I, P, ERR, SZ],
             "blz_materialize_column": [colp, colp, colp, P, ERR, SZ],
             "blz_gdf_sequence": [P, SZ, C.c_int32, C.c_int32, P, ERR, SZ],
             "blz_apply_boolean_mask": [colp, colp, colp, P, ERR, SZ],
@@ -397,6 +398,16 @@ def filter_project(frame: Sequence[DeviceColumn], filter_query_part: str,
           project_query_part.encode() if project_query_part else None, out, max_out, C.byref(n),
           C.byref(kept), scalar_mode, stream)
     return [DeviceColumn.adopt(out[i], owned=True) for i in range(n.value)], kept.value
+
+
+def process_aggregate(frame: Sequence[DeviceColumn], query_part: str, scalar_mode=SCALAR_INTENDED,
+                      stream=None, max_out=64):
+    """process_aggregate without GROUP BY (GroupBy.cpp:531-588): one-row columns."""
+    out = (GdfColumn * max_out)()
+    n = C.c_int(0)
+    _call(lib().blz_process_aggregate, _colpp(frame), len(frame), query_part.encode(), out, max_out,
+          C.byref(n), scalar_mode, stream)
+    return [DeviceColumn.adopt(out[i], owned=True) for i in range(n.value)]
 
 
 def materialize_column(inp: DeviceColumn, out: DeviceColumn, row_indices: DeviceColumn, stream=None):

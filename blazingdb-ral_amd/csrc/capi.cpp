@@ -521,6 +521,184 @@ int blz_process_filter(gdf_column *const *frame, int n_frame, const char *query_
   });
 }
 
+namespace {
+// One aggregation of a LogicalAggregate line (the reference's gdf_agg_op, by name).
+enum AggOp { AGG_SUM, AGG_MIN, AGG_MAX, AGG_AVG, AGG_COUNT };
+struct AggSpec {
+  AggOp op;
+  std::string input;  // "$k", an expression, or empty
+};
+
+std::string trim(const std::string &s) {
+  const size_t a = s.find_first_not_of(" \t\n"), b = s.find_last_not_of(" \t\n");
+  return a == std::string::npos ? std::string() : s.substr(a, b - a + 1);
+}
+
+// process_aggregate's parsing (GroupBy.cpp:543-569, get_aggregation_operation
+// CalciteExpressionParsing.cpp:483-507, get_string_between_outer_parentheses :756-767).
+std::vector<AggSpec> parse_aggregate(const std::string &qp) {
+  const size_t a = qp.find('('), b = qp.rfind(')');
+  if (a == std::string::npos || b == std::string::npos || b < a) invalid("not a LogicalAggregate line");
+  const std::string combined = qp.substr(a + 1, b - a - 1);
+  const std::string group = trim(planner::get_named_expression(combined, "group"));
+  if (group.size() > 2) throw BlzError(BLZ_ERR_UNSUPPORTED, "aggregation with GROUP BY is not supported");
+  std::vector<AggSpec> out;
+  for (const std::string &e0 : planner::get_expressions_from_expression_list(combined)) {
+    const std::string e = trim(e0);
+    if (e.find("group=") != std::string::npos) continue;
+    const size_t eq = e.find("=[");
+    if (eq == std::string::npos) invalid("aggregation without =[...]: " + e);
+    const size_t lp = e.find('(', eq), rp = e.rfind(')');
+    if (lp == std::string::npos || rp == std::string::npos || rp < lp) invalid("aggregation without (...): " + e);
+    const std::string name = trim(e.substr(eq + 2, lp - eq - 2));
+    AggSpec s;
+    if (name == "SUM") s.op = AGG_SUM;
+    else if (name == "MIN") s.op = AGG_MIN;
+    else if (name == "MAX") s.op = AGG_MAX;
+    else if (name == "AVG") s.op = AGG_AVG;
+    else if (name == "COUNT") s.op = AGG_COUNT;
+    else if (name == "COUNT_DISTINCT") throw BlzError(BLZ_ERR_UNSUPPORTED, "COUNT_DISTINCT is not supported");
+    else invalid("aggregation type not supported, " + name);
+    s.input = trim(e.substr(lp + 1, rp - lp - 1));
+    out.push_back(s);
+  }
+  return out;
+}
+
+bool is_float_dtype(int t) { return t == GDF_FLOAT32 || t == GDF_FLOAT64; }
+double b2d_host(uint64_t u) {
+  double d;
+  std::memcpy(&d, &u, 8);
+  return d;
+}
+
+// get_aggregation_output_type (CalciteExpressionParsing.cpp:92-117), no GROUP BY.
+int aggregation_output_type(int input_type, AggOp op) {
+  switch (op) {
+    case AGG_COUNT: return GDF_INT64;
+    case AGG_SUM: return is_float_dtype(input_type) ? GDF_FLOAT64 : GDF_INT64;
+    case AGG_AVG: return GDF_FLOAT64;
+    default: return input_type;
+  }
+}
+
+// Reduction of the valid rows of `c` (op 0 SUM, 1 MIN, 2 MAX) on the GPU.
+RedPart reduce_column(const gdf_column *c, int op, hipStream_t st) {
+  const int load = column_load_kind(c->dtype);
+  if (load == LD_NONE) throw BlzError(BLZ_ERR_UNSUPPORTED, "aggregation over this dtype is not supported");
+  const int64_t n = c->size;
+  const int64_t bt = block_threads();
+  const int grid = (int)std::max<int64_t>(1, std::min<int64_t>(1024, (n + bt - 1) / bt));
+  RedPart *parts = nullptr;
+  HIPCHECK(hipMallocAsync((void **)&parts, sizeof(RedPart) * (grid + 1), st));
+  RedPart r{};
+  hipError_t e = launch_reduce(op, c->data, load, (const uint8_t *)c->valid, n, parts, grid, st);
+  if (e == hipSuccess) e = hipMemcpyAsync(&r, parts + grid, sizeof(r), hipMemcpyDeviceToHost, st);
+  hipError_t f = hipFreeAsync(parts, st);
+  hipError_t s = hipStreamSynchronize(st);
+  HIPCHECK(e);
+  HIPCHECK(f);
+  HIPCHECK(s);
+  return r;
+}
+
+// A one-row column holding `bits` (the first width bytes) and its validity.
+gdf_column scalar_column(int dtype, uint64_t bits, bool valid, hipStream_t st) {
+  gdf_column c = alloc_column(dtype, 1, true, st);
+  const uint8_t vbyte = valid ? 1 : 0;
+  HIPCHECK(hipMemcpyAsync(c.data, &bits, blz_dtype_width((gdf_dtype)dtype), hipMemcpyHostToDevice, st));
+  HIPCHECK(hipMemcpyAsync(c.valid, &vbyte, 1, hipMemcpyHostToDevice, st));
+  HIPCHECK(hipStreamSynchronize(st));
+  c.null_count = valid ? 0 : 1;
+  return c;
+}
+}  // namespace
+
+int blz_process_aggregate(gdf_column *const *frame, int n_frame, const char *query_part,
+                          gdf_column *out_columns, int max_out, int *n_out, int scalar_mode,
+                          blz_stream_t stream, char *err, size_t err_len) {
+  return guarded(err, err_len, [&] {
+    check_frame(frame, n_frame);
+    if (!query_part || !out_columns || !n_out) invalid("bad arguments");
+    const std::vector<AggSpec> aggs = parse_aggregate(query_part);
+    if ((int)aggs.size() > max_out) invalid("more aggregations than out_columns can hold");
+    hipStream_t st = S(stream);
+    const int64_t n = frame[0]->size;
+    const auto dt = frame_dtypes(frame, n_frame);
+    std::vector<gdf_column> made;
+    try {
+      for (const AggSpec &a : aggs) {
+        // the aggregation's input column (compute_aggregations, GroupBy.cpp:303-315)
+        gdf_column temp{};
+        const gdf_column *in = nullptr;
+        if (planner::contains_evaluation(a.input)) {
+          const int t = planner::get_output_type_expression(dt, a.input);
+          temp = alloc_column(t, n, true, st);
+          int32_t nulls = 0;
+          char e2[512] = {0};
+          const int rc = blz_evaluate_expression(frame, n_frame, a.input.c_str(), &temp, scalar_mode, &nulls,
+                                                 stream, e2, sizeof(e2));
+          if (rc != BLZ_OK) {
+            blz_free_column(&temp);
+            throw BlzError(rc, std::string("aggregation input: ") + e2);
+          }
+          temp.null_count = nulls;
+          in = &temp;
+        } else {
+          int k = 0;
+          if (!a.input.empty()) {
+            if (a.input[0] != '$') invalid("aggregation input is neither a column nor an expression: " + a.input);
+            k = std::stoi(a.input.substr(1));
+          }
+          if (k < 0 || k >= n_frame) invalid("aggregation input column out of range: " + a.input);
+          in = frame[k];
+        }
+        const int ot = aggregation_output_type(in->dtype, a.op);
+        const int64_t rows = in->size, valid_rows = rows - (in->valid ? in->null_count : 0);
+        uint64_t bits = 0;
+        bool valid = true;
+        try {
+          if (a.op == AGG_COUNT) {
+            bits = (uint64_t)valid_rows;
+          } else if (a.op == AGG_AVG) {
+            // the sum over the valid rows in INT64 or FLOAT64, divided in binary64
+            // by size - null_count (GroupBy.cpp:251-270)
+            valid = rows > 0 && valid_rows > 0;
+            if (valid) {
+              const RedPart r = reduce_column(in, 0, st);
+              const double sum = is_float_dtype(in->dtype) ? b2d_host(r.v) : (double)(int64_t)r.v;
+              const double avg = sum / (double)valid_rows;
+              std::memcpy(&bits, &avg, 8);
+            }
+          } else {
+            const RedPart r = rows > 0 ? reduce_column(in, a.op == AGG_SUM ? 0 : a.op == AGG_MIN ? 1 : 2, st)
+                                       : RedPart{0, 0};
+            valid = r.cnt > 0;
+            bits = r.v;
+            if (valid && a.op != AGG_SUM && ot == GDF_FLOAT32) {  // widened float32 back
+              const float f = (float)b2d_host(r.v);
+              uint32_t u;
+              std::memcpy(&u, &f, 4);
+              bits = u;
+            }
+            if (!valid) bits = 0;
+          }
+        } catch (...) {
+          if (temp.data) blz_free_column(&temp);
+          throw;
+        }
+        if (temp.data) blz_free_column(&temp);
+        made.push_back(scalar_column(ot, bits, valid, st));
+      }
+    } catch (...) {
+      for (auto &c : made) blz_free_column(&c);
+      throw;
+    }
+    for (size_t i = 0; i < made.size(); i++) out_columns[i] = made[i];
+    *n_out = (int)made.size();
+  });
+}
+
 int blz_filter_project(gdf_column *const *frame, int n_frame, const char *filter_query_part,
                        const char *project_query_part, gdf_column *out_columns, int max_out, int *n_out,
                        int64_t *n_kept, int scalar_mode, blz_stream_t stream, char *err, size_t err_len) {

@@ -84,6 +84,17 @@ hipError_t launch_fill_zero(void *p, int64_t bytes, int grid, hipStream_t st);
 hipError_t launch_null_count(const uint8_t *valid, int64_t n, unsigned long long *out, int grid,
                              hipStream_t st);
 
+// Reduction of a column's valid rows (op 0 SUM, 1 MIN, 2 MAX; load: its LoadKind):
+// parts[0..grid) the workgroups' partials, parts[grid] the result (value as the
+// interpreter's 64-bit pattern: int64, or binary64 for float columns; cnt = valid
+// rows). Deterministic for a given grid (1 <= grid <= 65536).
+struct RedPart {
+  uint64_t v;
+  unsigned long long cnt;
+};
+hipError_t launch_reduce(int op, const void *data, int load, const uint8_t *valid, int64_t n, RedPart *parts,
+                         int grid, hipStream_t st);
+
 // dst[0..bytes) = src[0..bytes); bytes a multiple of 16, both 16-byte aligned.
 hipError_t launch_copy(void *dst, const void *src, int64_t bytes, int grid, hipStream_t st);
 

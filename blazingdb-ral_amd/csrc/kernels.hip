@@ -503,6 +503,80 @@ null_count_kernel(const uint8_t *valid, int64_t n, unsigned long long *nulls_out
 }
 
 // ---------------------------------------------------------------------------
+// Reduction of one column over its valid rows, cudf::reduce's role in an
+// aggregation without GROUP BY (GroupBy.cpp:230-290): op 0 SUM, 1 MIN, 2 MAX.
+// Values are widened as the interpreter widens them (integers to int64, floats to
+// binary64); SUM accumulates in int64 (wrapping) or binary64, MIN / MAX compare in
+// the widened type (a NaN never replaces the running value). Thread t of
+// workgroup b takes rows b*256 + t + k*grid*256 in that order, then a fixed tree;
+// the final pass combines the partials in a fixed order: a float sum is the same
+// on every run with the same grid.
+
+template <int OP, bool FLT>
+DEV uint64_t red_identity() {
+  if (FLT) return d2b(OP == 0 ? 0.0 : OP == 1 ? __builtin_inf() : -__builtin_inf());
+  return OP == 0 ? 0ull : OP == 1 ? (uint64_t)INT64_MAX : (uint64_t)INT64_MIN;
+}
+template <int OP, bool FLT>
+DEV uint64_t red_combine(uint64_t a, uint64_t b) {
+  if (FLT) {
+    const double x = b2d(a), y = b2d(b);
+    return d2b(OP == 0 ? x + y : OP == 1 ? (y < x ? y : x) : (y > x ? y : x));
+  }
+  const int64_t x = (int64_t)a, y = (int64_t)b;
+  return OP == 0 ? a + b : (uint64_t)(OP == 1 ? (y < x ? y : x) : (y > x ? y : x));
+}
+// value and valid-row count of this workgroup, combined over its waves in order
+template <int OP, bool FLT>
+DEV void red_block(uint64_t acc, unsigned long long cnt, RedPart *dst) {
+  __shared__ uint64_t s_v[kWaves];
+  __shared__ unsigned long long s_c[kWaves];
+  for (int off = 32; off > 0; off >>= 1) {
+    acc = red_combine<OP, FLT>(acc, __shfl_xor(acc, off));
+    cnt += __shfl_xor(cnt, off);
+  }
+  const int wave = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    s_v[wave] = acc;
+    s_c[wave] = cnt;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint64_t v = s_v[0];
+    unsigned long long c = s_c[0];
+    for (int w = 1; w < kWaves; w++) {
+      v = red_combine<OP, FLT>(v, s_v[w]);
+      c += s_c[w];
+    }
+    dst->v = v;
+    dst->cnt = c;
+  }
+}
+template <int OP, bool FLT>
+__global__ void __launch_bounds__(kBlock)
+reduce_kernel(const void *data, int load, const uint8_t *valid, int64_t n, RedPart *parts) {
+  uint64_t acc = red_identity<OP, FLT>();
+  unsigned long long cnt = 0;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
+    if (!valid_bit(valid, i)) continue;
+    const uint64_t v = is_wide(load) ? gload<uint64_t>(data, i) : widen32(load, load_narrow(load, data, i));
+    acc = red_combine<OP, FLT>(acc, v);
+    cnt++;
+  }
+  red_block<OP, FLT>(acc, cnt, parts + blockIdx.x);
+}
+template <int OP, bool FLT>
+__global__ void __launch_bounds__(kBlock) reduce_final_kernel(const RedPart *parts, int n_parts, RedPart *out) {
+  uint64_t acc = red_identity<OP, FLT>();
+  unsigned long long cnt = 0;
+  for (int i = threadIdx.x; i < n_parts; i += kBlock) {
+    acc = red_combine<OP, FLT>(acc, parts[i].v);
+    cnt += parts[i].cnt;
+  }
+  red_block<OP, FLT>(acc, cnt, out);
+}
+
+// ---------------------------------------------------------------------------
 // Host-side launchers.
 
 template <int S, int R>
@@ -596,6 +670,27 @@ hipError_t launch_null_count(const uint8_t *valid, int64_t n, unsigned long long
                              hipStream_t st) {
   hipLaunchKernelGGL(null_count_kernel, dim3(grid), dim3(kBlock), 0, st, valid, n, out);
   return hipGetLastError();
+}
+
+template <int OP, bool FLT>
+static hipError_t launch_reduce_t(const void *data, int load, const uint8_t *valid, int64_t n, RedPart *parts,
+                                  int grid, hipStream_t st) {
+  hipLaunchKernelGGL((reduce_kernel<OP, FLT>), dim3(grid), dim3(kBlock), 0, st, data, load, valid, n, parts);
+  hipLaunchKernelGGL((reduce_final_kernel<OP, FLT>), dim3(1), dim3(kBlock), 0, st, parts, grid, parts + grid);
+  return hipGetLastError();
+}
+hipError_t launch_reduce(int op, const void *data, int load, const uint8_t *valid, int64_t n, RedPart *parts,
+                         int grid, hipStream_t st) {
+  const bool flt = load == LD_F32 || load == LD_F64;
+  switch (op * 2 + flt) {
+    case 0: return launch_reduce_t<0, false>(data, load, valid, n, parts, grid, st);
+    case 1: return launch_reduce_t<0, true>(data, load, valid, n, parts, grid, st);
+    case 2: return launch_reduce_t<1, false>(data, load, valid, n, parts, grid, st);
+    case 3: return launch_reduce_t<1, true>(data, load, valid, n, parts, grid, st);
+    case 4: return launch_reduce_t<2, false>(data, load, valid, n, parts, grid, st);
+    case 5: return launch_reduce_t<2, true>(data, load, valid, n, parts, grid, st);
+    default: return hipErrorInvalidValue;
+  }
 }
 
 hipError_t launch_copy(void *dst, const void *src, int64_t bytes, int grid, hipStream_t st) {

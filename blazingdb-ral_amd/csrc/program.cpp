@@ -1021,6 +1021,8 @@ bool compile_vm(const CompileRequest &req, Builder &B, const std::vector<AccOp> 
 
 }  // namespace
 
+int column_load_kind(int dtype) { return load_kind(dtype); }
+
 Compiled compile_program(const CompileRequest &req) {
   Compiled out;
   out.n_rows = req.n_rows;

@@ -75,6 +75,7 @@ struct Compiled {
 Compiled compile_program(const CompileRequest &req);
 
 int unary_output_dtype(int l, int u);
+int column_load_kind(int dtype);  // LoadKind of a column dtype (LD_NONE: not read)
 int binary_output_dtype(int l, int r, int op);
 uint64_t scalar_bits(const gdf_scalar &s, int mode);
 

@@ -92,6 +92,20 @@ int blz_execute_project(gdf_column *const *frame, int n_frame, const char *query
                         gdf_column *out_columns, int32_t *is_alias, int max_out, int *n_out,
                         int scalar_mode, blz_stream_t stream, char *err, size_t err_len);
 
+/* process_aggregate without GROUP BY (src/operators/GroupBy.h:18, GroupBy.cpp:531-588
+ * with compute_aggregations :292-336 and aggregations_without_groupby :230-290):
+ * `query_part` is a "LogicalAggregate(group=[{}], name=[OP(input)], ...)" line, OP
+ * one of SUM, MIN, MAX, AVG, COUNT; input "$k", an expression (evaluated first) or
+ * empty (COUNT(): column 0). out_columns[0..*n_out) receive one-row callee-allocated
+ * columns (free with blz_free_column): COUNT -> INT64 rows - null_count (always
+ * valid); SUM -> INT64 (integer input, wrapping) or FLOAT64 over the valid rows;
+ * MIN / MAX -> the input dtype; AVG -> FLOAT64 sum / count. SUM, MIN, MAX, AVG of no
+ * valid row are null. A GROUP BY (group=[{...}] not empty) or COUNT_DISTINCT gives
+ * BLZ_ERR_UNSUPPORTED. */
+int blz_process_aggregate(gdf_column *const *frame, int n_frame, const char *query_part,
+                          gdf_column *out_columns, int max_out, int *n_out, int scalar_mode,
+                          blz_stream_t stream, char *err, size_t err_len);
+
 /* LogicalProject over LogicalFilter in ONE pass over the frame: the result equals
  * blz_process_filter followed by blz_execute_project on its output (projection of
  * row-local expressions commutes with the filter). Every output is callee-allocated

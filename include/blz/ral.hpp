@@ -220,6 +220,24 @@ inline void process_filter(blazing_frame &input, std::string query_part) {
   input.add_table(std::move(table));
 }
 
+// A LogicalAggregate without GROUP BY: the frame then holds one table of one-row
+// columns, one per aggregation (ral::operators::process_aggregate, GroupBy.h:18,
+// single-node path GroupBy.cpp:531-588; no communication context here).
+inline void process_aggregate(blazing_frame &input, std::string query_part) {
+  auto frame = input.column_pointers();
+  const int max_out = 256;
+  std::vector<gdf_column> outs(max_out);
+  int n_out = 0;
+  char err[512] = {0};
+  blz_detail::check(blz_process_aggregate(frame.data(), (int)frame.size(), query_part.c_str(), outs.data(),
+                                          max_out, &n_out, BLZ_SCALAR_INTENDED, nullptr, err, sizeof(err)),
+                    err, "process_aggregate");
+  std::vector<gdf_column_cpp> table(n_out);
+  for (int i = 0; i < n_out; i++) table[i].create_gdf_column(&outs[i]);
+  input.clear();
+  input.add_table(std::move(table));
+}
+
 // Evaluates a LogicalProject over the frame, which then holds the projected
 // columns; "$k" outputs alias frame column k (CalciteInterpreter.cpp:123-336).
 inline void execute_project_plan(blazing_frame &input, std::string query_part) {

@@ -535,3 +535,44 @@ def parse_project(column_dtypes, query_part):
         else:
             outputs.append(("alias", get_index(b)))
     return (plan if n_out > 0 else None), outputs, names
+
+
+# ---- LogicalAggregate (no GROUP BY) ----
+AGG_OPS = ("SUM", "MIN", "MAX", "AVG", "COUNT")
+
+
+def parse_aggregate(query_part):
+    """process_aggregate's parsing (GroupBy.cpp:543-569): the group list (get_group_columns
+    :41-56) and, for every other `name=[OP(input)]`, the operation
+    (get_aggregation_operation, CalciteExpressionParsing.cpp:483-507) and the input
+    between the outer parentheses (get_string_between_outer_parentheses :756-767).
+    Returns (group column indices, [(op, input)])."""
+    a, b = query_part.find("("), query_part.rfind(")")
+    if a < 0 or b < a:
+        raise PlanError("not a LogicalAggregate line")
+    combined = query_part[a + 1:b]
+    group = named_expression(combined, "group").strip()
+    groups = [] if len(group) <= 2 else [int(x) for x in group[1:-1].split(",")]
+    aggs = []
+    for e in get_expressions_from_expression_list(combined):
+        e = e.strip()
+        if "group=" in e:
+            continue
+        eq = e.find("=[")
+        lp, rp = e.find("(", eq), e.rfind(")")
+        op = e[eq + 2:lp].strip()
+        if op not in AGG_OPS:
+            raise PlanError(f"aggregation type not supported, {op}")
+        aggs.append((op, e[lp + 1:rp].strip()))
+    return groups, aggs
+
+
+def aggregation_output_type(input_type, op):
+    """get_aggregation_output_type (CalciteExpressionParsing.cpp:92-117) without GROUP BY."""
+    if op == "COUNT":
+        return GDF_INT64
+    if op == "SUM":
+        return GDF_FLOAT64 if _is_float(input_type) else GDF_INT64
+    if op == "AVG":
+        return GDF_FLOAT64
+    return input_type

@@ -259,3 +259,58 @@ def run_logical_plan(tables: List[HostColumn], plan_text: str, scalar_mode=0) ->
         else:
             raise ValueError(f"unsupported plan line: {line}")
     return frame
+
+
+def process_aggregate(frame: List[HostColumn], query_part: str, scalar_mode=0) -> List[HostColumn]:
+    """process_aggregate without GROUP BY (GroupBy.cpp:531-588 -> compute_aggregations
+    :292-336 -> aggregations_without_groupby :230-290): each aggregation's input is a
+    frame column ("$k"; empty = column 0, get_index CalciteExpressionParsing.cpp:600-608)
+    or an expression evaluated first; the result is a one-row column. COUNT = rows -
+    null_count; SUM / MIN / MAX over the valid rows (cudf::reduce skips nulls; no
+    valid row -> null); AVG = SUM / (rows - null_count) in binary64, null when no row
+    is valid. An integer SUM wraps in int64; a float SUM here is the exact sum rounded
+    once (math.fsum): the reference's summation order is cuDF's, not pinned."""
+    import math
+    groups, aggs = P.parse_aggregate(query_part)
+    if groups:
+        raise P.PlanError("aggregation with GROUP BY is not restated")
+    dts = [c.dtype for c in frame]
+    out = []
+    for op, inp in aggs:
+        if P.contains_evaluation(inp):
+            col = evaluate_expression(frame, inp, P.get_output_type_expression(dts, inp), True, scalar_mode)
+        else:
+            col = frame[P.get_index(inp) if inp else 0]
+        ot = P.aggregation_output_type(col.dtype, op)
+        n = len(col)
+        valid_rows = n - (col.null_count if col.valid is not None else 0)
+        vals = col.data if col.valid is None else col.data[col.valid]
+        is_f = col.dtype in (GDF_FLOAT32, GDF_FLOAT64)
+        ok = True
+        if op == "COUNT":
+            v = valid_rows
+        elif op == "AVG":
+            ok = n > 0 and valid_rows > 0
+            if ok:
+                s = math.fsum(vals.astype(np.float64)) if is_f else float(_wrap_sum(vals))
+                v = s / float(valid_rows)
+        else:
+            ok = n > 0 and len(vals) > 0
+            if ok and op == "SUM":
+                v = math.fsum(vals.astype(np.float64)) if is_f else _wrap_sum(vals)
+            elif ok and is_f:
+                finite = vals[~np.isnan(vals)]   # a NaN never replaces the running min / max
+                v = (finite.min() if op == "MIN" else finite.max()) if len(finite) else \
+                    (np.inf if op == "MIN" else -np.inf)
+            elif ok:
+                v = vals.astype(np.int64).min() if op == "MIN" else vals.astype(np.int64).max()
+        data = np.zeros(1, NP_OF[ot])
+        if ok:
+            data[0] = v
+        out.append(HostColumn(data, ot, np.array([ok])))
+    return out
+
+
+def _wrap_sum(vals):
+    """Sum of integers in int64 with wrap-around (two's complement)."""
+    return int(((int(np.sum(vals.astype(object))) + 2**63) % 2**64) - 2**63)

@@ -76,6 +76,11 @@ int main(int argc, char **argv) {
     auto e1 = download<int32_t>(frame.get_column(1));
     auto e2 = download<int32_t>(frame.get_column(2));
     for (size_t i = 0; i < e0.size(); i++) std::printf("row %.17g %d %d\n", e0[i], e1[i], e2[i]);
+    process_aggregate(frame, "LogicalAggregate(group=[{}], n=[COUNT()], m=[MAX($1)], s=[SUM($1)])");
+    auto n_ = download<int64_t>(frame.get_column(0));
+    auto m_ = download<int32_t>(frame.get_column(1));
+    auto s_ = download<int64_t>(frame.get_column(2));
+    std::printf("agg %lld %d %lld\n", (long long)n_[0], m_[0], (long long)s_[0]);
   } catch (const std::runtime_error &e) {
     std::printf("runtime_error: %s\n", e.what());
     return 2;

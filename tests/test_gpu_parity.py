@@ -562,8 +562,8 @@ def test_synth_column_deterministic(B):
 
 def test_cpp_operator_api_filter_project(B):
     """The reference's call sequence (evaluate_expression, process_filter,
-    execute_project_plan) from C++ through include/blz/ral.hpp, against the oracle
-    on the same generated frame, bit-exact."""
+    execute_project_plan, process_aggregate) from C++ through include/blz/ral.hpp,
+    against the oracle on the same generated frame, bit-exact."""
     import os
     import subprocess
     shim = os.path.join(os.path.dirname(os.path.abspath(__file__)), "cpp", "ral_shim_test")
@@ -585,8 +585,10 @@ def test_cpp_operator_api_filter_project(B):
     proj = "LogicalProject(EXPR$0=[*($1, -(1, $2))], EXPR$1=[+($0, 1)], D=[$3])"
     ref = O.execute_project(O.process_filter(host, filt), proj)
     assert lines[1] == f"kept {len(ref[0])}"
-    rows = [ln.split() for ln in lines[2:]]
+    rows = [ln.split() for ln in lines[2:] if ln.startswith("row ")]
     assert len(rows) == len(ref[0])
+    agg = O.process_aggregate(ref, "LogicalAggregate(group=[{}], n=[COUNT()], m=[MAX($1)], s=[SUM($1)])")
+    assert lines[-1] == f"agg {agg[0].data[0]} {agg[1].data[0]} {agg[2].data[0]}"
     got0 = np.array([float(x[1]) for x in rows])
     got1 = np.array([int(x[2]) for x in rows], dtype=np.int32)
     got2 = np.array([int(x[3]) for x in rows], dtype=np.int32)
