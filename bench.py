@@ -275,6 +275,39 @@ def run_aggregate(B, frame, prog, kept, n, iters=20):
                                      "path": "process_aggregate over the filter+project outputs"}}
 
 
+def run_gather(B, n=100_000_000, iters=5):
+    """Join / order-by materialisation: 6 columns (3 int64, 3 float64) of n rows
+    gathered by a random permutation, through one blz_materialize_columns call and
+    through the reference's loop of materialize_column calls. Bytes per row: 4 B of
+    index + 6 x (8 B read + 8 B written); wall milliseconds per gather of the table."""
+    rng = np.random.default_rng(77)
+    perm = rng.permutation(n).astype(np.int32)
+    idx = B.DeviceColumn.from_numpy(perm, B.GDF_INT32)
+    del perm
+    cols = []
+    for k in range(6):
+        c = B.DeviceColumn(B.GDF_INT64 if k < 3 else B.GDF_FLOAT64, n)
+        B.synth_column(c, 900 + k, 5 if k < 3 else 1, 0.0, 1.0, 0.0)
+        cols.append(c)
+    outs = [B.DeviceColumn(c.col.dtype, n) for c in cols]
+
+    def timed(f):
+        f()
+        t0 = time.perf_counter()
+        for _ in range(iters):
+            f()
+        return (time.perf_counter() - t0) / iters * 1e3
+
+    multi = timed(lambda: B.materialize_columns(cols, outs, idx))
+    loop = timed(lambda: [B.materialize_column(c, o, idx) for c, o in zip(cols, outs)])
+    for c in cols + outs + [idx]:
+        c.free()
+    by = n * (4 + 6 * 16)
+    return {"workload": f"{n} rows x 6 columns (3 int64, 3 f64), random permutation", "bytes": by,
+            "materialize_columns": {"ms": multi, "GBps": by / (multi * 1e-3) / 1e9},
+            "materialize_column_loop": {"ms": loop, "GBps": by / (loop * 1e-3) / 1e9}}
+
+
 def run_filter_gather(B, args):
     """500M-row filter + gather: one JSON line, value = (52 N + 48 kept) bytes per step
     over the step's wall time (inputs generated on the device, resident); roofline
@@ -401,6 +434,7 @@ def main():
     prog.close()
     for c in frame:
         c.free()
+    gather = run_gather(B)
     batch = None if args.no_batch else run_batch(B, args.batch_plans, 1e4, 5e7)
     serve = None if args.no_serve else run_serve(B, args.serve_gb, [200, 1000, 5000], args.serve_seconds)
     cpu = None if args.no_cpu else cpu_baseline()
@@ -431,6 +465,7 @@ def main():
                      "frac": bytes_step / (total_ms * 1e-3) / 1e9 / HBM_PEAK_GBS},
         "cpu_baseline": cpu,
         "aggregate": aggregate,
+        "gather": gather,
         "batch": batch,
         "serve": serve,
     }

@@ -101,6 +101,7 @@ def lib():
                                    C.POINTER(C.c_int64), I, P, ERR, SZ],
             "blz_process_aggregate": [colpp, I, C.c_char_p, colp, I, C.POINTER(C.c_int), I, P, ERR, SZ],
             "blz_materialize_column": [colp, colp, colp, P, ERR, SZ],
+            "blz_materialize_columns": [colpp, colpp, I, colp, P, ERR, SZ],
             "blz_gdf_sequence": [P, SZ, C.c_int32, C.c_int32, P, ERR, SZ],
             "blz_apply_boolean_mask": [colp, colp, colp, P, ERR, SZ],
             "blz_update_null_count": [colp, P, ERR, SZ],
@@ -415,6 +416,14 @@ def materialize_column(inp: DeviceColumn, out: DeviceColumn, row_indices: Device
     _call(lib().blz_materialize_column, C.byref(inp.col), C.byref(out.col), C.byref(row_indices.col),
           stream)
     return out.col.null_count
+
+
+def materialize_columns(inputs: Sequence[DeviceColumn], outputs: Sequence[DeviceColumn],
+                        row_indices: DeviceColumn, stream=None):
+    """materialize_column over several columns sharing the indices, one blocking call."""
+    _call(lib().blz_materialize_columns, _colpp(inputs), _colpp(outputs), len(inputs), C.byref(row_indices.col),
+          stream)
+    return [o.col.null_count for o in outputs]
 
 
 def gdf_sequence(col: DeviceColumn, init_val=0, step=1, stream=None):

@@ -122,6 +122,14 @@ int blz_filter_project(gdf_column *const *frame, int n_frame, const char *filter
  * input validity bit row_indices[i] (when output->valid != NULL); sets
  * output->null_count. row_indices is an int32 column; output->size must equal
  * row_indices->size. */
+/* materialize_column over n_columns columns of one table sharing row_indices, as one
+ * blocking call (no reference counterpart as one call: its callers loop over
+ * materialize_column, JoinOperator.cpp:130-135, OrderBy.cpp:66,
+ * distribution/primitives.cpp:239,426,667). Each outputs[c] as
+ * blz_materialize_column(inputs[c], outputs[c], row_indices); every input has the
+ * same size. */
+int blz_materialize_columns(const gdf_column *const *inputs, gdf_column *const *outputs, int n_columns,
+                            const gdf_column *row_indices, blz_stream_t stream, char *err, size_t err_len);
 int blz_materialize_column(const gdf_column *input, gdf_column *output,
                            const gdf_column *row_indices, blz_stream_t stream, char *err,
                            size_t err_len);
