@@ -788,7 +788,7 @@ int blz_materialize_column(const gdf_column *input, gdf_column *output, const gd
     hipError_t e = launch_fill_zero(d, 8, 1, S(stream));
     if (e == hipSuccess && m > 0) {
       const int64_t words = (m + 63) / 64;
-      const int grid = (int)std::max<int64_t>(1, std::min<int64_t>((words + 3) / 4, (int64_t)device_cus() * 16));
+      const int grid = (int)std::max<int64_t>(1, std::min<int64_t>((words + 15) / 16, (int64_t)device_cus() * 16));
       e = launch_gather(w, input->data, input->valid, input->size, (const int32_t *)row_indices->data, m, output->data,
                         output->valid, d, grid, S(stream));
     }
@@ -825,7 +825,7 @@ int blz_materialize_columns(const gdf_column *const *inputs, gdf_column *const *
     HIPCHECK(hipMallocAsync((void **)&d, 8 * (size_t)n_columns, st));
     hipError_t e = launch_fill_zero(d, 8 * (int64_t)n_columns, 1, st);
     const int64_t words = (m + 63) / 64;
-    const int grid = (int)std::max<int64_t>(1, std::min<int64_t>((words + 3) / 4, (int64_t)device_cus() * 16));
+    const int grid = (int)std::max<int64_t>(1, std::min<int64_t>((words + 15) / 16, (int64_t)device_cus() * 16));
     for (int c = 0; e == hipSuccess && m > 0 && c < n_columns; c++)
       e = launch_gather(blz_dtype_width(inputs[c]->dtype), inputs[c]->data, inputs[c]->valid, n_in,
                         (const int32_t *)row_indices->data, m, outputs[c]->data, outputs[c]->valid, d + c, grid, st);

@@ -371,7 +371,10 @@ interp_kernel(const uint8_t *__restrict__ blob_arg, Workspace ws) {
 }
 
 // ---------------------------------------------------------------------------
-// gather: out[i] = in[idx[i]] (idx >= 0), validity bits gathered, null count.
+// gather: out[i] = in[idx[i]] (idx >= 0), validity bits gathered, null count. A
+// wave takes four 64-index words at a time and issues every index load, then every
+// value load, before its stores: random accesses need many requests in flight.
+constexpr int kGatherWords = 4;
 template <typename T>
 __global__ void __launch_bounds__(kBlock)
 gather_kernel(const T *__restrict__ in, const uint8_t *__restrict__ in_valid, int64_t n_in,
@@ -382,21 +385,32 @@ gather_kernel(const T *__restrict__ in, const uint8_t *__restrict__ in_valid, in
   const int64_t wave_id = ((int64_t)blockIdx.x * kBlock + threadIdx.x) >> 6;
   const int64_t n_waves = ((int64_t)gridDim.x * kBlock) >> 6;
   uint64_t nulls = 0;
-  for (int64_t w = wave_id; w < n_words; w += n_waves) {
-    const int64_t i = w * 64 + lane;
-    bool vbit = false;
-    if (i < n_idx) {
-      const int32_t j = idx[i];
-      if (j >= 0 && j < n_in) {  // an index past the input is treated like a negative one
-        out[i] = in[j];
-        vbit = valid_bit(in_valid, j);
-      }
+  for (int64_t w0 = wave_id * kGatherWords; w0 < n_words; w0 += n_waves * kGatherWords) {
+    int32_t j[kGatherWords];
+    bool ok[kGatherWords];
+    T v[kGatherWords];
+#pragma unroll
+    for (int u = 0; u < kGatherWords; u++) {
+      const int64_t i = (w0 + u) * 64 + lane;
+      j[u] = i < n_idx ? idx[i] : -1;
     }
-    if (out_valid != nullptr) {
-      const uint64_t word = __builtin_amdgcn_ballot_w64(vbit);
-      if (lane == 0) {
-        store_vword(out_valid, w, n_idx, word);
-        nulls += (uint64_t)__builtin_popcountll(~word & low_mask(n_idx - w * 64));
+#pragma unroll
+    for (int u = 0; u < kGatherWords; u++) {
+      ok[u] = j[u] >= 0 && j[u] < n_in;  // an index past the input is treated like a negative one
+      v[u] = ok[u] ? in[j[u]] : T(0);
+    }
+#pragma unroll
+    for (int u = 0; u < kGatherWords; u++) {
+      const int64_t w = w0 + u;
+      if (w >= n_words) break;  // wave-uniform
+      const int64_t i = w * 64 + lane;
+      if (ok[u]) out[i] = v[u];
+      if (out_valid != nullptr) {
+        const uint64_t word = __builtin_amdgcn_ballot_w64(ok[u] && valid_bit(in_valid, j[u]));
+        if (lane == 0) {
+          store_vword(out_valid, w, n_idx, word);
+          nulls += (uint64_t)__builtin_popcountll(~word & low_mask(n_idx - w * 64));
+        }
       }
     }
   }
