@@ -64,10 +64,10 @@ hipError_t launch_vm(int r, int mode, int ops, int grid, int lds_bytes, hipStrea
 int vm_blocks_per_cu(int r, int mode, int ops, int lds_bytes);
 // Exclusive prefix of every program's slot counts (ws.wave_count; total to
 // ws.n_sel) and, from it and the keep words, the row of every kept row in output
-// order (ws.idx): chunks of 4096 slots, combined within a program by a single-pass
+// order (ws.idx): chunks of kScanChunk slots, combined within a program by a single-pass
 // decoupled look-back over its ws.tile_state (zero before the scan).
 hipError_t launch_slot_scan(const VmBatch *bd, int64_t n_chunks, hipStream_t st);
-constexpr int64_t kScanChunk = 1024;  // slots per scan chunk
+constexpr int64_t kScanChunk = 512;  // slots per scan chunk
 // Seeded synthetic column in place (dist: 0 int uniform [a,b], 1 real uniform [a,b),
 // 2 normal(a,b), 3 lognormal(a,b), 4 k/100 for k in [a,b], 5 a + row).
 hipError_t launch_synth(void *data, uint8_t *valid, int64_t n, int dtype, uint64_t seed, int dist, double a,

@@ -18,11 +18,11 @@ BLZ_VM_SHAPES(BLZ_VM_DECL)
 
 // Exclusive prefix of every program's per-wave-slot kept counts (VM_PRED) in one
 // pass, expanded into the row index of every kept row in output order: each
-// workgroup scans a chunk of 1024 counts of one program and combines its total
+// workgroup scans a chunk of 512 counts of one program and combines its total
 // with the chunks before it by a single-pass decoupled look-back over that
 // program's state words. Chunks are taken in the order workgroups start, so
 // every chunk a look-back waits for belongs to a workgroup that is running.
-constexpr int kScanItems = 4;
+constexpr int kScanItems = 2;
 static_assert(kScanItems * kBlock == kScanChunk, "scan chunk");
 __global__ void __launch_bounds__(kBlock)
 slot_scan_kernel(const VmBatch *__restrict__ bd_arg) {
