@@ -13,7 +13,9 @@ value = algorithmic bytes per step / wall time per step (GB/s), where algorithmi
 bytes = 24 B per input row (a, b, c, d read once) + 12 B per kept row (f64 + i32
 written). roofline is for the dominant kernel (the predicate kernel): its
 algorithmic bytes (predicate columns a, c, d read once: 16 B/row, keep words
-written: 1/8 B/row, slot counts: 16 B per 1024-row tile) over its own duration
+written: 1/8 B/row, slot counts: 16 B per 1024-row tile; it reads the columns
+into registers, pred_direct_kernel, since the predicate is a conjunction of
+column-versus-constant comparisons) over its own duration
 (HIP events on the launch stream); "pipeline" gives the same figure for all three
 kernels together. traffic is the dominant kernel's HBM bytes per launch from the
 rocprofv3 FETCH_SIZE / WRITE_SIZE summary in profiles/. cpu_baseline times the
@@ -418,9 +420,12 @@ def main():
     names = STAGES if len(stage_ms) == 3 else ["program"]
     sbytes = stage_bytes(n, kept, prog.tile_rows) if len(stage_ms) == 3 else [algorithmic_bytes(n, kept)]
     dom = int(np.argmax(stage_ms))
-    # demangled kernel name prefix: vm_kernel<rows per lane, mode (1 predicate, 2 gather), op set>
-    kernel_name = (f"vm_kernel<{prog.r}, {1 if dom == 0 else 2}," if names[dom] != "slot_scan"
-                   else "slot_scan_kernel") if len(stage_ms) == 3 else f"kernel {prog.kernel}"
+    # demangled kernel name prefix: vm_kernel<rows per lane, mode (1 predicate, 2 gather), op set>;
+    # a conjunction of column-versus-constant comparisons runs as pred_direct_kernel
+    pred_name = ("pred_direct_kernel" if traffic_from_profile("pred_direct_kernel") is not None
+                 else f"vm_kernel<{prog.r}, 1,")
+    kernel_name = ((pred_name if dom == 0 else f"vm_kernel<{prog.r}, 2," if dom == 2 else "slot_scan_kernel")
+                   if len(stage_ms) == 3 else f"kernel {prog.kernel}")
     achieved = sbytes[dom] / (stage_ms[dom] * 1e-3) / 1e9
     bytes_step = algorithmic_bytes(n, kept)
     total_ms = float(sum(stage_ms))

@@ -62,6 +62,11 @@ struct VmBatch {
 // rows (after launch_slot_scan).
 hipError_t launch_vm(int r, int mode, int ops, int grid, int lds_bytes, hipStream_t st, const VmBatch *bd);
 int vm_blocks_per_cu(int r, int mode, int ops, int lds_bytes);
+// The predicate of a single program whose ops are compare-and / range leaves over
+// columns (Compiled::vm_direct_pred), from registers: same keep words and slot
+// counts as launch_vm mode 1.
+// n_terms <= 3; wmask bit k: term k reads an 8-byte column.
+hipError_t launch_pred_direct(const VmBatch *bd, int n_terms, int wmask, int grid, hipStream_t st);
 // Exclusive prefix of every program's slot counts (ws.wave_count; total to
 // ws.n_sel) and, from it and the keep words, the row of every kept row in output
 // order (ws.idx): chunks of kScanChunk slots, combined within a program by a single-pass

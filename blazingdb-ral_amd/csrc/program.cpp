@@ -865,6 +865,21 @@ bool compile_vm(const CompileRequest &req, Builder &B, const std::vector<AccOp> 
     if (op.cls >= VC_L64 && op.cls <= VC_L8U) op.off = req.compact ? goff[op.arg2] : lds_off[op.arg2];
   for (auto *v : {&vm_p, &vm_q})
     for (auto &op : *v) assign_leaf(op, R);
+  // A conjunction of column-versus-constant comparisons without validity (every
+  // predicate op a compare-and / range leaf over a column) runs from registers.
+  bool direct = req.compact && req.pred != nullptr && vops_p.empty() && !vm_p.empty() && vm_p.size() <= 3 &&
+                stencil_v == V_ALL;
+  for (const auto &op : vm_p) {
+    const int ho = op.leaf / HC_COUNT_, cl = op.leaf % HC_COUNT_;
+    if (op.leaf >= kHotLeaves || ho < HO_CMPAND || ho > HO_RANGE_F || cl < HC_L64) direct = false;
+  }
+  out.vm_direct_pred = direct;
+  out.vm_direct_terms = direct ? (int)vm_p.size() : 0;
+  out.vm_direct_wmask = 0;
+  for (size_t k = 0; direct && k < vm_p.size(); k++) {
+    const int cl = vm_p[k].leaf % HC_COUNT_;
+    if (cl == HC_L64 || cl == HC_L64_F) out.vm_direct_wmask |= 1 << k;
+  }
 
   std::vector<Piece> pieces;
   for (int c = 0; c < n_dev; c++) {

@@ -69,6 +69,9 @@ struct Compiled {
   int vm_g_buf_off = 0, vm_g_tmp_off = 0, vm_g_cpre_off = 0;  // tile VM LDS layout (gather)
   int vm_g_wave_bytes = 0, vm_nvout = 0;  // gather pass buffer per wave, outputs with validity
   int vm_vr_off = 0, vm_g_vr_off = 0;     // validity registers (four waves' regions)
+  bool vm_direct_pred = false;            // tile VM filter: predicate by launch_pred_direct
+  int vm_direct_terms = 0;                //   with this many terms (the predicate's ops),
+  int vm_direct_wmask = 0;                //   bit k: term k reads an 8-byte column
 };
 
 // Throws std::runtime_error with a message on an unsupported or malformed plan.

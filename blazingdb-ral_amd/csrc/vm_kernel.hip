@@ -126,6 +126,157 @@ slot_scan_kernel(const VmBatch *__restrict__ bd_arg) {
   }
 }
 
+// Predicate of one program whose ops are all compare-and / range leaves over
+// columns (a conjunction of column-versus-constant comparisons, no validity, at
+// most kDirectTerms ops): the operands are loaded from the columns straight into
+// registers instead of being staged through LDS, every op's loads of a step in
+// flight together and widened only after the last is issued; one instantiation
+// per number of terms and pattern of 8-byte / 4-byte columns (WMASK bit k: term k
+// reads 8 bytes), so no load sits behind a branch. The keep words and per-slot
+// kept counts are the predicate kernel's (slot s = rows s T/4 .. (s + 1) T/4, keep
+// word s GW + g), so the slot scan and gather follow unchanged. The comparisons
+// are the tile VM's leaves.
+constexpr int kDirectTerms = 3;
+constexpr int kDirectR = 8;  // 64-row groups per wave step
+
+// One term over the step's rows: the operands widened as the VM's operand class
+// widens them, then the comparison ANDed into keep (one uniform branch per term,
+// straight-line code per row).
+DEV void direct_term(int ho, int cls, const uint64_t (&raw)[kDirectR], uint64_t im, uint64_t im2, int rf,
+                     bool (&keep)[kDirectR]) {
+  uint64_t b[kDirectR];
+#pragma unroll
+  for (int u = 0; u < kDirectR; u++) {
+    switch (cls) {
+      case HC_L64: b[u] = raw[u]; break;
+      case HC_L32I: b[u] = (uint64_t)(int64_t)(int32_t)(uint32_t)raw[u]; break;
+      case HC_L32F: b[u] = d2b((double)__builtin_bit_cast(float, (uint32_t)raw[u])); break;
+      case HC_L32I_F: b[u] = d2b((double)(int32_t)(uint32_t)raw[u]); break;
+      default: b[u] = d2b((double)(int64_t)raw[u]); break;  // HC_L64_F
+    }
+  }
+  const int64_t ii = (int64_t)im2;
+  const double fi = b2d(im2), hf = b2d(im);
+#define BLZ_TERM(CASE, TEST)                                                    \
+  case CASE:                                                                    \
+    _Pragma("unroll") for (int u = 0; u < kDirectR; u++) keep[u] = keep[u] & (TEST); \
+    break;
+  switch (ho) {
+    BLZ_TERM(HO_CMPAND + 0, (int64_t)b[u] < ii)
+    BLZ_TERM(HO_CMPAND + 1, (int64_t)b[u] <= ii)
+    BLZ_TERM(HO_CMPAND + 2, (int64_t)b[u] > ii)
+    BLZ_TERM(HO_CMPAND + 3, (int64_t)b[u] >= ii)
+    BLZ_TERM(HO_CMPAND + 4, b[u] == im2)
+    BLZ_TERM(HO_CMPAND + 5, b[u] != im2)
+    BLZ_TERM(HO_CMPAND + 6, b2d(b[u]) < fi)
+    BLZ_TERM(HO_CMPAND + 7, b2d(b[u]) <= fi)
+    BLZ_TERM(HO_CMPAND + 8, b2d(b[u]) > fi)
+    BLZ_TERM(HO_CMPAND + 9, b2d(b[u]) >= fi)
+    BLZ_TERM(HO_CMPAND + 10, b2d(b[u]) == fi)
+    BLZ_TERM(HO_CMPAND + 11, b2d(b[u]) != fi)
+    BLZ_TERM(HO_RANGE_I, ((int64_t)b[u] > ii) & ((int64_t)b[u] < (int64_t)im))
+    BLZ_TERM(HO_RANGE_F, ((b2d(b[u]) > fi) | ((rf & 1) != 0 && b2d(b[u]) == fi)) &
+                             ((b2d(b[u]) < hf) | ((rf & 2) != 0 && b2d(b[u]) == hf)))
+    default: break;
+  }
+#undef BLZ_TERM
+}
+
+template <int NT, int WMASK>
+__global__ void __launch_bounds__(kBlock) pred_direct_kernel(const VmBatch *__restrict__ bd_arg) {
+  typedef uint32_t u32x8 __attribute__((ext_vector_type(8)));
+  const CONSTANT VmBatch &bd = *(const CONSTANT VmBatch *)bd_arg;
+  const CONSTANT VmPlanRef *plans = (const CONSTANT VmPlanRef *)bd.plans;
+  const VmCtx cx = load_ctx(plans[0].blob);
+  const Workspace ws = load_ws(plans[0].ws);
+  const int lane = threadIdx.x & 63;
+  const int64_t n = cx.n;
+  const int GW = cx.T / 256;  // 64-row groups per slot
+  const int64_t n_slots = ((n + cx.T - 1) / cx.T) * kWaves;
+  // the terms, decoded once (scalar registers): column, operand class, leaf, bounds
+  const CONSTANT u32x8 *opw = (const CONSTANT u32x8 *)cx.ops;
+  const void *col[NT];
+  int cls[NT], ho[NT], rf[NT];
+  uint64_t im[NT], im2[NT];
+#pragma unroll
+  for (int k = 0; k < NT; k++) {
+    const u32x8 w = opw[k];
+    cls[k] = (int)w.s0 % HC_COUNT_;
+    ho[k] = (int)w.s0 / HC_COUNT_;
+    rf[k] = (int)(w.s3 & 0xFF);
+    col[k] = cx.in[(w.s3 >> 16) & 0xFF].data;
+    im[k] = (uint64_t)w.s4 | ((uint64_t)w.s5 << 32);
+    im2[k] = (uint64_t)w.s6 | ((uint64_t)w.s7 << 32);
+  }
+  // Work unit: whole slots and whole steps (max(GW, 8) groups); a step takes 8
+  // consecutive groups, which may span several slots (GW < 8) or be part of one.
+  const int U = GW > kDirectR ? GW : kDirectR;
+  const int64_t n_groups = n_slots * GW;
+  const int64_t n_units = (n_groups + U - 1) / U;
+  const int64_t n_waves = ((int64_t)gridDim.x * kBlock) >> 6;
+  for (int64_t unit = ((int64_t)blockIdx.x * kBlock + threadIdx.x) >> 6; unit < n_units; unit += n_waves) {
+    uint64_t slot_cnt = 0;  // GW >= 8: the unit's one slot
+    for (int64_t g0 = unit * U; g0 < (unit + 1) * U && g0 < n_groups; g0 += kDirectR) {
+      int64_t row[kDirectR];
+#pragma unroll
+      for (int u = 0; u < kDirectR; u++) {
+        const int64_t r = (g0 + u) * 64 + lane;
+        row[u] = r < n ? r : n - 1;  // loads stay in bounds; the row is not kept
+      }
+      uint64_t raw[NT][kDirectR];
+#pragma unroll
+      for (int k = 0; k < NT; k++) {
+#pragma unroll
+        for (int u = 0; u < kDirectR; u++)
+          raw[k][u] = ((WMASK >> k) & 1) ? gload<uint64_t>(col[k], row[u]) : (uint64_t)gload<uint32_t>(col[k], row[u]);
+      }
+      bool keep[kDirectR];
+#pragma unroll
+      for (int u = 0; u < kDirectR; u++) keep[u] = (g0 + u) * 64 + lane < n;
+#pragma unroll
+      for (int k = 0; k < NT; k++) direct_term(ho[k], cls[k], raw[k], im[k], im2[k], rf[k], keep);
+      uint64_t myword = 0;
+      uint32_t c[kDirectR];
+#pragma unroll
+      for (int u = 0; u < kDirectR; u++) {
+        const uint64_t km = __builtin_amdgcn_ballot_w64(keep[u]);
+        if (lane == u) myword = km;
+        c[u] = (uint32_t)__builtin_popcountll(km);
+      }
+      if (lane < kDirectR && g0 + lane < n_groups) gstore<uint64_t>(ws.keep, g0 + lane, myword);
+      if (GW >= kDirectR) {
+#pragma unroll
+        for (int u = 0; u < kDirectR; u++) slot_cnt += c[u];
+      } else if (lane == 0) {  // 8 / GW slots in this step
+        for (int j = 0; j * GW < kDirectR && g0 + j * GW < n_groups; j++) {
+          uint32_t t = 0;
+#pragma unroll
+          for (int u = 0; u < kDirectR; u++) t += (u / GW == j) ? c[u] : 0u;
+          gstore<uint32_t>(ws.wave_count, g0 / GW + j, t);
+        }
+      }
+    }
+    if (GW >= kDirectR && lane == 0) gstore<uint32_t>(ws.wave_count, unit * U / GW, (uint32_t)slot_cnt);
+  }
+}
+
+template <int NT, int WMASK>
+static void launch_pred_direct_t(const VmBatch *bd, int grid, hipStream_t st) {
+  hipLaunchKernelGGL((pred_direct_kernel<NT, WMASK>), dim3(grid), dim3(kBlock), 0, st, bd);
+}
+hipError_t launch_pred_direct(const VmBatch *bd, int n_terms, int wmask, int grid, hipStream_t st) {
+  typedef void (*Fn)(const VmBatch *, int, hipStream_t);
+  static const Fn t1[2] = {launch_pred_direct_t<1, 0>, launch_pred_direct_t<1, 1>};
+  static const Fn t2[4] = {launch_pred_direct_t<2, 0>, launch_pred_direct_t<2, 1>, launch_pred_direct_t<2, 2>,
+                           launch_pred_direct_t<2, 3>};
+  static const Fn t3[8] = {launch_pred_direct_t<3, 0>, launch_pred_direct_t<3, 1>, launch_pred_direct_t<3, 2>,
+                           launch_pred_direct_t<3, 3>, launch_pred_direct_t<3, 4>, launch_pred_direct_t<3, 5>,
+                           launch_pred_direct_t<3, 6>, launch_pred_direct_t<3, 7>};
+  if (n_terms < 1 || n_terms > kDirectTerms || wmask < 0 || wmask >= (1 << n_terms)) return hipErrorInvalidValue;
+  (n_terms == 1 ? t1 : n_terms == 2 ? t2 : t3)[wmask](bd, grid, st);
+  return hipGetLastError();
+}
+
 hipError_t launch_vm(int r, int mode, int slow, int grid, int lds_bytes, hipStream_t st, const VmBatch *bd) {
 #define BLZ_VM_LAUNCH(RR) \
   if (r == RR) return launch_vm_r##RR(mode, slow, grid, lds_bytes, st, bd);
