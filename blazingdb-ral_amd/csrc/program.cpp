@@ -865,13 +865,35 @@ bool compile_vm(const CompileRequest &req, Builder &B, const std::vector<AccOp> 
     if (op.cls >= VC_L64 && op.cls <= VC_L8U) op.off = req.compact ? goff[op.arg2] : lds_off[op.arg2];
   for (auto *v : {&vm_p, &vm_q})
     for (auto &op : *v) assign_leaf(op, R);
-  // A conjunction of column-versus-constant comparisons without validity (every
-  // predicate op a compare-and / range leaf over a column) runs from registers.
-  bool direct = req.compact && req.pred != nullptr && vops_p.empty() && !vm_p.empty() && vm_p.size() <= 3 &&
-                stencil_v == V_ALL;
+  // A conjunction of column-versus-constant comparisons (every predicate op a
+  // compare-and / range leaf over a column) runs from registers. Its validity
+  // must be an AND of raw column bitmaps (columns the input rule forces valid
+  // are absent from it); each bitmap is applied by one term over that column,
+  // named in the term's vcol (which compare-and / range leaves otherwise ignore).
+  bool direct = req.compact && req.pred != nullptr && !vm_p.empty() && vm_p.size() <= 3;
   for (const auto &op : vm_p) {
     const int ho = op.leaf / HC_COUNT_, cl = op.leaf % HC_COUNT_;
     if (op.leaf >= kHotLeaves || ho < HO_CMPAND || ho > HO_RANGE_F || cl < HC_L64) direct = false;
+  }
+  if (direct) {
+    std::vector<int> todo{stencil_v};
+    while (direct && !todo.empty()) {
+      const int v = todo.back();
+      todo.pop_back();
+      if (v == V_ALL) continue;
+      if (v < 0) { direct = false; break; }
+      const VNode &vn = B.vnodes[v];
+      if (vn.kind == 1) {
+        if (vn.code != VOP_AND) direct = false;
+        todo.push_back(vn.a);
+        todo.push_back(vn.b);
+        continue;
+      }
+      bool placed = false;
+      for (auto &op : vm_p)
+        if (op.arg2 == vn.aux && (op.vcol == 0xFF || op.vcol == vn.aux)) { op.vcol = (uint8_t)vn.aux; placed = true; break; }
+      if (!placed) direct = false;
+    }
   }
   out.vm_direct_pred = direct;
   out.vm_direct_terms = direct ? (int)vm_p.size() : 0;

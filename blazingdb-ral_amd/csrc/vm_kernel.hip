@@ -127,8 +127,10 @@ slot_scan_kernel(const VmBatch *__restrict__ bd_arg) {
 }
 
 // Predicate of one program whose ops are all compare-and / range leaves over
-// columns (a conjunction of column-versus-constant comparisons, no validity, at
-// most kDirectTerms ops): the operands are loaded from the columns straight into
+// columns (a conjunction of column-versus-constant comparisons, at most
+// kDirectTerms ops; its validity, an AND of column bitmaps, is applied by the
+// terms whose vcol names a column, so a row is kept only where each such column
+// is valid): the operands are loaded from the columns straight into
 // registers instead of being staged through LDS, every op's loads of a step in
 // flight together and widened only after the last is issued; one instantiation
 // per number of terms and pattern of 8-byte / 4-byte columns (WMASK bit k: term k
@@ -196,6 +198,7 @@ __global__ void __launch_bounds__(kBlock) pred_direct_kernel(const VmBatch *__re
   // the terms, decoded once (scalar registers): column, operand class, leaf, bounds
   const CONSTANT u32x8 *opw = (const CONSTANT u32x8 *)cx.ops;
   const void *col[NT];
+  const uint8_t *valid[NT];
   int cls[NT], ho[NT], rf[NT];
   uint64_t im[NT], im2[NT];
 #pragma unroll
@@ -205,6 +208,8 @@ __global__ void __launch_bounds__(kBlock) pred_direct_kernel(const VmBatch *__re
     ho[k] = (int)w.s0 / HC_COUNT_;
     rf[k] = (int)(w.s3 & 0xFF);
     col[k] = cx.in[(w.s3 >> 16) & 0xFF].data;
+    const int vc = (int)((w.s3 >> 8) & 0xFF);
+    valid[k] = vc != 0xFF ? (const uint8_t *)cx.in[vc].valid : nullptr;
     im[k] = (uint64_t)w.s4 | ((uint64_t)w.s5 << 32);
     im2[k] = (uint64_t)w.s6 | ((uint64_t)w.s7 << 32);
   }
@@ -233,6 +238,12 @@ __global__ void __launch_bounds__(kBlock) pred_direct_kernel(const VmBatch *__re
       bool keep[kDirectR];
 #pragma unroll
       for (int u = 0; u < kDirectR; u++) keep[u] = (g0 + u) * 64 + lane < n;
+#pragma unroll
+      for (int k = 0; k < NT; k++) {
+        if (valid[k] == nullptr) continue;  // uniform
+#pragma unroll
+        for (int u = 0; u < kDirectR; u++) keep[u] = keep[u] & valid_bit(valid[k], row[u]);
+      }
 #pragma unroll
       for (int k = 0; k < NT; k++) direct_term(ho[k], cls[k], raw[k], im[k], im2[k], rf[k], keep);
       uint64_t myword = 0;

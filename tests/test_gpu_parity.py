@@ -301,6 +301,33 @@ def test_range_conjunctions(B, cond):
         prog.close()
 
 
+@pytest.mark.parametrize("nc0", [None, 0])
+def test_range_conjunction_bitmap_null_count(B, nc0):
+    """A range conjunction over nullable columns (the register-load predicate
+    applies the columns' bitmaps itself); with null_count 0 a bitmap that has
+    cleared bits still counts as all valid (interpreter_ops.cuh:1312-1327)."""
+    rng = np.random.default_rng(7100)
+    n = 70_001
+    host = [gen_column(rng, P.GDF_INT32, n, 0.05), gen_column(rng, P.GDF_INT64, n),
+            gen_column(rng, P.GDF_FLOAT64, n, 0.05, -5.0, 5.0)]
+    if nc0 is not None:
+        host[0].null_count = nc0
+    filt = "LogicalFilter(condition=[AND(>=($0, -500), <($0, 500), >($2, -4.0))])"
+    proj = "LogicalProject(B=[$1], C=[*($2, 2.0)])"
+    _print_case(test="range_conjunction_bitmap_null_count", nc0=nc0)
+    ref = _oracle(host, filt, proj)
+    dev = [to_device(B, h) for h in host]
+    for kernel in (0, 3):
+        prog = B.Program(dev, filt, proj, kernel=kernel)
+        prog.launch()
+        kept, nulls = prog.result()
+        assert kept == len(ref[0]), f"kernel {kernel}"
+        for g, r, nc in zip(prog.outputs, ref, nulls):
+            g.col.null_count = nc
+            assert_column_equal(g, r, what=f"kernel {kernel} nc0={nc0}")
+        prog.close()
+
+
 WIDE_MIX = [P.GDF_INT32, P.GDF_INT32, P.GDF_INT64, P.GDF_INT64, P.GDF_FLOAT32, P.GDF_FLOAT32,
             P.GDF_FLOAT64, P.GDF_FLOAT64]
 
