@@ -139,16 +139,25 @@ slot_scan_kernel(const VmBatch *__restrict__ bd_arg) {
 // word s GW + g), so the slot scan and gather follow unchanged. The comparisons
 // are the tile VM's leaves.
 constexpr int kDirectTerms = 3;
-constexpr int kDirectR = 8;  // 64-row groups per wave step
+
+// 64-row groups per wave step: 16 when the terms read at most 8 bytes per row,
+// else 8 (more groups per step for wider rows costs occupancy in registers).
+template <int NT, int WMASK>
+constexpr int direct_step() {
+  int bytes = 0;
+  for (int k = 0; k < NT; k++) bytes += ((WMASK >> k) & 1) ? 8 : 4;
+  return bytes <= 8 ? 16 : 8;
+}
 
 // One term over the step's rows: the operands widened as the VM's operand class
 // widens them, then the comparison ANDed into keep (one uniform branch per term,
 // straight-line code per row).
-DEV void direct_term(int ho, int cls, const uint64_t (&raw)[kDirectR], uint64_t im, uint64_t im2, int rf,
-                     bool (&keep)[kDirectR]) {
-  uint64_t b[kDirectR];
+template <int RS>
+DEV void direct_term(int ho, int cls, const uint64_t (&raw)[RS], uint64_t im, uint64_t im2, int rf,
+                     bool (&keep)[RS]) {
+  uint64_t b[RS];
 #pragma unroll
-  for (int u = 0; u < kDirectR; u++) {
+  for (int u = 0; u < RS; u++) {
     switch (cls) {
       case HC_L64: b[u] = raw[u]; break;
       case HC_L32I: b[u] = (uint64_t)(int64_t)(int32_t)(uint32_t)raw[u]; break;
@@ -161,7 +170,7 @@ DEV void direct_term(int ho, int cls, const uint64_t (&raw)[kDirectR], uint64_t 
   const double fi = b2d(im2), hf = b2d(im);
 #define BLZ_TERM(CASE, TEST)                                                    \
   case CASE:                                                                    \
-    _Pragma("unroll") for (int u = 0; u < kDirectR; u++) keep[u] = keep[u] & (TEST); \
+    _Pragma("unroll") for (int u = 0; u < RS; u++) keep[u] = keep[u] & (TEST); \
     break;
   switch (ho) {
     BLZ_TERM(HO_CMPAND + 0, (int64_t)b[u] < ii)
@@ -191,6 +200,7 @@ __global__ void __launch_bounds__(kBlock) pred_direct_kernel(const VmBatch *__re
   const CONSTANT VmPlanRef *plans = (const CONSTANT VmPlanRef *)bd.plans;
   const VmCtx cx = load_ctx(plans[0].blob);
   const Workspace ws = load_ws(plans[0].ws);
+  constexpr int RS = direct_step<NT, WMASK>();
   const int lane = threadIdx.x & 63;
   const int64_t n = cx.n;
   const int GW = cx.T / 256;  // 64-row groups per slot
@@ -213,61 +223,79 @@ __global__ void __launch_bounds__(kBlock) pred_direct_kernel(const VmBatch *__re
     im[k] = (uint64_t)w.s4 | ((uint64_t)w.s5 << 32);
     im2[k] = (uint64_t)w.s6 | ((uint64_t)w.s7 << 32);
   }
-  // Work unit: whole slots and whole steps (max(GW, 8) groups); a step takes 8
+  // Work unit: whole slots and whole steps (max(GW, RS) groups); a step takes RS
   // consecutive groups, which may span several slots (GW < 8) or be part of one.
-  const int U = GW > kDirectR ? GW : kDirectR;
+  const int U = GW > RS ? GW : RS;
   const int64_t n_groups = n_slots * GW;
   const int64_t n_units = (n_groups + U - 1) / U;
   const int64_t n_waves = ((int64_t)gridDim.x * kBlock) >> 6;
-  for (int64_t unit = ((int64_t)blockIdx.x * kBlock + threadIdx.x) >> 6; unit < n_units; unit += n_waves) {
+  // the wave's index made visibly uniform, so step bases live in scalar registers
+  const int64_t wave = __builtin_amdgcn_readfirstlane((int)((blockIdx.x * kBlock + threadIdx.x) >> 6));
+  for (int64_t unit = wave; unit < n_units; unit += n_waves) {
     uint64_t slot_cnt = 0;  // GW >= 8: the unit's one slot
-    for (int64_t g0 = unit * U; g0 < (unit + 1) * U && g0 < n_groups; g0 += kDirectR) {
-      int64_t row[kDirectR];
+    for (int64_t g0 = unit * U; g0 < (unit + 1) * U && g0 < n_groups; g0 += RS) {
+      uint64_t raw[NT][RS];
+      if ((g0 + RS) * 64 <= n) {  // uniform: every row of the step exists
+        // scalar base at the step's first row, lane offset, constant per group
 #pragma unroll
-      for (int u = 0; u < kDirectR; u++) {
-        const int64_t r = (g0 + u) * 64 + lane;
-        row[u] = r < n ? r : n - 1;  // loads stay in bounds; the row is not kept
+        for (int k = 0; k < NT; k++) {
+          if ((WMASK >> k) & 1) {
+            const GLOBAL uint64_t *p = (const GLOBAL uint64_t *)col[k] + g0 * 64;
+#pragma unroll
+            for (int u = 0; u < RS; u++) raw[k][u] = p[u * 64 + lane];
+          } else {
+            const GLOBAL uint32_t *p = (const GLOBAL uint32_t *)col[k] + g0 * 64;
+#pragma unroll
+            for (int u = 0; u < RS; u++) raw[k][u] = p[u * 64 + lane];
+          }
+        }
+      } else {
+#pragma unroll
+        for (int k = 0; k < NT; k++) {
+#pragma unroll
+          for (int u = 0; u < RS; u++) {
+            const int64_t r = (g0 + u) * 64 + lane;
+            const int64_t rc = r < n ? r : n - 1;  // loads stay in bounds; the row is not kept
+            raw[k][u] = ((WMASK >> k) & 1) ? gload<uint64_t>(col[k], rc) : (uint64_t)gload<uint32_t>(col[k], rc);
+          }
+        }
       }
-      uint64_t raw[NT][kDirectR];
+      bool keep[RS];
 #pragma unroll
-      for (int k = 0; k < NT; k++) {
-#pragma unroll
-        for (int u = 0; u < kDirectR; u++)
-          raw[k][u] = ((WMASK >> k) & 1) ? gload<uint64_t>(col[k], row[u]) : (uint64_t)gload<uint32_t>(col[k], row[u]);
-      }
-      bool keep[kDirectR];
-#pragma unroll
-      for (int u = 0; u < kDirectR; u++) keep[u] = (g0 + u) * 64 + lane < n;
+      for (int u = 0; u < RS; u++) keep[u] = (g0 + u) * 64 + lane < n;
 #pragma unroll
       for (int k = 0; k < NT; k++) {
         if (valid[k] == nullptr) continue;  // uniform
 #pragma unroll
-        for (int u = 0; u < kDirectR; u++) keep[u] = keep[u] & valid_bit(valid[k], row[u]);
+        for (int u = 0; u < RS; u++) {
+          const int64_t r = (g0 + u) * 64 + lane;
+          keep[u] = keep[u] & valid_bit(valid[k], r < n ? r : n - 1);
+        }
       }
 #pragma unroll
       for (int k = 0; k < NT; k++) direct_term(ho[k], cls[k], raw[k], im[k], im2[k], rf[k], keep);
       uint64_t myword = 0;
-      uint32_t c[kDirectR];
+      uint32_t c[RS];
 #pragma unroll
-      for (int u = 0; u < kDirectR; u++) {
+      for (int u = 0; u < RS; u++) {
         const uint64_t km = __builtin_amdgcn_ballot_w64(keep[u]);
         if (lane == u) myword = km;
         c[u] = (uint32_t)__builtin_popcountll(km);
       }
-      if (lane < kDirectR && g0 + lane < n_groups) gstore<uint64_t>(ws.keep, g0 + lane, myword);
-      if (GW >= kDirectR) {
+      if (lane < RS && g0 + lane < n_groups) gstore<uint64_t>(ws.keep, g0 + lane, myword);
+      if (GW >= RS) {
 #pragma unroll
-        for (int u = 0; u < kDirectR; u++) slot_cnt += c[u];
-      } else if (lane == 0) {  // 8 / GW slots in this step
-        for (int j = 0; j * GW < kDirectR && g0 + j * GW < n_groups; j++) {
+        for (int u = 0; u < RS; u++) slot_cnt += c[u];
+      } else if (lane == 0) {  // RS / GW slots in this step
+        for (int j = 0; j * GW < RS && g0 + j * GW < n_groups; j++) {
           uint32_t t = 0;
 #pragma unroll
-          for (int u = 0; u < kDirectR; u++) t += (u / GW == j) ? c[u] : 0u;
+          for (int u = 0; u < RS; u++) t += (u / GW == j) ? c[u] : 0u;
           gstore<uint32_t>(ws.wave_count, g0 / GW + j, t);
         }
       }
     }
-    if (GW >= kDirectR && lane == 0) gstore<uint32_t>(ws.wave_count, unit * U / GW, (uint32_t)slot_cnt);
+    if (GW >= RS && lane == 0) gstore<uint32_t>(ws.wave_count, unit * U / GW, (uint32_t)slot_cnt);
   }
 }
 
