@@ -224,8 +224,12 @@ hipError_t launch_program(const Compiled &c, bool compact, int grid, hipStream_t
   if (c.vm) {
     const int64_t tiles = (c.n_rows + c.T - 1) / c.T;
     const int64_t chunks = (c.n_rows + 64 * c.R - 1) / (64 * c.R);  // most gather chunks (all rows kept)
+    // Predicate tiles: 8x the resident workgroups, so CUs that finish early take
+    // queued ones (a general predicate over 100M rows 0.294 -> 0.254 ms). Projection
+    // tiles and the gather of kept rows stay at the resident count (8x made a mix of
+    // 256 projections 11% slower; 2x and 4x slowed the gather).
     auto grid_for = [&](int mode, int lds, int64_t units) {
-      const int64_t resident = (int64_t)device_cus() * vm_per_cu(c, mode, lds);
+      const int64_t resident = (int64_t)device_cus() * vm_per_cu(c, mode, lds) * (mode == 1 ? 8 : 1);
       return (int)std::max<int64_t>(1, std::min<int64_t>(units, resident));
     };
     if (!compact) {
@@ -1414,7 +1418,8 @@ blz_batch batch_build(const blz_query *qs, int n, int r, int t, bool use_graph, 
         Compiled probe = c0;  // occupancy depends on (r, mode, slow, lds) only
         probe.vm_slow = sb.slow;
         const int mode = sb.members.empty() ? 0 : (b->progs[sb.members[0]]->compact ? 1 : 0);
-        sb.grid = (int)std::max<int64_t>(1, std::min<int64_t>(tile0, (int64_t)cus * vm_per_cu(probe, mode, sb.lds)));
+        sb.grid = (int)std::max<int64_t>(
+            1, std::min<int64_t>(tile0, (int64_t)cus * vm_per_cu(probe, mode, sb.lds) * (mode == 1 ? 8 : 1)));
         sb.grid_gather = (int)std::max<int64_t>(
             1, std::min<int64_t>((gchunks + 3) / 4, (int64_t)cus * vm_per_cu(probe, 2, std::max(sb.lds_gather, 1))));
       }
