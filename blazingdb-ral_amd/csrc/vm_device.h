@@ -1113,13 +1113,19 @@ vm_kernel(const VmBatch *__restrict__ bd_arg) {
                     orel, live);
     if (cx.n_copy) copy_kept<R>(cx.copies, cx.n_copy, grow, j0, orel, live);
     if (cx.any_valid_out) {
-      // input validity of the chunk's rows: word u (outputs j0 + 64 u ..) to lane 0
+      // input validity of the chunk's rows: word u (outputs j0 + 64 u ..) to lane 0;
+      // a column's R bit loads are all issued before the first ballot waits
+      // (rows of lanes that are not live are row 0: in range, masked after)
+      for (int ci = 0; ci < cx.n_in; ci++) {
+        const int vs = cx.in[ci].vslot;
+        if (vs < 0) continue;
+        const void *vp = cx.in[ci].valid;
+        bool vb[R];
 #pragma unroll
-      for (int u = 0; u < R; u++) {
-        for (int ci = 0; ci < cx.n_in; ci++) {
-          const int vs = cx.in[ci].vslot;
-          if (vs < 0) continue;
-          const uint64_t word = __builtin_amdgcn_ballot_w64(live[u] && valid_bit(cx.in[ci].valid, grow[u]));
+        for (int u = 0; u < R; u++) vb[u] = valid_bit(vp, grow[u]);
+#pragma unroll
+        for (int u = 0; u < R; u++) {
+          const uint64_t word = __builtin_amdgcn_ballot_w64(live[u] & vb[u]);
           if (lane == 0) vr[vs * R + u] = word;
         }
       }
