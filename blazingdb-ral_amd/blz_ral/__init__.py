@@ -126,7 +126,7 @@ def lib():
                                 C.POINTER(C.c_double), ERR, SZ],
             "blz_server_release": [P, C.c_int64],
             "blz_server_replay": [P, C.POINTER(BlzQuery), C.POINTER(C.c_double), I, C.POINTER(C.c_double),
-                                  C.POINTER(C.c_int64), C.POINTER(C.c_double), ERR, SZ],
+                                  C.POINTER(C.c_int64), C.POINTER(C.c_double), C.POINTER(C.c_int64), ERR, SZ],
             "blz_server_stats": [P, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(SZ)],
             "blz_server_destroy": [P],
             "blz_synth_column": [colp, C.c_uint64, I, C.c_double, C.c_double, C.c_double, P, ERR, SZ],
@@ -618,9 +618,10 @@ class Server:
         lib().blz_server_release(self.h, ticket)
         self._keep.pop(ticket, None)
 
-    def replay(self, queries, arrival_s, scalar_mode=SCALAR_INTENDED):
+    def replay(self, queries, arrival_s, scalar_mode=SCALAR_INTENDED, keep_results=False):
         """Submits queries at the given offsets (seconds); returns (latency ms array,
-        rows kept array, wall seconds)."""
+        rows kept array, wall seconds), and with keep_results the tickets too: those
+        queries stay resident until wait() has read them and release() frees them."""
         n = len(queries)
         keep = []
         arr = _queries(queries, scalar_mode, keep)
@@ -628,7 +629,12 @@ class Server:
         lat = (C.c_double * n)()
         kept = (C.c_int64 * n)()
         wall = C.c_double(0)
-        _call(lib().blz_server_replay, self.h, arr, arr_t, n, lat, kept, C.byref(wall))
+        tickets = (C.c_int64 * n)() if keep_results else None
+        _call(lib().blz_server_replay, self.h, arr, arr_t, n, lat, kept, C.byref(wall), tickets)
+        if keep_results:
+            for i, (frame, _, _) in enumerate(queries):
+                self._keep[tickets[i]] = (keep, list(frame))
+            return np.array(lat[:]), np.array(kept[:]), wall.value, list(tickets)
         return np.array(lat[:]), np.array(kept[:]), wall.value
 
     def stats(self):

@@ -1844,11 +1844,14 @@ int blz_server_wait(blz_server s, int64_t ticket, int64_t *n_kept, int32_t *null
       std::unique_lock<std::mutex> lk(s->mu);
       auto it = s->qs.find(ticket);
       if (it == s->qs.end()) invalid("unknown ticket");
-      s->cv_done.wait(lk, [&] { return it->second.state >= 2; });
-      if (it->second.state == 3) throw BlzError(BLZ_ERR_HIP, "query failed: " + it->second.err);
-      if (latency_ms) *latency_ms = (it->second.t_done - it->second.t_submit) * 1e3;
-      sb = it->second.batch;
-      idx = it->second.index;
+      // a reference stays valid when a concurrent submission rehashes the map (an
+      // iterator does not)
+      SrvQuery &sq = it->second;
+      s->cv_done.wait(lk, [&] { return sq.state >= 2; });
+      if (sq.state == 3) throw BlzError(BLZ_ERR_HIP, "query failed: " + sq.err);
+      if (latency_ms) *latency_ms = (sq.t_done - sq.t_submit) * 1e3;
+      sb = sq.batch;
+      idx = sq.index;
     }
     char e2[256];
     const int rc = blz_batch_result(sb->b, sb->st, idx, n_kept, null_counts, out_columns, max_out, n_out, e2,
@@ -1863,7 +1866,8 @@ int blz_server_release(blz_server s, int64_t ticket) {
   auto it = s->qs.find(ticket);
   if (it == s->qs.end() || it->second.state < 2) return BLZ_ERR_INVALID;
   SrvBatch *sb = it->second.batch;
-  s->used -= std::min(s->used, it->second.out_bytes);
+  // a query whose batch never started returned its budget when the build failed
+  if (sb) s->used -= std::min(s->used, it->second.out_bytes);
   if (sb && it->second.index >= 0) {
     blz_program p = sb->b->progs[it->second.index];
     for (auto &c : p->outs) {
@@ -1880,10 +1884,11 @@ int blz_server_release(blz_server s, int64_t ticket) {
 }
 
 int blz_server_replay(blz_server s, const blz_query *queries, const double *arrival_s, int n, double *latency_ms,
-                      int64_t *n_kept, double *wall_s, char *err, size_t err_len) {
+                      int64_t *n_kept, double *wall_s, int64_t *tickets_out, char *err, size_t err_len) {
   return guarded(err, err_len, [&] {
     if (!s || !queries || !arrival_s || n <= 0) invalid("bad arguments");
     std::vector<int64_t> tickets(n, 0);
+    if (tickets_out) std::fill(tickets_out, tickets_out + n, (int64_t)0);
     std::thread waiter;
     std::string werr;
     std::mutex tm;
@@ -1907,7 +1912,8 @@ int blz_server_replay(blz_server s, const blz_query *queries, const double *arri
         }
         if (latency_ms) latency_ms[i] = lat;
         if (n_kept) n_kept[i] = kept;
-        blz_server_release(s, tk);
+        if (tickets_out) tickets_out[i] = tk;  // the caller reads the results and releases
+        else blz_server_release(s, tk);
       }
     });
     for (int i = 0; i < n; i++) {

@@ -247,9 +247,13 @@ int blz_server_wait(blz_server server, int64_t ticket, int64_t *n_kept, int32_t 
 /* Frees the query's outputs and returns their budget. */
 int blz_server_release(blz_server server, int64_t ticket);
 /* Submits queries[i] at arrival_s[i] seconds after the call (a client thread),
- * waits for and releases each, and reports per-query latency and rows kept. */
+ * waits for each, and reports per-query latency and rows kept. tickets_out NULL:
+ * each query is released once it has completed; otherwise tickets_out[i] receives
+ * query i's ticket (0 if its submission failed) and its results stay resident
+ * until the caller reads them (blz_server_wait) and releases it. */
 int blz_server_replay(blz_server server, const blz_query *queries, const double *arrival_s, int n,
-                      double *latency_ms, int64_t *n_kept, double *wall_s, char *err, size_t err_len);
+                      double *latency_ms, int64_t *n_kept, double *wall_s, int64_t *tickets_out, char *err,
+                      size_t err_len);
 int blz_server_stats(blz_server server, int64_t *n_batches, int64_t *peak_inflight, size_t *used_bytes);
 int blz_server_destroy(blz_server server);
 

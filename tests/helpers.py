@@ -65,3 +65,54 @@ def c1_frame(n, seed=1):
 C1_FILTER = ("LogicalFilter(condition=[AND(>=($3, 730), <($3, 1095), >=($2, 0.03), "
              "<=($2, 0.05), <($0, 30))])")
 C1_PROJECT = "LogicalProject(EXPR$0=[*($1, -(1, $2))], EXPR$1=[+($0, 1)])"
+
+
+def ulp_diff(got, want):
+    """Per-row distance in units in the last place between two float arrays of one
+    dtype (float32 or float64): 0 where the bits are equal or both rows are NaN;
+    inf where exactly one is NaN or two zeros differ in sign; otherwise the number
+    of representable values between them (the bit patterns mapped onto a monotone
+    integer line, negative values reflected)."""
+    g = np.ascontiguousarray(got)
+    w = np.ascontiguousarray(want)
+    assert g.dtype == w.dtype and g.dtype in (np.float32, np.float64), (g.dtype, w.dtype)
+    it = np.int64 if g.dtype == np.float64 else np.int32
+    gi = g.view(it).astype(np.int64)
+    wi = w.view(it).astype(np.int64)
+    out = np.zeros(len(g), dtype=np.float64)
+    diff = np.nonzero(gi != wi)[0]
+    if len(diff) == 0:
+        return out
+    mag = (1 << (63 if it == np.int64 else 31)) - 1
+    for i in diff.tolist():
+        gn, wn = bool(np.isnan(g[i])), bool(np.isnan(w[i]))
+        if gn and wn:
+            continue
+        if gn or wn:
+            out[i] = np.inf
+            continue
+        a, b = int(gi[i]), int(wi[i])
+        oa = -(a & mag) if a < 0 else a
+        ob = -(b & mag) if b < 0 else b
+        out[i] = np.inf if oa == ob else abs(oa - ob)   # equal ordinals, different bits: +0 / -0
+    return out
+
+
+C2_KINDS = {
+    # dtype -> host generator of the wide-expression column mix (int32 uniform
+    # [-1e6, 1e6], int64 uniform [0, 2^40], float32 normal(0, 1), float64 lognormal(0, 2))
+    P.GDF_INT32: lambda rng, n: rng.integers(-1_000_000, 1_000_001, n).astype(np.int32),
+    P.GDF_INT64: lambda rng, n: rng.integers(0, 2 ** 40 + 1, n).astype(np.int64),
+    P.GDF_FLOAT32: lambda rng, n: rng.normal(0.0, 1.0, n).astype(np.float32),
+    P.GDF_FLOAT64: lambda rng, n: rng.lognormal(0.0, 2.0, n),
+}
+
+
+def c2_mix_table(rng, dtypes, null_rates, n):
+    """Host columns of the wide-expression mix with the given null rates."""
+    cols = []
+    for dt, nr in zip(dtypes, null_rates):
+        data = C2_KINDS[dt](rng, n)
+        valid = (rng.random(n) >= nr) if nr > 0 else None
+        cols.append(O.HostColumn(data, dt, valid))
+    return cols
