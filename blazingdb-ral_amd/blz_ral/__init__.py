@@ -143,6 +143,11 @@ def lib():
             "blz_device_count": [],
             "blz_plan_expression": [I32P, I, C.c_char_p, I, I16P, I16P, I16P, I32P, I32P, scp, scp,
                                     C.POINTER(C.c_int), I32P, C.POINTER(C.c_int), I16P, ERR, SZ],
+            "blz_add_expression_to_plan": [C.c_char_p, C.c_int16, C.c_int16, C.c_int16, I16P, I, I, I16P, I16P,
+                                           I16P, I32P, I32P, scp, scp, C.POINTER(C.c_int), ERR, SZ],
+            "blz_parse_project_plan": [I32P, I, C.c_char_p, I, I16P, I16P, I16P, I32P, I32P, scp, scp,
+                                       C.POINTER(C.c_int), I32P, C.POINTER(C.c_int), I16P, C.POINTER(C.c_int),
+                                       I32P, I32P, I32P, I, C.POINTER(C.c_int), ERR, SZ],
         }
         for name, args in sig.items():
             f = getattr(L, name)
@@ -385,7 +390,13 @@ def execute_project(frame: Sequence[DeviceColumn], query_part: str,
     n = C.c_int(0)
     _call(lib().blz_execute_project, _colpp(frame), len(frame), query_part.encode(), out, alias,
           max_out, C.byref(n), scalar_mode, stream)
-    return [DeviceColumn.adopt(out[i], owned=not alias[i]) for i in range(n.value)]
+    res = []
+    for i in range(n.value):
+        c = DeviceColumn.adopt(out[i], owned=not alias[i])
+        if alias[i]:  # shares a frame column's buffers: keep that column alive
+            c._base = next((f for f in frame if f.col.data == out[i].data), None)
+        res.append(c)
+    return res
 
 
 def filter_project(frame: Sequence[DeviceColumn], filter_query_part: str,
@@ -517,6 +528,46 @@ def plan_expression(frame_dtypes: Sequence[int], expression: str, cap=256):
                 binops=list(bops)[:k], unops=list(uops)[:k], lscalars=[ls[i] for i in range(k)],
                 rscalars=[rs[i] for i in range(k)], input_columns=list(in_cols)[:n_in.value],
                 final_position=final[0])
+
+
+def add_expression_to_plan(expression: str, expression_position: int, num_outputs: int, num_inputs: int,
+                           new_input_indices: Sequence[int], cap=256):
+    """add_expression_to_plan (LogicalFilter.h:26-42): the ops one expression appends
+    to a plan (host only)."""
+    I16 = C.c_int16 * cap
+    I32 = C.c_int32 * cap
+    SC = GdfScalar * cap
+    left, right, outs, bops, uops, ls, rs = I16(), I16(), I16(), I32(), I32(), SC(), SC()
+    idx = (C.c_int16 * max(len(new_input_indices), 1))(*new_input_indices)
+    n = C.c_int(0)
+    _call(lib().blz_add_expression_to_plan, expression.encode(), expression_position, num_outputs, num_inputs,
+          idx, len(new_input_indices), cap, left, right, outs, bops, uops, ls, rs, C.byref(n))
+    k = n.value
+    return dict(left=list(left)[:k], right=list(right)[:k], outputs=list(outs)[:k], binops=list(bops)[:k],
+                unops=list(uops)[:k], lscalars=[ls[i] for i in range(k)], rscalars=[rs[i] for i in range(k)])
+
+
+def parse_project_plan(frame_dtypes: Sequence[int], query_part: str, cap=256, max_out=64):
+    """parse_project_plan (CalciteInterpreter.h:63) for a frame of dtypes (host only):
+    the plan, the frame columns it reads in register order, the evaluated outputs'
+    registers and each output's (kind, index, dtype), kind 0 evaluated, 1 "$k"."""
+    I16 = C.c_int16 * cap
+    I32 = C.c_int32 * cap
+    SC = GdfScalar * cap
+    left, right, outs, bops, uops, ls, rs = I16(), I16(), I16(), I32(), I32(), SC(), SC()
+    in_cols, fin = I32(), (C.c_int16 * max_out)()
+    kind, index, odt = (C.c_int32 * max_out)(), (C.c_int32 * max_out)(), (C.c_int32 * max_out)()
+    n_ops, n_in, n_eval, n_out = C.c_int(0), C.c_int(0), C.c_int(0), C.c_int(0)
+    dt = (C.c_int32 * max(len(frame_dtypes), 1))(*frame_dtypes)
+    _call(lib().blz_parse_project_plan, dt, len(frame_dtypes), query_part.encode(), cap, left, right, outs, bops,
+          uops, ls, rs, C.byref(n_ops), in_cols, C.byref(n_in), fin, C.byref(n_eval), kind, index, odt, max_out,
+          C.byref(n_out))
+    k = n_ops.value
+    return dict(left=list(left)[:k], right=list(right)[:k], outputs=list(outs)[:k], binops=list(bops)[:k],
+                unops=list(uops)[:k], lscalars=[ls[i] for i in range(k)], rscalars=[rs[i] for i in range(k)],
+                input_columns=list(in_cols)[:n_in.value], final_positions=list(fin)[:n_eval.value],
+                outputs_kind=list(kind)[:n_out.value], outputs_index=list(index)[:n_out.value],
+                outputs_dtype=list(odt)[:n_out.value])
 
 
 class Batch:

@@ -57,6 +57,9 @@ int write_err(char *err, size_t len, const std::string &m) {
 
 template <class F>
 int guarded(char *err, size_t len, F &&f) {
+  // a failure left by an earlier call (hipGetLastError reports it once) must not be
+  // taken for a failure of this call's kernel launches
+  (void)hipGetLastError();
   try {
     f();
     write_err(err, len, "");
@@ -592,7 +595,9 @@ int aggregation_output_type(int input_type, AggOp op) {
   }
 }
 
-// Reduction of the valid rows of `c` (op 0 SUM, 1 MIN, 2 MAX) on the GPU.
+// Reduction of the valid rows of `c` (op 0 SUM, 1 MIN, 2 MAX) on the GPU. A column
+// whose null_count is 0 is read as all valid, its bitmap unread (the interpreter's
+// rule for its inputs, interpreter_ops.cuh:1312-1327).
 RedPart reduce_column(const gdf_column *c, int op, hipStream_t st) {
   const int load = column_load_kind(c->dtype);
   if (load == LD_NONE) throw BlzError(BLZ_ERR_UNSUPPORTED, "aggregation over this dtype is not supported");
@@ -602,7 +607,8 @@ RedPart reduce_column(const gdf_column *c, int op, hipStream_t st) {
   RedPart *parts = nullptr;
   HIPCHECK(hipMallocAsync((void **)&parts, sizeof(RedPart) * (grid + 1), st));
   RedPart r{};
-  hipError_t e = launch_reduce(op, c->data, load, (const uint8_t *)c->valid, n, parts, grid, st);
+  const uint8_t *valid = c->null_count > 0 ? (const uint8_t *)c->valid : nullptr;
+  hipError_t e = launch_reduce(op, c->data, load, valid, n, parts, grid, st);
   if (e == hipSuccess) e = hipMemcpyAsync(&r, parts + grid, sizeof(r), hipMemcpyDeviceToHost, st);
   hipError_t f = hipFreeAsync(parts, st);
   hipError_t s = hipStreamSynchronize(st);
@@ -652,7 +658,12 @@ int blz_process_aggregate(gdf_column *const *frame, int n_frame, const char *que
             blz_free_column(&temp);
             throw BlzError(rc, std::string("aggregation input: ") + e2);
           }
-          temp.null_count = nulls;
+          // evaluate_expression leaves the column's null_count at 0 (the call that
+          // would update it is commented out, LogicalFilter.cpp:368), so COUNT and
+          // AVG's divisor count every row (GroupBy.cpp:263, :284) and the reduction
+          // reads every row, as for any column whose null_count is 0
+          (void)nulls;
+          temp.null_count = 0;
           in = &temp;
         } else {
           int k = 0;
@@ -789,7 +800,9 @@ int blz_materialize_column(const gdf_column *input, gdf_column *output, const gd
     check_column(input, "input column", -1);
     check_column(row_indices, "row_indices", -1);
     check_column(output, "output column", row_indices->size);
-    if (row_indices->dtype != GDF_INT32) throw BlzError(BLZ_ERR_UNSUPPORTED, "row_indices must be INT32");
+    // the indices are read as a signed integer of their width (ColumnManipulation.cu:120-134)
+    const int iw = blz_dtype_width(row_indices->dtype);
+    if (iw == 0) throw BlzError(BLZ_ERR_UNSUPPORTED, "row_indices must have a 1, 2, 4 or 8-byte dtype");
     const int w = blz_dtype_width(input->dtype);
     if (w == 0 || blz_dtype_width(output->dtype) != w) invalid("input and output widths differ");
     const int64_t m = row_indices->size;
@@ -799,7 +812,7 @@ int blz_materialize_column(const gdf_column *input, gdf_column *output, const gd
     if (e == hipSuccess && m > 0) {
       const int64_t words = (m + 63) / 64;
       const int grid = (int)std::max<int64_t>(1, std::min<int64_t>((words + 15) / 16, (int64_t)device_cus() * 16));
-      e = launch_gather(w, input->data, input->valid, input->size, (const int32_t *)row_indices->data, m, output->data,
+      e = launch_gather(w, input->data, input->valid, input->size, row_indices->data, iw, m, output->data,
                         output->valid, d, grid, S(stream));
     }
     if (e == hipSuccess) e = hipMemcpyAsync(&h, d, 8, hipMemcpyDeviceToHost, S(stream));
@@ -816,7 +829,8 @@ int blz_materialize_columns(const gdf_column *const *inputs, gdf_column *const *
   return guarded(err, err_len, [&] {
     if (!inputs || !outputs || n_columns < 0) invalid("bad arguments");
     check_column(row_indices, "row_indices", -1);
-    if (row_indices->dtype != GDF_INT32) throw BlzError(BLZ_ERR_UNSUPPORTED, "row_indices must be INT32");
+    const int iw = blz_dtype_width(row_indices->dtype);
+    if (iw == 0) throw BlzError(BLZ_ERR_UNSUPPORTED, "row_indices must have a 1, 2, 4 or 8-byte dtype");
     if (n_columns == 0) return;
     const int64_t m = row_indices->size;
     const int64_t n_in = inputs[0] ? inputs[0]->size : 0;
@@ -838,7 +852,7 @@ int blz_materialize_columns(const gdf_column *const *inputs, gdf_column *const *
     const int grid = (int)std::max<int64_t>(1, std::min<int64_t>((words + 15) / 16, (int64_t)device_cus() * 16));
     for (int c = 0; e == hipSuccess && m > 0 && c < n_columns; c++)
       e = launch_gather(blz_dtype_width(inputs[c]->dtype), inputs[c]->data, inputs[c]->valid, n_in,
-                        (const int32_t *)row_indices->data, m, outputs[c]->data, outputs[c]->valid, d + c, grid, st);
+                        row_indices->data, iw, m, outputs[c]->data, outputs[c]->valid, d + c, grid, st);
     if (e == hipSuccess) e = hipMemcpyAsync(h.data(), d, 8 * (size_t)n_columns, hipMemcpyDeviceToHost, st);
     (void)hipFreeAsync(d, st);
     hipError_t s = hipStreamSynchronize(st);
@@ -1227,6 +1241,67 @@ int blz_plan_expression(const int32_t *frame_dtypes_in, int n_frame, const char 
     for (size_t i = 0; i < ep.spec.in_dev.size(); i++) in_cols[i] = ep.spec.in_dev[i];
     *n_in = (int)ep.spec.in_dev.size();
     *final_pos = ep.spec.final_pos[0];
+  });
+}
+
+namespace {
+// Copies the ops of p from op `first` on into the caller's arrays (capacity cap).
+void export_ops(const PlanSpec &p, size_t first, int cap, column_index_type *left, column_index_type *right,
+                column_index_type *outputs, int32_t *binops, int32_t *unops, gdf_scalar *lscalars,
+                gdf_scalar *rscalars, int *n_ops) {
+  const int k = (int)(p.left.size() - first);
+  if (k > cap) invalid("plan larger than cap (" + std::to_string(k) + " ops)");
+  for (int i = 0; i < k; i++) {
+    left[i] = p.left[first + i];
+    right[i] = p.right[first + i];
+    outputs[i] = p.outs[first + i];
+    binops[i] = p.binops[first + i];
+    unops[i] = p.unops[first + i];
+    lscalars[i] = p.lsc[first + i];
+    rscalars[i] = p.rsc[first + i];
+  }
+  *n_ops = k;
+}
+}  // namespace
+
+int blz_add_expression_to_plan(const char *expression, column_index_type expression_position,
+                               column_index_type num_outputs, column_index_type num_inputs,
+                               const column_index_type *new_input_indices, int n_frame, int cap,
+                               column_index_type *left, column_index_type *right, column_index_type *outputs,
+                               int32_t *binops, int32_t *unops, gdf_scalar *lscalars, gdf_scalar *rscalars,
+                               int *n_ops, char *err, size_t err_len) {
+  return guarded(err, err_len, [&] {
+    if (!expression || !n_ops || n_frame < 0 || (n_frame > 0 && !new_input_indices)) invalid("bad arguments");
+    std::vector<int> idx(new_input_indices, new_input_indices + n_frame);
+    PlanSpec p;
+    planner::add_expression_to_plan(p, expression, expression_position, num_outputs, num_inputs, idx);
+    export_ops(p, 0, cap, left, right, outputs, binops, unops, lscalars, rscalars, n_ops);
+  });
+}
+
+int blz_parse_project_plan(const int32_t *frame_dtypes_in, int n_frame, const char *query_part, int cap,
+                           column_index_type *left, column_index_type *right, column_index_type *outputs,
+                           int32_t *binops, int32_t *unops, gdf_scalar *lscalars, gdf_scalar *rscalars, int *n_ops,
+                           int32_t *in_cols, int *n_in, column_index_type *final_pos, int *n_eval,
+                           int32_t *out_kind, int32_t *out_index, int32_t *out_dtype, int max_out, int *n_out,
+                           char *err, size_t err_len) {
+  return guarded(err, err_len, [&] {
+    if (!query_part || !n_ops || !n_in || !n_eval || !n_out) invalid("bad arguments");
+    std::vector<int> dt(frame_dtypes_in, frame_dtypes_in + n_frame);
+    const planner::ProjectPlan pp = planner::parse_project(dt, query_part);
+    if ((int)pp.outputs.size() > max_out) invalid("more outputs than max_out");
+    if ((int)pp.spec.in_dev.size() > cap) invalid("more input columns than cap");
+    export_ops(pp.spec, 0, cap, left, right, outputs, binops, unops, lscalars, rscalars, n_ops);
+    for (size_t i = 0; i < pp.spec.in_dev.size(); i++) in_cols[i] = pp.spec.in_dev[i];
+    *n_in = (int)pp.spec.in_dev.size();
+    for (size_t i = 0; i < pp.spec.final_pos.size(); i++) final_pos[i] = pp.spec.final_pos[i];
+    *n_eval = (int)pp.spec.final_pos.size();
+    for (size_t i = 0; i < pp.outputs.size(); i++) {
+      out_kind[i] = pp.outputs[i].kind;
+      out_index[i] = pp.outputs[i].index;
+      out_dtype[i] = pp.outputs[i].dtype;
+    }
+    *n_out = (int)pp.outputs.size();
   });
 }
 

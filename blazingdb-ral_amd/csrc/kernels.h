@@ -80,10 +80,10 @@ hipError_t launch_synth(void *data, uint8_t *valid, int64_t n, int dtype, uint64
 // Zeroes `n` byte ranges (dst[i], bytes[i]; 8-byte multiples, 8-byte aligned) in
 // one launch.
 hipError_t launch_fill_ranges(void *const *dst, const int64_t *bytes, int n, int grid, hipStream_t st);
-// materialize_column: width in {1,2,4,8}.
-hipError_t launch_gather(int width, const void *in, const uint8_t *in_valid, int64_t n_in, const int32_t *idx,
-                         int64_t n_idx, void *out, uint8_t *out_valid, unsigned long long *null_count,
-                         int grid, hipStream_t st);
+// materialize_column: value width and index width (a signed integer) in {1,2,4,8}.
+hipError_t launch_gather(int width, const void *in, const uint8_t *in_valid, int64_t n_in, const void *idx,
+                         int idx_width, int64_t n_idx, void *out, uint8_t *out_valid,
+                         unsigned long long *null_count, int grid, hipStream_t st);
 hipError_t launch_sequence(int32_t *out, int64_t n, int32_t init, int32_t step, int grid, hipStream_t st);
 hipError_t launch_fill_zero(void *p, int64_t bytes, int grid, hipStream_t st);
 hipError_t launch_null_count(const uint8_t *valid, int64_t n, unsigned long long *out, int grid,

@@ -375,10 +375,10 @@ interp_kernel(const uint8_t *__restrict__ blob_arg, Workspace ws) {
 // wave takes four 64-index words at a time and issues every index load, then every
 // value load, before its stores: random accesses need many requests in flight.
 constexpr int kGatherWords = 4;
-template <typename T>
+template <typename T, typename I>
 __global__ void __launch_bounds__(kBlock)
 gather_kernel(const T *__restrict__ in, const uint8_t *__restrict__ in_valid, int64_t n_in,
-              const int32_t *__restrict__ idx, int64_t n_idx, T *__restrict__ out,
+              const I *__restrict__ idx, int64_t n_idx, T *__restrict__ out,
               uint8_t *__restrict__ out_valid, unsigned long long *null_count) {
   const int lane = threadIdx.x & 63;
   const int64_t n_words = (n_idx + 63) >> 6;
@@ -386,13 +386,13 @@ gather_kernel(const T *__restrict__ in, const uint8_t *__restrict__ in_valid, in
   const int64_t n_waves = ((int64_t)gridDim.x * kBlock) >> 6;
   uint64_t nulls = 0;
   for (int64_t w0 = wave_id * kGatherWords; w0 < n_words; w0 += n_waves * kGatherWords) {
-    int32_t j[kGatherWords];
+    int64_t j[kGatherWords];
     bool ok[kGatherWords];
     T v[kGatherWords];
 #pragma unroll
     for (int u = 0; u < kGatherWords; u++) {
       const int64_t i = (w0 + u) * 64 + lane;
-      j[u] = i < n_idx ? idx[i] : -1;
+      j[u] = i < n_idx ? (int64_t)idx[i] : -1;
     }
 #pragma unroll
     for (int u = 0; u < kGatherWords; u++) {
@@ -630,30 +630,44 @@ int interp_blocks_per_cu(int s, int r, bool compact, int lds_bytes) {
   return (e == hipSuccess && blocks > 0) ? blocks : 1;
 }
 
-hipError_t launch_gather(int width, const void *in, const uint8_t *in_valid, int64_t n_in, const int32_t *idx,
-                         int64_t n_idx, void *out, uint8_t *out_valid, unsigned long long *null_count,
-                         int grid, hipStream_t st) {
-  switch (width) {
+template <typename T>
+static hipError_t launch_gather_t(const void *in, const uint8_t *in_valid, int64_t n_in, const void *idx,
+                                  int idx_width, int64_t n_idx, void *out, uint8_t *out_valid,
+                                  unsigned long long *null_count, int grid, hipStream_t st) {
+  const T *i = (const T *)in;
+  T *o = (T *)out;
+  switch (idx_width) {
     case 1:
-      hipLaunchKernelGGL(gather_kernel<uint8_t>, dim3(grid), dim3(kBlock), 0, st, (const uint8_t *)in,
-                         in_valid, n_in, idx, n_idx, (uint8_t *)out, out_valid, null_count);
+      hipLaunchKernelGGL((gather_kernel<T, int8_t>), dim3(grid), dim3(kBlock), 0, st, i, in_valid, n_in,
+                         (const int8_t *)idx, n_idx, o, out_valid, null_count);
       break;
     case 2:
-      hipLaunchKernelGGL(gather_kernel<uint16_t>, dim3(grid), dim3(kBlock), 0, st, (const uint16_t *)in,
-                         in_valid, n_in, idx, n_idx, (uint16_t *)out, out_valid, null_count);
+      hipLaunchKernelGGL((gather_kernel<T, int16_t>), dim3(grid), dim3(kBlock), 0, st, i, in_valid, n_in,
+                         (const int16_t *)idx, n_idx, o, out_valid, null_count);
       break;
     case 4:
-      hipLaunchKernelGGL(gather_kernel<uint32_t>, dim3(grid), dim3(kBlock), 0, st, (const uint32_t *)in,
-                         in_valid, n_in, idx, n_idx, (uint32_t *)out, out_valid, null_count);
+      hipLaunchKernelGGL((gather_kernel<T, int32_t>), dim3(grid), dim3(kBlock), 0, st, i, in_valid, n_in,
+                         (const int32_t *)idx, n_idx, o, out_valid, null_count);
       break;
     case 8:
-      hipLaunchKernelGGL(gather_kernel<uint64_t>, dim3(grid), dim3(kBlock), 0, st, (const uint64_t *)in,
-                         in_valid, n_in, idx, n_idx, (uint64_t *)out, out_valid, null_count);
+      hipLaunchKernelGGL((gather_kernel<T, int64_t>), dim3(grid), dim3(kBlock), 0, st, i, in_valid, n_in,
+                         (const int64_t *)idx, n_idx, o, out_valid, null_count);
       break;
     default:
       return hipErrorInvalidValue;
   }
   return hipGetLastError();
+}
+hipError_t launch_gather(int width, const void *in, const uint8_t *in_valid, int64_t n_in, const void *idx,
+                         int idx_width, int64_t n_idx, void *out, uint8_t *out_valid,
+                         unsigned long long *null_count, int grid, hipStream_t st) {
+  switch (width) {
+    case 1: return launch_gather_t<uint8_t>(in, in_valid, n_in, idx, idx_width, n_idx, out, out_valid, null_count, grid, st);
+    case 2: return launch_gather_t<uint16_t>(in, in_valid, n_in, idx, idx_width, n_idx, out, out_valid, null_count, grid, st);
+    case 4: return launch_gather_t<uint32_t>(in, in_valid, n_in, idx, idx_width, n_idx, out, out_valid, null_count, grid, st);
+    case 8: return launch_gather_t<uint64_t>(in, in_valid, n_in, idx, idx_width, n_idx, out, out_valid, null_count, grid, st);
+    default: return hipErrorInvalidValue;
+  }
 }
 
 hipError_t launch_sequence(int32_t *out, int64_t n, int32_t init, int32_t step, int grid, hipStream_t st) {

@@ -369,8 +369,6 @@ int get_output_type_expression(const std::vector<int> &dtypes, const std::string
   return st.back();
 }
 
-namespace {
-
 // LogicalFilter.cpp:44-267
 void add_expression_to_plan(PlanSpec &p, const std::string &expression, int expression_position,
                             int num_outputs, int num_inputs, const std::vector<int> &new_input_indices) {
@@ -453,6 +451,8 @@ void add_expression_to_plan(PlanSpec &p, const std::string &expression, int expr
     }
   }
 }
+
+namespace {
 
 void used_columns(const std::vector<std::string> &exprs, int ncols, std::vector<int> &new_idx,
                   std::vector<int> &cols) {

@@ -33,6 +33,13 @@ int get_output_type_expression(const std::vector<int> &dtypes, const std::string
 std::string get_named_expression(const std::string &query_part, const std::string &name);
 bool contains_evaluation(const std::string &expression);
 
+// add_expression_to_plan (LogicalFilter.cpp:44-267): appends the ops of one
+// expression to p (its registers: inputs [0, num_inputs), outputs from num_inputs,
+// temporaries from num_inputs + num_outputs, lowest free first).
+// new_input_indices[frame column] = its input register, or -1.
+void add_expression_to_plan(PlanSpec &p, const std::string &expression, int expression_position,
+                            int num_outputs, int num_inputs, const std::vector<int> &new_input_indices);
+
 // evaluate_expression's plan. `spec.in_dev` holds FRAME column indices.
 struct ExprPlan {
   bool bare = false;  // "$k": the expression is column k itself

@@ -100,8 +100,11 @@ int blz_execute_project(gdf_column *const *frame, int n_frame, const char *query
  * columns (free with blz_free_column): COUNT -> INT64 rows - null_count (always
  * valid); SUM -> INT64 (integer input, wrapping) or FLOAT64 over the valid rows;
  * MIN / MAX -> the input dtype; AVG -> FLOAT64 sum / count. SUM, MIN, MAX, AVG of no
- * valid row are null. A GROUP BY (group=[{...}] not empty) or COUNT_DISTINCT gives
- * BLZ_ERR_UNSUPPORTED. */
+ * valid row are null. A column whose null_count is 0 counts every row as valid, and
+ * an evaluated input has null_count 0 whatever its bitmap holds, as in the reference
+ * (evaluate_expression does not update it, LogicalFilter.cpp:368): COUNT(expr) is the
+ * row count and AVG(expr) divides the sum of every row by it. A GROUP BY
+ * (group=[{...}] not empty) or COUNT_DISTINCT gives BLZ_ERR_UNSUPPORTED. */
 int blz_process_aggregate(gdf_column *const *frame, int n_frame, const char *query_part,
                           gdf_column *out_columns, int max_out, int *n_out, int scalar_mode,
                           blz_stream_t stream, char *err, size_t err_len);
@@ -120,8 +123,9 @@ int blz_filter_project(gdf_column *const *frame, int n_frame, const char *filter
 /* materialize_column (src/ColumnManipulation.cuh:16-18, ColumnManipulation.cu:83-150):
  * output[i] = input[row_indices[i]] for row_indices[i] >= 0; output validity bit i =
  * input validity bit row_indices[i] (when output->valid != NULL); sets
- * output->null_count. row_indices is an int32 column; output->size must equal
- * row_indices->size. */
+ * output->null_count. row_indices holds signed integers of its dtype's width (1, 2, 4
+ * or 8 bytes, as ColumnManipulation.cu:120-134 dispatches them); output->size must
+ * equal row_indices->size. */
 /* materialize_column over n_columns columns of one table sharing row_indices, as one
  * blocking call (no reference counterpart as one call: its callers loop over
  * materialize_column, JoinOperator.cpp:130-135, OrderBy.cpp:66,
@@ -295,6 +299,33 @@ int blz_plan_expression(const int32_t *frame_dtypes, int n_frame, const char *ex
                         gdf_scalar *lscalars, gdf_scalar *rscalars, int *n_ops,
                         int32_t *in_cols, int *n_in, column_index_type *final_pos,
                         char *err, size_t err_len);
+
+/* add_expression_to_plan (src/LogicalFilter.h:26-42, LogicalFilter.cpp:44-267): the
+ * ops of one expression as the reference appends them to a plan: registers
+ * [0, num_inputs) are the inputs, num_inputs + expression_position the
+ * expression's output, temporaries from num_inputs + num_outputs (lowest free
+ * first); new_input_indices[k] (k < n_frame) is frame column k's input register or
+ * -1. Writes the ops (at most cap) to the arrays and their count to *n_ops; the
+ * caller appends them to its plan. */
+int blz_add_expression_to_plan(const char *expression, column_index_type expression_position,
+                               column_index_type num_outputs, column_index_type num_inputs,
+                               const column_index_type *new_input_indices, int n_frame, int cap,
+                               column_index_type *left, column_index_type *right, column_index_type *outputs,
+                               int32_t *binops, int32_t *unops, gdf_scalar *lscalars, gdf_scalar *rscalars,
+                               int *n_ops, char *err, size_t err_len);
+
+/* parse_project_plan (src/CalciteInterpreter.h:63, CalciteInterpreter.cpp:123-309)
+ * for a frame of the given dtypes: every evaluated expression of the LogicalProject
+ * in one register plan (ops: at most cap), the frame columns it reads in register
+ * order (in_cols[0..*n_in)), the evaluated outputs' registers (final_pos[0..*n_eval))
+ * and, for each output i of the projection, out_kind[i] (0 evaluated, 1 a "$k"
+ * column), out_index[i] (the evaluated output's number, or k) and out_dtype[i]. */
+int blz_parse_project_plan(const int32_t *frame_dtypes, int n_frame, const char *query_part, int cap,
+                           column_index_type *left, column_index_type *right, column_index_type *outputs,
+                           int32_t *binops, int32_t *unops, gdf_scalar *lscalars, gdf_scalar *rscalars, int *n_ops,
+                           int32_t *in_cols, int *n_in, column_index_type *final_pos, int *n_eval,
+                           int32_t *out_kind, int32_t *out_index, int32_t *out_dtype, int max_out, int *n_out,
+                           char *err, size_t err_len);
 
 #ifdef __cplusplus
 }

@@ -106,6 +106,15 @@ typedef enum {
   BLZ_CAST_FLOAT64
 } gdf_unary_operator;
 
+/* cuDF's legacy status code, as project_plan_params carries it
+ * (src/CalciteInterpreter.h:31); this path sets GDF_SUCCESS or throws. */
+typedef enum {
+  GDF_SUCCESS = 0,
+  GDF_CUDA_ERROR,
+  GDF_UNSUPPORTED_DTYPE,
+  GDF_COLUMN_SIZE_MISMATCH
+} gdf_error;
+
 typedef union {
   int8_t si08;
   int16_t si16;

@@ -7,6 +7,8 @@
  *   evaluate_expression    src/LogicalFilter.h:20-23
  *   process_filter         src/CalciteInterpreter.cpp:509 (no header in the reference)
  *   execute_project_plan   src/CalciteInterpreter.h:61-63
+ *   parse_project_plan     src/CalciteInterpreter.h:63 (with project_plan_params :17-32)
+ *   add_expression_to_plan src/LogicalFilter.h:26-42
  *   materialize_column     src/ColumnManipulation.cuh:16-18
  *   gdf_sequence           src/Utils.cuh:187-190
  * with the frame and column types they take (blazing_frame, DataFrame.h:16-181;
@@ -187,6 +189,119 @@ inline void perform_operation(std::vector<gdf_column *> output_columns, std::vec
                             unary_operators.data(), left_scalars.data(), right_scalars.data(), BLZ_SCALAR_INTENDED,
                             nullptr, nullptr, err, sizeof(err)),
       err, "perform_operation");
+}
+
+// Appends the ops of one expression to a plan (LogicalFilter.h:26-42,
+// LogicalFilter.cpp:44-267): registers [0, num_inputs) are the inputs,
+// num_inputs + expression_position the expression's output, temporaries from
+// num_inputs + num_outputs; new_input_indices[k] is frame column k's input register
+// (or -1). `inputs` and `input_columns` are not read, as in the reference.
+inline void add_expression_to_plan(blazing_frame &inputs, std::vector<gdf_column *> &input_columns,
+                                   std::string expression, column_index_type expression_position,
+                                   column_index_type num_outputs, column_index_type num_inputs,
+                                   std::vector<column_index_type> &left_inputs,
+                                   std::vector<column_index_type> &right_inputs,
+                                   std::vector<column_index_type> &outputs,
+                                   std::vector<gdf_binary_operator> &operators,
+                                   std::vector<gdf_unary_operator> &unary_operators,
+                                   std::vector<gdf_scalar> &left_scalars, std::vector<gdf_scalar> &right_scalars,
+                                   std::vector<column_index_type> &new_input_indices) {
+  (void)inputs;
+  (void)input_columns;
+  const int cap = 1024;
+  std::vector<column_index_type> l(cap), r(cap), o(cap);
+  std::vector<int32_t> b(cap), u(cap);
+  std::vector<gdf_scalar> ls(cap), rs(cap);
+  int n = 0;
+  char err[512] = {0};
+  blz_detail::check(blz_add_expression_to_plan(expression.c_str(), expression_position, num_outputs, num_inputs,
+                                               new_input_indices.data(), (int)new_input_indices.size(), cap, l.data(),
+                                               r.data(), o.data(), b.data(), u.data(), ls.data(), rs.data(), &n, err,
+                                               sizeof(err)),
+                    err, "add_expression_to_plan");
+  for (int i = 0; i < n; i++) {
+    left_inputs.push_back(l[i]);
+    right_inputs.push_back(r[i]);
+    outputs.push_back(o[i]);
+    operators.push_back((gdf_binary_operator)b[i]);
+    unary_operators.push_back((gdf_unary_operator)u[i]);
+    left_scalars.push_back(ls[i]);
+    right_scalars.push_back(rs[i]);
+  }
+}
+
+// The plan of a LogicalProject (CalciteInterpreter.h:17-32): every evaluated
+// expression in one op stream over input_columns, into output_columns (allocated
+// here, one per evaluated expression, every input row); `columns` holds the
+// projection's outputs in order ("$k" outputs share frame column k's buffers).
+struct project_plan_params {
+  size_t num_expressions_out;
+  std::vector<gdf_column *> output_columns;
+  std::vector<gdf_column *> input_columns;
+  std::vector<column_index_type> left_inputs;
+  std::vector<column_index_type> right_inputs;
+  std::vector<column_index_type> outputs;
+  std::vector<column_index_type> final_output_positions;
+  std::vector<gdf_binary_operator> operators;
+  std::vector<gdf_unary_operator> unary_operators;
+  std::vector<gdf_scalar> left_scalars;
+  std::vector<gdf_scalar> right_scalars;
+  std::vector<column_index_type> new_column_indices;
+  std::vector<gdf_column_cpp> columns;
+  gdf_error error;
+};
+
+// parse_project_plan (CalciteInterpreter.h:63, CalciteInterpreter.cpp:123-309): the
+// caller then runs perform_operation on the params' vectors, as the reference's
+// interops benchmark does (benchmarks/interops/interops_benchmark.cpp:89-95).
+inline project_plan_params parse_project_plan(blazing_frame &input, std::string query_part) {
+  auto frame = input.column_pointers();
+  std::vector<int32_t> dt(frame.size());
+  for (size_t i = 0; i < frame.size(); i++) dt[i] = frame[i]->dtype;
+  const int cap = 1024, max_out = 1024;
+  std::vector<column_index_type> l(cap), r(cap), o(cap), fin(max_out);
+  std::vector<int32_t> b(cap), u(cap), in_cols(cap), kind(max_out), index(max_out), odt(max_out);
+  std::vector<gdf_scalar> ls(cap), rs(cap);
+  int n_ops = 0, n_in = 0, n_eval = 0, n_out = 0;
+  char err[512] = {0};
+  blz_detail::check(blz_parse_project_plan(dt.data(), (int)dt.size(), query_part.c_str(), cap, l.data(), r.data(),
+                                           o.data(), b.data(), u.data(), ls.data(), rs.data(), &n_ops, in_cols.data(),
+                                           &n_in, fin.data(), &n_eval, kind.data(), index.data(), odt.data(),
+                                           max_out, &n_out, err, sizeof(err)),
+                    err, "parse_project_plan");
+  project_plan_params p{};
+  p.num_expressions_out = (size_t)n_eval;
+  p.left_inputs.assign(l.begin(), l.begin() + n_ops);
+  p.right_inputs.assign(r.begin(), r.begin() + n_ops);
+  p.outputs.assign(o.begin(), o.begin() + n_ops);
+  for (int i = 0; i < n_ops; i++) {
+    p.operators.push_back((gdf_binary_operator)b[i]);
+    p.unary_operators.push_back((gdf_unary_operator)u[i]);
+  }
+  p.left_scalars.assign(ls.begin(), ls.begin() + n_ops);
+  p.right_scalars.assign(rs.begin(), rs.begin() + n_ops);
+  p.final_output_positions.assign(fin.begin(), fin.begin() + n_eval);
+  p.new_column_indices.assign(frame.size(), (column_index_type)-1);
+  for (int i = 0; i < n_in; i++) {
+    p.input_columns.push_back(frame[in_cols[i]]);
+    p.new_column_indices[in_cols[i]] = (column_index_type)i;
+  }
+  const size_t rows = frame.empty() ? 0 : (size_t)frame[0]->size;
+  std::vector<gdf_column *> evaluated(n_eval, nullptr);
+  p.columns.resize(n_out);
+  for (int i = 0; i < n_out; i++) {
+    if (kind[i] == 0) {
+      gdf_column_cpp c;
+      c.create_gdf_column((gdf_dtype)odt[i], rows, nullptr, blz_dtype_width((gdf_dtype)odt[i]));
+      evaluated[index[i]] = c.get_gdf_column();
+      p.columns[i] = c;
+    } else {
+      p.columns[i] = input.get_column(index[i]);
+    }
+  }
+  p.output_columns = evaluated;
+  p.error = GDF_SUCCESS;
+  return p;
 }
 
 // Plans and evaluates one Calcite expression over the frame into `output`

@@ -268,8 +268,13 @@ def process_aggregate(frame: List[HostColumn], query_part: str, scalar_mode=0) -
     or an expression evaluated first; the result is a one-row column. COUNT = rows -
     null_count; SUM / MIN / MAX over the valid rows (cudf::reduce skips nulls; no
     valid row -> null); AVG = SUM / (rows - null_count) in binary64, null when no row
-    is valid. An integer SUM wraps in int64; a float SUM here is the exact sum rounded
-    once (math.fsum): the reference's summation order is cuDF's, not pinned."""
+    is valid. An evaluated input's null_count stays 0 (evaluate_expression never
+    updates it, LogicalFilter.cpp:368), and a column whose null_count is 0 is read
+    as all valid (the interpreter's input rule, interpreter_ops.cuh:1312-1327), so
+    COUNT(expr) counts every row and AVG(expr) divides the sum of every row by it
+    (GroupBy.cpp:263, :284). An integer SUM wraps in int64; a float SUM here is the
+    exact sum rounded once (math.fsum): the reference's summation order is cuDF's,
+    not pinned."""
     import math
     groups, aggs = P.parse_aggregate(query_part)
     if groups:
@@ -279,12 +284,13 @@ def process_aggregate(frame: List[HostColumn], query_part: str, scalar_mode=0) -
     for op, inp in aggs:
         if P.contains_evaluation(inp):
             col = evaluate_expression(frame, inp, P.get_output_type_expression(dts, inp), True, scalar_mode)
+            col = HostColumn(col.data, col.dtype, col.valid, 0)
         else:
             col = frame[P.get_index(inp) if inp else 0]
         ot = P.aggregation_output_type(col.dtype, op)
         n = len(col)
         valid_rows = n - (col.null_count if col.valid is not None else 0)
-        vals = col.data if col.valid is None else col.data[col.valid]
+        vals = col.data if (col.valid is None or col.null_count == 0) else col.data[col.valid]
         is_f = col.dtype in (GDF_FLOAT32, GDF_FLOAT64)
         ok = True
         if op == "COUNT":

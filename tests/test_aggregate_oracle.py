@@ -73,3 +73,33 @@ def test_all_null_input_and_expression_input():
     col = O.HostColumn(np.arange(10, dtype=np.int32), P.GDF_INT32)
     s = O.process_aggregate([col], "LogicalAggregate(group=[{}], EXPR$0=[SUM(*($0, 2))])")[0]
     assert s.dtype == P.GDF_INT64 and s.data[0] == 90
+
+
+def test_count_and_avg_over_a_nullable_expression_count_every_row():
+    """compute_aggregations evaluates an expression input into a new column
+    (GroupBy.cpp:318-319) whose null_count evaluate_expression never updates
+    (LogicalFilter.cpp:368), so COUNT = size - 0 (:284) and AVG's divisor is the
+    row count (:263): the expression's nulls are counted, and its sum reads every
+    row, as for any column whose null_count is 0."""
+    rng = np.random.default_rng(6)
+    n = 3000
+    x = rng.integers(-1000, 1000, n).astype(np.int32)
+    v = rng.random(n) > 0.3
+    col = O.HostColumn(x, P.GDF_INT32, v)
+    qp = ("LogicalAggregate(group=[{}], c=[COUNT(+($0, 1))], a=[AVG(+($0, 1))], s=[SUM(+($0, 1))], "
+          "cc=[COUNT($0)], ac=[AVG($0)])")
+    got = O.process_aggregate([col], qp)
+    y = x.astype(np.int64) + 1                      # every row, null ones included
+    assert got[0].data[0] == n
+    assert got[1].data[0] == float(int(y.sum())) / n
+    assert got[2].data[0] == int(y.sum())
+    # the column itself keeps its null count: its nulls are not counted
+    assert got[3].data[0] == int(v.sum())
+    assert got[4].data[0] == float(int(x[v].astype(np.int64).sum())) / int(v.sum())
+
+
+def test_null_count_zero_bitmap_is_read_as_all_valid():
+    x = np.arange(8, dtype=np.int64)
+    col = O.HostColumn(x, P.GDF_INT64, np.array([True, False] * 4), 0)   # inconsistent: null_count 0
+    got = O.process_aggregate([col], "LogicalAggregate(group=[{}], s=[SUM($0)], c=[COUNT($0)])")
+    assert got[0].data[0] == 28 and got[1].data[0] == 8

@@ -131,3 +131,25 @@ def test_q6_filter_project_sum(B):
     assert again[0].data().tobytes() == got[0].data().tobytes()
     for c in got + again + outs + dev:
         c.free()
+
+
+def test_count_avg_sum_over_a_nullable_expression(B):
+    """An expression input keeps null_count 0 after evaluate_expression
+    (LogicalFilter.cpp:368): COUNT(expr) is the row count, AVG(expr) divides the sum
+    of every row by it (GroupBy.cpp:263, :284), SUM / MIN / MAX read every row; the
+    nullable column itself counts only its valid rows. Integer inputs: bit-exact."""
+    rng = np.random.default_rng(91)
+    n = 200_003
+    x = gen_column(rng, P.GDF_INT32, n, 0.3)
+    y = gen_column(rng, P.GDF_INT64, n, 0.1)
+    host = [x, y]
+    q = ("LogicalAggregate(group=[{}], c=[COUNT(+($0, 1))], a=[AVG(+($0, 1))], s=[SUM(*($0, $1))], "
+         "mx=[MAX(-($1, $0))], mn=[MIN(+($0, 1))], cc=[COUNT($0)], ac=[AVG($0)], sc=[SUM($1)])")
+    _case(test="aggregate_nullable_expression", n=n)
+    ref = O.process_aggregate(host, q)
+    assert ref[0].data[0] == n                      # every row counted
+    dev = [to_device(B, h) for h in host]
+    got = B.process_aggregate(dev, q)
+    _check(got, ref, [None] * 8, "|".join(["x"] * 8))
+    for c in got + dev:
+        c.free()

@@ -21,6 +21,11 @@ from tests.helpers import assert_column_equal, to_device, ulp_diff
 pytestmark = pytest.mark.gpu
 
 ULP_TOL = 1   # transcendental and POWER results: at most 1 ulp from the oracle
+# COTAN is cos(x) / sin(x) (interpreter_ops.cuh:765-767): a quotient of two library
+# results, each within 1 ulp between the two libraries, rounded once more: up to
+# 2 ulp of relative difference, which is 4 ulp of the quotient where the quotient
+# sits at the bottom of its binade and the operands at the top of theirs
+ULP_TOL_OP = {"COTAN": 4}
 
 
 @pytest.fixture(scope="module")
@@ -66,12 +71,12 @@ def _int_inputs(rng, n, np_dtype):
     return x.astype(np_dtype)
 
 
-def _check_float(got, ref, what, dtype_name):
+def _check_float(got, ref, what, dtype_name, tol=ULP_TOL):
     d = ulp_diff(got, ref)
     worst = float(d.max()) if len(d) else 0.0
     n_diff = int(np.count_nonzero(d))
     print(f"ULP {what} dtype={dtype_name} max={worst:g} rows_differing={n_diff} of {len(d)}", flush=True)
-    if worst > ULP_TOL:
+    if worst > tol:
         i = int(np.argmax(d))
         raise AssertionError(f"{what}: {worst:g} ulp at row {i}: gpu={got[i]!r} oracle={ref[i]!r}")
 
@@ -275,7 +280,7 @@ def test_unary_register_plans_within_one_ulp(B, dtype):
         out = B.DeviceColumn(dtype, n, with_valid=True)
         nulls = B.perform_operation([out], dev, [0], [-1], [1], [1], [B.GDF_INVALID_BINARY], [u],
                                     [dummy], [dummy])
-        _check_float(out.data(), ref.data, f"{name} plan", str(np.dtype(npd)))
+        _check_float(out.data(), ref.data, f"{name} plan", str(np.dtype(npd)), ULP_TOL_OP.get(name, ULP_TOL))
         assert np.array_equal(out.valid(), ref.valid) and nulls[0] == ref.null_count
         out.free()
 
@@ -293,14 +298,16 @@ def test_register_plan_float_scalars_and_slot_reuse(B, scalar_mode):
     c = rng.uniform(-50.0, 50.0, n)
     host = [O.HostColumn(a, P.GDF_INT32), O.HostColumn(b, P.GDF_FLOAT32, rng.random(n) > 0.1),
             O.HostColumn(c, P.GDF_FLOAT64)]
-    # registers: inputs 0..2, outputs 3..4, temporaries from 5
-    # t5 = c * 2.5 ; t6 = SIN(b) ; t5 = t5 + t6 ; out3 = t5 - a ; t6 = 7 - a ; out4 = t6 * 3
+    # registers: inputs 0..2, outputs 3..4, temporaries from 5; the transcendental is
+    # the last op of its output (a 1-ulp difference before a cancelling subtraction
+    # would grow without bound)
+    # t5 = c * 2.5 ; t6 = t5 + b ; t5 = t6 - a ; out3 = SIN(t5) ; t6 = 7 - a ; out4 = t6 * 3
     f25 = P.Scalar(P.GDF_FLOAT64, 2.5, True)
     i7 = P.Scalar(P.GDF_INT64, 7, True)
     i3 = P.Scalar(P.GDF_INT64, 3, True)
-    plan = P.Plan(left=[2, 1, 5, 5, -2, 6], right=[-2, -1, 6, 0, 0, -2], outputs=[5, 6, 5, 3, 6, 4],
-                  binops=[P.GDF_MUL, P.GDF_INVALID_BINARY, P.GDF_ADD, P.GDF_SUB, P.GDF_SUB, P.GDF_MUL],
-                  unops=[P.BLZ_INVALID_UNARY, P.BLZ_SIN] + [P.BLZ_INVALID_UNARY] * 4,
+    plan = P.Plan(left=[2, 5, 6, 5, -2, 6], right=[-2, 1, 0, -1, 0, -2], outputs=[5, 6, 5, 3, 6, 4],
+                  binops=[P.GDF_MUL, P.GDF_ADD, P.GDF_SUB, P.GDF_INVALID_BINARY, P.GDF_SUB, P.GDF_MUL],
+                  unops=[P.BLZ_INVALID_UNARY] * 3 + [P.BLZ_SIN] + [P.BLZ_INVALID_UNARY] * 2,
                   lscalars=[P.DUMMY, P.DUMMY, P.DUMMY, P.DUMMY, i7, P.DUMMY],
                   rscalars=[f25, P.DUMMY, P.DUMMY, P.DUMMY, P.DUMMY, i3], final_positions=[3, 4])
     out_dt = [P.GDF_FLOAT64, P.GDF_INT32]

@@ -158,3 +158,66 @@ def test_cpp_operator_api_compiles_and_reports_errors():
     r = subprocess.run([_shim(), "errors"], capture_output=True, text=True, timeout=60)
     assert r.returncode == 0, r.stdout + r.stderr
     assert r.stdout.startswith("runtime_error: evaluate_expression:"), r.stdout
+
+
+INTEROP_PLANS = [  # benchmarks/interops/interops_benchmark.cpp:32-37
+    "LogicalProject(EXPR$0=[+($0, $2)])",
+    "LogicalProject(EXPR$0=[*(SIN(/(+($0, $2), $1)), POWER(/(-($0, $2), $1), 2))])",
+    "LogicalProject(EXPR$1=[-(FLOOR(+(POWER(SIN($1), 2), POWER(COS($1), 2))), CEIL(+(POWER(SIN(+($1, 53.42)), 2), "
+    "POWER(COS(+($1, 53.42)), 2))))])",
+    "LogicalProject(EXPR$0=[*(*(10, SIN(/(+($0, $2), $1))), POWER(/(-($0, $2), $1), 2))], EXPR$1=[-(FLOOR(+(POWER("
+    "SIN($1), 2), POWER(COS($1), 2))), CEIL(+(POWER(SIN(+($1, 53.42)), 2), POWER(COS(+($1, 53.42)), 2))))], "
+    "EXPR$2=[+(-(/(*(FLOOR(+($1, 0.1)), POWER(MOD($0, 13), 2.0)), 5), CEIL(MOD(*(2, $2), 57))), 0.001)])",
+    "LogicalProject(a=[$2], b=[*($0, 2)], c=[$0], d=[CASE(IS NOT NULL($1), $1, 7.5)], e=[>($2, $0)])",
+]
+
+
+@pytest.mark.parametrize("qp", INTEROP_PLANS)
+def test_parse_project_plan_matches_oracle_planner(B, qp):
+    """blz_parse_project_plan (parse_project_plan, CalciteInterpreter.cpp:123-309)
+    against the oracle's restatement: the same register plan, input columns in
+    frame order, final registers and per-output kind / index / dtype."""
+    dts = [P.GDF_INT32, P.GDF_FLOAT64, P.GDF_INT32]
+    want, outs, _ = P.parse_project(dts, qp)
+    got = B.parse_project_plan(dts, qp)
+    for key in ("left", "right", "outputs", "binops", "unops"):
+        assert got[key] == getattr(want, key), key
+    assert got["input_columns"] == want.input_columns
+    assert got["final_positions"] == want.final_positions
+    assert got["outputs_kind"] == [0 if k == "expr" else 1 for k, _ in outs]
+    assert got["outputs_index"] == [i for _, i in outs]
+    ev = iter(want.output_dtypes)
+    assert got["outputs_dtype"] == [next(ev) if k == "expr" else dts[i] for k, i in outs]
+
+
+@pytest.mark.parametrize("seed", range(10))
+def test_add_expression_to_plan_matches_oracle_planner(B, seed):
+    """blz_add_expression_to_plan (LogicalFilter.cpp:44-267) appends exactly the ops
+    the oracle's add_expression_to_plan appends, for an expression placed at an
+    arbitrary output position of a multi-output plan."""
+    rng = np.random.default_rng(950 + seed)
+    k = int(rng.integers(2, 7))
+    dtypes = exprgen.rand_frame_dtypes(rng, k)
+    n_done = 0
+    for _ in range(10):
+        expr = exprgen.rand_expr(rng, dtypes, int(rng.integers(1, 5)), True, list(range(k)))
+        if expr.startswith("$"):
+            continue
+        used = sorted({int(m) for m in re.findall(r"\$(\d+)", expr)})
+        idx = [-1] * k
+        for slot, c in enumerate(used):
+            idx[c] = slot
+        num_outputs = int(rng.integers(1, 4))
+        pos = int(rng.integers(0, num_outputs))
+        want = P.Plan()
+        try:
+            P.add_expression_to_plan(want, expr, pos, num_outputs, len(used), idx)
+        except P.PlanError:
+            with pytest.raises(B.BlzError):
+                B.add_expression_to_plan(expr, pos, num_outputs, len(used), idx)
+            continue
+        got = B.add_expression_to_plan(expr, pos, num_outputs, len(used), idx)
+        for key in ("left", "right", "outputs", "binops", "unops"):
+            assert got[key] == getattr(want, key), (expr, key)
+        n_done += 1
+    assert n_done > 0
