@@ -93,6 +93,8 @@ def lib():
         sig = {
             "blz_perform_operation": [colpp, I, colpp, I, I16P, I16P, I16P, I, I16P, I32P, I32P,
                                       scp, scp, I, I32P, P, ERR, SZ],
+            "blz_perform_operation_noalloc": [colpp, I, colpp, I, I16P, I16P, I16P, I, I16P, I32P, I32P,
+                                              scp, scp, I, I32P, P, ERR, SZ],
             "blz_evaluate_expression": [colpp, I, C.c_char_p, colp, I, I32P, P, ERR, SZ],
             "blz_process_filter": [colpp, I, C.c_char_p, colp, C.POINTER(C.c_int64), I, P, ERR, SZ],
             "blz_execute_project": [colpp, I, C.c_char_p, colp, I32P, I, C.POINTER(C.c_int), I, P,
@@ -358,6 +360,24 @@ def perform_operation(outputs: Sequence[DeviceColumn], inputs: Sequence[DeviceCo
     fp = (C.c_int16 * max(len(final_positions), 1))(*final_positions)
     nulls = (C.c_int32 * max(len(outputs), 1))()
     _call(lib().blz_perform_operation, _colpp(outputs), len(outputs), _colpp(inputs), len(inputs),
+          I16(*left), I16(*right), I16(*outs), n, fp, I32(*operators), I32(*unary_operators),
+          SC(*left_scalars), SC(*right_scalars), scalar_mode, nulls, stream)
+    return list(nulls)[:len(outputs)]
+
+
+def perform_operation_noalloc(outputs: Sequence[DeviceColumn], inputs: Sequence[DeviceColumn],
+                              left, right, outs, final_positions, operators, unary_operators,
+                              left_scalars: Sequence[GdfScalar], right_scalars: Sequence[GdfScalar],
+                              scalar_mode=SCALAR_INTENDED, stream=None) -> List[int]:
+    """perform_operation without device allocation (the library's per-device scratch
+    holds the launch buffers). Returns per-output null counts."""
+    n = len(left)
+    I16 = C.c_int16 * max(n, 1)
+    I32 = C.c_int32 * max(n, 1)
+    SC = GdfScalar * max(n, 1)
+    fp = (C.c_int16 * max(len(final_positions), 1))(*final_positions)
+    nulls = (C.c_int32 * max(len(outputs), 1))()
+    _call(lib().blz_perform_operation_noalloc, _colpp(outputs), len(outputs), _colpp(inputs), len(inputs),
           I16(*left), I16(*right), I16(*outs), n, fp, I32(*operators), I32(*unary_operators),
           SC(*left_scalars), SC(*right_scalars), scalar_mode, nulls, stream)
     return list(nulls)[:len(outputs)]

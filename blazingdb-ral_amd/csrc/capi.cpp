@@ -294,6 +294,42 @@ RunResult run_once(const Compiled &c, bool compact, int64_t n, int n_out, hipStr
   return r;
 }
 
+// One execution of a projection program (no compaction) whose launch buffers are
+// the device's allocation-free scratch: no device allocation, blocking. Calls on
+// one device are serialised by the scratch's lock, so two never share it in flight.
+RunResult run_in_scratch(const Compiled &c, int64_t n, int n_out, hipStream_t st) {
+  RunResult r;
+  r.nulls.assign(n_out, 0);
+  uint8_t *base = nullptr;
+  size_t cap = 0;
+  std::mutex *mu = nullptr;
+  HIPCHECK(noalloc_scratch(&base, &cap, &mu));
+  LaunchBuffers lb;
+  layout(lb, c, n_out, false, n);
+  if (lb.bytes > cap)
+    invalid("the plan's launch buffers need " + std::to_string(lb.bytes) + " bytes; the allocation-free scratch holds " +
+            std::to_string(cap));
+  if (n == 0) return r;
+  std::lock_guard<std::mutex> g(*mu);
+  lb.base = base;
+  bind(lb, n_out);
+  std::vector<unsigned long long> host(n_out + 1, 0);
+  const std::vector<uint8_t> img = device_image(lb, c);
+  hipError_t e = hipMemcpyAsync(lb.blob, img.data(), img.size(), hipMemcpyHostToDevice, st);
+  // the null counts and counters start at zero on every call (the scratch keeps
+  // the last call's contents)
+  if (e == hipSuccess) e = launch_fill_zero(lb.base + lb.zero_off, (int64_t)lb.zero_bytes, 1, st);
+  if (e == hipSuccess) e = launch_program(c, false, interp_grid(c, false, n), st, lb.blob, lb.ws, lb.bd, lb.n_chunks);
+  if (e == hipSuccess && n_out > 0)
+    e = hipMemcpyAsync(host.data(), lb.ws.null_counts, sizeof(unsigned long long) * n_out, hipMemcpyDeviceToHost, st);
+  hipError_t sy = hipStreamSynchronize(st);
+  HIPCHECK(e);
+  HIPCHECK(sy);
+  for (int o = 0; o < n_out; o++) r.nulls[o] = (int32_t)host[o];
+  r.n_sel = n;
+  return r;
+}
+
 std::vector<int> frame_dtypes(gdf_column *const *frame, int n) {
   std::vector<int> d(n);
   for (int i = 0; i < n; i++) d[i] = frame[i]->dtype;
@@ -371,6 +407,65 @@ int blz_device_count(void) {
   return n;
 }
 
+}  // extern "C"
+
+namespace {
+// perform_operation's plan compiled for the given columns (throws on error).
+Compiled compile_register_plan(gdf_column *const *output_columns, int n_outputs, gdf_column *const *input_columns,
+                               int n_inputs, const column_index_type *left_inputs,
+                               const column_index_type *right_inputs, const column_index_type *outputs,
+                               int n_operations, const column_index_type *final_output_positions,
+                               const gdf_binary_operator *operators, const gdf_unary_operator *unary_operators,
+                               const gdf_scalar *left_scalars, const gdf_scalar *right_scalars, int scalar_mode) {
+  if (n_inputs <= 0) invalid("perform_operation needs at least one input column");
+  if (n_inputs > kMaxSlots) invalid("more than 64 input columns");
+  if (n_outputs < 0 || n_operations < 0) invalid("negative count");
+  const int64_t n = input_columns[0]->size;
+  for (int i = 0; i < n_inputs; i++) check_column(input_columns[i], "input column", n);
+  for (int i = 0; i < n_outputs; i++) check_column(output_columns[i], "output column", n);
+  PlanSpec p;
+  for (int c = 0; c < n_inputs; c++) p.in_dev.push_back(c);
+  p.left.assign(left_inputs, left_inputs + n_operations);
+  p.right.assign(right_inputs, right_inputs + n_operations);
+  p.outs.assign(outputs, outputs + n_operations);
+  p.final_pos.assign(final_output_positions, final_output_positions + n_outputs);
+  for (int k = 0; k < n_operations; k++) {
+    p.binops.push_back(operators[k]);
+    p.unops.push_back(unary_operators[k]);
+    p.lsc.push_back(left_scalars[k]);
+    p.rsc.push_back(right_scalars[k]);
+  }
+  CompileRequest req;
+  req.n_rows = n;
+  for (int c = 0; c < n_inputs; c++) req.inputs.push_back(dev_info(input_columns[c]));
+  req.proj = &p;
+  req.scalar_mode = scalar_mode;
+  for (int o = 0; o < n_outputs; o++)
+    req.outputs.push_back({output_columns[o]->data, output_columns[o]->valid, output_columns[o]->dtype, 0, o});
+  return compile_program(req);
+}
+}  // namespace
+
+extern "C" {
+
+int blz_perform_operation_noalloc(gdf_column *const *output_columns, int n_outputs,
+                                  gdf_column *const *input_columns, int n_inputs,
+                                  const column_index_type *left_inputs, const column_index_type *right_inputs,
+                                  const column_index_type *outputs, int n_operations,
+                                  const column_index_type *final_output_positions,
+                                  const gdf_binary_operator *operators, const gdf_unary_operator *unary_operators,
+                                  const gdf_scalar *left_scalars, const gdf_scalar *right_scalars, int scalar_mode,
+                                  int32_t *out_null_counts, blz_stream_t stream, char *err, size_t err_len) {
+  return guarded(err, err_len, [&] {
+    Compiled c = compile_register_plan(output_columns, n_outputs, input_columns, n_inputs, left_inputs, right_inputs,
+                                       outputs, n_operations, final_output_positions, operators, unary_operators,
+                                       left_scalars, right_scalars, scalar_mode);
+    RunResult r = run_in_scratch(c, input_columns[0]->size, n_outputs, S(stream));
+    if (out_null_counts)
+      for (int o = 0; o < n_outputs; o++) out_null_counts[o] = output_columns[o]->valid ? r.nulls[o] : 0;
+  });
+}
+
 int blz_perform_operation(gdf_column *const *output_columns, int n_outputs,
                           gdf_column *const *input_columns, int n_inputs,
                           const column_index_type *left_inputs, const column_index_type *right_inputs,
@@ -380,33 +475,10 @@ int blz_perform_operation(gdf_column *const *output_columns, int n_outputs,
                           const gdf_scalar *left_scalars, const gdf_scalar *right_scalars, int scalar_mode,
                           int32_t *out_null_counts, blz_stream_t stream, char *err, size_t err_len) {
   return guarded(err, err_len, [&] {
-    if (n_inputs <= 0) invalid("perform_operation needs at least one input column");
-    if (n_inputs > kMaxSlots) invalid("more than 64 input columns");
-    if (n_outputs < 0 || n_operations < 0) invalid("negative count");
-    const int64_t n = input_columns[0]->size;
-    for (int i = 0; i < n_inputs; i++) check_column(input_columns[i], "input column", n);
-    for (int i = 0; i < n_outputs; i++) check_column(output_columns[i], "output column", n);
-    PlanSpec p;
-    for (int c = 0; c < n_inputs; c++) p.in_dev.push_back(c);
-    p.left.assign(left_inputs, left_inputs + n_operations);
-    p.right.assign(right_inputs, right_inputs + n_operations);
-    p.outs.assign(outputs, outputs + n_operations);
-    p.final_pos.assign(final_output_positions, final_output_positions + n_outputs);
-    for (int k = 0; k < n_operations; k++) {
-      p.binops.push_back(operators[k]);
-      p.unops.push_back(unary_operators[k]);
-      p.lsc.push_back(left_scalars[k]);
-      p.rsc.push_back(right_scalars[k]);
-    }
-    CompileRequest req;
-    req.n_rows = n;
-    for (int c = 0; c < n_inputs; c++) req.inputs.push_back(dev_info(input_columns[c]));
-    req.proj = &p;
-    req.scalar_mode = scalar_mode;
-    for (int o = 0; o < n_outputs; o++)
-      req.outputs.push_back({output_columns[o]->data, output_columns[o]->valid, output_columns[o]->dtype, 0, o});
-    Compiled c = compile_program(req);
-    RunResult r = run_once(c, false, n, n_outputs, S(stream));
+    Compiled c = compile_register_plan(output_columns, n_outputs, input_columns, n_inputs, left_inputs, right_inputs,
+                                       outputs, n_operations, final_output_positions, operators, unary_operators,
+                                       left_scalars, right_scalars, scalar_mode);
+    RunResult r = run_once(c, false, input_columns[0]->size, n_outputs, S(stream));
     if (out_null_counts)
       for (int o = 0; o < n_outputs; o++) out_null_counts[o] = output_columns[o]->valid ? r.nulls[o] : 0;
   });

@@ -5,6 +5,8 @@
 #define BLZ_KERNELS_H_
 
 #include <hip/hip_runtime.h>
+
+#include <mutex>
 #include <stdint.h>
 
 namespace blz {
@@ -102,6 +104,12 @@ hipError_t launch_reduce(int op, const void *data, int load, const uint8_t *vali
 
 // dst[0..bytes) = src[0..bytes); bytes a multiple of 16, both 16-byte aligned.
 hipError_t launch_copy(void *dst, const void *src, int64_t bytes, int grid, hipStream_t st);
+
+// The allocation-free entry point's scratch on the current device: a file-scope
+// device array of *bytes bytes (each device holds its own copy until the process
+// ends), resolved to its device address once per device. *lock serialises the
+// calls that use it on one device.
+hipError_t noalloc_scratch(uint8_t **ptr, size_t *bytes, std::mutex **lock);
 
 int blob_capacity();   // bytes of LDS the program blob may occupy
 int block_threads();   // threads per workgroup of every kernel

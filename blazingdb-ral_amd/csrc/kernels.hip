@@ -32,6 +32,10 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <map>
+#include <mutex>
+#include <utility>
+
 #include "program.h"
 #include "kernels.h"
 #include "device_common.h"
@@ -724,6 +728,36 @@ hipError_t launch_reduce(int op, const void *data, int load, const uint8_t *vali
 hipError_t launch_copy(void *dst, const void *src, int64_t bytes, int grid, hipStream_t st) {
   hipLaunchKernelGGL(copy_kernel, dim3(grid), dim3(kBlock), 0, st, (uint4 *)dst, (const uint4 *)src, bytes / 16);
   return hipGetLastError();
+}
+
+// Scratch of the allocation-free entry point (blz_perform_operation_noalloc): the
+// launch buffers of one projection program (blob, launch descriptor, null counts,
+// counters) with nothing that grows with the rows; sized for the largest blob
+// (kLdsBlobBytes) plus descriptor, 64 null counts and counters.
+constexpr size_t kNoAllocScratchBytes = 16 << 10;
+static_assert(kNoAllocScratchBytes >= kLdsBlobBytes + 256 + 512 + 128, "no-alloc scratch too small");
+static __device__ __align__(256) uint8_t scratch[kNoAllocScratchBytes];
+
+hipError_t noalloc_scratch(uint8_t **ptr, size_t *bytes, std::mutex **lock) {
+  static std::mutex mu;
+  static std::map<int, std::pair<uint8_t *, std::mutex *>> per_device;
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  std::lock_guard<std::mutex> g(mu);
+  auto it = per_device.find(dev);
+  if (it == per_device.end()) {
+    // the array's name is a host-side stand-in: its device address on this device
+    // (the current one) is what kernels and copies take
+    void *p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    it = per_device.emplace(dev, std::make_pair((uint8_t *)p, new std::mutex())).first;
+  }
+  *ptr = it->second.first;
+  *bytes = kNoAllocScratchBytes;
+  *lock = it->second.second;
+  return hipSuccess;
 }
 
 int blob_capacity() { return kLdsBlobBytes; }

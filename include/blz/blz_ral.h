@@ -61,6 +61,27 @@ int blz_perform_operation(gdf_column *const *output_columns, int n_outputs,
                           int scalar_mode, int32_t *out_null_counts, blz_stream_t stream,
                           char *err, size_t err_len);
 
+/* As blz_perform_operation, allocating no device memory (no counterpart in the
+ * reference: for callers that cannot call hipMalloc, such as code under HIP graph
+ * capture, tight serving loops and harnesses that forbid it). The plan's launch
+ * buffers (program blob, launch descriptor, null counts, counters: at most 16 KiB,
+ * whatever the row count) live in a file-scope device array that each GPU holds
+ * until the process ends; a plan whose buffers would not fit gives BLZ_ERR_INVALID.
+ * Calls on one GPU share that array and keep the last call's contents: the library
+ * clears what must start at zero and serialises the calls on one GPU (each one
+ * blocks until its work is done), so they never use the array at the same time. */
+int blz_perform_operation_noalloc(gdf_column *const *output_columns, int n_outputs,
+                                  gdf_column *const *input_columns, int n_inputs,
+                                  const column_index_type *left_inputs,
+                                  const column_index_type *right_inputs,
+                                  const column_index_type *outputs, int n_operations,
+                                  const column_index_type *final_output_positions,
+                                  const gdf_binary_operator *operators,
+                                  const gdf_unary_operator *unary_operators,
+                                  const gdf_scalar *left_scalars, const gdf_scalar *right_scalars,
+                                  int scalar_mode, int32_t *out_null_counts, blz_stream_t stream,
+                                  char *err, size_t err_len);
+
 /* ---- expression level (host planner + kernels) ------------------------------- */
 
 /* evaluate_expression (src/LogicalFilter.h:20-23, LogicalFilter.cpp:271-371):
