@@ -110,6 +110,7 @@ def lib():
             "blz_program_create": [C.POINTER(P), colpp, I, C.c_char_p, C.c_char_p, colp, I,
                                    C.POINTER(C.c_int), I, I, I, I, I, ERR, SZ],
             "blz_program_tile": [P, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)],
+            "blz_program_predicate": [P] + [C.POINTER(C.c_int)] * 3,
             "blz_program_launch": [P, P, ERR, SZ],
             "blz_program_result": [P, P, C.POINTER(C.c_int64), I32P, ERR, SZ],
             "blz_program_time": [P, P, I, C.POINTER(C.c_double), ERR, SZ],
@@ -493,6 +494,10 @@ class Program:
         tr, lb, kk = C.c_int(0), C.c_int(0), C.c_int(0)
         lib().blz_program_tile(self.h, C.byref(tr), C.byref(lb), C.byref(kk))
         self.tile_rows, self.lds_bytes, self.kernel = tr.value, lb.value, kk.value
+        pd = [C.c_int(0) for _ in range(3)]
+        lib().blz_program_predicate(self.h, *[C.byref(x) for x in pd])
+        # register-load predicate: (terms, 8-byte term mask), or None
+        self.direct_pred = (pd[1].value, pd[2].value) if pd[0].value else None
         self.outputs = [DeviceColumn.adopt(out[i], owned=False) for i in range(n_out.value)]
 
     def launch(self, stream=None):

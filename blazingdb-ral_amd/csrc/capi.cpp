@@ -239,11 +239,12 @@ hipError_t launch_program(const Compiled &c, bool compact, int grid, hipStream_t
       e = launch_vm(c.R, 0, c.vm_slow, grid_for(0, c.lds_bytes, tiles), c.lds_bytes, st, bd);
       return e == hipSuccess ? mark(1) : e;
     }
-    if (c.vm_direct_pred)  // 64 workgroups per CU: more than stay resident, so CUs that
-                           // finish early take queued ones (0.315 -> 0.279 ms for C1's
-                           // predicate against 4 per CU; one per tile is slower again)
+    if (c.vm_direct_pred)  // 32 workgroups per CU: more than stay resident, so CUs that
+                           // finish early take queued ones (100M rows of three columns,
+                           // non-temporal loads: 0.264 / 0.259 / 0.262 / 0.267 / 0.274 ms
+                           // at 16 / 32 / 64 / 8 / 4 per CU)
       e = launch_pred_direct(bd, c.vm_direct_terms, c.vm_direct_wmask,
-                             (int)std::max<int64_t>(1, std::min<int64_t>(tiles, (int64_t)device_cus() * 64)), st);
+                             (int)std::max<int64_t>(1, std::min<int64_t>(tiles, (int64_t)device_cus() * 32)), st);
     else
       e = launch_vm(c.R, 1, c.vm_slow, grid_for(1, c.lds_bytes, tiles), c.lds_bytes, st, bd);
     if (e == hipSuccess) e = mark(1);
@@ -1207,6 +1208,15 @@ int blz_program_tile(blz_program prog, int *tile_rows, int *lds_bytes, int *kern
   if (tile_rows) *tile_rows = prog->c.T;
   if (lds_bytes) *lds_bytes = prog->c.lds_bytes;
   if (kernel) *kernel = prog->c.vm ? 4 : 3;
+  return BLZ_OK;
+}
+
+int blz_program_predicate(blz_program prog, int *direct, int *terms, int *wmask) {
+  if (!prog) return BLZ_ERR_INVALID;
+  const bool d = prog->compact && prog->c.vm && prog->c.vm_direct_pred;
+  if (direct) *direct = d ? 1 : 0;
+  if (terms) *terms = d ? prog->c.vm_direct_terms : 0;
+  if (wmask) *wmask = d ? prog->c.vm_direct_wmask : 0;
   return BLZ_OK;
 }
 

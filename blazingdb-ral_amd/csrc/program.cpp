@@ -224,7 +224,7 @@ void fail(const std::string &m) { throw std::runtime_error(m); }
 
 // Builds the SSA graph of one reference-format plan.
 PlanResult build_plan(Builder &B, const PlanSpec &p, const std::vector<DevInputInfo> &dev_in,
-                      int program, int scalar_mode) {
+                      int program, int scalar_mode, bool after_filter = false) {
   const int n_ops = (int)p.left.size();
   const int n_in = (int)p.in_dev.size();
   if ((int)p.right.size() != n_ops || (int)p.outs.size() != n_ops || (int)p.binops.size() != n_ops ||
@@ -246,8 +246,12 @@ PlanResult build_plan(Builder &B, const PlanSpec &p, const std::vector<DevInputI
     cur[c] = B.in_node(p.in_dev[c]);
     // process_valids input rule (interpreter_ops.cuh:1312-1327): forced all-valid
     // when there is no bitmap, null_count == 0, or the op with the column's index
-    // is IS_NULL / IS_NOT_NULL.
-    bool forced = d.valid == nullptr || d.null_count == 0 ||
+    // is IS_NULL / IS_NOT_NULL. A projection after a filter reads the filtered
+    // columns, whose null counts materialize_column recomputes from the kept rows'
+    // bits (ColumnManipulation.cu:105-113): there the bitmap counts even when the
+    // unfiltered column's null_count is 0 (forcing would change only rows whose
+    // bit is clear, and those make the recomputed count non-zero).
+    bool forced = d.valid == nullptr || (d.null_count == 0 && !after_filter) ||
                   (c < n_ops && (p.unops[c] == BLZ_IS_NULL || p.unops[c] == BLZ_IS_NOT_NULL));
     vcur[c] = forced ? V_ALL : B.vraw(p.in_dev[c]);
   }
@@ -1076,7 +1080,7 @@ Compiled compile_program(const CompileRequest &req) {
   }
   PlanResult pr, qr;
   if (req.compact && req.pred) pr = build_plan(B, *req.pred, req.inputs, 0, req.scalar_mode);
-  if (req.proj) qr = build_plan(B, *req.proj, req.inputs, 1, req.scalar_mode);
+  if (req.proj) qr = build_plan(B, *req.proj, req.inputs, 1, req.scalar_mode, req.compact);
 
   // Roots.
   int stencil_node = -1, stencil_v = V_ALL;

@@ -116,6 +116,15 @@ DEV uint64_t load_raw(int width, const void *base, int64_t i) {
     default: return gload<uint8_t>(base, i);
   }
 }
+// The same, non-temporal (rows read once: the gather of a filter's kept rows).
+DEV uint64_t load_raw_nt(int width, const void *base, int64_t i) {
+  switch (width) {
+    case 8: return __builtin_nontemporal_load((const GLOBAL uint64_t *)base + i);
+    case 4: return __builtin_nontemporal_load((const GLOBAL uint32_t *)base + i);
+    case 2: return __builtin_nontemporal_load((const GLOBAL uint16_t *)base + i);
+    default: return __builtin_nontemporal_load((const GLOBAL uint8_t *)base + i);
+  }
+}
 DEV void store_raw(int width, lds_ptr p, uint64_t v) {
   switch (width) {
     case 8: *(LDSAS uint64_t *)p = v; break;
@@ -139,7 +148,7 @@ DEV void stage_kept(const CONSTANT Piece *gcols, int ng, lds_ptr gb, int plrow, 
         const int w = gcols[g0 + j].width;
         const void *base = gcols[g0 + j].base;
 #pragma unroll
-        for (int u = 0; u < R; u++) v[j][u] = keep[u] ? load_raw(w, base, grow[u]) : 0;
+        for (int u = 0; u < R; u++) v[j][u] = keep[u] ? load_raw_nt(w, base, grow[u]) : 0;
       }
     }
 #pragma unroll
@@ -1099,12 +1108,13 @@ vm_kernel(const VmBatch *__restrict__ bd_arg) {
       live[u] = j0 + orel[u] < n_sel;
     }
 #pragma unroll
-    for (int u = 0; u < R; u++) grow[u] = !live[u] ? 0 : have_next ? (int64_t)nidx[u] : (int64_t)idx[j0 + orel[u]];
+    for (int u = 0; u < R; u++)
+      grow[u] = !live[u] ? 0 : have_next ? (int64_t)nidx[u] : (int64_t)__builtin_nontemporal_load(idx + j0 + orel[u]);
     have_next = ch + cstride < p_end;
     if (have_next) {
       const int64_t jn = j0 + cstride * CR;
 #pragma unroll
-      for (int u = 0; u < R; u++) nidx[u] = jn + orel[u] < n_sel ? idx[jn + orel[u]] : 0u;
+      for (int u = 0; u < R; u++) nidx[u] = jn + orel[u] < n_sel ? __builtin_nontemporal_load(idx + jn + orel[u]) : 0u;
     }
 #pragma unroll
     for (int u = 0; u < R; u++) acc[u] = 0;

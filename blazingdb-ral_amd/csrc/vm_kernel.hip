@@ -4,6 +4,8 @@
  * dispatch over rows per lane. The VM kernels themselves are in vm_device.h,
  * instantiated per rows-per-lane value by vm_r2.hip, vm_r4.hip and vm_r8.hip.
  */
+#include <type_traits>
+
 #include "vm_device.h"
 
 namespace blz {
@@ -139,6 +141,9 @@ slot_scan_kernel(const VmBatch *__restrict__ bd_arg) {
 // word s GW + g), so the slot scan and gather follow unchanged. The comparisons
 // are the tile VM's leaves.
 constexpr int kDirectTerms = 3;
+// Units per wave block: a wave takes this many consecutive units (100M rows of
+// three columns: 0.259 / 0.255 / 0.251 / 0.262 ms at 1 / 2 / 4 / 8).
+constexpr int kDirectBlock = 4;
 
 // 64-row groups per wave step: 16 when the terms read exactly 8 bytes per row,
 // else 8 (measured on 500M rows: one 4-byte column 0.39 ms at 8 against 0.42 at
@@ -232,22 +237,28 @@ __global__ void __launch_bounds__(kBlock) pred_direct_kernel(const VmBatch *__re
   const int64_t n_waves = ((int64_t)gridDim.x * kBlock) >> 6;
   // the wave's index made visibly uniform, so step bases live in scalar registers
   const int64_t wave = __builtin_amdgcn_readfirstlane((int)((blockIdx.x * kBlock + threadIdx.x) >> 6));
-  for (int64_t unit = wave; unit < n_units; unit += n_waves) {
-    uint64_t slot_cnt = 0;  // GW >= 8: the unit's one slot
+  // One unit's steps. CLAMP: the unit reaches past the column's last row (row
+  // indices are clamped; those rows are not kept); every other unit loads from a
+  // scalar base plus the lane offset and a constant per group. The two are separate
+  // loops, so the fast one keeps its immediate offsets. Every load is non-temporal:
+  // each row is read once (measured: 0.277 -> 0.259 ms on 100M rows of three
+  // columns).
+  auto run_unit = [&](auto clamp_tag, int64_t unit) {
+    constexpr bool CLAMP = decltype(clamp_tag)::value;
+    uint64_t slot_cnt = 0;  // GW >= RS: the unit's one slot
     for (int64_t g0 = unit * U; g0 < (unit + 1) * U && g0 < n_groups; g0 += RS) {
       uint64_t raw[NT][RS];
-      if ((g0 + RS) * 64 <= n) {  // uniform: every row of the step exists
-        // scalar base at the step's first row, lane offset, constant per group
+      if (!CLAMP) {
 #pragma unroll
         for (int k = 0; k < NT; k++) {
           if ((WMASK >> k) & 1) {
             const GLOBAL uint64_t *p = (const GLOBAL uint64_t *)col[k] + g0 * 64;
 #pragma unroll
-            for (int u = 0; u < RS; u++) raw[k][u] = p[u * 64 + lane];
+            for (int u = 0; u < RS; u++) raw[k][u] = __builtin_nontemporal_load(p + u * 64 + lane);
           } else {
             const GLOBAL uint32_t *p = (const GLOBAL uint32_t *)col[k] + g0 * 64;
 #pragma unroll
-            for (int u = 0; u < RS; u++) raw[k][u] = p[u * 64 + lane];
+            for (int u = 0; u < RS; u++) raw[k][u] = __builtin_nontemporal_load(p + u * 64 + lane);
           }
         }
       } else {
@@ -257,20 +268,22 @@ __global__ void __launch_bounds__(kBlock) pred_direct_kernel(const VmBatch *__re
           for (int u = 0; u < RS; u++) {
             const int64_t r = (g0 + u) * 64 + lane;
             const int64_t rc = r < n ? r : n - 1;  // loads stay in bounds; the row is not kept
-            raw[k][u] = ((WMASK >> k) & 1) ? gload<uint64_t>(col[k], rc) : (uint64_t)gload<uint32_t>(col[k], rc);
+            raw[k][u] = ((WMASK >> k) & 1)
+                            ? __builtin_nontemporal_load((const GLOBAL uint64_t *)col[k] + rc)
+                            : (uint64_t)__builtin_nontemporal_load((const GLOBAL uint32_t *)col[k] + rc);
           }
         }
       }
       bool keep[RS];
 #pragma unroll
-      for (int u = 0; u < RS; u++) keep[u] = (g0 + u) * 64 + lane < n;
+      for (int u = 0; u < RS; u++) keep[u] = CLAMP ? (g0 + u) * 64 + lane < n : true;
 #pragma unroll
       for (int k = 0; k < NT; k++) {
         if (valid[k] == nullptr) continue;  // uniform
 #pragma unroll
         for (int u = 0; u < RS; u++) {
           const int64_t r = (g0 + u) * 64 + lane;
-          keep[u] = keep[u] & valid_bit(valid[k], r < n ? r : n - 1);
+          keep[u] = keep[u] & valid_bit(valid[k], CLAMP && r >= n ? n - 1 : r);
         }
       }
 #pragma unroll
@@ -297,6 +310,15 @@ __global__ void __launch_bounds__(kBlock) pred_direct_kernel(const VmBatch *__re
       }
     }
     if (GW >= RS && lane == 0) gstore<uint32_t>(ws.wave_count, unit * U / GW, (uint32_t)slot_cnt);
+  };
+  // Units in blocks of kDirectBlock consecutive units per wave (blocks strided over
+  // the waves): a wave's loads and stores stay in consecutive lines.
+  const int64_t n_blocks = (n_units + kDirectBlock - 1) / kDirectBlock;
+  for (int64_t blk = wave; blk < n_blocks; blk += n_waves) {
+    for (int64_t unit = blk * kDirectBlock; unit < (blk + 1) * kDirectBlock && unit < n_units; unit++) {
+      if ((unit + 1) * U * 64 <= n) run_unit(std::false_type{}, unit);
+      else run_unit(std::true_type{}, unit);
+    }
   }
 }
 

@@ -208,6 +208,11 @@ int blz_program_info(blz_program prog, int *s, int *r, int *n_ops, int *n_vops, 
 /* Tile rows (0 = unstaged kernel), dynamic LDS bytes per workgroup and the kernel
  * kind chosen (1 accumulator, 2 staged register file, 3 unstaged). */
 int blz_program_tile(blz_program prog, int *tile_rows, int *lds_bytes, int *kernel);
+/* How a tile-VM filter evaluates its predicate: *direct = 1 when it reads the
+ * columns into registers (a conjunction of at most three column-versus-constant
+ * comparisons, validity an AND of column bitmaps), with *terms terms (*wmask bit k:
+ * term k reads an 8-byte column). */
+int blz_program_predicate(blz_program prog, int *direct, int *terms, int *wmask);
 int blz_program_destroy(blz_program prog);
 
 /* ---- batches (no reference counterpart: many independent queries, one launch) ----

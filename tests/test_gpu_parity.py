@@ -701,3 +701,57 @@ def test_bare_column_outputs_keep_their_bytes(B, kernel, mode):
         prog.close()
         for d in dev:
             d.free()
+
+
+DIRECT_CASES = [
+    # (predicate, projection): 4- and 8-byte terms, nullable and null_count-0 columns
+    ("AND(>=($3, 730), <($3, 1095), >=($2, 0.03), <=($2, 0.05), <($0, 30))",
+     "LogicalProject(E0=[*($1, -(1, $2))], E1=[+($0, 1)])"),
+    ("AND(>=($3, 100), <($3, 2000), >($2, 0.02))", "LogicalProject(D=[$3], C=[$2], B=[$1], E=[*($2, 2.0)])"),
+    ("<($4, 5000)", "LogicalProject(K=[$4], A=[$0])"),                            # nullable int64 term
+    ("AND(>($5, -50.0), <($0, 40))", "LogicalProject(F=[+($5, 1.0)], B=[$1])"),   # float32 + nullable
+    ("AND(<>($5, 0.0), =($0, 7), >=($4, 100))", "LogicalProject(A=[$0], K=[+($4, 1)])"),
+]
+
+
+@pytest.mark.parametrize("t", [0, 512, 2048, 4096])
+@pytest.mark.parametrize("case", range(len(DIRECT_CASES)))
+def test_register_load_predicate(B, case, t):
+    """Filters the register-load predicate runs (asserted through the program's
+    description): 4- and 8-byte terms, nullable columns and a bitmap whose
+    null_count is 0 (forced valid in the predicate; the projection of the filtered
+    rows reads its bits, as after the reference's materialize_column), tile rows from
+    512 to 4096 (a step spanning several slots, one slot, or a slot spanning several
+    steps) and row counts that are not a multiple of the tile, against the oracle."""
+    cond, proj = DIRECT_CASES[case]
+    filt = f"LogicalFilter(condition=[{cond}])"
+    for n in (1, 777, 70_001, 300_013):
+        rng = np.random.default_rng(8200 + case * 10 + n % 97)
+        host = c1_frame(n, seed=30 + case)
+        host.append(gen_column(rng, P.GDF_INT64, n, 0.1, 0, 10_000))     # $4: nullable int64
+        host.append(gen_column(rng, P.GDF_FLOAT32, n, 0.05, -60.0, 60.0))  # $5: nullable float32
+        host[0] = O.HostColumn(host[0].data, P.GDF_INT32, rng.random(n) > 0.02, 0)  # bitmap, null_count 0
+        _print_case(test="register_load_predicate", case=case, t=t, n=n)
+        ref = _oracle(host, filt, proj)
+        dev = [to_device(B, h) for h in host]
+        prog = B.Program(dev, filt, proj, t=t)
+        if t and prog.kernel != 4:
+            prog.close()
+            pytest.skip(f"{t}-row tiles do not fit this plan's staging")
+        assert prog.kernel == 4 and prog.direct_pred is not None, "not the register-load predicate"
+        if t:
+            assert prog.tile_rows == t
+        for rep in range(2):
+            prog.launch()
+            kept, nulls = prog.result()
+            assert kept == len(ref[0])
+            for g, r, nc in zip(prog.outputs, ref, nulls):
+                g.col.null_count = nc
+                assert_column_equal(g, r, what=f"case {case} t={t} n={n} rep={rep}")
+        prog.close()
+        outs, kept = B.filter_project(dev, filt, proj)  # the drop-in call: the same bytes
+        assert kept == len(ref[0])
+        for g, r in zip(outs, ref):
+            assert g.data().tobytes() == r.data.tobytes()
+        for c in outs + dev:
+            c.free()
