@@ -22,6 +22,8 @@ rocprofv3 FETCH_SIZE / WRITE_SIZE summary in profiles/. cpu_baseline times the
 scalar oracle on a 10M-row sample of the same workload on this host.
 
 python bench.py [--gpus 1] [--steps K] [--warmup W] [--rows N] [--calibrate]
+python bench.py --workload wide-expression   (100M rows x 8 nullable columns, the 24-op
+    four-output projection; its own JSON line)
 python bench.py --workload filter-gather [--selectivity S]   (500M-row filter + gather:
     int32 key U[0, 2^31), 3 int64 sequential and 3 float64 U[0, 1) payload columns,
     key < S * 2^31, the payload compacted in row order; its own JSON line)
@@ -46,6 +48,7 @@ FILTER = ("LogicalFilter(condition=[AND(>=($3, 730), <($3, 1095), >=($2, 0.03), 
 PROJECT = "LogicalProject(EXPR$0=[*($1, -(1, $2))], EXPR$1=[+($0, 1)])"
 PROFILE = os.path.join(ROOT, "profiles", "vm_kernel_c1_filter_project_100M.txt")
 PROFILE_FG = os.path.join(ROOT, "profiles", "vm_kernel_filter_gather_500M_s0.9.txt")
+PROFILE_WIDE = os.path.join(ROOT, "profiles", "vm_kernel_wide_expression_100M.txt")
 STAGES = ["predicate", "slot_scan", "gather"]
 
 
@@ -310,6 +313,62 @@ def run_gather(B, n=100_000_000, iters=5):
             "materialize_column_loop": {"ms": loop, "GBps": by / (loop * 1e-3) / 1e9}}
 
 
+def run_wide_expression(B, args):
+    """Wide-expression evaluation: 100M rows x 8 nullable columns (generated on the
+    device), the 24-op four-output projection, one launch of the tile VM per step;
+    one JSON line. value = algorithmic bytes (the referenced input columns and their
+    bitmaps read once, the outputs and their bitmaps written) per step over the
+    step's wall time."""
+    from blz_ral import workloads as W
+    n = args.rows
+    print(f"[bench] generating {n} rows x 8 columns on the device", file=sys.stderr, flush=True)
+    frame = []
+    for i, (dt, dist, a, b) in enumerate(W.WIDE_COLUMNS):
+        c = B.DeviceColumn(dt, n, with_valid=True)
+        B.synth_column(c, 7001 + i, dist, a, b, W.WIDE_NULL_RATE)
+        B.update_null_count(c)
+        frame.append(c)
+    prog = B.Program(frame, None, W.WIDE_PROJECT)
+    for _ in range(args.warmup):
+        prog.launch()
+    prog.result()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        prog.launch()
+    prog.result()
+    dt = (time.perf_counter() - t0) / args.steps
+    kernel_ms = prog.time(iters=max(10, args.steps // 2))
+    width = lambda d: np.dtype(B.NP_OF[d]).itemsize
+    used = sorted({int(m) for m in re.findall(r"\$(\d+)", W.WIDE_PROJECT)})
+    bytes_in = sum(n * width(frame[i].col.dtype) + (n + 7) // 8 for i in used)
+    bytes_out = sum(n * width(o.col.dtype) + (n + 7) // 8 for o in prog.outputs)
+    bytes_step = bytes_in + bytes_out
+    kernel_name = f"vm_kernel<{prog.r}, 0,"
+    out = {
+        "metric": "wide-expression GB/s at 100M rows x 8 columns", "value": bytes_step / dt / 1e9, "unit": "GB/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": dt * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "int64/f64",
+        "data": "synthetic",
+        "config": {"workload": "wide expression: 2 i32, 2 i64, 2 f32, 2 f64 columns, 5% nulls, 24-op plan, "
+                               "4 outputs", "rows": n, "ops": prog.n_ops, "algorithmic_bytes_per_step": bytes_step,
+                   "program": f"tile VM R={prog.r} tile={prog.tile_rows} rows"},
+        "roofline": {"bound": "hbm", "achieved": bytes_step / (kernel_ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": bytes_step / (kernel_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                     "traffic": traffic_from_profile(kernel_name, PROFILE_WIDE), "kernel": kernel_name,
+                     "kernel_ms": kernel_ms, "algorithmic_bytes": bytes_step},
+    }
+    prog.close()
+    for c in frame:
+        c.free()
+    if args.calibrate:
+        buf_a = B.DeviceBuffer(1 << 30)
+        buf_b = B.DeviceBuffer(1 << 30)
+        for _ in range(3):
+            B._check(B.lib().blz_device_copy(buf_b.ptr, buf_a.ptr, 1 << 30, None), "copy")
+        print("[bench] calibration copy: 3 x 1 GiB", file=sys.stderr, flush=True)
+    print(json.dumps(out))
+
+
 def run_filter_gather(B, args):
     """500M-row filter + gather: one JSON line, value = (52 N + 48 kept) bytes per step
     over the step's wall time (inputs generated on the device, resident); roofline
@@ -342,7 +401,8 @@ def run_filter_gather(B, args):
     tiles = (n + prog.tile_rows - 1) // prog.tile_rows
     sbytes = [4 * n + n / 8 + 16 * tiles, 16 * tiles + n / 8 + 4 * kept, 4 * kept + 48 * kept + 48 * kept]
     dom = int(np.argmax(stage_ms))
-    kernel_name = (f"vm_kernel<{prog.r}, {1 if dom == 0 else 2}," if dom != 1 else "slot_scan_kernel")
+    pred_name = "pred_direct_kernel" if prog.direct_pred else f"vm_kernel<{prog.r}, 1,"
+    kernel_name = ((pred_name if dom == 0 else f"vm_kernel<{prog.r}, 2,") if dom != 1 else "slot_scan_kernel")
     achieved = sbytes[dom] / (stage_ms[dom] * 1e-3) / 1e9
     bytes_step = 52 * n + 48 * kept
     prog.close()
@@ -385,7 +445,8 @@ def main():
     ap.add_argument("--batch-plans", type=int, default=256)
     ap.add_argument("--serve-gb", type=float, default=100.0)
     ap.add_argument("--serve-seconds", type=float, default=1.0)
-    ap.add_argument("--workload", choices=["filter-project", "filter-gather"], default="filter-project")
+    ap.add_argument("--workload", choices=["filter-project", "filter-gather", "wide-expression"],
+                    default="filter-project")
     ap.add_argument("--selectivity", type=float, default=0.9, help="filter-gather: fraction of rows kept")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -397,6 +458,9 @@ def main():
         raise SystemExit("no HIP device visible")
     if args.workload == "filter-gather":
         run_filter_gather(B, args)
+        return
+    if args.workload == "wide-expression":
+        run_wide_expression(B, args)
         return
 
     n = args.rows

@@ -23,6 +23,20 @@ COLUMN_KINDS = {
     GDF_FLOAT32: (2, 0.0, 1.0),
     GDF_FLOAT64: (3, 0.0, 2.0),
 }
+# The wide-expression workload: 8 columns (2 int32 uniform [-1e6, 1e6], 2 int64
+# uniform [0, 2^40], 2 float32 normal(0, 1), 2 float64 lognormal(0, 2)), 5 % nulls
+# each, as (dtype, blz_synth_column distribution, a, b); and one 24-op projection
+# with MOD, CAST int64 -> float64, comparisons, AND / OR / NOT and a nested
+# conditional (CASE -> COALESCE), four outputs.
+WIDE_COLUMNS = [(GDF_INT32, 0, -1e6, 1e6), (GDF_INT32, 0, -1e6, 1e6), (GDF_INT64, 0, 0, 2.0 ** 40),
+                (GDF_INT64, 0, 0, 2.0 ** 40), (GDF_FLOAT32, 2, 0.0, 1.0), (GDF_FLOAT32, 2, 0.0, 1.0),
+                (GDF_FLOAT64, 3, 0.0, 2.0), (GDF_FLOAT64, 3, 0.0, 2.0)]
+WIDE_NULL_RATE = 0.05
+WIDE_PROJECT = ("LogicalProject(E0=[+(*($0, 3), MOD(-($1, $0), 7))], "
+                "E1=[CASE(IS NOT NULL($6), $6, +(BL_CAST_DOUBLE($2), -($7, 1.0)))], "
+                "E2=[AND(<($4, 0.5), OR(>($5, -0.5), NOT(=($0, $1))), <>($2, 0))], "
+                "E3=[-(*(+($2, MOD($3, 1000)), 2), /(*($6, $7), +(*($4, $5), 1.5)))])")
+
 ARITH = ["+", "-", "*"]
 CMP = ["=", "<>", "<", ">", "<=", ">="]
 

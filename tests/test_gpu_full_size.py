@@ -50,10 +50,12 @@ def _host(c):
 # 2 float32 normal(0, 1), 2 float64 lognormal(0, 2)), 5 % nulls each; one 24-op
 # plan with MOD, CAST int64 -> float64, comparisons, AND / OR / NOT and a nested
 # conditional (CASE -> COALESCE); four outputs.
-C2_PROJECT = ("LogicalProject(E0=[+(*($0, 3), MOD(-($1, $0), 7))], "
-              "E1=[CASE(IS NOT NULL($6), $6, +(BL_CAST_DOUBLE($2), -($7, 1.0)))], "
-              "E2=[AND(<($4, 0.5), OR(>($5, -0.5), NOT(=($0, $1))), <>($2, 0))], "
-              "E3=[-(*(+($2, MOD($3, 1000)), 2), /(*($6, $7), +(*($4, $5), 1.5)))])")
+def _c2_project():
+    from blz_ral import workloads as W
+    return W.WIDE_PROJECT
+
+
+C2_PROJECT = _c2_project()
 
 
 def test_wide_expression_100M_rows(B):
