@@ -557,6 +557,77 @@ void assign_leaf(VmOp &op, int R) {
   op.leaf = hc >= 0 && ho >= 0 ? ho * HC_COUNT_ + hc : kLeafGeneral;
 }
 
+// Folds the accumulator moves of an op stream into the ops beside them (kFusePre,
+// kFuseStore, kFuseEmit): a LOAD of a temporary or a tile column into the op after
+// it, a STORE or an EMIT into the op before it. A LOAD of an integer column followed
+// by I2F becomes one LOAD of the column widened to binary64 first. The values every
+// op computes are unchanged; only the number of dispatches falls.
+std::vector<VmOp> fuse_moves(const std::vector<VmOp> &in) {
+  auto target = [](const VmOp &o) {
+    const int l = o.leaf & kLeafMask;
+    return l < kHotLeaves || (l >= kLeafI2F && l <= kLeafModK);
+  };
+  auto is_load = [](const VmOp &o) {
+    const int l = o.leaf;  // no flags yet
+    return l < kHotLeaves && l / HC_COUNT_ == HO_LOAD && l % HC_COUNT_ != HC_IMM;
+  };
+  // LOAD of an integer column, then I2F: one LOAD of the widened column
+  std::vector<VmOp> a;
+  for (size_t k = 0; k < in.size(); k++) {
+    VmOp op = in[k];
+    if (is_load(op) && k + 1 < in.size() && in[k + 1].leaf == kLeafI2F) {
+      const int hc = op.leaf % HC_COUNT_;
+      if (hc == HC_L64 || hc == HC_L32I) {
+        op.leaf = HO_LOAD * HC_COUNT_ + (hc == HC_L64 ? HC_L64_F : HC_L32I_F);
+        op.cv = CV_I2F;
+        a.push_back(op);
+        k++;
+        continue;
+      }
+    }
+    a.push_back(op);
+  }
+  // LOAD into the next op
+  std::vector<VmOp> b;
+  for (size_t k = 0; k < a.size(); k++) {
+    const VmOp &op = a[k];
+    if (is_load(op) && k + 1 < a.size() && target(a[k + 1]) && (uint32_t)op.off < (1u << 20)) {
+      VmOp nx = a[k + 1];
+      nx.leaf |= kFusePre;
+      const uint32_t w = (uint32_t)op.off | ((uint32_t)(op.leaf % HC_COUNT_) << 20);
+      std::memcpy(&nx.code, &w, 4);  // code, arg, cls, cv
+      b.push_back(nx);
+      k++;
+      continue;
+    }
+    b.push_back(op);
+  }
+  // STORE / EMIT into the previous op
+  std::vector<VmOp> c;
+  for (const VmOp &op : b) {
+    if (!c.empty() && target(c.back())) {
+      VmOp &pv = c.back();
+      if (op.leaf == kLeafStore && !(pv.leaf & kFuseStore) && op.arg < 256) {
+        uint32_t w;
+        std::memcpy(&w, &pv.code, 4);
+        if (!(pv.leaf & kFusePre)) w = 0;
+        w = (w & 0x00FFFFFFu) | ((uint32_t)op.arg << 24);
+        std::memcpy(&pv.code, &w, 4);
+        pv.leaf |= kFuseStore;
+        continue;
+      }
+      if (op.leaf == kLeafEmit && !(pv.leaf & kFuseEmit)) {
+        pv.vcol = op.arg;   // gen word bits 8-15: output
+        pv.arg2 = op.gen;   // bits 16-23: StoreKind
+        pv.leaf |= kFuseEmit;
+        continue;
+      }
+    }
+    c.push_back(op);
+  }
+  return c;
+}
+
 // Conjunctions of comparisons (filters such as a date range, a discount range and
 // a quantity bound): a predicate opening with LOAD x; CMP imm opens with
 // CMPAND(x CMP imm) instead (the predicate kernel starts with acc = 1), the
@@ -906,6 +977,9 @@ bool compile_vm(const CompileRequest &req, Builder &B, const std::vector<AccOp> 
     const int cl = vm_p[k].leaf % HC_COUNT_;
     if (cl == HC_L64 || cl == HC_L64_F) out.vm_direct_wmask |= 1 << k;
   }
+  // the register-load predicate reads its terms as they are
+  if (!direct) vm_p = fuse_moves(vm_p);
+  vm_q = fuse_moves(vm_q);
 
   std::vector<Piece> pieces;
   for (int c = 0; c < n_dev; c++) {
@@ -929,7 +1003,8 @@ bool compile_vm(const CompileRequest &req, Builder &B, const std::vector<AccOp> 
   const size_t off_in = al(sizeof(ProgHeader));
   const size_t off_out = al(off_in + sizeof(InCol) * n_dev);
   const size_t off_ops = al(off_out + sizeof(OutCol) * n_out);
-  const size_t off_pieces = al(off_ops + sizeof(VmOp) * (vm_p.size() + vm_q.size()));
+  // one descriptor past the last op: the kernels load op k + 1 while op k runs
+  const size_t off_pieces = al(off_ops + sizeof(VmOp) * (vm_p.size() + vm_q.size() + 1));
   // bare-column outputs (no ops: the projection / gather kernel copies them)
   std::vector<CopyCol> copies;
   for (int o = 0; o < n_out; o++) {

@@ -175,6 +175,20 @@ constexpr int kLeafNotF = kHotLeaves + 4;   // acc = !(binary64 acc != 0)
 constexpr int kLeafDivK = kHotLeaves + 5;   // acc = acc / imm (int64, imm not in {-1, 0, 1}): by
 constexpr int kLeafModK = kHotLeaves + 6;   // acc = acc % imm   the magic multiplier imm2, shift gen
 constexpr int kLeafGeneral = kHotLeaves + 7;
+// Fused moves. A hot leaf or one of kLeafI2F .. kLeafModK may carry, in bits 16-18
+// of its leaf word, the accumulator moves around it that would otherwise be ops of
+// their own (each op costs one dispatch, whatever it does):
+//   kFusePre    before the op, acc = operand of hot class (code word bits 20-23) at
+//               byte offset (bits 0-19), as HO_LOAD does;
+//   kFuseStore  after it, temporary (code word bits 24-31) = acc, as kLeafStore;
+//   kFuseEmit   after it, output (gen word bits 8-15) <- acc with StoreKind (bits
+//               16-23), as kLeafEmit.
+// The code word (code, arg, cls, cv) is not read by these leaves; it then holds the
+// fused fields.
+constexpr int kLeafMask = 0xFFFF;
+constexpr int kFusePre = 1 << 16;
+constexpr int kFuseStore = 1 << 17;
+constexpr int kFuseEmit = 1 << 18;
 struct VmOp {
   int32_t leaf;   // dispatch id (set once the tile shape is known)
   int32_t off;    // VC_L*: byte offset of the column slice in the tile buffer;

@@ -448,7 +448,9 @@ DEV void run_rare(int code, int arg, uint32_t w3, uint64_t (&acc)[R], const uint
 // it is used, so no operand value crosses a join and the accumulator is updated in
 // place. The rest decode their fields in the general leaf. Scalar work per op is
 // the switch's compare tree; the next op's descriptor is loaded while this one
-// executes.
+// executes. Accumulator moves (a load before an op, a store or an output after
+// it) ride on the op beside them as flags (kFusePre, kFuseStore, kFuseEmit), so
+// they cost no dispatch of their own.
 //
 // OPS selects what the general leaf can run: 0 the common set (any operand class
 // and conversion of the common ops); 1 adds the rest of the hot set (division,
@@ -516,6 +518,36 @@ DEV void run_rare(int code, int arg, uint32_t w3, uint64_t (&acc)[R], const uint
   BLZ_LEAF(OP, HC_L32I_F, BLZ_B_L32I_F, STMT)           \
   BLZ_LEAF(OP, HC_L64_F, BLZ_B_L64_F, STMT)
 
+// Output `arg` <- acc (StoreKind `kind`): rows u with live[u] at obase + orel[u].
+template <int R>
+DEV void vm_emit(const CONSTANT OutCol *out, int arg, int kind, const uint64_t (&acc)[R], int64_t obase,
+                 const int (&orel)[R], const bool (&live)[R]) {
+  void *base = out[arg].data;
+  switch (kind) {
+    case ST_I64: case ST_F64:
+#pragma unroll
+      for (int u = 0; u < R; u++) if (live[u]) gstore<uint64_t>(base, obase + orel[u], acc[u]);
+      break;
+    case ST_I32:
+#pragma unroll
+      for (int u = 0; u < R; u++) if (live[u]) gstore<int32_t>(base, obase + orel[u], (int32_t)acc[u]);
+      break;
+    case ST_F32:
+#pragma unroll
+      for (int u = 0; u < R; u++) if (live[u]) gstore<float>(base, obase + orel[u], (float)b2d(acc[u]));
+      break;
+    case ST_I16:
+#pragma unroll
+      for (int u = 0; u < R; u++) if (live[u]) gstore<int16_t>(base, obase + orel[u], (int16_t)acc[u]);
+      break;
+    case ST_I8:
+#pragma unroll
+      for (int u = 0; u < R; u++) if (live[u]) gstore<int8_t>(base, obase + orel[u], (int8_t)acc[u]);
+      break;
+    default: break;
+  }
+}
+
 template <int R, int OPS>
 DEV void run_vm(const CONSTANT VmOp *ops, int nops, uint64_t (&acc)[R], lds_ptr tmpb, int tstride,
                 const CONSTANT InCol *in, const CONSTANT OutCol *out, clds_ptr tb, int lrow,
@@ -524,13 +556,46 @@ DEV void run_vm(const CONSTANT VmOp *ops, int nops, uint64_t (&acc)[R], lds_ptr 
   typedef uint32_t u32x8 __attribute__((ext_vector_type(8)));
   static_assert(sizeof(VmOp) == sizeof(u32x8), "VmOp is one 32-byte descriptor");
   const CONSTANT u32x8 *opw = (const CONSTANT u32x8 *)ops;
-  u32x8 w = nops > 0 ? opw[0] : u32x8{0u, 0u, 0u, 0u, 0u, 0u, 0u, 0u};
+  // the op list is followed by one spare descriptor, so op k + 1 is always loadable
+  u32x8 w = opw[0];
   clds_ptr b4 = tb + 4 * lrow;  // this lane's row in a 4- / 8-byte column slice
   clds_ptr b8 = tb + 8 * lrow;
   for (int k = 0; k < nops; k++) {
-    const u32x8 w_next = opw[k + 1 < nops ? k + 1 : k];
-    const int leaf = (int)w.s0;
+    const u32x8 w_next = opw[k + 1];
+    const uint32_t s0 = w.s0;
+    const int leaf = (int)(s0 & kLeafMask);
     const int off = (int)w.s1;
+    if (s0 & kFusePre) {  // acc = operand (as HO_LOAD), then the op
+      const int poff = (int)(w.s2 & 0xFFFFF);
+      clds_ptr tp = (clds_ptr)tmpb + poff;
+      clds_ptr lp4 = b4 + poff, lp8 = b8 + poff;
+      switch ((w.s2 >> 20) & 0xF) {
+        case HC_TMP:
+#pragma unroll
+          for (int u = 0; u < R; u++) acc[u] = BLZ_B_TMP;
+          break;
+        case HC_L64:
+#pragma unroll
+          for (int u = 0; u < R; u++) acc[u] = BLZ_B_L64;
+          break;
+        case HC_L32I:
+#pragma unroll
+          for (int u = 0; u < R; u++) acc[u] = BLZ_B_L32I;
+          break;
+        case HC_L32F:
+#pragma unroll
+          for (int u = 0; u < R; u++) acc[u] = BLZ_B_L32F;
+          break;
+        case HC_L32I_F:
+#pragma unroll
+          for (int u = 0; u < R; u++) acc[u] = BLZ_B_L32I_F;
+          break;
+        default:  // HC_L64_F
+#pragma unroll
+          for (int u = 0; u < R; u++) acc[u] = BLZ_B_L64_F;
+          break;
+      }
+    }
     const uint64_t im = (uint64_t)w.s4 | ((uint64_t)w.s5 << 32);
     const uint64_t im2 = (uint64_t)w.s6 | ((uint64_t)w.s7 << 32);
     const double fi = b2d(im2), hf = b2d(im);
@@ -545,33 +610,9 @@ DEV void run_vm(const CONSTANT VmOp *ops, int nops, uint64_t (&acc)[R], lds_ptr 
 #pragma unroll
         for (int u = 0; u < R; u++) *(LDSAS uint64_t *)(p + 512 * u) = acc[u];
       } break;
-      case kLeafEmit: {
-        const int arg = (int)((w.s2 >> 8) & 0xFF);
-        void *base = out[arg].data;
-        switch ((int)(w.s3 & 0xFF)) {  // StoreKind
-          case ST_I64: case ST_F64:
-#pragma unroll
-            for (int u = 0; u < R; u++) if (live[u]) gstore<uint64_t>(base, obase + orel[u], acc[u]);
-            break;
-          case ST_I32:
-#pragma unroll
-            for (int u = 0; u < R; u++) if (live[u]) gstore<int32_t>(base, obase + orel[u], (int32_t)acc[u]);
-            break;
-          case ST_F32:
-#pragma unroll
-            for (int u = 0; u < R; u++) if (live[u]) gstore<float>(base, obase + orel[u], (float)b2d(acc[u]));
-            break;
-          case ST_I16:
-#pragma unroll
-            for (int u = 0; u < R; u++) if (live[u]) gstore<int16_t>(base, obase + orel[u], (int16_t)acc[u]);
-            break;
-          case ST_I8:
-#pragma unroll
-            for (int u = 0; u < R; u++) if (live[u]) gstore<int8_t>(base, obase + orel[u], (int8_t)acc[u]);
-            break;
-          default: break;
-        }
-      } break;
+      case kLeafEmit:
+        vm_emit<R>(out, (int)((w.s2 >> 8) & 0xFF), (int)(w.s3 & 0xFF), acc, obase, orel, live);
+        break;
       case kLeafI2F:
 #pragma unroll
         for (int u = 0; u < R; u++) acc[u] = d2b((double)(int64_t)acc[u]);
@@ -678,6 +719,14 @@ DEV void run_vm(const CONSTANT VmOp *ops, int nops, uint64_t (&acc)[R], lds_ptr 
             break;
         }
       } break;
+    }
+    if (s0 & (kFuseStore | kFuseEmit)) {
+      if (s0 & kFuseStore) {
+        lds_ptr p = tmpb + (int)(w.s2 >> 24) * tstride;
+#pragma unroll
+        for (int u = 0; u < R; u++) *(LDSAS uint64_t *)(p + 512 * u) = acc[u];
+      }
+      if (s0 & kFuseEmit) vm_emit<R>(out, (int)((w.s3 >> 8) & 0xFF), (int)((w.s3 >> 16) & 0xFF), acc, obase, orel, live);
     }
     w = w_next;
   }
