@@ -132,6 +132,22 @@ def lib():
                                   C.POINTER(C.c_int64), C.POINTER(C.c_double), C.POINTER(C.c_int64), ERR, SZ],
             "blz_server_stats": [P, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(SZ)],
             "blz_server_destroy": [P],
+            "blz_server_submit_token": [P, C.POINTER(BlzQuery), C.c_uint64, ERR, SZ],
+            "blz_csv_read": [P, SZ, C.c_char, C.c_char, I, I32P, I, I32P, I, colp, C.POINTER(C.c_int64), P, ERR,
+                             SZ],
+            "blz_csv_parse_device": [P, SZ, C.c_char, C.c_char, I, I32P, I, I32P, I, colp,
+                                     C.POINTER(C.c_int64), P, ERR, SZ],
+            "blz_csv_load_files": [C.POINTER(C.c_char_p), I, C.c_char, C.c_char, I, I32P, I, I32P, I, colp,
+                                   C.POINTER(C.c_int64), P, ERR, SZ],
+            "blz_rs_init_session": [C.POINTER(C.c_uint64), ERR, SZ],
+            "blz_rs_register_query": [C.c_uint64, C.c_uint64, ERR, SZ],
+            "blz_rs_update_token": [C.c_uint64, colp, C.POINTER(C.c_char_p), I, C.c_double, C.c_char_p, ERR, SZ],
+            "blz_rs_get_result": [C.c_uint64, C.c_uint64, colp, C.POINTER(C.c_uint64), I, C.POINTER(C.c_int),
+                                  C.POINTER(C.c_double), C.c_char_p, SZ, ERR, SZ],
+            "blz_rs_column_name": [C.c_uint64, I, C.c_char_p, SZ],
+            "blz_rs_try_free_result": [C.c_uint64, C.c_uint64, C.POINTER(C.c_int), ERR, SZ],
+            "blz_rs_remove_all_connection_tokens": [C.c_uint64],
+            "blz_rs_get_column": [C.c_uint64, C.c_uint64, colp, ERR, SZ],
             "blz_synth_column": [colp, C.c_uint64, I, C.c_double, C.c_double, C.c_double, P, ERR, SZ],
             "blz_device_malloc": [C.POINTER(P), SZ],
             "blz_device_free": [P],
@@ -713,6 +729,14 @@ class Server:
             return np.array(lat[:]), np.array(kept[:]), wall.value, list(tickets)
         return np.array(lat[:]), np.array(kept[:]), wall.value
 
+    def submit_token(self, frame, filter_query_part, project_query_part, token, scalar_mode=SCALAR_INTENDED):
+        """evaluate_query's asynchronous form: the query's outputs are parked in the
+        result repository under `token` (registered) once it has run."""
+        keep = []
+        arr = _queries([(frame, filter_query_part, project_query_part)], scalar_mode, keep)
+        _call(lib().blz_server_submit_token, self.h, arr, C.c_uint64(token))
+        self._keep[("token", token)] = (keep, list(frame))
+
     def stats(self):
         nb, pk, used = C.c_int64(0), C.c_int64(0), C.c_size_t(0)
         lib().blz_server_stats(self.h, C.byref(nb), C.byref(pk), C.byref(used))
@@ -728,6 +752,131 @@ class Server:
             self.close()
         except Exception:
             pass
+
+
+def _csv_args(dtypes, column_indices):
+    dt = (C.c_int32 * len(dtypes))(*dtypes)
+    if column_indices is None:
+        return dt, None, len(dtypes), 0
+    ci = (C.c_int32 * max(len(column_indices), 1))(*column_indices)
+    return dt, ci, len(column_indices), len(column_indices)
+
+
+def read_csv(data, dtypes, column_indices=None, delimiter="|", line_terminator="\n", skip_rows=0,
+             stream=None, on_device=False):
+    """csv_parser::parse over `data` (bytes in host memory, or with on_device a
+    DeviceBuffer / device address and `data_len`): ([DeviceColumn], rows)."""
+    dt, ci, n_out, n_idx = _csv_args(dtypes, column_indices)
+    out = (GdfColumn * max(n_out, 1))()
+    rows = C.c_int64(0)
+    if on_device:
+        ptr, nbytes = data
+        _call(lib().blz_csv_parse_device, ptr, nbytes, delimiter.encode(), line_terminator.encode(), skip_rows,
+              dt, len(dtypes), ci, n_idx, out, C.byref(rows), stream)
+    else:
+        buf = bytes(data)
+        _call(lib().blz_csv_read, buf, len(buf), delimiter.encode(), line_terminator.encode(), skip_rows, dt,
+              len(dtypes), ci, n_idx, out, C.byref(rows), stream)
+    return [DeviceColumn.adopt(out[i], owned=True) for i in range(n_out)], rows.value
+
+
+def load_csv_files(paths, dtypes, column_indices=None, delimiter="|", line_terminator="\n", skip_rows=0,
+                   stream=None):
+    """data_loader::load_data: every file parsed, columns concatenated in file order."""
+    dt, ci, n_out, n_idx = _csv_args(dtypes, column_indices)
+    ps = (C.c_char_p * len(paths))(*[p.encode() for p in paths])
+    out = (GdfColumn * max(n_out, 1))()
+    rows = C.c_int64(0)
+    _call(lib().blz_csv_load_files, ps, len(paths), delimiter.encode(), line_terminator.encode(), skip_rows, dt,
+          len(dtypes), ci, n_idx, out, C.byref(rows), stream)
+    return [DeviceColumn.adopt(out[i], owned=True) for i in range(n_out)], rows.value
+
+
+class ResultSet:
+    """result_set_t (ResultSetRepository.h:19-25) as get_result returns it: the
+    parked columns (views of the repository's buffers), their names and column
+    tokens, the duration (ms) and the error text ("" on success)."""
+
+    def __init__(self, columns, names, column_tokens, duration, error_msg):
+        self.columns = columns
+        self.names = names
+        self.column_tokens = column_tokens
+        self.duration = duration
+        self.errorMsg = error_msg
+
+    @property
+    def is_ready(self):
+        return True
+
+
+class ResultSetRepository:
+    """result_set_repository (src/ResultSetRepository.cpp): the process-wide
+    repository of asynchronous results behind blz_rs_*."""
+
+    @staticmethod
+    def init_session():
+        c = C.c_uint64(0)
+        _call(lib().blz_rs_init_session, C.byref(c))
+        return c.value
+
+    @staticmethod
+    def register_query(connection, token):
+        _call(lib().blz_rs_register_query, C.c_uint64(connection), C.c_uint64(token))
+        return token
+
+    @staticmethod
+    def update_token(token, columns: Sequence[DeviceColumn], duration=0.0, error_msg="", names=None):
+        """Parks `columns`: the repository takes their device buffers (the Python
+        objects no longer free them)."""
+        n = len(columns)
+        arr = (GdfColumn * max(n, 1))()
+        for i, c in enumerate(columns):
+            arr[i] = c.col
+        nm = (C.c_char_p * max(n, 1))()
+        for i in range(n):
+            nm[i] = (names[i] if names else "").encode()
+        _call(lib().blz_rs_update_token, C.c_uint64(token), arr, nm, n, float(duration),
+              error_msg.encode() if error_msg else None)
+        for c in columns:  # ownership moved to the repository
+            c._owned = False
+            for b in (c._data, c._valid):
+                if b is not None:
+                    b.ptr = C.c_void_p(0)
+
+    @staticmethod
+    def get_result(connection, token, max_columns=64) -> ResultSet:
+        """Blocks until the token's frame is parked; counts one reference."""
+        out = (GdfColumn * max_columns)()
+        toks = (C.c_uint64 * max_columns)()
+        n = C.c_int(0)
+        dur = C.c_double(0)
+        msg = C.create_string_buffer(1024)
+        _call(lib().blz_rs_get_result, C.c_uint64(connection), C.c_uint64(token), out, toks, max_columns,
+              C.byref(n), C.byref(dur), msg, len(msg))
+        names = []
+        for i in range(n.value):
+            b = C.create_string_buffer(256)
+            _check(lib().blz_rs_column_name(C.c_uint64(token), i, b, len(b)), "column name")
+            names.append(b.value.decode())
+        cols = [DeviceColumn.adopt(out[i], owned=False) for i in range(n.value)]
+        return ResultSet(cols, names, [toks[i] for i in range(n.value)], dur.value,
+                         msg.value.decode(errors="replace"))
+
+    @staticmethod
+    def try_free_result(connection, token) -> bool:
+        found = C.c_int(0)
+        _call(lib().blz_rs_try_free_result, C.c_uint64(connection), C.c_uint64(token), C.byref(found))
+        return bool(found.value)
+
+    @staticmethod
+    def remove_all_connection_tokens(connection):
+        _check(lib().blz_rs_remove_all_connection_tokens(C.c_uint64(connection)), "remove tokens")
+
+    @staticmethod
+    def get_column(connection, column_token) -> DeviceColumn:
+        out = GdfColumn()
+        _call(lib().blz_rs_get_column, C.c_uint64(connection), C.c_uint64(column_token), C.byref(out))
+        return DeviceColumn.adopt(out, owned=False)
 
 
 def synth_column(col: "DeviceColumn", seed: int, dist: int, a: float, b: float, null_rate=0.0, stream=None):

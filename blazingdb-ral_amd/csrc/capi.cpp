@@ -16,6 +16,8 @@
 #include <condition_variable>
 #include <chrono>
 #include <map>
+#include <set>
+#include <functional>
 #include <mutex>
 #include <tuple>
 #include <new>
@@ -27,6 +29,7 @@
 #include "kernels.h"
 #include "planner.h"
 #include "program_compiler.h"
+#include "result_sets.h"
 
 using namespace blz;
 
@@ -1799,9 +1802,18 @@ struct SrvQuery {
 
 }  // namespace
 
+// A served query whose result goes to the result repository under `token`.
+struct SrvHandoff {
+  int64_t ticket = 0;
+  uint64_t token = 0;
+  std::vector<std::string> names;
+};
+
 struct blz_server_s {
   std::mutex mu;
-  std::condition_variable cv_work, cv_done;
+  std::condition_variable cv_work, cv_done, cv_handoff;
+  std::deque<SrvHandoff> handoff;  // tickets to park in the result repository, in submission order
+  std::thread finisher;            // started by the first blz_server_submit_token
   std::deque<int64_t> pending;
   std::unordered_map<int64_t, SrvQuery> qs;
   std::vector<SrvBatch *> inflight, held;
@@ -1816,6 +1828,10 @@ struct blz_server_s {
 };
 
 namespace {
+
+// Servers alive: a parked result releases its ticket only while its server exists.
+std::mutex g_live_mu;
+std::set<blz_server> g_live;
 
 void srv_free_batch(blz_server s, SrvBatch *sb) {
   batch_free(sb->b);
@@ -1965,6 +1981,10 @@ int blz_server_create(blz_server *server, size_t hbm_budget_bytes, int max_batch
       s->streams.push_back(st);
     }
     s->worker = std::thread(srv_loop, s);
+    {
+      std::lock_guard<std::mutex> g(g_live_mu);
+      g_live.insert(s);
+    }
     *server = s;
   });
 }
@@ -2096,6 +2116,88 @@ int blz_server_replay(blz_server s, const blz_query *queries, const double *arri
   });
 }
 
+namespace {
+
+// The result repository's producer (CalciteInterpreter.cpp:920-960: the query runs
+// on its own thread, then update_token parks its frame or its error). Hand-offs are
+// taken in submission order; each waits for its query, describes its outputs and
+// parks them. The parked frame holds the ticket: freeing the result releases it,
+// which frees the outputs and returns their budget.
+void srv_finisher(blz_server s) {
+  (void)hipSetDevice(s->device);
+  std::unique_lock<std::mutex> lk(s->mu);
+  while (true) {
+    s->cv_handoff.wait(lk, [&] { return s->stop || !s->handoff.empty(); });
+    if (s->handoff.empty()) break;  // stopping, nothing left
+    SrvHandoff h = std::move(s->handoff.front());
+    s->handoff.pop_front();
+    auto it = s->qs.find(h.ticket);
+    bool ran = false;
+    if (it != s->qs.end()) {
+      SrvQuery &sq = it->second;
+      s->cv_done.wait(lk, [&] { return sq.state >= 2 || s->stop; });
+      ran = sq.state >= 2;
+    }
+    lk.unlock();
+    std::vector<gdf_column> cols(64);
+    int n_out = 0;
+    int64_t kept = 0;
+    double lat = 0;
+    char e2[512] = {0};
+    std::string error;
+    if (!ran) error = "the server was destroyed before the query ran";
+    else if (blz_server_wait(s, h.ticket, &kept, nullptr, cols.data(), (int)cols.size(), &n_out, &lat, e2,
+                             sizeof(e2)) != BLZ_OK)
+      error = e2;
+    cols.resize(error.empty() ? n_out : 0);
+    h.names.resize(cols.size());
+    const int64_t tk = h.ticket;
+    std::function<void()> release;
+    if (ran)
+      release = [s, tk] {
+        std::lock_guard<std::mutex> g(g_live_mu);
+        if (g_live.count(s)) blz_server_release(s, tk);
+      };
+    try {
+      rs_park(h.token, std::move(cols), std::move(h.names), error.empty() ? lat : 0.0, error, std::move(release));
+    } catch (const std::exception &) {
+      // the token was freed before its query finished: the release above ran
+    }
+    lk.lock();
+  }
+}
+
+}  // namespace
+
+int blz_server_submit_token(blz_server s, const blz_query *q, uint64_t token, char *err, size_t err_len) {
+  return guarded(err, err_len, [&] {
+    if (!s || !q) invalid("bad arguments");
+    rs_require_token(token);
+    // as in the reference, a query that cannot run parks its error under the token
+    // (the call itself succeeds)
+    std::vector<std::string> names;
+    int64_t tk = 0;
+    try {
+      if (q->project_query_part) {
+        const auto pp = planner::parse_project(frame_dtypes(q->frame, q->n_frame), q->project_query_part);
+        for (auto &o : pp.outputs) names.push_back(o.name);
+      } else {
+        for (int i = 0; i < q->n_frame; i++) names.push_back("$" + std::to_string(i));
+      }
+      char e2[512];
+      const int rc = blz_server_submit(s, q, &tk, e2, sizeof(e2));
+      if (rc != BLZ_OK) throw std::runtime_error(e2);
+    } catch (const std::exception &e) {
+      rs_park(token, {}, {}, 0.0, e.what(), nullptr);
+      return;
+    }
+    std::lock_guard<std::mutex> g(s->mu);
+    if (!s->finisher.joinable()) s->finisher = std::thread(srv_finisher, s);
+    s->handoff.push_back(SrvHandoff{tk, token, std::move(names)});
+    s->cv_handoff.notify_one();
+  });
+}
+
 int blz_server_stats(blz_server s, int64_t *n_batches, int64_t *peak_inflight, size_t *used_bytes) {
   if (!s) return BLZ_ERR_INVALID;
   std::lock_guard<std::mutex> g(s->mu);
@@ -2108,11 +2210,18 @@ int blz_server_stats(blz_server s, int64_t *n_batches, int64_t *peak_inflight, s
 int blz_server_destroy(blz_server s) {
   if (!s) return BLZ_OK;
   {
+    std::lock_guard<std::mutex> g(g_live_mu);
+    g_live.erase(s);
+  }
+  {
     std::lock_guard<std::mutex> g(s->mu);
     s->stop = true;
     s->cv_work.notify_all();
+    s->cv_done.notify_all();
+    s->cv_handoff.notify_all();
   }
   if (s->worker.joinable()) s->worker.join();
+  if (s->finisher.joinable()) s->finisher.join();
   (void)hipDeviceSynchronize();
   for (auto sb : s->inflight) {
     batch_free(sb->b);

@@ -285,7 +285,89 @@ int blz_server_replay(blz_server server, const blz_query *queries, const double 
                       double *latency_ms, int64_t *n_kept, double *wall_s, int64_t *tickets_out, char *err,
                       size_t err_len);
 int blz_server_stats(blz_server server, int64_t *n_batches, int64_t *peak_inflight, size_t *used_bytes);
+/* ---- ingest: delimited text to device columns ----
+ * csv_parser::parse (io/data_parser/CSVParser.cpp:178-238) with the reader options
+ * of init_default_csv_args (:26-47): `delimiter` and `line_terminator` bytes, quote
+ * '"' (minimal quoting, no doubled quotes), blank lines skipped, no header, no NA
+ * detection. The file's columns have dtypes[0 .. n_file_columns); the columns
+ * column_indices[0 .. n_indices) (NULL: all, in order) are parsed into callee-
+ * allocated out_columns (free with blz_free_column; no validity, null_count 0).
+ * Parsed dtypes: INT8/16/32/64, FLOAT32/64, DATE32, DATE64, TIMESTAMP (ms);
+ * others (strings) may be skipped but not requested (BLZ_ERR_UNSUPPORTED).
+ * skip_rows drops that many rows first (csv_parser's constructor takes it and
+ * leaves it unused, CSVParser.cpp:153-169). Integers: sign and digits up to the
+ * first other byte, wrapping; reals: correctly rounded when the decimal has at most
+ * 15-16 significant digits and a small exponent (binary64: m <= 2^53, |e| <= 22;
+ * binary32: m <= 2^24, |e| <= 10), otherwise within a few ulp; dates
+ * YYYY-MM-DD[(T| )hh:mm:ss[.fff]][Z]. A missing field reads as 0. *n_rows = rows. */
+int blz_csv_read(const void *host_bytes, size_t n_bytes, char delimiter, char line_terminator, int skip_rows,
+                 const int32_t *dtypes, int n_file_columns, const int32_t *column_indices, int n_indices,
+                 gdf_column *out_columns, int64_t *n_rows, blz_stream_t stream, char *err, size_t err_len);
+/* The same over bytes already in device memory (nothing is read past n_bytes). */
+int blz_csv_parse_device(const void *device_bytes, size_t n_bytes, char delimiter, char line_terminator,
+                         int skip_rows, const int32_t *dtypes, int n_file_columns, const int32_t *column_indices,
+                         int n_indices, gdf_column *out_columns, int64_t *n_rows, blz_stream_t stream, char *err,
+                         size_t err_len);
+/* data_loader::load_data (io/DataLoader.cpp:26-118): parses each file and
+ * concatenates their columns in file order. */
+int blz_csv_load_files(const char *const *paths, int n_paths, char delimiter, char line_terminator, int skip_rows,
+                       const int32_t *dtypes, int n_file_columns, const int32_t *column_indices, int n_indices,
+                       gdf_column *out_columns, int64_t *n_rows, blz_stream_t stream, char *err, size_t err_len);
+
+/* Destroying a server frees what its queries still hold; results it parked in the
+ * result repository must be freed first. */
 int blz_server_destroy(blz_server server);
+
+/* ---- asynchronous result hand-off: the result repository ----
+ * result_set_repository (ResultSetRepository.h:29-63, .cpp), one per process.
+ * Connections group query tokens; a token's frame is parked once (update_token)
+ * and read by any number of readers (get_result, which blocks until it is parked
+ * and counts a reference). Errors are the reference's std::runtime_error texts
+ * ("Connection does not exist", "Token does not exist", "Result set does not
+ * exist", "Column does not exist") with BLZ_ERR_INVALID. CUDA-IPC export of parked
+ * columns (ral-message.cuh:23-46) is not provided: parked columns are device
+ * pointers of this process. */
+
+/* init_session (ResultSetRepository.cpp:115-133): a new random connection id. */
+int blz_rs_init_session(uint64_t *connection, char *err, size_t err_len);
+/* register_query (:36-44, add_token :22-34): token not ready, listed under the
+ * connection (which is created when unknown, as add_token does). */
+int blz_rs_register_query(uint64_t connection, uint64_t token, char *err, size_t err_len);
+/* update_token (:45-112): parks n_columns columns (names may be NULL), the query's
+ * duration and error text (NULL or "" for none), gives every column a column token
+ * and wakes the token's readers. The repository takes the columns' device buffers
+ * (allocated with hipMalloc) and frees them with the result; buffers shared by two
+ * columns are freed once (the reference deduplicates the frame first, :59-60). An
+ * unknown token leaves the buffers with the caller. */
+int blz_rs_update_token(uint64_t token, const gdf_column *columns, const char *const *names, int n_columns,
+                        double duration_ms, const char *error_message, char *err, size_t err_len);
+/* get_result (:186-205): blocks until the token's frame is parked, counts one
+ * reference and describes it: columns[i] (no copy: the repository's buffers),
+ * column_tokens[i], the duration and the error text (empty on success). Either
+ * array may be NULL; columns must hold the frame's columns. */
+int blz_rs_get_result(uint64_t connection, uint64_t token, gdf_column *columns, uint64_t *column_tokens,
+                      int max_columns, int *n_columns, double *duration_ms, char *error_message,
+                      size_t error_len, char *err, size_t err_len);
+/* Name of parked column `column` of a token (the projection's expression name). */
+int blz_rs_column_name(uint64_t token, int column, char *name, size_t name_len);
+/* try_free_result (:166-184): *found = whether the token exists; the last
+ * reference (or none: a result never read) frees the frame, any other drops one.
+ * Unknown connection: "Connection does not exist". */
+int blz_rs_try_free_result(uint64_t connection, uint64_t token, int *found, char *err, size_t err_len);
+/* remove_all_connection_tokens (:152-164): frees the connection's results that no
+ * reader holds and forgets the connection. */
+int blz_rs_remove_all_connection_tokens(uint64_t connection);
+/* get_column (:207-226): the parked column with that column token. */
+int blz_rs_get_column(uint64_t connection, uint64_t column_token, gdf_column *out, char *err, size_t err_len);
+/* evaluate_query's asynchronous form (CalciteInterpreter.cpp:920-960) over the
+ * serving loop: submits the query and, once it has run, parks its outputs (named
+ * after the projection's expressions; "$i" for a bare filter) and its latency in
+ * ms, or its error, under `token`, which must be registered. A query that cannot
+ * be planned parks its error at once (the call returns BLZ_OK, as evaluate_query
+ * does). The parked frame keeps the query's outputs and HBM budget until the
+ * result is freed. */
+int blz_server_submit_token(blz_server server, const blz_query *query, uint64_t token, char *err,
+                            size_t err_len);
 
 /* ---- benchmark inputs (no reference counterpart) ----
  * Fills an allocated column in place with seeded synthetic values: row i depends
