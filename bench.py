@@ -24,6 +24,8 @@ scalar oracle on a 10M-row sample of the same workload on this host.
 python bench.py [--gpus 1] [--steps K] [--warmup W] [--rows N] [--calibrate]
 python bench.py --workload wide-expression   (100M rows x 8 nullable columns, the 24-op
     four-output projection; its own JSON line)
+python bench.py --workload csv-ingest   (a 20M-row lineitem-shaped text file in HBM parsed
+    into seven device columns; its own JSON line)
 python bench.py --workload filter-gather [--selectivity S]   (500M-row filter + gather:
     int32 key U[0, 2^31), 3 int64 sequential and 3 float64 U[0, 1) payload columns,
     key < S * 2^31, the payload compacted in row order; its own JSON line)
@@ -237,24 +239,54 @@ def run_serve(B, total_gb, rates, seconds, seed=505):
     return out
 
 
-def cpu_baseline(sample_rows=10_000_000, min_seconds=10.0):
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    return ln.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(sample_rows=10_000_000, min_seconds=10.0, max_threads=16):
+    """The scalar oracle on a bounded sample of the same workload: one thread, and
+    all cores (row partitions on a thread pool; the oracle's C calls release the
+    GIL), at most max_threads (this host's share of a shared box)."""
+    from concurrent.futures import ThreadPoolExecutor
     from oracle import pyoracle as O
     from oracle import planner as P
     a, b, c, d = gen_c1(sample_rows, 99)
     host = [O.HostColumn(a, P.GDF_INT32), O.HostColumn(b, P.GDF_FLOAT64),
             O.HostColumn(c, P.GDF_FLOAT64), O.HostColumn(d, P.GDF_INT32)]
-    times, kept = [], 0
-    t_end = time.perf_counter() + min_seconds
-    while len(times) < 3 or time.perf_counter() < t_end:
-        t0 = time.perf_counter()
-        out = O.execute_project(O.process_filter(host, FILTER), PROJECT)
-        times.append(time.perf_counter() - t0)
-        kept = len(out[0])
-    t = float(np.median(times))
-    return {"value": algorithmic_bytes(sample_rows, kept) / t / 1e9, "unit": "GB/s", "cores": 1,
-            "kind": "port",
-            "sample": f"{sample_rows} rows of the same workload, scalar oracle (1 thread, -O2 "
-                      f"-ffp-contract=off), median of {len(times)} passes ({t:.2f} s each)"}
+
+    def run(cols):
+        return len(O.execute_project(O.process_filter(cols, FILTER), PROJECT)[0])
+
+    def timed(fn):
+        times, kept = [], 0
+        t_end = time.perf_counter() + min_seconds / 2
+        while len(times) < 3 or time.perf_counter() < t_end:
+            t0 = time.perf_counter()
+            kept = fn()
+            times.append(time.perf_counter() - t0)
+        return float(np.median(times)), kept, len(times)
+
+    t1, kept, n1 = timed(lambda: run(host))
+    threads = max(1, min(os.cpu_count() or 1, max_threads))
+    bounds = np.linspace(0, sample_rows, threads + 1).astype(np.int64)
+    parts = [[O.HostColumn(h.data[bounds[i]:bounds[i + 1]], h.dtype) for h in host] for i in range(threads)]
+    with ThreadPoolExecutor(threads) as pool:
+        tn, kept_n, nn = timed(lambda: sum(pool.map(run, parts)))
+    if kept_n != kept:
+        raise SystemExit("cpu baseline: partitioned run kept a different row count")
+    gbs = lambda t: algorithmic_bytes(sample_rows, kept) / t / 1e9
+    return {"value": gbs(tn), "unit": "GB/s", "cores": threads, "kind": "port",
+            "sample": f"{sample_rows} rows of the same workload, scalar oracle (-O2 -ffp-contract=off) on "
+                      f"{threads} threads over row partitions, median of {nn} passes ({tn:.2f} s each); "
+                      f"one thread: {gbs(t1):.3f} GB/s, median of {n1} passes ({t1:.2f} s each)",
+            "single_thread_value": gbs(t1), "cpu": cpu_model(), "nproc": os.cpu_count()}
 
 
 def run_aggregate(B, frame, prog, kept, n, iters=20):
@@ -369,6 +401,136 @@ def run_wide_expression(B, args):
     print(json.dumps(out))
 
 
+PROFILE_CSV = os.path.join(ROOT, "profiles", "csv_fields_kernel_lineitem_20M.txt")
+
+
+def gen_lineitem_text(n, seed=77):
+    """A lineitem-shaped pipe-delimited file of n fixed-width rows (59 bytes each):
+    orderkey (10 digits), partkey (8), quantity NN.NN, extendedprice NNNNNN.NN,
+    discount 0.NN, tax 0.0N, returnflag (one letter), shipdate YYYY-MM-DD; and the
+    values it was written from (reals in hundredths)."""
+    rng = np.random.default_rng(seed)
+    v = {"okey": rng.integers(0, 10 ** 10, n), "pkey": rng.integers(0, 10 ** 8, n),
+         "qty": rng.integers(100, 5001, n), "price": rng.integers(90_000, 99_999_999, n),
+         "disc": rng.integers(0, 11, n), "tax": rng.integers(0, 9, n), "days": rng.integers(8035, 10591, n)}
+    fields = [("okey", 10, None), ("pkey", 8, None), ("qty", 5, 2), ("price", 9, 2), ("disc", 4, 2),
+              ("tax", 4, 2), ("flag", 1, None), ("date", 10, None)]
+    width = sum(w for _, w, _ in fields) + len(fields)
+    buf = np.empty((n, width), np.uint8)
+    col = 0
+
+    def digits(vals, c0, w):
+        x = vals.copy()
+        for k in range(w - 1, -1, -1):
+            buf[:, c0 + k] = 48 + (x % 10)
+            x //= 10
+
+    for name, w, dec in fields:
+        if name == "flag":
+            buf[:, col] = np.array([ord("A"), ord("N"), ord("R")], np.uint8)[v["okey"] % 3]
+        elif name == "date":
+            d = v["days"].astype("datetime64[D]")
+            y = d.astype("datetime64[Y]").astype(np.int64) + 1970
+            m = d.astype("datetime64[M]").astype(np.int64) % 12 + 1
+            dd = (d - d.astype("datetime64[M]")).astype(np.int64) + 1
+            digits(y, col, 4)
+            buf[:, col + 4] = ord("-")
+            digits(m, col + 5, 2)
+            buf[:, col + 7] = ord("-")
+            digits(dd, col + 8, 2)
+        elif dec:
+            digits(v[name] // 100, col, w - 3)
+            buf[:, col + w - 3] = ord(".")
+            digits(v[name] % 100, col + w - 2, 2)
+        else:
+            digits(v[name], col, w)
+        col += w
+        buf[:, col] = ord("|") if name != "date" else ord("\n")
+        col += 1
+    return buf.reshape(-1), v
+
+
+def run_csv_ingest(B, args):
+    """Ingest: a 20M-row lineitem-shaped text file resident in HBM parsed into seven
+    device columns (int64, int32, four binary64, date32; the one-letter flag is
+    skipped) per step: the four kernels and the output allocations. value = text
+    bytes per step over the step's wall time. roofline: the field kernel (one
+    thread per row), text bytes read + column bytes written over its own duration.
+    cpu_baseline: the oracle's reader (pure Python) on the first 20,000 rows."""
+    from oracle import planner as P
+    n = args.csv_rows
+    print(f"[bench] generating a {n}-row text file", file=sys.stderr, flush=True)
+    text, vals = gen_lineitem_text(n)
+    dtypes = [P.GDF_INT64, P.GDF_INT32, P.GDF_FLOAT64, P.GDF_FLOAT64, P.GDF_FLOAT64, P.GDF_FLOAT64,
+              P.GDF_STRING, P.GDF_DATE32]
+    want = [0, 1, 2, 3, 4, 5, 7]
+    buf = B.DeviceBuffer(len(text))
+    buf.upload(text)
+    nbytes = len(text)
+
+    def step():
+        cols, rows = B.read_csv((buf.value, nbytes), dtypes, want, on_device=True)
+        return cols, rows
+
+    cols, rows = step()
+    assert rows == n
+    checks = [cols[0].data()[:1000] == vals["okey"][:1000], cols[3].data()[-1000:] == vals["price"][-1000:] / 100.0,
+              cols[6].data()[:1000] == vals["days"][:1000]]
+    if not all(c.all() for c in checks):
+        raise SystemExit("csv ingest: parsed values differ from the values written")
+    out_bytes = sum(c.col.size * np.dtype(B.NP_OF[c.col.dtype]).itemsize for c in cols)
+    del cols
+    for _ in range(args.warmup):
+        step()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        cols, _ = step()
+        del cols
+    dt = (time.perf_counter() - t0) / args.steps
+    stage = np.zeros(4)
+    reps = max(3, args.steps // 4)
+    for _ in range(reps):
+        cols, _, ms = B.parse_csv_device_timed(buf.value, nbytes, dtypes, want)
+        stage += np.array(ms)
+        del cols
+    stage /= reps
+    names = ["csv_count_kernel", "csv_scan_kernel", "csv_rows_kernel", "csv_fields_kernel"]
+    k = int(np.argmax(stage))
+    alg = {0: nbytes, 1: 0, 2: nbytes + rows * 8, 3: nbytes + out_bytes}[k]
+    cpu = None
+    if not args.no_cpu:
+        from oracle import csv_oracle as CO
+        sample = bytes(text[: 20_000 * (nbytes // n)])
+        t1 = time.perf_counter()
+        CO.read_csv(sample, dtypes, want)
+        ct = time.perf_counter() - t1
+        cpu = {"value": len(sample) / ct / 1e9, "unit": "GB/s", "cores": 1, "kind": "port",
+               "sample": f"oracle/csv_oracle.read_csv (pure Python) on the first 20000 rows ({len(sample)} bytes) "
+                         f"of the same file, {ct:.2f} s", "cpu": cpu_model(), "nproc": os.cpu_count()}
+    out = {
+        "metric": "csv ingest GB/s of text at 20M lineitem rows", "value": nbytes / dt / 1e9, "unit": "GB/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": dt * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u8->int64/f64",
+        "data": "synthetic",
+        "config": {"workload": "lineitem-shaped text, 8 fields, 7 parsed (int64, int32, 4 binary64, date32)",
+                   "rows": n, "text_bytes": nbytes, "output_bytes": out_bytes,
+                   "stage_ms": dict(zip(names, [round(x, 4) for x in stage]))},
+        "roofline": {"bound": "hbm", "achieved": alg / (stage[k] * 1e-3) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": alg / (stage[k] * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                     "traffic": traffic_from_profile(names[k], PROFILE_CSV), "kernel": names[k],
+                     "kernel_ms": float(stage[k]), "algorithmic_bytes": alg},
+        "cpu_baseline": cpu,
+    }
+    buf.free()
+    if args.calibrate:
+        buf_a = B.DeviceBuffer(1 << 30)
+        buf_b = B.DeviceBuffer(1 << 30)
+        for _ in range(3):
+            B._check(B.lib().blz_device_copy(buf_b.ptr, buf_a.ptr, 1 << 30, None), "copy")
+        print("[bench] calibration copy: 3 x 1 GiB", file=sys.stderr, flush=True)
+    print(json.dumps(out))
+
+
 def run_filter_gather(B, args):
     """500M-row filter + gather: one JSON line, value = (52 N + 48 kept) bytes per step
     over the step's wall time (inputs generated on the device, resident); roofline
@@ -445,8 +607,9 @@ def main():
     ap.add_argument("--batch-plans", type=int, default=256)
     ap.add_argument("--serve-gb", type=float, default=100.0)
     ap.add_argument("--serve-seconds", type=float, default=1.0)
-    ap.add_argument("--workload", choices=["filter-project", "filter-gather", "wide-expression"],
+    ap.add_argument("--workload", choices=["filter-project", "filter-gather", "wide-expression", "csv-ingest"],
                     default="filter-project")
+    ap.add_argument("--csv-rows", type=int, default=20_000_000, help="csv-ingest: rows of the text file")
     ap.add_argument("--selectivity", type=float, default=0.9, help="filter-gather: fraction of rows kept")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -461,6 +624,9 @@ def main():
         return
     if args.workload == "wide-expression":
         run_wide_expression(B, args)
+        return
+    if args.workload == "csv-ingest":
+        run_csv_ingest(B, args)
         return
 
     n = args.rows

@@ -137,6 +137,8 @@ def lib():
                              SZ],
             "blz_csv_parse_device": [P, SZ, C.c_char, C.c_char, I, I32P, I, I32P, I, colp,
                                      C.POINTER(C.c_int64), P, ERR, SZ],
+            "blz_csv_parse_device_timed": [P, SZ, C.c_char, C.c_char, I, I32P, I, I32P, I, colp,
+                                           C.POINTER(C.c_int64), C.POINTER(C.c_float), P, ERR, SZ],
             "blz_csv_load_files": [C.POINTER(C.c_char_p), I, C.c_char, C.c_char, I, I32P, I, I32P, I, colp,
                                    C.POINTER(C.c_int64), P, ERR, SZ],
             "blz_rs_init_session": [C.POINTER(C.c_uint64), ERR, SZ],
@@ -778,6 +780,19 @@ def read_csv(data, dtypes, column_indices=None, delimiter="|", line_terminator="
         _call(lib().blz_csv_read, buf, len(buf), delimiter.encode(), line_terminator.encode(), skip_rows, dt,
               len(dtypes), ci, n_idx, out, C.byref(rows), stream)
     return [DeviceColumn.adopt(out[i], owned=True) for i in range(n_out)], rows.value
+
+
+def parse_csv_device_timed(ptr, nbytes, dtypes, column_indices=None, delimiter="|", line_terminator="\n",
+                           skip_rows=0, stream=None):
+    """read_csv over device bytes, also returning the device time (ms) of its four
+    kernels (row-start counts, scan, row starts, fields)."""
+    dt, ci, n_out, n_idx = _csv_args(dtypes, column_indices)
+    out = (GdfColumn * max(n_out, 1))()
+    rows = C.c_int64(0)
+    ms = (C.c_float * 4)()
+    _call(lib().blz_csv_parse_device_timed, ptr, nbytes, delimiter.encode(), line_terminator.encode(), skip_rows,
+          dt, len(dtypes), ci, n_idx, out, C.byref(rows), ms, stream)
+    return [DeviceColumn.adopt(out[i], owned=True) for i in range(n_out)], rows.value, list(ms)
 
 
 def load_csv_files(paths, dtypes, column_indices=None, delimiter="|", line_terminator="\n", skip_rows=0,

@@ -9,13 +9,15 @@
  * (io/DataLoader.cpp:26-118) concatenates the columns parsed from each file.
  *
  * The bytes sit in HBM; three passes over them and one over the rows:
- *   1. per 4 KiB chunk (one wave; a lane takes 64 contiguous bytes): the parity of
- *      its quote characters and the number of rows that start in it, for either
- *      quote state at the chunk's first byte (the state is not known yet);
- *   2. one workgroup scans the chunk summaries: the quote state at each chunk's
+ *   1. per group of 16 chunks of 4 KiB (one wave walks them in order; a lane takes
+ *      64 contiguous bytes of a chunk): the parity of its quote characters and
+ *      the number of rows that start in it, for either quote state at its first
+ *      byte (the state is not known yet);
+ *   2. one workgroup scans the group summaries: the quote state at each group's
  *      start and the index of its first row;
- *   3. each chunk again, now with its quote state: the byte offset of every row
- *      start, written at the row's index;
+ *   3. each group again, now with its quote state: the byte offset of every row
+ *      start, written at the row's index (a lane gathers its starts in a bit mask
+ *      and stores one word per row);
  *   4. one thread per row walks its fields (delimiters outside quotes), parses
  *      the requested ones and stores them, row-major across threads so stores of
  *      a wave are contiguous.
@@ -58,6 +60,7 @@ typedef uint32_t csv_u32x4 __attribute__((ext_vector_type(4)));
 constexpr int kCsvLaneBytes = 64;
 constexpr int kCsvChunk = 64 * kCsvLaneBytes;  // bytes per wave
 constexpr int kCsvBlock = 256;
+constexpr int kCsvGroup = 16;                  // chunks per summary (one wave walks them in order)
 constexpr int kCsvMaxCols = 256;               // file columns a row may be walked through
 
 struct CsvSum {        // a run of bytes, for either quote state s at its start:
@@ -144,47 +147,56 @@ CSV_DEV LaneScan scan_lane(const LaneBytes &b, int64_t p, int64_t n, uint8_t ter
   return r;
 }
 
+// A chunk's summary (all lanes hold it): quote parity and rows for either start state.
+CSV_DEV CsvSum chunk_sum(const LaneScan &ls, uint64_t pm, uint32_t pre) {
+  uint64_t c0 = pre ? ls.cnt[1] : ls.cnt[0];  // lane counts for chunk start state s: state s ^ pre
+  uint64_t c1 = pre ? ls.cnt[0] : ls.cnt[1];
+  for (int o = 32; o > 0; o >>= 1) {
+    c0 += __shfl_xor(c0, o);
+    c1 += __shfl_xor(c1, o);
+  }
+  CsvSum s;
+  s.par = (uint32_t)__builtin_popcountll(pm) & 1;
+  s.pad = 0;
+  s.rows[0] = c0;
+  s.rows[1] = c1;
+  return s;
+}
+
+// Pass 1: one summary per group of kCsvGroup consecutive chunks (one wave walks
+// the group's chunks in order).
 __global__ void __launch_bounds__(kCsvBlock) csv_count_kernel(const uint8_t *__restrict__ buf, int64_t n,
                                                               uint8_t term, uint8_t quote, CsvSum *sums,
-                                                              int64_t n_chunks) {
+                                                              int64_t n_chunks, int64_t n_groups) {
   const int lane = threadIdx.x & 63;
   const int64_t wave0 = ((int64_t)blockIdx.x * kCsvBlock + threadIdx.x) >> 6;
   const int64_t n_waves = ((int64_t)gridDim.x * kCsvBlock) >> 6;
-  for (int64_t c = wave0; c < n_chunks; c += n_waves) {
-    const int64_t p = c * kCsvChunk + (int64_t)lane * kCsvLaneBytes;
-    LaneBytes b;
-    load_lane(buf, n, p, term, b);
-    const LaneScan ls = scan_lane(b, p, n, term, quote);
-    // lanes before this one: quote parity (exclusive prefix)
-    const uint64_t pm = __builtin_amdgcn_ballot_w64(ls.par != 0);
-    const uint32_t pre = (uint32_t)__builtin_popcountll(pm & ((1ull << lane) - 1)) & 1;
-    // rows of the chunk for start state s: lane counts for state s ^ pre
-    uint64_t c0 = pre ? ls.cnt[1] : ls.cnt[0];
-    uint64_t c1 = pre ? ls.cnt[0] : ls.cnt[1];
-    for (int o = 32; o > 0; o >>= 1) {
-      c0 += __shfl_xor(c0, o);
-      c1 += __shfl_xor(c1, o);
+  for (int64_t g = wave0; g < n_groups; g += n_waves) {
+    CsvSum acc{0, 0, {0, 0}};
+    const int64_t c_end = (g + 1) * kCsvGroup < n_chunks ? (g + 1) * kCsvGroup : n_chunks;
+    for (int64_t c = g * kCsvGroup; c < c_end; c++) {
+      const int64_t p = c * kCsvChunk + (int64_t)lane * kCsvLaneBytes;
+      LaneBytes b;
+      load_lane(buf, n, p, term, b);
+      const LaneScan ls = scan_lane(b, p, n, term, quote);
+      // lanes before this one: quote parity (exclusive prefix)
+      const uint64_t pm = __builtin_amdgcn_ballot_w64(ls.par != 0);
+      const uint32_t pre = (uint32_t)__builtin_popcountll(pm & ((1ull << lane) - 1)) & 1;
+      acc = csv_combine(acc, chunk_sum(ls, pm, pre));
     }
-    if (lane == 0) {
-      CsvSum s;
-      s.par = (uint32_t)__builtin_popcountll(pm) & 1;
-      s.pad = 0;
-      s.rows[0] = c0;
-      s.rows[1] = c1;
-      sums[c] = s;
-    }
+    if (lane == 0) sums[g] = acc;
   }
 }
 
-// One workgroup: exclusive scan of the chunk summaries (the file starts outside
-// quotes): starts[c] = quote state and first row of chunk c; *total = rows.
-__global__ void __launch_bounds__(1024) csv_scan_kernel(const CsvSum *__restrict__ sums, int64_t n_chunks,
+// Pass 2, one workgroup: exclusive scan of the group summaries (the file starts
+// outside quotes): starts[g] = quote state and first row of group g; *total = rows.
+__global__ void __launch_bounds__(1024) csv_scan_kernel(const CsvSum *__restrict__ sums, int64_t n_groups,
                                                         CsvStart *starts, uint64_t *total) {
   __shared__ CsvSum part[1024];
   const int t = threadIdx.x;
-  const int64_t per = (n_chunks + 1023) / 1024;
-  const int64_t lo = t * per < n_chunks ? t * per : n_chunks;
-  const int64_t hi = lo + per < n_chunks ? lo + per : n_chunks;
+  const int64_t per = (n_groups + 1023) / 1024;
+  const int64_t lo = t * per < n_groups ? t * per : n_groups;
+  const int64_t hi = lo + per < n_groups ? lo + per : n_groups;
   CsvSum acc{0, 0, {0, 0}};
   for (int64_t c = lo; c < hi; c++) acc = csv_combine(acc, sums[c]);
   part[t] = acc;
@@ -210,49 +222,63 @@ __global__ void __launch_bounds__(1024) csv_scan_kernel(const CsvSum *__restrict
   }
 }
 
-// Row start offsets: rows [skip, skip + n_out) of the file, at index row - skip.
+// Pass 3: row start offsets, rows [skip, skip + n_out) of the file at index
+// row - skip. A wave walks its group's chunks in order from the group's state.
 __global__ void __launch_bounds__(kCsvBlock) csv_rows_kernel(const uint8_t *__restrict__ buf, int64_t n,
                                                              uint8_t term, uint8_t quote,
                                                              const CsvStart *__restrict__ starts,
-                                                             int64_t n_chunks, int64_t skip, int64_t n_out,
-                                                             int64_t *row_start) {
+                                                             int64_t n_chunks, int64_t n_groups, int64_t skip,
+                                                             int64_t n_out, int64_t *row_start) {
   const int lane = threadIdx.x & 63;
   const int64_t wave0 = ((int64_t)blockIdx.x * kCsvBlock + threadIdx.x) >> 6;
   const int64_t n_waves = ((int64_t)gridDim.x * kCsvBlock) >> 6;
-  for (int64_t c = wave0; c < n_chunks; c += n_waves) {
-    const int64_t p = c * kCsvChunk + (int64_t)lane * kCsvLaneBytes;
-    LaneBytes b;
-    load_lane(buf, n, p, term, b);
-    const LaneScan ls = scan_lane(b, p, n, term, quote);
-    const uint64_t pm = __builtin_amdgcn_ballot_w64(ls.par != 0);
-    const CsvStart st = starts[c];
-    const uint32_t s = st.par ^ ((uint32_t)__builtin_popcountll(pm & ((1ull << lane) - 1)) & 1);
-    // rows of this lane, then the lanes before it
-    const uint64_t mine = ls.cnt[s];
-    uint64_t incl = mine;
-    for (int o = 1; o < 64; o <<= 1) {
-      const uint64_t y = __shfl_up(incl, o);
-      if (lane >= o) incl += y;
-    }
-    int64_t r = (int64_t)(st.rows + incl - mine);
-    if (mine == 0) continue;
-    uint8_t prev = b.prev;
-    uint32_t q = s;
-    const int64_t lim = n - p;
+  for (int64_t g = wave0; g < n_groups; g += n_waves) {
+    const CsvStart st = starts[g];
+    uint64_t rows = st.rows;
+    uint32_t par = st.par;
+    const int64_t c_end = (g + 1) * kCsvGroup < n_chunks ? (g + 1) * kCsvGroup : n_chunks;
+    for (int64_t c = g * kCsvGroup; c < c_end; c++) {
+      const int64_t p = c * kCsvChunk + (int64_t)lane * kCsvLaneBytes;
+      LaneBytes b;
+      load_lane(buf, n, p, term, b);
+      // this lane's row starts (bit k: a row starts at byte p + k) for its actual
+      // quote state, and its quote parity
+      const int64_t lim = n - p;
+      uint32_t q = 0;
+      uint64_t hit0 = 0, hit1 = 0;  // for lane start state 0 / 1
+      uint8_t prev = b.prev;
 #pragma unroll
-    for (int k = 0; k < kCsvLaneBytes; k++) {
-      const uint8_t ch = b.at(k);
-      const uint8_t nx = k + 1 < kCsvLaneBytes ? b.at(k + 1) : b.next;
-      const bool in = k < lim;
-      const bool nx_end = k + 1 >= lim;
-      const bool blank = ch == term || (ch == '\r' && (nx_end || nx == term));
-      if (in && prev == term && !blank && q == 0) {
+      for (int k = 0; k < kCsvLaneBytes; k++) {
+        const uint8_t ch = b.at(k);
+        const uint8_t nx = k + 1 < kCsvLaneBytes ? b.at(k + 1) : b.next;
+        const bool in = k < lim;
+        const bool nx_end = k + 1 >= lim;
+        const bool blank = ch == term || (ch == '\r' && (nx_end || nx == term));
+        const uint64_t h = (in && prev == term && !blank) ? (1ull << k) : 0ull;
+        hit0 |= q == 0 ? h : 0ull;
+        hit1 |= q == 1 ? h : 0ull;
+        q ^= (in && ch == quote) ? 1u : 0u;
+        prev = ch;
+      }
+      const uint64_t pm = __builtin_amdgcn_ballot_w64(q != 0);
+      const uint32_t s = par ^ ((uint32_t)__builtin_popcountll(pm & ((1ull << lane) - 1)) & 1);
+      uint64_t hits = s ? hit1 : hit0;
+      const uint64_t mine = (uint64_t)__builtin_popcountll(hits);
+      uint64_t incl = mine;
+      for (int o = 1; o < 64; o <<= 1) {
+        const uint64_t y = __shfl_up(incl, o);
+        if (lane >= o) incl += y;
+      }
+      int64_t r = (int64_t)(rows + incl - mine);
+      while (hits) {
+        const int k = __builtin_ctzll(hits);
+        hits &= hits - 1;
         const int64_t o = r - skip;
         if (o >= 0 && o < n_out) row_start[o] = p + k;
         r++;
       }
-      q ^= (in && ch == quote) ? 1u : 0u;
-      prev = ch;
+      rows += (uint64_t)__shfl(incl, 63);
+      par ^= (uint32_t)__builtin_popcountll(pm) & 1;
     }
   }
 }
@@ -545,7 +571,24 @@ struct ParseArgs {
 
 // Parses n bytes at device address d_buf; fills out[0 .. n_indices) with
 // callee-allocated columns. Returns the row count.
-int64_t parse_device(const uint8_t *d_buf, int64_t n, const ParseArgs &a, gdf_column *out, hipStream_t st) {
+int64_t parse_device(const uint8_t *d_buf, int64_t n, const ParseArgs &a, gdf_column *out, hipStream_t st,
+                     float *stage_ms = nullptr) {
+  // stage_ms (measurement): device time of the count, scan, row and field kernels
+  hipEvent_t ev[6] = {};  // before count | after count | after scan | before rows | after rows | after fields
+  auto mark = [&](int i) {
+    if (stage_ms) CSV_HIP(hipEventRecord(ev[i], st));
+  };
+  if (stage_ms) {
+    for (auto &e : ev) CSV_HIP(hipEventCreate(&e));
+    for (int i = 0; i < 4; i++) stage_ms[i] = 0.0f;
+  }
+  struct EvFree {
+    hipEvent_t *e;
+    ~EvFree() {
+      for (int i = 0; i < 6; i++)
+        if (e[i]) (void)hipEventDestroy(e[i]);
+    }
+  } ev_free{ev};
   if (a.n_file_columns <= 0 || a.n_file_columns > kCsvMaxCols)
     throw CsvError(BLZ_ERR_INVALID, "1 to 256 file columns");
   if (!a.dtypes) throw CsvError(BLZ_ERR_INVALID, "dtypes is NULL");
@@ -570,7 +613,8 @@ int64_t parse_device(const uint8_t *d_buf, int64_t n, const ParseArgs &a, gdf_co
   try {
     if (n > 0) {
       const int64_t n_chunks = (n + kCsvChunk - 1) / kCsvChunk;
-      const size_t sums_b = (size_t)n_chunks * sizeof(CsvSum), starts_b = (size_t)n_chunks * sizeof(CsvStart);
+      const int64_t n_groups = (n_chunks + kCsvGroup - 1) / kCsvGroup;
+      const size_t sums_b = (size_t)n_groups * sizeof(CsvSum), starts_b = (size_t)n_groups * sizeof(CsvStart);
       CSV_HIP(hipMallocAsync(&d_tmp, sums_b + starts_b + 64, st));
       CsvSum *sums = (CsvSum *)d_tmp;
       CsvStart *starts = (CsvStart *)((uint8_t *)d_tmp + sums_b);
@@ -578,13 +622,16 @@ int64_t parse_device(const uint8_t *d_buf, int64_t n, const ParseArgs &a, gdf_co
       int dev = 0, cus = 0;
       CSV_HIP(hipGetDevice(&dev));
       CSV_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-      const int64_t want = (n_chunks + kCsvBlock / 64 - 1) / (kCsvBlock / 64);
+      const int64_t want = (n_groups + kCsvBlock / 64 - 1) / (kCsvBlock / 64);
       const int grid = (int)std::max<int64_t>(1, std::min<int64_t>(want, (int64_t)cus * 16));
+      mark(0);
       hipLaunchKernelGGL(csv_count_kernel, dim3(grid), dim3(kCsvBlock), 0, st, d_buf, n, term, quote, sums,
-                         n_chunks);
+                         n_chunks, n_groups);
       CSV_HIP(hipGetLastError());
-      hipLaunchKernelGGL(csv_scan_kernel, dim3(1), dim3(1024), 0, st, sums, n_chunks, starts, d_total);
+      mark(1);
+      hipLaunchKernelGGL(csv_scan_kernel, dim3(1), dim3(1024), 0, st, sums, n_groups, starts, d_total);
       CSV_HIP(hipGetLastError());
+      mark(2);
       uint64_t total = 0;
       CSV_HIP(hipMemcpyAsync(&total, d_total, 8, hipMemcpyDeviceToHost, st));
       CSV_HIP(hipStreamSynchronize(st));
@@ -604,18 +651,27 @@ int64_t parse_device(const uint8_t *d_buf, int64_t n, const ParseArgs &a, gdf_co
         CSV_HIP(hipMallocAsync((void **)&rows, (size_t)n_rows * 8 + sizeof(CsvOut), st));
         CsvOut *d_out = (CsvOut *)(rows + n_rows);
         CSV_HIP(hipMemcpyAsync(d_out, &h, sizeof(CsvOut), hipMemcpyHostToDevice, st));
+        mark(3);  // after the output allocations
         hipLaunchKernelGGL(csv_rows_kernel, dim3(grid), dim3(kCsvBlock), 0, st, d_buf, n, term, quote, starts,
-                           n_chunks, (int64_t)a.skip_rows, n_rows, rows);
+                           n_chunks, n_groups, (int64_t)a.skip_rows, n_rows, rows);
         CSV_HIP(hipGetLastError());
+        mark(4);
         const int64_t g2 = (n_rows + kCsvBlock - 1) / kCsvBlock;
         hipLaunchKernelGGL(csv_fields_kernel, dim3((unsigned)g2), dim3(kCsvBlock), 0, st, d_buf, n, delim, term,
                            quote, rows, n_rows, last, d_out);
         CSV_HIP(hipGetLastError());
+        mark(5);
         CSV_HIP(hipFreeAsync(rows, st));
       }
       CSV_HIP(hipFreeAsync(d_tmp, st));
       d_tmp = nullptr;
       CSV_HIP(hipStreamSynchronize(st));
+      if (stage_ms && n_rows > 0) {
+        CSV_HIP(hipEventElapsedTime(&stage_ms[0], ev[0], ev[1]));
+        CSV_HIP(hipEventElapsedTime(&stage_ms[1], ev[1], ev[2]));
+        CSV_HIP(hipEventElapsedTime(&stage_ms[2], ev[3], ev[4]));
+        CSV_HIP(hipEventElapsedTime(&stage_ms[3], ev[4], ev[5]));
+      }
     } else {
       for (size_t i = 0; i < cols.size(); i++) {
         void *p = nullptr;
@@ -684,6 +740,19 @@ int blz_csv_parse_device(const void *device_bytes, size_t n_bytes, char delimite
     const ParseArgs a{delimiter, line_terminator, skip_rows, dtypes, n_file_columns, column_indices, n_indices};
     const int64_t r = parse_device((const uint8_t *)device_bytes, (int64_t)n_bytes, a, out_columns,
                                    (hipStream_t)stream);
+    if (n_rows) *n_rows = r;
+  });
+}
+
+int blz_csv_parse_device_timed(const void *device_bytes, size_t n_bytes, char delimiter, char line_terminator,
+                               int skip_rows, const int32_t *dtypes, int n_file_columns,
+                               const int32_t *column_indices, int n_indices, gdf_column *out_columns,
+                               int64_t *n_rows, float *stage_ms, blz_stream_t stream, char *err, size_t err_len) {
+  return csv_guard(err, err_len, [&] {
+    if (!out_columns || !stage_ms || (!device_bytes && n_bytes)) throw CsvError(BLZ_ERR_INVALID, "bad arguments");
+    const ParseArgs a{delimiter, line_terminator, skip_rows, dtypes, n_file_columns, column_indices, n_indices};
+    const int64_t r = parse_device((const uint8_t *)device_bytes, (int64_t)n_bytes, a, out_columns,
+                                   (hipStream_t)stream, stage_ms);
     if (n_rows) *n_rows = r;
   });
 }

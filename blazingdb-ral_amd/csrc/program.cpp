@@ -696,7 +696,7 @@ void fuse_conjunctions(std::vector<VmOp> &v, bool predicate) {
   }
 }
 
-// Lowers the accumulator form to VmOps and lays out the double-buffered tile.
+// Lowers the accumulator form to VmOps and lays out the tile buffer (kTileBuffers).
 // Returns false (with the reason) when the plan does not fit the tile VM.
 bool compile_vm(const CompileRequest &req, Builder &B, const std::vector<AccOp> &acc_p,
                 const std::vector<AccOp> &acc_q, int acc_temps, const std::vector<VOp> &vops_p,
@@ -885,8 +885,8 @@ bool compile_vm(const CompileRequest &req, Builder &B, const std::vector<AccOp> 
   // Shape: rows per lane R and tile rows T = 256 R P (P passes of the four waves).
   // Temporaries live in LDS (256 R rows each), after the validity registers (four
   // waves x registers x R words). The default is R = 4 and one pass
-  // per tile (T = 1024): small double buffers keep four workgroups per CU, which
-  // hides more latency than longer tiles do; the autotuner may choose otherwise.
+  // per tile (T = 1024): a small tile buffer keeps two workgroups per CU, whose
+  // loads and evaluations overlap; the autotuner may choose otherwise.
   static const int shapes[] = {4, 8, 2};
   const int budget = std::min(req.lds_budget > 0 ? req.lds_budget : 64 * 1024, 160 * 1024 - 4096);
   int R = 0, T = 0, buf = 0;
@@ -1440,7 +1440,7 @@ Compiled compile_program(const CompileRequest &req) {
   }
 
   if (req.kernel == 1 || req.kernel == 2) fail("kernel kind not available (1 and 2 are superseded by 4)");
-  // ---- Tile VM: the accumulator form, pre-decoded, over double-buffered tiles ----
+  // ---- Tile VM: the accumulator form, pre-decoded, over LDS-staged tiles ----
   if (req.kernel == 4 && !acc_ok) fail("plan does not fit the tile VM");
   if (acc_ok && (req.kernel == 0 || req.kernel == 4)) {
     std::string why;

@@ -8,9 +8,10 @@
  * every 64-row group is one validity word and one ballot.
  *
  * Staging. The columns an op reads are moved HBM -> LDS with direct-to-LDS
- * 16-byte loads (no VGPR round trip), double buffered: while the waves evaluate
- * tile i from one buffer, the loads of their next tile fill the other. Only the
- * last, partial tile of a table is staged with bounded ordinary loads.
+ * 16-byte loads (no VGPR round trip) into one tile buffer per workgroup
+ * (kTileBuffers): a workgroup's loads and its evaluation alternate, and the other
+ * workgroups resident on the CU overlap them. Only the last, partial tile of a
+ * table is staged with bounded ordinary loads.
  *
  * Evaluation. The plan is in accumulator form (every op combines the running
  * value with at most one operand) pre-decoded on the host into 32-byte VmOps read

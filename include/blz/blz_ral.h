@@ -181,7 +181,7 @@ typedef struct blz_program_s *blz_program;
  * device buffers the program allocates and owns: out_columns[0..*n_out) receive
  * them (capacity: every frame row; sizes valid after blz_program_result).
  * kernel: 0 automatic (the tile VM when the plan fits it, else the direct kernel),
- * 3 direct register-file kernel, 4 tile VM (double-buffered direct-to-LDS staging;
+ * 3 direct register-file kernel, 4 tile VM (direct-to-LDS tile staging;
  * a filter runs as predicate, slot-scan and gather launches). r / t (0 =
  * automatic) force the tile VM's rows per lane (2, 4, 8) and tile rows (256 r P);
  * s / r force the direct kernel's slots x rows (8,1) (8,2) (16,1) (32,1) (64,1). */
@@ -308,6 +308,13 @@ int blz_csv_parse_device(const void *device_bytes, size_t n_bytes, char delimite
                          int skip_rows, const int32_t *dtypes, int n_file_columns, const int32_t *column_indices,
                          int n_indices, gdf_column *out_columns, int64_t *n_rows, blz_stream_t stream, char *err,
                          size_t err_len);
+/* Measurement (no reference counterpart): as blz_csv_parse_device, and stage_ms[0..4)
+ * receives the device time of its four kernels (row-start counts, their scan, row
+ * starts, fields), from HIP events on `stream`. */
+int blz_csv_parse_device_timed(const void *device_bytes, size_t n_bytes, char delimiter, char line_terminator,
+                               int skip_rows, const int32_t *dtypes, int n_file_columns,
+                               const int32_t *column_indices, int n_indices, gdf_column *out_columns,
+                               int64_t *n_rows, float *stage_ms, blz_stream_t stream, char *err, size_t err_len);
 /* data_loader::load_data (io/DataLoader.cpp:26-118): parses each file and
  * concatenates their columns in file order. */
 int blz_csv_load_files(const char *const *paths, int n_paths, char delimiter, char line_terminator, int skip_rows,

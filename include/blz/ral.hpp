@@ -11,6 +11,9 @@
  *   add_expression_to_plan src/LogicalFilter.h:26-42
  *   materialize_column     src/ColumnManipulation.cuh:16-18
  *   gdf_sequence           src/Utils.cuh:187-190
+ *   ral::io::csv_parser    src/io/data_parser/CSVParser.h:21-44 (ingest)
+ *   ral::io::data_loader   src/io/DataLoader.h (files -> concatenated columns)
+ *   result_set_repository  src/ResultSetRepository.h:29-63 (asynchronous results)
  * with the frame and column types they take (blazing_frame, DataFrame.h:16-181;
  * gdf_column_cpp, GDFColumn.cuh:17-95) reduced to the members this path uses.
  *
@@ -93,6 +96,22 @@ class gdf_column_cpp {
     auto h = std::make_shared<blz_detail::ColumnHolder>();
     h->col = *column;
     h_ = h;
+  }
+
+  // A column over buffers someone else owns (a parked result's columns).
+  static gdf_column_cpp view_of(const gdf_column &col, const std::string &name = "") {
+    gdf_column_cpp c;
+    auto h = std::make_shared<blz_detail::ColumnHolder>();
+    h->col = col;
+    h->owned = false;
+    c.h_ = h;
+    c.name_ = name;
+    return c;
+  }
+  // Hands the buffers to another owner: no copy of this column, nor the column it
+  // aliases, frees them.
+  void disown() {
+    for (auto h = h_; h; h = h->base) h->owned = false;
   }
 
   gdf_column *get_gdf_column() const { return h_ ? &h_->col : nullptr; }
@@ -395,5 +414,190 @@ inline void gdf_sequence(int32_t *data, size_t size, int32_t init_val, int32_t s
   char err[256] = {0};
   blz_detail::check(blz_gdf_sequence(data, size, init_val, step, nullptr, err, sizeof(err)), err, "gdf_sequence");
 }
+
+namespace ral {
+namespace io {
+
+// csv_parser (io/data_parser/CSVParser.h:21-44, .cpp:153-238): the reader options
+// of init_default_csv_args (CSVParser.cpp:26-47) with the given delimiter, line
+// terminator, column names and dtypes. parse takes a file path (the reference
+// takes an arrow file handle and a Schema that repeats the names and dtypes).
+// column_indices empty: every column; requested columns must be numeric or dates.
+class csv_parser {
+ public:
+  csv_parser(std::string delimiter, std::string line_terminator, int skip_rows, std::vector<std::string> names,
+             std::vector<gdf_dtype> dtypes)
+      : delimiter_(delimiter.empty() ? '|' : delimiter[0]),
+        line_terminator_(line_terminator.empty() ? '\n' : line_terminator[0]),
+        skip_rows_(skip_rows),
+        names_(std::move(names)),
+        dtypes_(std::move(dtypes)) {}
+
+  void parse(const std::string &path, std::vector<gdf_column_cpp> &columns_out,
+             std::vector<size_t> column_indices = {}) const {
+    load({path}, columns_out, column_indices);
+  }
+  // The same over text in host memory.
+  void parse_buffer(const void *data, size_t n_bytes, std::vector<gdf_column_cpp> &columns_out,
+                    std::vector<size_t> column_indices = {}) const {
+    std::vector<int32_t> dt(dtypes_.begin(), dtypes_.end()), ci = indices(column_indices);
+    std::vector<gdf_column> out(ci.size());
+    int64_t rows = 0;
+    char err[512] = {0};
+    blz_detail::check(blz_csv_read(data, n_bytes, delimiter_, line_terminator_, skip_rows_, dt.data(), (int)dt.size(),
+                                   ci.data(), (int)ci.size(), out.data(), &rows, nullptr, err, sizeof(err)),
+                      err, "csv_parser::parse");
+    adopt(out, ci, columns_out);
+  }
+  // data_loader::load_data (DataLoader.cpp:26-118): every file, columns concatenated.
+  void load(const std::vector<std::string> &paths, std::vector<gdf_column_cpp> &columns_out,
+            std::vector<size_t> column_indices = {}) const {
+    std::vector<int32_t> dt(dtypes_.begin(), dtypes_.end()), ci = indices(column_indices);
+    std::vector<const char *> ps;
+    for (auto &p : paths) ps.push_back(p.c_str());
+    std::vector<gdf_column> out(ci.size());
+    int64_t rows = 0;
+    char err[512] = {0};
+    blz_detail::check(blz_csv_load_files(ps.data(), (int)ps.size(), delimiter_, line_terminator_, skip_rows_,
+                                         dt.data(), (int)dt.size(), ci.data(), (int)ci.size(), out.data(), &rows,
+                                         nullptr, err, sizeof(err)),
+                      err, "csv_parser::parse");
+    adopt(out, ci, columns_out);
+  }
+
+ private:
+  std::vector<int32_t> indices(const std::vector<size_t> &column_indices) const {
+    std::vector<int32_t> ci;
+    if (column_indices.empty())
+      for (size_t i = 0; i < dtypes_.size(); i++) ci.push_back((int32_t)i);
+    else
+      for (size_t c : column_indices) ci.push_back((int32_t)c);
+    return ci;
+  }
+  void adopt(std::vector<gdf_column> &out, const std::vector<int32_t> &ci,
+             std::vector<gdf_column_cpp> &columns_out) const {
+    columns_out.clear();
+    for (size_t i = 0; i < out.size(); i++) {
+      gdf_column_cpp c;
+      c.create_gdf_column(&out[i]);
+      if ((size_t)ci[i] < names_.size()) c.set_name(names_[ci[i]]);
+      columns_out.push_back(c);
+    }
+  }
+  char delimiter_, line_terminator_;
+  int skip_rows_;
+  std::vector<std::string> names_;
+  std::vector<gdf_dtype> dtypes_;
+};
+
+// data_loader (io/DataLoader.h): a parser over a list of files.
+class data_loader {
+ public:
+  data_loader(std::shared_ptr<csv_parser> parser, std::vector<std::string> paths)
+      : parser_(std::move(parser)), paths_(std::move(paths)) {}
+  void load_data(std::vector<gdf_column_cpp> &columns, const std::vector<size_t> &column_indices) {
+    parser_->load(paths_, columns, column_indices);
+  }
+
+ private:
+  std::shared_ptr<csv_parser> parser_;
+  std::vector<std::string> paths_;
+};
+
+}  // namespace io
+}  // namespace ral
+
+typedef uint64_t connection_id_t;
+typedef uint64_t query_token_t;
+typedef uint64_t column_token_t;
+
+// result_set_t (ResultSetRepository.h:19-25): the frame's columns are views of
+// the repository's buffers.
+struct result_set_t {
+  bool is_ready;
+  blazing_frame result_frame;
+  double duration;
+  std::string errorMsg;
+  size_t ref_counter;
+};
+
+// result_set_repository (ResultSetRepository.h:29-63), the process's one
+// repository (blz_rs_*). Errors are the reference's std::runtime_error texts.
+class result_set_repository {
+ public:
+  static result_set_repository &get_instance() {
+    static result_set_repository instance;
+    return instance;
+  }
+  query_token_t register_query(connection_id_t connection, query_token_t token) {
+    char err[256] = {0};
+    blz_detail::check(blz_rs_register_query(connection, token, err, sizeof(err)), err, "register_query");
+    return token;
+  }
+  // Parks the frame: the repository takes its columns' buffers (the frame's
+  // gdf_column_cpp no longer free them).
+  void update_token(query_token_t token, blazing_frame frame, double duration, std::string errorMsg = "") {
+    std::vector<gdf_column> cols;
+    std::vector<std::string> names;
+    for (auto &t : frame.get_columns())
+      for (auto &c : t) {
+        cols.push_back(*c.get_gdf_column());
+        names.push_back(c.name());
+      }
+    std::vector<const char *> np;
+    for (auto &n : names) np.push_back(n.c_str());
+    char err[256] = {0};
+    blz_detail::check(blz_rs_update_token(token, cols.data(), np.data(), (int)cols.size(), duration, errorMsg.c_str(),
+                                          err, sizeof(err)),
+                      err, "update_token");
+    for (auto &t : frame.get_columns())
+      for (auto &c : t) c.disown();
+  }
+  connection_id_t init_session() {
+    connection_id_t c = 0;
+    char err[256] = {0};
+    blz_detail::check(blz_rs_init_session(&c, err, sizeof(err)), err, "init_session");
+    return c;
+  }
+  void remove_all_connection_tokens(connection_id_t connection) { blz_rs_remove_all_connection_tokens(connection); }
+  // Blocks until the token's frame is parked; counts one reference.
+  result_set_t get_result(connection_id_t connection, query_token_t token) {
+    std::vector<gdf_column> cols(256);
+    std::vector<uint64_t> toks(256);
+    int n = 0;
+    double dur = 0;
+    char msg[1024] = {0}, err[256] = {0};
+    blz_detail::check(blz_rs_get_result(connection, token, cols.data(), toks.data(), (int)cols.size(), &n, &dur, msg,
+                                        sizeof(msg), err, sizeof(err)),
+                      err, "get_result");
+    result_set_t r{true, blazing_frame{}, dur, msg, 1};
+    std::vector<gdf_column_cpp> t;
+    for (int i = 0; i < n; i++) {
+      char name[256] = {0};
+      blz_rs_column_name(token, i, name, sizeof(name));
+      t.push_back(gdf_column_cpp::view_of(cols[i], name));
+    }
+    r.result_frame.add_table(t);
+    return r;
+  }
+  bool try_free_result(connection_id_t connection, query_token_t token) {
+    int found = 0;
+    char err[256] = {0};
+    blz_detail::check(blz_rs_try_free_result(connection, token, &found, err, sizeof(err)), err, "try_free_result");
+    return found != 0;
+  }
+  gdf_column_cpp get_column(connection_id_t connection, column_token_t column_token) {
+    gdf_column c{};
+    char err[256] = {0};
+    blz_detail::check(blz_rs_get_column(connection, column_token, &c, err, sizeof(err)), err, "get_column");
+    return gdf_column_cpp::view_of(c);
+  }
+
+  result_set_repository(result_set_repository const &) = delete;
+  void operator=(result_set_repository const &) = delete;
+
+ private:
+  result_set_repository() = default;
+};
 
 #endif  // BLZ_RAL_HPP_

@@ -8,6 +8,11 @@
 //                                  expression planned by add_expression_to_plan
 //                                  against evaluate_expression)
 //        ral_shim_test errors     (no device needed: shows the error behaviour)
+//        ral_shim_test csv FILE   (tests/tpch/tpch_csv_tests.cu TEST_01 over FILE:
+//                                  data_loader + csv_parser, process_filter,
+//                                  execute_project_plan; the frame parked in the
+//                                  result repository and read back; FILE loaded
+//                                  twice by one data_loader)
 // Column i of the frame is generated from the row index alone, so the caller can
 // rebuild the same frame for the oracle:
 //   a[i] = 1 + (i * 7919) % 50                      int32
@@ -17,6 +22,7 @@
 #include <cinttypes>
 #include <cstdio>
 #include <cstring>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -123,7 +129,58 @@ static int interops(int64_t n) {
   return 0;
 }
 
+static int csv_mode(const char *path) {
+  const std::vector<std::string> names = {"c_custkey", "c_name",    "c_address",    "c_nationkey",
+                                          "c_phone",   "c_acctbal", "c_mktsegment", "c_comment"};
+  const std::vector<gdf_dtype> types = {GDF_INT32, GDF_INT64, GDF_INT64, GDF_INT32,
+                                        GDF_INT64, GDF_FLOAT32, GDF_INT64, GDF_INT64};
+  auto parser = std::make_shared<ral::io::csv_parser>("|", "\n", 0, names, types);
+  std::vector<gdf_column_cpp> cols;
+  ral::io::data_loader(parser, {path}).load_data(cols, {});
+  std::printf("rows %d\n", (int)cols[0].size());
+  std::vector<gdf_column_cpp> twice;
+  ral::io::data_loader(parser, {path, path}).load_data(twice, {0, 5});
+  std::printf("rows_twice %d %s %s\n", (int)twice[0].size(), twice[0].name().c_str(), twice[1].name().c_str());
+  blazing_frame frame;
+  frame.add_table(cols);
+  process_filter(frame, "LogicalFilter(condition=[AND(<($0, 150), =($3, 5))])");
+  execute_project_plan(frame, "LogicalProject(c_custkey=[$0], c_nationkey=[$3], c_acctbal=[$5])");
+  auto &repo = result_set_repository::get_instance();
+  const connection_id_t conn = repo.init_session();
+  repo.register_query(conn, 77);
+  repo.update_token(77, frame, 1.5);
+  frame.clear();
+  cols.clear();
+  result_set_t r = repo.get_result(conn, 77);
+  std::printf("duration %g error '%s'\n", r.duration, r.errorMsg.c_str());
+  for (size_t i = 0; i < r.result_frame.get_width(); i++) {
+    gdf_column_cpp &c = r.result_frame.get_column((int)i);
+    std::printf("col %s", c.name().c_str());
+    if (c.dtype() == GDF_FLOAT32) {
+      for (float v : download<float>(c)) std::printf(" %a", (double)v);
+    } else {
+      for (int32_t v : download<int32_t>(c)) std::printf(" %d", v);
+    }
+    std::printf("\n");
+  }
+  std::printf("freed %d\n", (int)repo.try_free_result(conn, 77));
+  try {
+    repo.get_result(conn, 77);
+  } catch (const std::runtime_error &e) {
+    std::printf("runtime_error: %s\n", e.what());
+  }
+  return 0;
+}
+
 int main(int argc, char **argv) {
+  if (argc > 2 && std::strcmp(argv[1], "csv") == 0) {
+    try {
+      return csv_mode(argv[2]);
+    } catch (const std::runtime_error &e) {
+      std::printf("runtime_error: %s\n", e.what());
+      return 2;
+    }
+  }
   if (argc > 2 && std::strcmp(argv[1], "interops") == 0) {
     try {
       return interops(std::stoll(argv[2]));

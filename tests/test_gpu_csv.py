@@ -179,3 +179,32 @@ def test_generated_file_round_trip(B, tmp_path):
     assert rows2 == n
     assert cols2[0].data().tobytes() == days.astype(np.int32).tobytes()
     assert cols2[1].data().tobytes() == a.tobytes()
+
+
+def test_cpp_loader_filter_project_and_result_repository(B, tmp_path):
+    """The C++ operator API (ral.hpp): data_loader over csv_parser reads the
+    reference's customer file (TEST_01 types), process_filter and
+    execute_project_plan run TEST_01, the frame is parked in the result repository
+    and read back: the reference's expected table. The same file loaded twice by
+    one data_loader gives twice the rows."""
+    import subprocess
+    g = json.load(open(GOLD))
+    test = next(t for t in g["tests"] if t["name"] == "TEST_01")
+    path = tmp_path / "customer.psv"
+    path.write_text(g["customer_psv"])
+    shim = os.path.join(os.path.dirname(os.path.abspath(__file__)), "cpp", "ral_shim_test")
+    print("CASE csv cpp loader", flush=True)
+    r = subprocess.run([shim, "csv", str(path)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    lines = r.stdout.splitlines()
+    assert "rows 150" in lines
+    assert "rows_twice 300 c_custkey c_acctbal" in lines
+    assert lines[lines.index("rows 150") + 2] == "duration 1.5 error ''"
+    cols = {ln.split()[1]: ln.split()[2:] for ln in lines if ln.startswith("col ")}
+    assert [int(v) for v in cols["c_custkey"]] == test["expected"][0]["values"]
+    assert [int(v) for v in cols["c_nationkey"]] == test["expected"][1]["values"]
+    got = np.array([float.fromhex(v) for v in cols["c_acctbal"]], dtype=np.float32)
+    want = np.array([_f32_exact(t) for t in test["expected"][2]["text"]], dtype=np.float32)
+    assert got.tobytes() == want.tobytes()
+    assert "freed 1" in lines
+    assert any("Result set does not exist" in ln for ln in lines)
