@@ -117,6 +117,8 @@ def lib():
             "blz_program_time_stages": [P, P, I, C.POINTER(C.c_double), C.POINTER(C.c_int), ERR, SZ],
             "blz_program_info": [P] + [C.POINTER(C.c_int)] * 6,
             "blz_program_destroy": [P],
+            "blz_program_source": [colpp, I, C.c_char_p, C.c_char_p, I, C.c_char_p, SZ, C.POINTER(SZ), ERR, SZ],
+            "blz_jit_compile": [C.c_char_p, ERR, SZ],
             "blz_batch_create": [C.POINTER(P), C.POINTER(BlzQuery), I, I, I, I, ERR, SZ],
             "blz_batch_launch": [P, P, ERR, SZ],
             "blz_batch_result": [P, P, I, C.POINTER(C.c_int64), I32P, colp, I, C.POINTER(C.c_int), ERR, SZ],
@@ -552,6 +554,41 @@ class Program:
             self.close()
         except Exception:
             pass
+
+
+def program_source(frame_dtypes: Sequence[int], n_rows: int, filter_query_part=None, project_query_part=None,
+                   has_valid=None, null_counts=None, scalar_mode=SCALAR_INTENDED) -> str:
+    """The compiled tier's kernel source for a frame of this shape (no device: the
+    column descriptors carry stand-in pointers that are never read)."""
+    n = len(frame_dtypes)
+    has_valid = has_valid or [False] * n
+    null_counts = null_counts or [0] * n
+    cols = []
+    for t, v, nc in zip(frame_dtypes, has_valid, null_counts):
+        g = GdfColumn()
+        g.data = 4096
+        g.valid = 4096 if v else None
+        g.size = n_rows
+        g.dtype = t
+        g.null_count = nc
+        cols.append(g)
+    arr = (C.POINTER(GdfColumn) * n)(*[C.pointer(g) for g in cols])
+    length = C.c_size_t(0)
+    f = filter_query_part.encode() if filter_query_part else None
+    q = project_query_part.encode() if project_query_part else None
+    _call(lib().blz_program_source, arr, n, f, q, scalar_mode, None, 0, C.byref(length))
+    buf = C.create_string_buffer(length.value + 1)
+    _call(lib().blz_program_source, arr, n, f, q, scalar_mode, buf, len(buf), C.byref(length))
+    return buf.value.decode()
+
+
+def jit_compile(src: str):
+    """Compiles a generated kernel source for gfx950 (no device needed); raises
+    BlzError with the compiler's log when it does not compile."""
+    err = C.create_string_buffer(8192)
+    rc = lib().blz_jit_compile(src.encode(), err, len(err))
+    if rc != BLZ_OK:
+        raise BlzError(rc, err.value.decode(errors="replace"))
 
 
 def plan_expression(frame_dtypes: Sequence[int], expression: str, cap=256):

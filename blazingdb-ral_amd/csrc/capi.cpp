@@ -26,6 +26,7 @@
 #include <vector>
 
 #include "../../include/blz/blz_ral.h"
+#include "jit.h"
 #include "kernels.h"
 #include "planner.h"
 #include "program_compiler.h"
@@ -140,7 +141,7 @@ struct LaunchBuffers {
   size_t bytes = 0;
   size_t bd_off = 0;
   size_t zero_off = 0, zero_bytes = 0;
-  size_t keep_off = 0, count_off = 0, idx_off = 0;
+  size_t keep_off = 0, count_off = 0, idx_off = 0, side_off = 0;
   Workspace ws{};
   uint8_t *blob = nullptr;
   VmBatch *bd = nullptr;  // device address of the launch descriptor
@@ -160,7 +161,7 @@ void layout(LaunchBuffers &lb, const Compiled &c, int n_out, bool compact, int64
   lb.zero_off = blob_sz + bd_sz;
   lb.zero_bytes = nc + cnt + ts;
   size_t off = lb.zero_off + lb.zero_bytes;
-  const bool vmf = c.vm && compact;
+  const bool vmf = (c.vm || c.jit) && compact;  // predicate, slot scan, gather
   const int64_t slots = vmf ? n_tiles * 4 : 0;
   lb.n_chunks = (slots + kScanChunk - 1) / kScanChunk;
   lb.keep_off = off;
@@ -169,6 +170,8 @@ void layout(LaunchBuffers &lb, const Compiled &c, int n_out, bool compact, int64
   off += round_up(sizeof(uint32_t) * slots, 256);
   lb.idx_off = off;
   off += round_up(vmf ? sizeof(uint32_t) * n : 0, 256);
+  lb.side_off = off;  // compiled filter: staged predicate columns, n_tiles * T rows each
+  off += round_up(vmf && c.jit ? (size_t)c.jit_side_bytes * n_tiles * c.T : 0, 256);
   lb.bytes = std::max<size_t>(off, 256);
 }
 void bind(LaunchBuffers &lb, int n_out) {
@@ -184,13 +187,14 @@ void bind(LaunchBuffers &lb, int n_out) {
   lb.ws.keep = (uint64_t *)(p + lb.keep_off);
   lb.ws.wave_count = (uint32_t *)(p + lb.count_off);
   lb.ws.idx = (uint32_t *)(p + lb.idx_off);
+  lb.ws.side = p + lb.side_off;
 }
 // Host image of the blob and the single-program launch descriptor, copied to
 // lb.blob in one transfer.
 std::vector<uint8_t> device_image(const LaunchBuffers &lb, const Compiled &c) {
   std::vector<uint8_t> img(lb.zero_off, 0);
   std::memcpy(img.data(), c.blob.data(), c.blob.size());
-  if (c.vm) {
+  if (c.vm || (c.jit && c.T > 0)) {  // the tile VM's kernels, or a compiled filter's slot scan
     const int64_t tiles = (c.n_rows + c.T - 1) / c.T;
     VmBatch b{};
     b.n_plans = 1;
@@ -216,9 +220,9 @@ std::vector<uint8_t> device_image(const LaunchBuffers &lb, const Compiled &c) {
   return img;
 }
 
-// Kernel launches of one execution of a program: 3 for a tile-VM filter
-// (predicate, slot scan, gather), 1 otherwise.
-int program_stages(const Compiled &c, bool compact) { return c.vm && compact ? 3 : 1; }
+// Kernel launches of one execution of a program: 3 for a tile-VM or compiled
+// filter (predicate, slot scan, gather), 1 otherwise.
+int program_stages(const Compiled &c, bool compact) { return (c.vm || c.jit) && compact ? 3 : 1; }
 
 // Enqueues the program's kernels. evs (optional, program_stages + 1 events) are
 // recorded before each kernel and after the last.
@@ -227,6 +231,33 @@ hipError_t launch_program(const Compiled &c, bool compact, int grid, hipStream_t
   auto mark = [&](int i) { return evs ? hipEventRecord(evs[i], st) : hipSuccess; };
   hipError_t e = mark(0);
   if (e != hipSuccess) return e;
+  if (c.jit) {
+    const hipFunction_t f = jit_function(c.jit_src, c.jit_name);
+    if (!compact) {
+      // Projection: blocks of four 64 R-row units per wave, 32 workgroups per CU at most.
+      const int64_t units = (c.n_rows + 64 * c.R - 1) / (64 * c.R);
+      const int64_t waves = (units + 3) / 4;
+      const int64_t g = std::min<int64_t>((waves + 3) / 4, (int64_t)device_cus() * 32);
+      e = launch_jit(f, (int)std::max<int64_t>(1, g), st, blob, ws);
+      return e == hipSuccess ? mark(1) : e;
+    }
+    // Filter: the predicate kernel over blocks of four slots per wave (32 workgroups
+    // per CU at most), the slot scan, then the gather of the kept rows by as many
+    // workgroups as stay resident (the kept count is known on the device only).
+    const hipFunction_t fg = jit_function(c.jit_src, "blz_cg_gather");
+    const int64_t slots = ((c.n_rows + c.T - 1) / c.T) * 4;
+    const int64_t g = std::min<int64_t>((slots + 15) / 16, (int64_t)device_cus() * 32);
+    e = launch_jit(f, (int)std::max<int64_t>(1, g), st, blob, ws);
+    if (e == hipSuccess) e = mark(1);
+    if (e == hipSuccess) e = launch_slot_scan(bd, n_chunks, st);
+    if (e == hipSuccess) e = mark(2);
+    if (e == hipSuccess) {
+      const int64_t chunks = (c.n_rows + 255) / 256;  // most gather chunks (every row kept)
+      const int64_t gg = std::min<int64_t>((chunks + 3) / 4, (int64_t)device_cus() * jit_blocks_per_cu(fg));
+      e = launch_jit(fg, (int)std::max<int64_t>(1, gg), st, blob, ws);
+    }
+    return e == hipSuccess ? mark(3) : e;
+  }
   if (c.vm) {
     const int64_t tiles = (c.n_rows + c.T - 1) / c.T;
     const int64_t chunks = (c.n_rows + 64 * c.R - 1) / (64 * c.R);  // most gather chunks (all rows kept)
@@ -1071,6 +1102,10 @@ blz_program make_program(gdf_column *const *frame, int n_frame, const char *filt
     if (proj_qp && pp.has_eval) req.proj = &pp.spec;
     if (req.inputs.empty()) invalid("nothing to evaluate");
     p->c = compile_program(req);
+    if (p->c.jit) {  // compiled and loaded now, not at launch
+      (void)jit_function(p->c.jit_src, p->c.jit_name);
+      if (has_filter) (void)jit_function(p->c.jit_src, "blz_cg_gather");
+    }
     p->compact = has_filter;
     p->n = n;
     p->n_out = (int)outs.size();
@@ -1116,8 +1151,9 @@ hipError_t program_enqueue(blz_program p, hipStream_t st) {
   hipError_t e = launch_fill_zero(p->lb.base + p->lb.zero_off, (int64_t)p->lb.zero_bytes,
                                   (int)std::min<size_t>(1024, p->lb.zero_bytes / 2048 + 1), st);
   // the direct kernel's compaction ORs boundary validity words: clear the outputs'
-  // validity first (the tile VM writes every word of its outputs whole)
-  for (size_t i = 0; e == hipSuccess && p->compact && !p->c.vm && i < p->outs.size(); i++)
+  // validity first (the tile VM writes every word of its outputs whole; the
+  // compiled filter clears each bit range it owns before setting it)
+  for (size_t i = 0; e == hipSuccess && p->compact && !p->c.vm && !p->c.jit && i < p->outs.size(); i++)
     if (p->outs[i].valid) {
       const size_t vb = gdf_valid_allocation_size((gdf_size_type)p->n);
       e = launch_fill_zero(p->outs[i].valid, (int64_t)vb, (int)std::min<size_t>(1024, vb / 2048 + 1), st);
@@ -1210,7 +1246,7 @@ int blz_program_tile(blz_program prog, int *tile_rows, int *lds_bytes, int *kern
   if (!prog) return BLZ_ERR_INVALID;
   if (tile_rows) *tile_rows = prog->c.T;
   if (lds_bytes) *lds_bytes = prog->c.lds_bytes;
-  if (kernel) *kernel = prog->c.vm ? 4 : 3;
+  if (kernel) *kernel = prog->c.jit ? 5 : prog->c.vm ? 4 : 3;
   return BLZ_OK;
 }
 
@@ -1221,6 +1257,70 @@ int blz_program_predicate(blz_program prog, int *direct, int *terms, int *wmask)
   if (terms) *terms = d ? prog->c.vm_direct_terms : 0;
   if (wmask) *wmask = d ? prog->c.vm_direct_wmask : 0;
   return BLZ_OK;
+}
+
+// The kernel source the compiled tier writes for a plan over a frame of this shape
+// (dtypes, bitmaps present, null counts, rows); nothing is allocated or launched,
+// column data is not read.
+int blz_program_source(gdf_column *const *frame, int n_frame, const char *filter_qp, const char *proj_qp,
+                       int scalar_mode, char *src, size_t src_cap, size_t *src_len, char *err, size_t err_len) {
+  return guarded(err, err_len, [&] {
+    check_frame(frame, n_frame);
+    const int64_t n = frame[0]->size;
+    const auto dt = frame_dtypes(frame, n_frame);
+    InputMap im(n_frame);
+    planner::ExprPlan fp;
+    const bool has_filter = filter_qp != nullptr;
+    if (has_filter) {
+      fp = planner::plan_expression(dt, condition_of(filter_qp));
+      if (fp.bare) im.use(frame, fp.bare_col);
+      else im.remap(frame, fp.spec);
+    }
+    planner::ProjectPlan pp;
+    std::vector<planner::ProjOut> outs;
+    if (proj_qp) {
+      pp = planner::parse_project(dt, proj_qp);
+      if (pp.has_eval) im.remap(frame, pp.spec);
+      outs = pp.outputs;
+    } else {
+      for (int i = 0; i < n_frame; i++) outs.push_back({1, i, dt[i], ""});
+    }
+    CompileRequest req;
+    req.n_rows = n;
+    req.scalar_mode = scalar_mode;
+    req.compact = has_filter;
+    req.kernel = 5;
+    // stand-in output buffers: the source depends only on which are present
+    static uint8_t stand_in[64];
+    for (const auto &o : outs) {
+      const bool with_valid = o.kind == 0 ? true : frame[o.index]->valid != nullptr;
+      if (o.kind == 0) req.outputs.push_back({stand_in, with_valid ? stand_in : nullptr, o.dtype, 0, o.index});
+      else req.outputs.push_back({stand_in, with_valid ? stand_in : nullptr, o.dtype, 1, im.use(frame, o.index)});
+    }
+    req.inputs = im.inputs;
+    if (has_filter) {
+      if (fp.bare) req.bare_stencil_input = im.frame_to_dev[fp.bare_col];
+      else req.pred = &fp.spec;
+    }
+    if (proj_qp && pp.has_eval) req.proj = &pp.spec;
+    if (req.inputs.empty()) invalid("nothing to evaluate");
+    const Compiled c = compile_program(req);
+    if (src_len) *src_len = c.jit_src.size();
+    if (src && src_cap > 0) {
+      const size_t k = std::min(src_cap - 1, c.jit_src.size());
+      std::memcpy(src, c.jit_src.data(), k);
+      src[k] = 0;
+    }
+  });
+}
+
+// Compiles a kernel source for the device (no device needed): 0 when it compiles,
+// the compiler's log in err otherwise.
+int blz_jit_compile(const char *src, char *err, size_t err_len) {
+  return guarded(err, err_len, [&] {
+    if (!src) invalid("src is NULL");
+    jit_compile_only(src);
+  });
 }
 
 int blz_program_destroy(blz_program prog) {

@@ -10,8 +10,12 @@
 #ifndef BLZ_DEVICE_COMMON_H_
 #define BLZ_DEVICE_COMMON_H_
 
+// Run-time compiled kernels (hiprtc) get the HIP runtime and fixed-width types
+// from the compiler itself.
+#ifndef __HIPCC_RTC__
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#endif
 
 #include "program.h"
 
@@ -180,6 +184,43 @@ DEV void store_narrow(int kind, void *base, int64_t row, uint64_t v) {
   }
 }
 
+// Single-pass decoupled look-back, run by one wave: publishes `total` for block b
+// of a sequence whose state words (bits 62-63: 0 nothing yet, 1 aggregate, 2
+// inclusive prefix; bits 0-61 the value) are zero before the pass, and returns the
+// exclusive prefix of block b (in every lane). Blocks are taken in the order
+// workgroups start, so each block it waits for belongs to a running workgroup.
+DEV uint64_t lookback_prefix(unsigned long long *state, int64_t b, uint64_t total, int lane) {
+  constexpr uint64_t FA = 1ull << 62, FP = 2ull << 62, VMASK = (1ull << 62) - 1;
+  if (b == 0) {
+    if (lane == 0) __hip_atomic_store(state, FP | total, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return 0;
+  }
+  if (lane == 0) __hip_atomic_store(state + b, FA | total, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  uint64_t pre = 0;
+  int64_t look = b - 1;
+  while (true) {
+    const int64_t i = look - lane;
+    uint64_t st = FP;
+    if (i >= 0) st = __hip_atomic_load(state + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const uint64_t flag = st >> 62;
+    const uint64_t is_x = __builtin_amdgcn_ballot_w64(flag == 0);
+    const uint64_t is_p = __builtin_amdgcn_ballot_w64(flag == 2);
+    // lanes up to and including the nearest inclusive prefix
+    const uint64_t upto = is_p ? (((is_p & (~is_p + 1)) << 1) - 1) : ~0ull;
+    if (is_x & upto) {
+      __builtin_amdgcn_s_sleep(1);
+      continue;
+    }
+    uint64_t x = ((upto >> lane) & 1) ? (st & VMASK) : 0;
+    for (int s = 32; s > 0; s >>= 1) x += __shfl_xor(x, s);
+    pre += x;
+    if (is_p) break;
+    look -= 64;
+  }
+  if (lane == 0) __hip_atomic_store(state + b, FP | (pre + total), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  return pre;
+}
+
 // Every op outside the hot set (division, floor/ceil/abs, integer power and
 // modulo, date parts, transcendentals): out of line and one row at a time, so
 // their long instruction sequences do not set the interpreter's register budget.
@@ -267,6 +308,67 @@ DEV void exec_op(int code, const uint64_t (&A)[R], const uint64_t (&B)[R], const
       break;
   }
 #undef BLZ_CASE
+}
+
+// One fully typed op for one row, every code inline: the run-time compiled
+// kernels call it with a constant `code`, so it folds to that op's expression (the
+// same expressions as exec_op and slow_op).
+DEV uint64_t op_eval(int code, uint64_t a, uint64_t b, bool vb) {
+  const double x = b2d(a), y = b2d(b);
+  switch (code) {
+    case OP_ADD_I: return a + b;
+    case OP_ADD_F: return d2b(x + y);
+    case OP_SUB_I: return a - b;
+    case OP_SUB_F: return d2b(x - y);
+    case OP_MUL_I: return a * b;
+    case OP_MUL_F: return d2b(x * y);
+    case OP_DIV_I: return (uint64_t)idiv((int64_t)a, (int64_t)b);
+    case OP_DIV_F: return d2b(x / y);
+    case OP_MOD_I: return (uint64_t)imod((int64_t)a, (int64_t)b);
+    case OP_POW_I: return ipow(a, (int64_t)b);
+    case OP_POW_F: return d2b(pow(x, y));
+    case OP_EQ_I: return (uint64_t)(a == b);
+    case OP_EQ_F: return (uint64_t)(x == y);
+    case OP_NE_I: return (uint64_t)(a != b);
+    case OP_NE_F: return (uint64_t)(x != y);
+    case OP_LT_I: return (uint64_t)((int64_t)a < (int64_t)b);
+    case OP_LT_F: return (uint64_t)(x < y);
+    case OP_GT_I: return (uint64_t)((int64_t)a > (int64_t)b);
+    case OP_GT_F: return (uint64_t)(x > y);
+    case OP_LE_I: return (uint64_t)((int64_t)a <= (int64_t)b);
+    case OP_LE_F: return (uint64_t)(x <= y);
+    case OP_GE_I: return (uint64_t)((int64_t)a >= (int64_t)b);
+    case OP_GE_F: return (uint64_t)(x >= y);
+    case OP_COALESCE: return vb ? a : b;
+    case OP_AND_B: return a & b;
+    case OP_FLOOR: return d2b(floor(x));
+    case OP_CEIL: return d2b(ceil(x));
+    case OP_SIN: return d2b(sin(x));
+    case OP_COS: return d2b(cos(x));
+    case OP_ASIN: return d2b(asin(x));
+    case OP_ACOS: return d2b(acos(x));
+    case OP_TAN: return d2b(tan(x));
+    case OP_COTAN: return d2b(cos(x) / sin(x));
+    case OP_ATAN: return d2b(atan(x));
+    case OP_ABS: return d2b(fabs(x));
+    case OP_LN: return d2b(log(x));
+    case OP_LOG10: return d2b(log10(x));
+    case OP_NOT_I: return (uint64_t)(a == 0);
+    case OP_NOT_F: return (uint64_t)(!(x != 0.0));
+    case OP_YEAR32: return (uint64_t)year_d((int64_t)a);
+    case OP_MONTH32: return (uint64_t)month_d((int64_t)a);
+    case OP_DAY32: return (uint64_t)day_d((int64_t)a);
+    case OP_YEAR64: return (uint64_t)year_d(ms_days((int64_t)a));
+    case OP_MONTH64: return (uint64_t)month_d(ms_days((int64_t)a));
+    case OP_DAY64: return (uint64_t)day_d(ms_days((int64_t)a));
+    case OP_HOUR64: return (uint64_t)hour_ms((int64_t)a);
+    case OP_MINUTE64: return (uint64_t)minute_ms((int64_t)a);
+    case OP_SECOND64: return (uint64_t)second_ms((int64_t)a);
+    case OP_ZERO: return 0ull;
+    case OP_IS_NULL: return (uint64_t)(!vb);
+    case OP_IS_NOT_NULL: return (uint64_t)vb;
+    default: return a;  // OP_IDENT
+  }
 }
 
 // Validity program: one 64-row word per lane (lanes [0, R) of each wave).

@@ -9,19 +9,10 @@
 #include <mutex>
 #include <stdint.h>
 
+#include "program.h"
+
 namespace blz {
 
-// Per-launch device scratch. Every counter must be zero at launch.
-struct Workspace {
-  unsigned long long *null_counts;  // [n_out] nulls per output (outputs with validity)
-  unsigned int *tile_counter;       // compaction: dynamic tile ids
-  unsigned long long *tile_state;   // compaction: [n_tiles] look-back state words
-  unsigned long long *n_sel;        // compaction: rows kept
-  // tile VM filter (written before they are read; no zeroing needed)
-  uint64_t *keep;                   // keep word per 64-row group
-  uint32_t *wave_count;             // kept rows per (tile, wave) slot
-  uint32_t *idx;                    // row of each kept row, in output order (n_sel entries)
-};
 
 // Register-file interpreter (project or fused filter+project/gather), the general
 // fallback for plans outside the tile VM's limits.
@@ -69,7 +60,8 @@ int vm_blocks_per_cu(int r, int mode, int ops, int lds_bytes);
 // counts as launch_vm mode 1.
 // n_terms <= 3; wmask bit k: term k reads an 8-byte column.
 hipError_t launch_pred_direct(const VmBatch *bd, int n_terms, int wmask, int grid, hipStream_t st);
-// Exclusive prefix of every program's slot counts (ws.wave_count; total to
+// Exclusive prefix of every program's slot counts (ws.wave_count, written over the
+// counts; total to
 // ws.n_sel) and, from it and the keep words, the row of every kept row in output
 // order (ws.idx): chunks of kScanChunk slots, combined within a program by a single-pass
 // decoupled look-back over its ws.tile_state (zero before the scan).

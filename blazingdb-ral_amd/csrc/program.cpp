@@ -27,7 +27,9 @@
 #include <map>
 #include <stdexcept>
 
+#include "codegen.h"
 #include "kernels.h"
+#include "ssa.h"
 
 namespace blz {
 namespace {
@@ -145,75 +147,6 @@ int store_kind_alias(int dtype) {
   if (dtype == GDF_BOOL8) return ST_I8;
   return store_kind_program(dtype);
 }
-
-// ---- SSA graph ----
-struct Operand {
-  bool imm = false;
-  uint64_t bits = 0;
-  int node = -1;  // value node when !imm
-};
-struct Node {
-  int kind = 0;  // 0 input column (dev input = aux), 1 op, 2 constant zero
-  int aux = 0;
-  DevOp op{};
-  Operand a, b;
-  bool is_bool = false;
-  int program = 0;  // 0 predicate, 1 projection
-  // scheduling / allocation
-  int need = 0;
-  int order = -1;
-  int last_use = -1;
-  int slot = -1;
-};
-
-// Validity values: -1 all valid, -2 all null, >= 0 dynamic vnode id.
-constexpr int V_ALL = -1, V_NONE = -2;
-struct VNode {
-  int kind = 0;  // 0 raw bitmap of dev input aux, 1 op
-  int aux = 0;
-  int code = VOP_AND;
-  int a = 0, b = 0;
-  int program = 0;
-  int vslot = -1;
-  bool needed = false;
-  int last_use = -1;
-  int order = -1;
-};
-
-struct Builder {
-  std::vector<Node> nodes;
-  std::vector<VNode> vnodes;
-  std::vector<int> input_node;   // dev input -> node id
-  std::vector<int> input_vnode;  // dev input -> vnode id (raw bitmap)
-  int zero_node = -1;
-
-  int in_node(int c) { return input_node[c]; }
-  int zero() {
-    if (zero_node < 0) {
-      Node z; z.kind = 2; z.op.code = OP_ZERO;
-      nodes.push_back(z);
-      zero_node = (int)nodes.size() - 1;
-    }
-    return zero_node;
-  }
-  int vraw(int c) { return input_vnode[c]; }
-  int vcombine(int code, int a, int b, int program) {
-    if (code == VOP_AND) {
-      if (a == V_NONE || b == V_NONE) return V_NONE;
-      if (a == V_ALL) return b;
-      if (b == V_ALL) return a;
-      if (a == b) return a;
-    } else {
-      if (a == V_ALL || b == V_ALL) return V_ALL;
-      if (a == V_NONE) return b;
-      if (b == V_NONE) return a;
-      if (a == b) return a;
-    }
-    VNode v; v.kind = 1; v.code = code; v.a = a; v.b = b; v.program = program;
-    vnodes.push_back(v);
-    return (int)vnodes.size() - 1;
-  }
-};
 
 struct PlanResult {
   std::vector<int> finals;   // value node per final output
@@ -1440,6 +1373,7 @@ Compiled compile_program(const CompileRequest &req) {
   }
 
   if (req.kernel == 1 || req.kernel == 2) fail("kernel kind not available (1 and 2 are superseded by 4)");
+  if (req.kernel < 0 || req.kernel > 5) fail("unknown kernel kind");
   // ---- Tile VM: the accumulator form, pre-decoded, over LDS-staged tiles ----
   if (req.kernel == 4 && !acc_ok) fail("plan does not fit the tile VM");
   if (acc_ok && (req.kernel == 0 || req.kernel == 4)) {
@@ -1565,6 +1499,78 @@ Compiled compile_program(const CompileRequest &req) {
   out.T = 0;
   out.lds_bytes = 0;
   out.acc = false;
+  if (req.kernel == 5) {
+    // ---- compiled tier: the SSA graph written out as a kernel ----
+    CodegenInput ci;
+    ci.B = &B;
+    ci.order_p = &order_p;
+    ci.order_q = &order_q;
+    ci.compact = req.compact;
+    ci.stencil_node = stencil_node;
+    ci.stencil_v = stencil_v;
+    ci.stencil_width = h.stencil_width;
+    for (int c = 0; c < n_dev; c++) {
+      ci.in_load.push_back(load_kind(req.inputs[c].dtype));
+      ci.in_width.push_back(wid(req.inputs[c].dtype));
+      ci.in_valid.push_back(req.inputs[c].valid != nullptr);
+    }
+    for (int o = 0; o < n_out; o++) {
+      const OutSpec &s = req.outputs[o];
+      ci.out_node.push_back(out_node[o]);
+      ci.out_v.push_back(out_v[o]);
+      ci.out_kind.push_back(s.kind);
+      ci.out_store.push_back(s.kind == 0 ? store_kind_program(s.dtype) : store_kind_alias(s.dtype));
+      ci.out_input.push_back(s.kind == 1 ? s.index : -1);
+      ci.out_valid.push_back(s.valid != nullptr);
+    }
+    // Rows per lane: about 64 bytes per lane of the columns a step holds (a filter's
+    // predicate and projection columns, a projection's), so every load of a step is
+    // in flight without spilling (one 4-byte column: 16 rows, 16 bytes per row: 8,
+    // 48: 2), unless forced (r).
+    int bpr = 0;
+    {
+      std::vector<bool> used(n_dev, false);
+      const std::vector<int> &ord = req.compact ? order_p : order_q;
+      for (int id : ord) {
+        const Node &nd = B.nodes[id];
+        if (nd.kind != 1) continue;
+        if (!nd.a.imm && nd.a.node >= 0 && nd.a.node < n_dev) used[nd.a.node] = true;
+        if (!nd.b.imm && nd.b.node >= 0 && nd.b.node < n_dev) used[nd.b.node] = true;
+      }
+      if (req.compact && stencil_node >= 0 && stencil_node < n_dev) used[stencil_node] = true;
+      if (req.compact)  // the kept rows' projection columns are held too
+        for (int id : order_q) {
+          const Node &nd = B.nodes[id];
+          if (nd.kind != 1) continue;
+          if (!nd.a.imm && nd.a.node >= 0 && nd.a.node < n_dev) used[nd.a.node] = true;
+          if (!nd.b.imm && nd.b.node >= 0 && nd.b.node < n_dev) used[nd.b.node] = true;
+        }
+      for (int o = 0; o < n_out; o++) {
+        if (req.outputs[o].kind == 1) used[req.outputs[o].index] = true;
+        else if (out_node[o] < n_dev) used[out_node[o]] = true;
+      }
+      for (int c = 0; c < n_dev; c++)
+        if (used[c]) bpr += wid(req.inputs[c].dtype);
+    }
+    int rs = bpr <= 4 ? 16 : bpr <= 16 ? 8 : bpr <= 32 ? 4 : 2;
+    if (req.force_r > 0) {
+      if (req.force_r != 1 && req.force_r != 2 && req.force_r != 4 && req.force_r != 8 && req.force_r != 16)
+        fail("compiled kernel: rows per lane must be 1, 2, 4, 8 or 16");
+      rs = req.force_r;
+    }
+    ci.rs = rs;
+    const CodegenOutput g = generate_kernel_source(ci);
+    out.jit = true;
+    out.jit_src = g.src;
+    out.jit_name = g.name;
+    out.jit_side_bytes = g.side_bytes;
+    out.R = g.rs;
+    out.T = req.compact ? g.tile_rows : 0;
+    if (req.compact) {  // the slot scan reads the slot shape from the header
+      h.tile_rows = out.T;
+      put(out.blob, 0, h);
+    }
+  }
   return out;
 }
 

@@ -260,6 +260,22 @@ struct OutCol {
   int32_t vbuf;          // staged compaction: index of this output's LDS validity buffer
 };
 
+// Per-launch device scratch. Every counter must be zero at launch.
+struct Workspace {
+  unsigned long long *null_counts;  // [n_out] nulls per output (outputs with validity)
+  unsigned int *tile_counter;       // compaction: dynamic tile ids
+  unsigned long long *tile_state;   // compaction: [n_tiles] look-back state words
+  unsigned long long *n_sel;        // compaction: rows kept
+  // tile VM filter (written before they are read; no zeroing needed)
+  uint64_t *keep;                   // keep word per 64-row group
+  uint32_t *wave_count;             // kept rows per (tile, wave) slot
+  uint32_t *idx;                    // row of each kept row, in output order (n_sel entries)
+  // compiled filter: the kept rows' values of the predicate columns the projection
+  // reads, compacted per slot (slot s's kept rows at s * slot_rows ..), one array of
+  // n_tiles * tile_rows values per column, back to back
+  uint8_t *side;
+};
+
 struct ProgHeader {
   int64_t n_rows;
   int32_t n_in, n_out;

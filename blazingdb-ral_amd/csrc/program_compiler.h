@@ -50,7 +50,8 @@ struct CompileRequest {
   int scalar_mode = BLZ_SCALAR_INTENDED;
   int force_s = 0, force_r = 0;     // register-file shape override (autotuner)
   int force_t = 0;                  // staged tile rows override (autotuner); -1 = direct kernel
-  int kernel = 0;                   // 0 automatic, 1 accumulator, 2 staged register file, 3 direct, 4 tile VM
+  int kernel = 0;                   // 0 automatic, 1 accumulator, 2 staged register file, 3 direct, 4 tile VM,
+                                    // 5 compiled (generated kernel, run-time compiled)
   int lds_budget = 64 * 1024;       // dynamic LDS per workgroup the staged kernel may use
 };
 
@@ -72,6 +73,12 @@ struct Compiled {
   bool vm_direct_pred = false;            // tile VM filter: predicate by launch_pred_direct
   int vm_direct_terms = 0;                //   with this many terms (the predicate's ops),
   int vm_direct_wmask = 0;                //   bit k: term k reads an 8-byte column
+  // compiled tier (kernel 5): generated kernel source and name (codegen.h); the
+  // blob is the register-file kernel's (header, input and output tables), T a
+  // filter's rows per tile (4 slots of 64 R rows, as the tile VM's)
+  bool jit = false;
+  std::string jit_src, jit_name;  // projection kernel, or a filter's predicate kernel
+  int jit_side_bytes = 0;         // filter: bytes per row of staged predicate columns
 };
 
 // Throws std::runtime_error with a message on an unsupported or malformed plan.

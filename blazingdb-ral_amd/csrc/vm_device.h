@@ -785,43 +785,6 @@ DEV uint64_t vm_vword(int vsrc, const LDSAS uint64_t *vr, int u) {
 //               into lanes, and store them, and their validity words, in row order.
 enum VmMode { VM_PROJECT = 0, VM_PRED = 1, VM_GATHER = 2 };
 
-// Single-pass decoupled look-back, run by one wave: publishes `total` for block b
-// of a sequence whose state words (bits 62-63: 0 nothing yet, 1 aggregate, 2
-// inclusive prefix; bits 0-61 the value) are zero before the pass, and returns the
-// exclusive prefix of block b (in every lane). Blocks are taken in the order
-// workgroups start, so each block it waits for belongs to a running workgroup.
-DEV uint64_t lookback_prefix(unsigned long long *state, int64_t b, uint64_t total, int lane) {
-  constexpr uint64_t FA = 1ull << 62, FP = 2ull << 62, VMASK = (1ull << 62) - 1;
-  if (b == 0) {
-    if (lane == 0) __hip_atomic_store(state, FP | total, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    return 0;
-  }
-  if (lane == 0) __hip_atomic_store(state + b, FA | total, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  uint64_t pre = 0;
-  int64_t look = b - 1;
-  while (true) {
-    const int64_t i = look - lane;
-    uint64_t st = FP;
-    if (i >= 0) st = __hip_atomic_load(state + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    const uint64_t flag = st >> 62;
-    const uint64_t is_x = __builtin_amdgcn_ballot_w64(flag == 0);
-    const uint64_t is_p = __builtin_amdgcn_ballot_w64(flag == 2);
-    // lanes up to and including the nearest inclusive prefix
-    const uint64_t upto = is_p ? (((is_p & (~is_p + 1)) << 1) - 1) : ~0ull;
-    if (is_x & upto) {
-      __builtin_amdgcn_s_sleep(1);
-      continue;
-    }
-    uint64_t x = ((upto >> lane) & 1) ? (st & VMASK) : 0;
-    for (int s = 32; s > 0; s >>= 1) x += __shfl_xor(x, s);
-    pre += x;
-    if (is_p) break;
-    look -= 64;
-  }
-  if (lane == 0) __hip_atomic_store(state + b, FP | (pre + total), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  return pre;
-}
-
 // Position of the r-th (0-based) set bit of m (r < popcount(m)).
 DEV int select_bit(uint64_t m, int r) {
   int p = 0;

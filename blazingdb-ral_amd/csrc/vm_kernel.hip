@@ -19,7 +19,8 @@ BLZ_VM_SHAPES(BLZ_VM_DECL)
 #undef BLZ_VM_DECL
 
 // Exclusive prefix of every program's per-wave-slot kept counts (VM_PRED) in one
-// pass, expanded into the row index of every kept row in output order: each
+// pass (written over the counts), expanded into the row index of every kept row in
+// output order: each
 // workgroup scans a chunk of 512 counts of one program and combines its total
 // with the chunks before it by a single-pass decoupled look-back over that
 // program's state words. Chunks are taken in the order workgroups start, so
@@ -84,6 +85,9 @@ slot_scan_kernel(const VmBatch *__restrict__ bd_arg) {
 #pragma unroll
   for (int i = 0; i < kScanItems; i++) {
     s_pre[threadIdx.x * kScanItems + i] = (uint32_t)run;
+    // each slot's count is replaced by its exclusive prefix (the compiled gather
+    // finds a kept row's place in its slot's staged values by it)
+    if (base + i < n_slots) ws.wave_count[base + i] = (uint32_t)run;
     run += v[i];
   }
   __syncthreads();

@@ -182,7 +182,9 @@ typedef struct blz_program_s *blz_program;
  * them (capacity: every frame row; sizes valid after blz_program_result).
  * kernel: 0 automatic (the tile VM when the plan fits it, else the direct kernel),
  * 3 direct register-file kernel, 4 tile VM (direct-to-LDS tile staging;
- * a filter runs as predicate, slot-scan and gather launches). r / t (0 =
+ * a filter runs as predicate, slot-scan and gather launches), 5 compiled: the
+ * plan written out as a kernel and compiled for the device at creation (one
+ * launch; a filter keeps rows in one pass with a decoupled look-back). r / t (0 =
  * automatic) force the tile VM's rows per lane (2, 4, 8) and tile rows (256 r P);
  * s / r force the direct kernel's slots x rows (8,1) (8,2) (16,1) (32,1) (64,1). */
 int blz_program_create(blz_program *prog, gdf_column *const *frame, int n_frame,
@@ -214,6 +216,16 @@ int blz_program_tile(blz_program prog, int *tile_rows, int *lds_bytes, int *kern
  * term k reads an 8-byte column). */
 int blz_program_predicate(blz_program prog, int *direct, int *terms, int *wmask);
 int blz_program_destroy(blz_program prog);
+/* The kernel source the compiled tier (kernel 5) writes for a filter / projection
+ * over a frame of this shape (dtypes, bitmaps present, null counts, rows); nothing
+ * is allocated or launched and column data is not read. *src_len receives the full
+ * length; src (src_cap bytes) the text, truncated. No reference counterpart. */
+int blz_program_source(gdf_column *const *frame, int n_frame, const char *filter_query_part,
+                       const char *project_query_part, int scalar_mode, char *src, size_t src_cap,
+                       size_t *src_len, char *err, size_t err_len);
+/* Compiles a generated kernel source for gfx950 without a device (the compiled
+ * tier's compiler, for inspection and tests). No reference counterpart. */
+int blz_jit_compile(const char *src, char *err, size_t err_len);
 
 /* ---- batches (no reference counterpart: many independent queries, one launch) ----
  *

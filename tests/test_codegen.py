@@ -1,0 +1,77 @@
+"""The compiled tier's kernel sources (no GPU): generated for random plans and the
+benchmark workloads, and compiled for gfx950 with the library's run-time
+compiler, so a plan shape the generator writes wrongly fails here, not on a
+device."""
+import numpy as np
+import pytest
+
+from tests import exprgen
+from oracle import planner as P
+
+C1_FILTER = ("LogicalFilter(condition=[AND(>=($3, 730), <($3, 1095), >=($2, 0.03), "
+             "<=($2, 0.05), <($0, 30))])")
+C1_PROJECT = "LogicalProject(EXPR$0=[*($1, -(1, $2))], EXPR$1=[+($0, 1)])"
+
+
+@pytest.fixture(scope="module")
+def B():
+    import blz_ral
+    blz_ral.lib()
+    return blz_ral
+
+
+def test_c1_source_shape(B):
+    src = B.program_source([P.GDF_INT32, P.GDF_FLOAT64, P.GDF_FLOAT64, P.GDF_INT32], 1000, C1_FILTER,
+                           C1_PROJECT)
+    assert "blz_cg_pred" in src and "blz_cg_gather" in src
+    # the predicate kernel stages a and c (the projection reads them) for the kept
+    # rows; the gather reads b at the kept rows' source rows
+    import re
+    assert len(re.findall(r"s\d\[p\] = r\d\[u\];", src)) == 2
+    assert len(re.findall(r"__builtin_nontemporal_load\(c\d \+ row\[u\]\)", src)) == 1
+    B.jit_compile(src)
+
+
+def test_projection_source_compiles(B):
+    from blz_ral import workloads as W
+    dts = [c[0] for c in W.WIDE_COLUMNS]
+    src = B.program_source(dts, 1000, None, W.WIDE_PROJECT, has_valid=[True] * len(dts),
+                           null_counts=[5] * len(dts))
+    assert "blz_cg_project" in src and "store_vword" in src
+    B.jit_compile(src)
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_random_plans_compile(B, seed):
+    """Random filter + projection plans over nullable frames of every integer and
+    float dtype (COALESCE, IS [NOT] NULL, POWER, the transcendentals' dtypes)."""
+    rng = np.random.default_rng(900 + seed)
+    k = int(rng.integers(2, 6))
+    dts = exprgen.rand_frame_dtypes(rng, k)
+    has_valid = [bool(rng.random() < 0.5) for _ in range(k)]
+    nullable = [i for i in range(k) if has_valid[i]]
+    cond = f"<({exprgen.rand_expr(rng, dts, 3, True, nullable)}, {int(rng.integers(-5, 6))})"
+    exprs = [exprgen.rand_expr(rng, dts, 3, True, nullable) for _ in range(int(rng.integers(1, 4)))]
+    items = [f"E{i}=[{e}]" for i, e in enumerate(exprs)] + [f"A=[${int(rng.integers(0, k))}]"]
+    proj = "LogicalProject(" + ", ".join(items) + ")"
+    filt = f"LogicalFilter(condition=[{cond}])" if seed % 2 == 0 else None
+    try:
+        src = B.program_source(dts, 5000, filt, proj, has_valid=has_valid, null_counts=[3] * k)
+    except B.BlzError as e:
+        pytest.skip(f"plan rejected by the planner: {e}")
+    B.jit_compile(src)
+
+
+def test_transcendental_and_date_sources_compile(B):
+    dts = [P.GDF_FLOAT64, P.GDF_DATE32, P.GDF_DATE64, P.GDF_INT32]
+    proj = ("LogicalProject(a=[SIN($0)], b=[LN(ABS($0))], c=[EXTRACT(FLAG(YEAR), $1)], "
+            "d=[EXTRACT(FLAG(HOUR), $2)], e=[POWER($3, 3)], f=[POWER($0, 0.5)], g=[LOG10($0)], "
+            "h=[EXTRACT(FLAG(MINUTE), $2)])")
+    src = B.program_source(dts, 777, "LogicalFilter(condition=[>($3, 0)])", proj)
+    B.jit_compile(src)
+
+
+def test_bad_source_reports_the_compiler_log(B):
+    with pytest.raises(B.BlzError) as e:
+        B.jit_compile("this is not HIP")
+    assert "hiprtc" in str(e.value)

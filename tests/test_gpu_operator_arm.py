@@ -155,7 +155,7 @@ def _trig_frame(rng, n):
 
 
 @pytest.mark.parametrize("mode", ["project", "filter"])
-@pytest.mark.parametrize("kernel", [4, 3])
+@pytest.mark.parametrize("kernel", [4, 3, 5])
 @pytest.mark.parametrize("col", [0, 1, 2, 3], ids=["f64", "f32", "i32", "i64"])
 def test_transcendentals_within_one_ulp(B, col, kernel, mode):
     """SIN ... LOG10 of a column (binary64 through the register slot, narrowed to the
@@ -209,7 +209,7 @@ POW_INT = ["POWER($4, $5)", "POWER($5, 3)", "POWER($4, 2)", "POWER($5, $4)"]
 
 
 @pytest.mark.parametrize("mode", ["project", "filter"])
-@pytest.mark.parametrize("kernel", [4, 3])
+@pytest.mark.parametrize("kernel", [4, 3, 5])
 def test_power(B, kernel, mode):
     """POWER: pow((double)L, (double)R) when an operand is floating point (within 1
     ulp), else the reference's loop `d = 1; for (i < R) d *= L` in wrapping int64
@@ -264,7 +264,7 @@ def _date_frame(rng, n):
 
 
 @pytest.mark.parametrize("mode", ["project", "filter"])
-@pytest.mark.parametrize("kernel", [4, 3])
+@pytest.mark.parametrize("kernel", [4, 3, 5])
 @pytest.mark.parametrize("col", [0, 1, 2], ids=["date32", "date64", "timestamp"])
 def test_date_parts(B, col, kernel, mode):
     """EXTRACT(FLAG(part), $k) for every part over DATE32 (days; HOUR / MINUTE /
@@ -297,7 +297,7 @@ def test_date_parts_in_a_predicate(B):
     proj = "LogicalProject(D=[$0], Y=[EXTRACT(FLAG(YEAR), $2)], S=[EXTRACT(FLAG(SECOND), $1)])"
     _print_case(test="date_predicate", n=n)
     ref = O.execute_project(O.process_filter(host, filt), proj)
-    for kernel in (0, 3):
+    for kernel in (0, 3, 5):
         kept, outs = _run_program(B, host, filt, proj, kernel if kernel else 4)
         assert kept == len(ref[0])
         for (gd, gv, nc), r in zip(outs, ref):

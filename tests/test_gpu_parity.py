@@ -223,7 +223,7 @@ def test_program_matches_one_shot(B):
     prog.close()
 
 
-@pytest.mark.parametrize("kernel", [3, 4])
+@pytest.mark.parametrize("kernel", [3, 4, 5])
 @pytest.mark.parametrize("seed", range(8))
 def test_kernel_kinds_random_filter_project(B, kernel, seed):
     """Both kernel kinds (direct register file / tile VM) on random plans."""
@@ -290,7 +290,7 @@ def test_range_conjunctions(B, cond):
     _print_case(test="range_conjunctions", cond=cond)
     ref = _oracle(host, filt, proj)
     dev = [to_device(B, h) for h in host]
-    for kernel in (0, 3):
+    for kernel in (0, 3, 5):
         prog = B.Program(dev, filt, proj, kernel=kernel)
         prog.launch()
         kept, nulls = prog.result()
@@ -317,7 +317,7 @@ def test_range_conjunction_bitmap_null_count(B, nc0):
     _print_case(test="range_conjunction_bitmap_null_count", nc0=nc0)
     ref = _oracle(host, filt, proj)
     dev = [to_device(B, h) for h in host]
-    for kernel in (0, 3):
+    for kernel in (0, 3, 5):
         prog = B.Program(dev, filt, proj, kernel=kernel)
         prog.launch()
         kept, nulls = prog.result()
@@ -332,7 +332,7 @@ WIDE_MIX = [P.GDF_INT32, P.GDF_INT32, P.GDF_INT64, P.GDF_INT64, P.GDF_FLOAT32, P
             P.GDF_FLOAT64, P.GDF_FLOAT64]
 
 
-@pytest.mark.parametrize("kernel", [0, 3, 4])
+@pytest.mark.parametrize("kernel", [0, 3, 4, 5])
 @pytest.mark.parametrize("seed", range(10))
 def test_wide_workload_queries(B, kernel, seed):
     """Queries of the batched / served workloads (4-40 ops over eight nullable
@@ -358,7 +358,7 @@ def test_wide_workload_queries(B, kernel, seed):
     prog.close()
 
 
-@pytest.mark.parametrize("kernel", [3, 4])
+@pytest.mark.parametrize("kernel", [3, 4, 5])
 def test_kernel_kinds_c1(B, kernel):
     n = 1_000_003
     _print_case(test="kernel_kinds_c1", kernel=kernel, n=n)
@@ -628,7 +628,7 @@ DIVISORS = [2, 3, 7, 10, 87, 1000, 65536, -2, -7, -1000, 6700417, 1099511627781,
             -9223372036854775807, 0, 1, -1]
 
 
-@pytest.mark.parametrize("kernel", [0, 3])
+@pytest.mark.parametrize("kernel", [0, 3, 5])
 def test_integer_division_by_constants(B, kernel):
     """Integer / and MOD by literal divisors (the tile VM divides by a magic
     multiplier; 0, 1 and -1 take the general path) against the oracle, with
@@ -664,7 +664,7 @@ BARE_DTYPES = [P.GDF_INT8, P.GDF_BOOL8, P.GDF_INT16, P.GDF_INT32, P.GDF_INT64, P
                P.GDF_FLOAT64, P.GDF_DATE32, P.GDF_DATE64, P.GDF_TIMESTAMP]
 
 
-@pytest.mark.parametrize("kernel", [0, 3])
+@pytest.mark.parametrize("kernel", [0, 3, 5])
 @pytest.mark.parametrize("mode", ["filter", "project"])
 def test_bare_column_outputs_keep_their_bytes(B, kernel, mode):
     """`$k` outputs are the (kept) rows' bytes, unchanged (the reference gathers them
