@@ -6,18 +6,19 @@ d int32 U[0,2555] (no nulls), filter
   AND(>=($3, 730), <($3, 1095), >=($2, 0.03), <=($2, 0.05), <($0, 30))
 and project  b*(1-c), a+1  over the rows kept. One step = one execution of the
 compiled filter+project program over the device-resident table: the predicate
-kernel (keep words + per-slot counts), the slot scan, and the gather kernel that
-evaluates the projection on the kept rows.
+kernel (keep words + per-slot counts, plus the kept rows' a and c staged per slot),
+the slot scan, and the gather kernel that evaluates the projection on the kept
+rows. By default the program is the compiled tier (the plan written out as kernels
+and compiled for the device when the program is prepared); --kernel 4 measures
+the tile VM instead.
 
 value = algorithmic bytes per step / wall time per step (GB/s), where algorithmic
 bytes = 24 B per input row (a, b, c, d read once) + 12 B per kept row (f64 + i32
 written). roofline is for the dominant kernel (the predicate kernel): its
 algorithmic bytes (predicate columns a, c, d read once: 16 B/row, keep words
-written: 1/8 B/row, slot counts: 16 B per 1024-row tile; it reads the columns
-into registers, pred_direct_kernel, since the predicate is a conjunction of
-column-versus-constant comparisons) over its own duration
-(HIP events on the launch stream); "pipeline" gives the same figure for all three
-kernels together. traffic is the dominant kernel's HBM bytes per launch from the
+written: 1/8 B/row, slot counts: 4 B per slot, and for the compiled tier the kept
+rows' a and c staged: 12 B per kept row) over its own duration (HIP events on the
+launch stream); "pipeline" gives the same figure for all three kernels together. traffic is the dominant kernel's HBM bytes per launch from the
 rocprofv3 FETCH_SIZE / WRITE_SIZE summary in profiles/. cpu_baseline times the
 scalar oracle on a 10M-row sample of the same workload on this host.
 
@@ -48,9 +49,16 @@ HBM_PEAK_GBS = 8000.0
 FILTER = ("LogicalFilter(condition=[AND(>=($3, 730), <($3, 1095), >=($2, 0.03), "
           "<=($2, 0.05), <($0, 30))])")
 PROJECT = "LogicalProject(EXPR$0=[*($1, -(1, $2))], EXPR$1=[+($0, 1)])"
-PROFILE = os.path.join(ROOT, "profiles", "vm_kernel_c1_filter_project_100M.txt")
-PROFILE_FG = os.path.join(ROOT, "profiles", "vm_kernel_filter_gather_500M_s0.9.txt")
-PROFILE_WIDE = os.path.join(ROOT, "profiles", "vm_kernel_wide_expression_100M.txt")
+# rocprofv3 summaries (kernel statistics and HBM counters) per workload and program kind
+PROFILES = {
+    ("c1", 5): "blz_cg_c1_filter_project_100M.txt", ("c1", 4): "vm_kernel_c1_filter_project_100M.txt",
+    ("fg", 5): "blz_cg_filter_gather_500M_s0.9.txt", ("fg", 4): "vm_kernel_filter_gather_500M_s0.9.txt",
+    ("wide", 5): "blz_cg_wide_expression_100M.txt", ("wide", 4): "vm_kernel_wide_expression_100M.txt",
+}
+
+
+def profile_path(workload, kernel):
+    return os.path.join(ROOT, "profiles", PROFILES.get((workload, kernel), "none"))
 STAGES = ["predicate", "slot_scan", "gather"]
 
 
@@ -67,16 +75,22 @@ def algorithmic_bytes(n, kept):
     return 24 * n + 12 * kept
 
 
-def stage_bytes(n, kept, tile_rows):
-    """Algorithmic bytes of each kernel of the filter+project program."""
+def stage_bytes(n, kept, tile_rows, compiled):
+    """Algorithmic bytes of each kernel of the filter+project program (four slots
+    per tile, one count each)."""
     tiles = (n + tile_rows - 1) // tile_rows
+    if compiled:
+        pred = 16 * n + n / 8 + 16 * tiles + 12 * kept   # a, c, d read; keep words, counts, staged a, c
+        scan = 32 * tiles + n / 8 + 4 * kept             # counts read, offsets written, keep words; indices
+        gather = 4 * kept + 20 * kept + 12 * kept + kept / 4  # indices; staged a, c and b; outputs + bitmaps
+        return [pred, scan, gather]
     pred = 16 * n + n / 8 + 16 * tiles          # a, c, d read; keep words + slot counts written
-    scan = 16 * tiles + n / 8 + 4 * kept        # slot counts, keep words read; kept-row indices written
+    scan = 32 * tiles + n / 8 + 4 * kept        # counts read, offsets written, keep words; kept-row indices
     gather = 4 * kept + 20 * kept + 12 * kept   # indices; a, b, c of kept rows; outputs
     return [pred, scan, gather]
 
 
-def traffic_from_profile(kernel, profile=PROFILE):
+def traffic_from_profile(kernel, profile):
     """HBM bytes per launch of `kernel` measured with rocprofv3 counters (profile file)."""
     if not os.path.exists(profile):
         return None
@@ -85,6 +99,23 @@ def traffic_from_profile(kernel, profile=PROFILE):
         if m and m.group(1).startswith(kernel):
             return float(m.group(2))
     return None
+
+
+def stage_kernel_names(prog):
+    """Kernel name prefixes (as rocprofv3 prints them) of a filter program's three
+    launches: predicate, slot scan, gather."""
+    if prog.kernel == 5:
+        return ["blz_cg_pred", "slot_scan_kernel", "blz_cg_gather"]
+    pred = "pred_direct_kernel" if prog.direct_pred else f"vm_kernel<{prog.r}, 1,"
+    return [pred, "slot_scan_kernel", f"vm_kernel<{prog.r}, 2,"]
+
+
+def program_desc(prog):
+    if prog.kernel == 5:
+        return f"compiled tier: {prog.r} rows per lane" + (f", tile {prog.tile_rows} rows" if prog.tile_rows else "")
+    if prog.kernel == 4:
+        return f"tile VM R={prog.r} tile={prog.tile_rows} rows"
+    return f"register-file kernel S={prog.s} R={prog.r}"
 
 
 def _query_bytes(frame, proj, outs, kept=None):
@@ -127,7 +158,7 @@ def run_batch(B, n_plans, rows_lo, rows_hi, seed=404):
     batch_r, batch_t, tuned = batch.r, batch.t, batch.tuned
     batch.close()
     # per-query: each query's prepared program, launched one after another
-    progs = [B.Program(f, fq, pq) for f, fq, pq in queries]
+    progs = [B.Program(f, fq, pq, kernel=4) for f, fq, pq in queries]  # the batch's tile-VM programs
     for p in progs:
         p.launch()
     progs[-1].result()
@@ -360,7 +391,7 @@ def run_wide_expression(B, args):
         B.synth_column(c, 7001 + i, dist, a, b, W.WIDE_NULL_RATE)
         B.update_null_count(c)
         frame.append(c)
-    prog = B.Program(frame, None, W.WIDE_PROJECT)
+    prog = B.Program(frame, None, W.WIDE_PROJECT, kernel=args.kernel)
     for _ in range(args.warmup):
         prog.launch()
     prog.result()
@@ -375,7 +406,7 @@ def run_wide_expression(B, args):
     bytes_in = sum(n * width(frame[i].col.dtype) + (n + 7) // 8 for i in used)
     bytes_out = sum(n * width(o.col.dtype) + (n + 7) // 8 for o in prog.outputs)
     bytes_step = bytes_in + bytes_out
-    kernel_name = f"vm_kernel<{prog.r}, 0,"
+    kernel_name = "blz_cg_project" if prog.kernel == 5 else f"vm_kernel<{prog.r}, 0,"
     out = {
         "metric": "wide-expression GB/s at 100M rows x 8 columns", "value": bytes_step / dt / 1e9, "unit": "GB/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": dt * 1e3,
@@ -383,10 +414,11 @@ def run_wide_expression(B, args):
         "data": "synthetic",
         "config": {"workload": "wide expression: 2 i32, 2 i64, 2 f32, 2 f64 columns, 5% nulls, 24-op plan, "
                                "4 outputs", "rows": n, "ops": prog.n_ops, "algorithmic_bytes_per_step": bytes_step,
-                   "program": f"tile VM R={prog.r} tile={prog.tile_rows} rows"},
+                   "program": program_desc(prog)},
         "roofline": {"bound": "hbm", "achieved": bytes_step / (kernel_ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": bytes_step / (kernel_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
-                     "traffic": traffic_from_profile(kernel_name, PROFILE_WIDE), "kernel": kernel_name,
+                     "traffic": traffic_from_profile(kernel_name, profile_path("wide", prog.kernel)),
+                     "kernel": kernel_name,
                      "kernel_ms": kernel_ms, "algorithmic_bytes": bytes_step},
     }
     prog.close()
@@ -550,7 +582,7 @@ def run_filter_gather(B, args):
              synth(B.GDF_FLOAT64, 7204, 1, 0.0, 1.0)]
     filt = f"LogicalFilter(condition=[<($0, {int(s * 2**31)})])"
     proj = "LogicalProject(p1=[$1], p2=[$2], p3=[$3], f1=[$4], f2=[$5], f3=[$6])"
-    prog = B.Program(frame, filt, proj)
+    prog = B.Program(frame, filt, proj, kernel=args.kernel)
     for _ in range(args.warmup):
         prog.launch()
     kept, _ = prog.result()
@@ -563,10 +595,11 @@ def run_filter_gather(B, args):
     tiles = (n + prog.tile_rows - 1) // prog.tile_rows
     sbytes = [4 * n + n / 8 + 16 * tiles, 16 * tiles + n / 8 + 4 * kept, 4 * kept + 48 * kept + 48 * kept]
     dom = int(np.argmax(stage_ms))
-    pred_name = "pred_direct_kernel" if prog.direct_pred else f"vm_kernel<{prog.r}, 1,"
-    kernel_name = ((pred_name if dom == 0 else f"vm_kernel<{prog.r}, 2,") if dom != 1 else "slot_scan_kernel")
+    kernel_name = stage_kernel_names(prog)[dom]
     achieved = sbytes[dom] / (stage_ms[dom] * 1e-3) / 1e9
     bytes_step = 52 * n + 48 * kept
+    desc = program_desc(prog)
+    kind = prog.kernel
     prog.close()
     for c in frame:
         c.free()
@@ -583,10 +616,10 @@ def run_filter_gather(B, args):
         "data": "synthetic",
         "config": {"workload": "filter + gather: int32 key, 3 int64 + 3 f64 payload, key < s*2^31",
                    "rows": n, "selectivity": s, "rows_kept": int(kept),
-                   "min_traffic_GBps": (4 * n + 96 * kept) / dt / 1e9,
-                   "program": f"tile VM R={prog.r} tile={prog.tile_rows} rows"},
+                   "min_traffic_GBps": (4 * n + 96 * kept) / dt / 1e9, "program": desc},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_from_profile(kernel_name, PROFILE_FG),
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": traffic_from_profile(kernel_name, profile_path("fg", kind)),
                      "kernel": kernel_name, "stage": STAGES[dom], "kernel_ms": stage_ms[dom],
                      "algorithmic_bytes": sbytes[dom]},
         "pipeline": {"stages": dict(zip(STAGES, stage_ms)), "kernel_ms": float(sum(stage_ms))},
@@ -611,6 +644,8 @@ def main():
                     default="filter-project")
     ap.add_argument("--csv-rows", type=int, default=20_000_000, help="csv-ingest: rows of the text file")
     ap.add_argument("--selectivity", type=float, default=0.9, help="filter-gather: fraction of rows kept")
+    ap.add_argument("--kernel", type=int, default=0, choices=[0, 3, 4, 5],
+                    help="program kind: 0 automatic (the compiled tier), 4 tile VM, 3 register-file kernel")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; multi-GPU runs are not part of it",
@@ -634,7 +669,9 @@ def main():
     a, b, c, d = gen_c1(n, 2024)
     frame = [B.DeviceColumn.from_numpy(x) for x in (a, b, c, d)]
     del a, b, c, d
-    prog = B.Program(frame, FILTER, PROJECT)
+    t_prep = time.perf_counter()
+    prog = B.Program(frame, FILTER, PROJECT, kernel=args.kernel)
+    t_prep = time.perf_counter() - t_prep
     print(f"[bench] program kernel={prog.kernel} R={prog.r} tile={prog.tile_rows} ops={prog.n_ops} "
           f"grid={prog.grid}", file=sys.stderr, flush=True)
     for _ in range(args.warmup):
@@ -648,14 +685,14 @@ def main():
     print(f"[bench] {args.steps} steps, {dt * 1e3:.3f} ms/step, kept {kept}", file=sys.stderr, flush=True)
     stage_ms = prog.time_stages(iters=max(10, args.steps // 2))
     names = STAGES if len(stage_ms) == 3 else ["program"]
-    sbytes = stage_bytes(n, kept, prog.tile_rows) if len(stage_ms) == 3 else [algorithmic_bytes(n, kept)]
+    sbytes = (stage_bytes(n, kept, prog.tile_rows, prog.kernel == 5) if len(stage_ms) == 3
+              else [algorithmic_bytes(n, kept)])
     dom = int(np.argmax(stage_ms))
-    # demangled kernel name prefix: vm_kernel<rows per lane, mode (1 predicate, 2 gather), op set>;
-    # a conjunction of column-versus-constant comparisons runs as pred_direct_kernel
-    pred_name = ("pred_direct_kernel" if traffic_from_profile("pred_direct_kernel") is not None
-                 else f"vm_kernel<{prog.r}, 1,")
-    kernel_name = ((pred_name if dom == 0 else f"vm_kernel<{prog.r}, 2," if dom == 2 else "slot_scan_kernel")
-                   if len(stage_ms) == 3 else f"kernel {prog.kernel}")
+    # kernel name prefixes as rocprofv3 prints them: blz_cg_* (compiled tier), or
+    # pred_direct_kernel / vm_kernel<rows per lane, mode (1 predicate, 2 gather), op set>
+    kernel_name = stage_kernel_names(prog)[dom] if len(stage_ms) == 3 else f"kernel {prog.kernel}"
+    desc = program_desc(prog)
+    kind = prog.kernel
     achieved = sbytes[dom] / (stage_ms[dom] * 1e-3) / 1e9
     bytes_step = algorithmic_bytes(n, kept)
     total_ms = float(sum(stage_ms))
@@ -690,9 +727,10 @@ def main():
                                "predicate, project b*(1-c), a+1",
                    "rows": n, "rows_kept": int(kept), "selectivity": kept / n,
                    "algorithmic_bytes_per_step": bytes_step,
-                   "program": f"tile VM R={prog.r} tile={prog.tile_rows} rows"},
+                   "program": desc, "prepare_s": t_prep},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_from_profile(kernel_name),
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": traffic_from_profile(kernel_name, profile_path("c1", kind)),
                      "kernel": kernel_name, "stage": names[dom], "kernel_ms": stage_ms[dom],
                      "algorithmic_bytes": sbytes[dom]},
         "pipeline": {"stages": dict(zip(names, stage_ms)), "kernel_ms": total_ms,

@@ -1101,7 +1101,24 @@ blz_program make_program(gdf_column *const *frame, int n_frame, const char *filt
     }
     if (proj_qp && pp.has_eval) req.proj = &pp.spec;
     if (req.inputs.empty()) invalid("nothing to evaluate");
-    p->c = compile_program(req);
+    // Automatic (kernel 0, no forced slots or tile rows): a prepared program is
+    // launched many times, so it takes the compiled tier; a plan the generator or
+    // the run-time compiler does not take falls back to the interpreters.
+    if (kernel == 0 && s == 0 && t == 0) {
+      req.kernel = 5;
+      try {
+        p->c = compile_program(req);
+        (void)jit_function(p->c.jit_src, p->c.jit_name);
+        if (has_filter) (void)jit_function(p->c.jit_src, "blz_cg_gather");
+      } catch (const BlzError &) {
+        throw;
+      } catch (const std::exception &) {
+        req.kernel = 0;
+        p->c = compile_program(req);
+      }
+    } else {
+      p->c = compile_program(req);
+    }
     if (p->c.jit) {  // compiled and loaded now, not at launch
       (void)jit_function(p->c.jit_src, p->c.jit_name);
       if (has_filter) (void)jit_function(p->c.jit_src, "blz_cg_gather");

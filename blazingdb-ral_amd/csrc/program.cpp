@@ -1524,8 +1524,8 @@ Compiled compile_program(const CompileRequest &req) {
       ci.out_valid.push_back(s.valid != nullptr);
     }
     // Rows per lane: about 64 bytes per lane of the columns a step holds (a filter's
-    // predicate and projection columns, a projection's), so every load of a step is
-    // in flight without spilling (one 4-byte column: 16 rows, 16 bytes per row: 8,
+    // predicate columns, a projection's columns), so every load of a step is in
+    // flight without spilling (one 4-byte column: 16 rows, 16 bytes per row: 8,
     // 48: 2), unless forced (r).
     int bpr = 0;
     {
@@ -1538,17 +1538,11 @@ Compiled compile_program(const CompileRequest &req) {
         if (!nd.b.imm && nd.b.node >= 0 && nd.b.node < n_dev) used[nd.b.node] = true;
       }
       if (req.compact && stencil_node >= 0 && stencil_node < n_dev) used[stencil_node] = true;
-      if (req.compact)  // the kept rows' projection columns are held too
-        for (int id : order_q) {
-          const Node &nd = B.nodes[id];
-          if (nd.kind != 1) continue;
-          if (!nd.a.imm && nd.a.node >= 0 && nd.a.node < n_dev) used[nd.a.node] = true;
-          if (!nd.b.imm && nd.b.node >= 0 && nd.b.node < n_dev) used[nd.b.node] = true;
+      if (!req.compact)
+        for (int o = 0; o < n_out; o++) {
+          if (req.outputs[o].kind == 1) used[req.outputs[o].index] = true;
+          else if (out_node[o] < n_dev) used[out_node[o]] = true;
         }
-      for (int o = 0; o < n_out; o++) {
-        if (req.outputs[o].kind == 1) used[req.outputs[o].index] = true;
-        else if (out_node[o] < n_dev) used[out_node[o]] = true;
-      }
       for (int c = 0; c < n_dev; c++)
         if (used[c]) bpr += wid(req.inputs[c].dtype);
     }

@@ -180,7 +180,8 @@ typedef struct blz_program_s *blz_program;
  * projection; project_query_part NULL keeps every frame column) against fixed
  * device buffers the program allocates and owns: out_columns[0..*n_out) receive
  * them (capacity: every frame row; sizes valid after blz_program_result).
- * kernel: 0 automatic (the tile VM when the plan fits it, else the direct kernel),
+ * kernel: 0 automatic (the compiled tier, 5, when s and t are 0 and the plan compiles;
+ * else the tile VM when the plan fits it, else the direct kernel),
  * 3 direct register-file kernel, 4 tile VM (direct-to-LDS tile staging;
  * a filter runs as predicate, slot-scan and gather launches), 5 compiled: the
  * plan written out as a kernel and compiled for the device at creation (one

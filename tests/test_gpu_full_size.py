@@ -58,18 +58,19 @@ def _c2_project():
 C2_PROJECT = _c2_project()
 
 
-def test_wide_expression_100M_rows(B):
+@pytest.mark.parametrize("kernel", [0, 4], ids=["compiled", "tile_vm"])
+def test_wide_expression_100M_rows(B, kernel):
     n = 100_000_000
-    _say("CASE test=wide_expression_100M n", n)
+    _say("CASE test=wide_expression_100M kernel", kernel, "n", n)
     kinds = [(P.GDF_INT32, 0, -1e6, 1e6), (P.GDF_INT32, 0, -1e6, 1e6), (P.GDF_INT64, 0, 0, 2.0**40),
              (P.GDF_INT64, 0, 0, 2.0**40), (P.GDF_FLOAT32, 2, 0.0, 1.0), (P.GDF_FLOAT32, 2, 0.0, 1.0),
              (P.GDF_FLOAT64, 3, 0.0, 2.0), (P.GDF_FLOAT64, 3, 0.0, 2.0)]
     dev = [_synth(B, dt, n, 7001 + i, dist, a, b, 0.05) for i, (dt, dist, a, b) in enumerate(kinds)]
     plan, _, _ = P.parse_project([c.col.dtype for c in dev], C2_PROJECT)
     assert len(plan.binops) == 24
-    prog = B.Program(dev, None, C2_PROJECT)
+    prog = B.Program(dev, None, C2_PROJECT, kernel=kernel)
     _say("program kernel", prog.kernel, "R", prog.r, "ops", prog.n_ops)
-    assert prog.kernel == 4  # the tile VM
+    assert prog.kernel == (5 if kernel == 0 else 4)  # the compiled tier by default, or the tile VM
     prog.launch()
     _, nulls = prog.result()
     _say("evaluated; checking")
