@@ -226,18 +226,24 @@ int program_stages(const Compiled &c, bool compact) { return (c.vm || c.jit) && 
 
 // Enqueues the program's kernels. evs (optional, program_stages + 1 events) are
 // recorded before each kernel and after the last.
+// sel_hint: rows a compiled filter kept the last time it ran (-1 unknown) — it sizes
+// the gather's grid, never its result.
 hipError_t launch_program(const Compiled &c, bool compact, int grid, hipStream_t st, const uint8_t *blob,
-                          const Workspace &ws, const VmBatch *bd, int64_t n_chunks, hipEvent_t *evs = nullptr) {
+                          const Workspace &ws, const VmBatch *bd, int64_t n_chunks, hipEvent_t *evs = nullptr,
+                          int64_t sel_hint = -1) {
   auto mark = [&](int i) { return evs ? hipEventRecord(evs[i], st) : hipSuccess; };
   hipError_t e = mark(0);
   if (e != hipSuccess) return e;
   if (c.jit) {
     const hipFunction_t f = jit_function(c.jit_src, c.jit_name);
     if (!compact) {
-      // Projection: blocks of four 64 R-row units per wave, 32 workgroups per CU at most.
+      // Projection: blocks of four 64 R-row units per wave, 8 workgroups per CU at most
+      // (the 100M-row wide expression: 1.50 / 1.49 / 1.52 / 2.00 / 3.33 ms at 4 / 8 /
+      // 16 / 32 / 64 per CU: a read and write stream runs best with few workgroups
+      // per CU walking many blocks).
       const int64_t units = (c.n_rows + 64 * c.R - 1) / (64 * c.R);
       const int64_t waves = (units + 3) / 4;
-      const int64_t g = std::min<int64_t>((waves + 3) / 4, (int64_t)device_cus() * 32);
+      const int64_t g = std::min<int64_t>((waves + 3) / 4, (int64_t)device_cus() * 8);
       e = launch_jit(f, (int)std::max<int64_t>(1, g), st, blob, ws);
       return e == hipSuccess ? mark(1) : e;
     }
@@ -252,8 +258,17 @@ hipError_t launch_program(const Compiled &c, bool compact, int grid, hipStream_t
     if (e == hipSuccess) e = launch_slot_scan(bd, n_chunks, st);
     if (e == hipSuccess) e = mark(2);
     if (e == hipSuccess) {
+      // The kept count is known on the device only. Without a hint: as many
+      // workgroups as stay resident (each walks chunks of 256 outputs per wave). With
+      // the last run's count: one workgroup per 1024 outputs, between the resident
+      // count and 256 per CU — many kept rows run faster with more workgroups in
+      // flight than stay resident (500M rows at s = 0.5: 8.3 -> 6.7 ms), few rows
+      // slower (C1's 2.3%: 0.070 ms at the resident count, 0.129 at 256 per CU).
       const int64_t chunks = (c.n_rows + 255) / 256;  // most gather chunks (every row kept)
-      const int64_t gg = std::min<int64_t>((chunks + 3) / 4, (int64_t)device_cus() * jit_blocks_per_cu(fg));
+      const int64_t resident = (int64_t)device_cus() * jit_blocks_per_cu(fg);
+      int64_t gg = std::min<int64_t>((chunks + 3) / 4, resident);
+      if (sel_hint >= 0)
+        gg = std::max<int64_t>(gg, std::min<int64_t>((sel_hint + 1023) / 1024, (int64_t)device_cus() * 256));
       e = launch_jit(fg, (int)std::max<int64_t>(1, gg), st, blob, ws);
     }
     return e == hipSuccess ? mark(3) : e;
@@ -1044,6 +1059,7 @@ struct blz_program_s {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   hipEvent_t evs[4] = {nullptr, nullptr, nullptr, nullptr};
   std::vector<uint8_t> img;  // host image of the blob + launch descriptor
+  int64_t last_sel = -1;     // rows kept by the last launch read back (compiled filter: sizes its gather)
   bool async_mem = false;    // device memory is stream-ordered on mem_st
   hipStream_t mem_st = nullptr;
 };
@@ -1176,7 +1192,8 @@ hipError_t program_enqueue(blz_program p, hipStream_t st) {
       e = launch_fill_zero(p->outs[i].valid, (int64_t)vb, (int)std::min<size_t>(1024, vb / 2048 + 1), st);
     }
   if (e == hipSuccess)
-    e = launch_program(p->c, p->compact, p->grid, st, p->lb.blob, p->lb.ws, p->lb.bd, p->lb.n_chunks);
+    e = launch_program(p->c, p->compact, p->grid, st, p->lb.blob, p->lb.ws, p->lb.bd, p->lb.n_chunks, nullptr,
+                       p->last_sel);
   return e;
 }
 }  // namespace
@@ -1202,6 +1219,7 @@ int blz_program_result(blz_program prog, blz_stream_t stream, int64_t *n_kept, i
     }
     HIPCHECK(hipStreamSynchronize(S(stream)));
     const int64_t kept = prog->compact ? (int64_t)h[prog->n_out] : prog->n;
+    prog->last_sel = kept;
     if (n_kept) *n_kept = kept;
     if (null_counts)
       for (int o = 0; o < prog->n_out; o++)
@@ -1219,14 +1237,14 @@ int blz_program_time_stages(blz_program prog, blz_stream_t stream, int iters, do
       // the zero-fill of counters is part of a launch but not of the kernel time
       HIPCHECK(launch_fill_zero(prog->lb.base + prog->lb.zero_off, (int64_t)prog->lb.zero_bytes,
                                 (int)std::min<size_t>(1024, prog->lb.zero_bytes / 2048 + 1), S(stream)));
-      for (size_t o = 0; prog->compact && !prog->c.vm && o < prog->outs.size(); o++)
+      for (size_t o = 0; prog->compact && !prog->c.vm && !prog->c.jit && o < prog->outs.size(); o++)
         if (prog->outs[o].valid) {
           const size_t vb = gdf_valid_allocation_size((gdf_size_type)prog->n);
           HIPCHECK(launch_fill_zero(prog->outs[o].valid, (int64_t)vb, (int)std::min<size_t>(1024, vb / 2048 + 1),
                                     S(stream)));
         }
       HIPCHECK(launch_program(prog->c, prog->compact, prog->grid, S(stream), prog->lb.blob, prog->lb.ws,
-                              prog->lb.bd, prog->lb.n_chunks, prog->evs));
+                              prog->lb.bd, prog->lb.n_chunks, prog->evs, prog->last_sel));
       HIPCHECK(hipEventSynchronize(prog->evs[ns]));
       for (int k = 0; k < ns; k++) {
         float ms = 0;
