@@ -26,6 +26,7 @@
 #include <vector>
 
 #include "../../include/blz/blz_ral.h"
+#include "device_arena.h"
 #include "jit.h"
 #include "kernels.h"
 #include "planner.h"
@@ -411,6 +412,24 @@ void check_frame(gdf_column *const *frame, int n_frame) {
   if (n_frame <= 0 || !frame) invalid("empty frame");
   const int64_t n = frame[0] ? frame[0]->size : 0;
   for (int i = 0; i < n_frame; i++) check_column(frame[i], "frame column", n);
+}
+
+// Device memory from the serving loop's arena, or from the driver.
+void *arena_or_malloc(DeviceArena *arena, size_t bytes) {
+  if (bytes == 0) return nullptr;
+  if (arena) {
+    void *p = arena->alloc(bytes);
+    if (!p) throw BlzError(BLZ_ERR_OOM, "serving arena: out of device memory");
+    return p;
+  }
+  void *p = nullptr;
+  HIPCHECK(hipMalloc(&p, bytes));
+  return p;
+}
+void arena_or_free(DeviceArena *arena, void *p) {
+  if (!p) return;
+  if (arena) arena->free(p);
+  else (void)hipFree(p);
 }
 
 // Device allocation; stream-ordered (no implicit device synchronisation) when
@@ -1056,20 +1075,25 @@ struct blz_program_s {
   int grid = 1;
   LaunchBuffers lb;
   std::vector<gdf_column> outs;
-  hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  hipEvent_t evs[4] = {nullptr, nullptr, nullptr, nullptr};
+  hipEvent_t evs[4] = {nullptr, nullptr, nullptr, nullptr};  // created by the first timing call
+  uint8_t *out_block = nullptr;  // every output's data and bitmap (one allocation)
   std::vector<uint8_t> img;  // host image of the blob + launch descriptor
   int64_t last_sel = -1;     // rows kept by the last launch read back (compiled filter: sizes its gather)
-  bool async_mem = false;    // device memory is stream-ordered on mem_st
+  DeviceArena *arena = nullptr;  // device memory from the serving loop's arena (else the driver)
   hipStream_t mem_st = nullptr;
 };
 
 namespace {
+// kernel argument of make_program: the interpreters' automatic choice (tile VM, else
+// the register-file kernel), never the compiled tier.
+constexpr int kInterpAuto = -1;
+
 // Compiles and allocates one program (throws on error); blz_program_create and
 // the batch share it.
 blz_program make_program(gdf_column *const *frame, int n_frame, const char *filter_qp, const char *proj_qp,
                          int scalar_mode, int kernel, int s, int r, int t, hipStream_t st = nullptr,
-                         bool sync = true) {
+                         DeviceArena *arena = nullptr) {
+  const bool sync = arena == nullptr;
   check_frame(frame, n_frame);
   const int64_t n = frame[0]->size;
   const auto dt = frame_dtypes(frame, n_frame);
@@ -1099,17 +1123,41 @@ blz_program make_program(gdf_column *const *frame, int n_frame, const char *filt
     req.force_s = s;
     req.force_r = r;
     req.force_t = t;
-    req.kernel = kernel;
-    for (size_t i = 0; i < outs.size(); i++) {
-      const auto &o = outs[i];
-      const bool with_valid = o.kind == 0 ? true : frame[o.index]->valid != nullptr;
-      gdf_column c = alloc_column(o.dtype, n, with_valid, st, !sync);
-      p->outs.push_back(c);
-      if (o.kind == 0) req.outputs.push_back({c.data, c.valid, o.dtype, 0, o.index});
-      else req.outputs.push_back({c.data, c.valid, o.dtype, 1, im.use(frame, o.index)});
-    }
-    p->async_mem = !sync;
+    req.kernel = kernel == kInterpAuto ? 0 : kernel;
+    // Every output (capacity: every row) in one allocation, freed with the program.
+    // No bitmap is cleared here: the kernels write every validity word of their
+    // outputs, and a launch of the register-file kernel's compaction (which ORs
+    // words at its tile edges) clears its outputs' bitmaps first.
+    p->arena = arena;
     p->mem_st = st;
+    {
+      std::vector<size_t> off_data(outs.size()), off_valid(outs.size(), SIZE_MAX);
+      size_t total = 0;
+      for (size_t i = 0; i < outs.size(); i++) {
+        const auto &o = outs[i];
+        const int w = blz_dtype_width((gdf_dtype)o.dtype);
+        if (w == 0) throw BlzError(BLZ_ERR_UNSUPPORTED, "dtype without a fixed width is not supported");
+        off_data[i] = total;
+        total += round_up((size_t)n * w, 256);
+        const bool with_valid = o.kind == 0 ? true : frame[o.index]->valid != nullptr;
+        if (with_valid && n > 0) {
+          off_valid[i] = total;
+          total += round_up(gdf_valid_allocation_size((gdf_size_type)n), 256);
+        }
+      }
+      p->out_block = (uint8_t *)arena_or_malloc(arena, std::max<size_t>(total, 256));
+      for (size_t i = 0; i < outs.size(); i++) {
+        const auto &o = outs[i];
+        gdf_column c{};
+        c.dtype = (gdf_dtype)o.dtype;
+        c.data = p->out_block + off_data[i];
+        c.valid = off_valid[i] == SIZE_MAX ? nullptr : (gdf_valid_type *)(p->out_block + off_valid[i]);
+        c.size = (gdf_size_type)n;
+        p->outs.push_back(c);
+        if (o.kind == 0) req.outputs.push_back({c.data, c.valid, o.dtype, 0, o.index});
+        else req.outputs.push_back({c.data, c.valid, o.dtype, 1, im.use(frame, o.index)});
+      }
+    }
     req.inputs = im.inputs;
     if (has_filter) {
       if (fp.bare) req.bare_stencil_input = im.frame_to_dev[fp.bare_col];
@@ -1144,16 +1192,16 @@ blz_program make_program(gdf_column *const *frame, int n_frame, const char *filt
     p->n_out = (int)outs.size();
     p->grid = interp_grid(p->c, has_filter, std::max<int64_t>(n, 1));
     layout(p->lb, p->c, p->n_out, has_filter, n);
-    p->lb.base = (uint8_t *)dalloc(p->lb.bytes, st, !sync);
+    p->lb.base = (uint8_t *)arena_or_malloc(arena, p->lb.bytes);
     bind(p->lb, p->n_out);
     p->img = device_image(p->lb, p->c);
     if (sync) HIPCHECK(hipMemcpy(p->lb.blob, p->img.data(), p->img.size(), hipMemcpyHostToDevice));
     else HIPCHECK(hipMemcpyAsync(p->lb.blob, p->img.data(), p->img.size(), hipMemcpyHostToDevice, st));
-    HIPCHECK(hipEventCreate(&p->ev0));
-    HIPCHECK(hipEventCreate(&p->ev1));
-    for (auto &e : p->evs) HIPCHECK(hipEventCreate(&e));
     if (sync) HIPCHECK(hipDeviceSynchronize());
   } catch (...) {
+    // copies this program enqueued may still target its memory: wait before the
+    // arena hands it out again
+    if (arena) (void)hipStreamSynchronize(st);
     blz_program_destroy(p);
     throw;
   }
@@ -1167,6 +1215,7 @@ int blz_program_create(blz_program *prog, gdf_column *const *frame, int n_frame,
   return guarded(err, err_len, [&] {
     if (!prog) invalid("prog is NULL");
     *prog = nullptr;
+    if (kernel < 0 || kernel > 5) invalid("kernel must be 0, 3, 4 or 5");
     blz_program p = make_program(frame, n_frame, filter_qp, proj_qp, scalar_mode, kernel, s, r, t);
     if ((int)p->outs.size() > n_out_cap) {
       blz_program_destroy(p);
@@ -1231,6 +1280,8 @@ int blz_program_time_stages(blz_program prog, blz_stream_t stream, int iters, do
                             char *err, size_t err_len) {
   return guarded(err, err_len, [&] {
     if (!prog || iters <= 0 || !stage_ms) invalid("bad arguments");
+    for (auto &e : prog->evs)
+      if (!e) HIPCHECK(hipEventCreate(&e));
     const int ns = program_stages(prog->c, prog->compact);
     std::vector<double> total(ns, 0.0);
     for (int i = 0; i < iters; i++) {
@@ -1360,21 +1411,11 @@ int blz_jit_compile(const char *src, char *err, size_t err_len) {
 
 int blz_program_destroy(blz_program prog) {
   if (!prog) return BLZ_OK;
-  if (prog->async_mem) {  // stream-ordered: freed after the stream's earlier work, no device-wide wait
-    for (auto &c : prog->outs) {
-      if (c.data) (void)hipFreeAsync(c.data, prog->mem_st);
-      if (c.valid) (void)hipFreeAsync(c.valid, prog->mem_st);
-    }
-    if (prog->lb.base) (void)hipFreeAsync(prog->lb.base, prog->mem_st);
-    prog->outs.clear();
-    prog->lb.base = nullptr;
-  } else {
-    (void)hipDeviceSynchronize();
-  }
-  for (auto &c : prog->outs) { (void)hipFree(c.data); (void)hipFree(c.valid); }
-  if (prog->lb.base) (void)hipFree(prog->lb.base);
-  if (prog->ev0) (void)hipEventDestroy(prog->ev0);
-  if (prog->ev1) (void)hipEventDestroy(prog->ev1);
+  // the serving loop destroys a program only after its batch completed; elsewhere
+  // the device finishes its work first
+  if (!prog->arena) (void)hipDeviceSynchronize();
+  arena_or_free(prog->arena, prog->out_block);
+  arena_or_free(prog->arena, prog->lb.base);
   for (auto &e : prog->evs)
     if (e) (void)hipEventDestroy(e);
   delete prog;
@@ -1560,7 +1601,7 @@ struct blz_batch_s {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   int tuned = 0;
   std::vector<uint8_t> img;  // host image of the descriptors
-  bool async_mem = false;
+  DeviceArena *arena = nullptr;  // device memory from the serving loop's arena (else the driver)
   hipStream_t mem_st = nullptr;
 };
 
@@ -1587,12 +1628,9 @@ hipError_t batch_enqueue(blz_batch b, hipStream_t st) {
 
 void batch_free(blz_batch b) {
   if (!b) return;
-  if (b->async_mem) {
-    if (b->d_mem) (void)hipFreeAsync(b->d_mem, b->mem_st);
-    b->d_mem = nullptr;
-  } else {
-    (void)hipDeviceSynchronize();
-  }
+  if (!b->arena) (void)hipDeviceSynchronize();
+  arena_or_free(b->arena, b->d_mem);
+  b->d_mem = nullptr;
   if (b->exec) (void)hipGraphExecDestroy(b->exec);
   if (b->graph) (void)hipGraphDestroy(b->graph);
   if (b->cap) (void)hipStreamDestroy(b->cap);
@@ -1605,7 +1643,8 @@ void batch_free(blz_batch b) {
 
 // Builds one batch of fixed shape (r, t). Throws on error.
 blz_batch batch_build(const blz_query *qs, int n, int r, int t, bool use_graph, hipStream_t st = nullptr,
-                      bool sync = true) {
+                      DeviceArena *arena = nullptr) {
+  const bool sync = arena == nullptr;
   auto *b = new blz_batch_s();
   try {
     b->r = r;
@@ -1615,7 +1654,7 @@ blz_batch batch_build(const blz_query *qs, int n, int r, int t, bool use_graph, 
       blz_program p = nullptr;
       try {
         p = make_program(q.frame, q.n_frame, q.filter_query_part, q.project_query_part, q.scalar_mode, 4, 0, r, t,
-                         st, sync);
+                         st, arena);
       } catch (const BlzError &e) {
         throw;  // device or argument errors
       } catch (const std::exception &e) {
@@ -1624,9 +1663,11 @@ blz_batch batch_build(const blz_query *qs, int n, int r, int t, bool use_graph, 
       if (p) {
         b->progs.push_back(p);
         (p->compact ? b->filt : b->proj).members.push_back(i);
-      } else {  // a plan outside the tile VM's limits: its own launch inside the batch
+      } else {  // a plan outside the shared shape: its own launch in the batch (the tile VM with
+                // a shape of its own, else the register-file kernel; not the compiled tier, whose
+                // compilation would stall the batch's construction)
         b->progs.push_back(make_program(q.frame, q.n_frame, q.filter_query_part, q.project_query_part,
-                                        q.scalar_mode, 0, 0, 0, 0, st, sync));
+                                        q.scalar_mode, kInterpAuto, 0, 0, 0, st, arena));
         b->others.push_back(i);
       }
     }
@@ -1734,8 +1775,8 @@ blz_batch batch_build(const blz_query *qs, int n, int r, int t, bool use_graph, 
     const size_t ptr_off = ctr_off + 256;
     const size_t bytes_off = round_up(ptr_off + sizeof(void *) * (rp.size() + 1), 256);
     const size_t total = round_up(bytes_off + sizeof(int64_t) * (rb.size() + 1), 256);
-    b->d_mem = (uint8_t *)dalloc(total, st, !sync);
-    b->async_mem = !sync;
+    b->d_mem = (uint8_t *)arena_or_malloc(arena, total);
+    b->arena = arena;
     b->mem_st = st;
     b->d_counter = (unsigned int *)(b->d_mem + ctr_off);
     rp.push_back(b->d_counter);
@@ -1768,6 +1809,7 @@ blz_batch batch_build(const blz_query *qs, int n, int r, int t, bool use_graph, 
     }
     if (sync) HIPCHECK(hipDeviceSynchronize());
   } catch (...) {
+    if (arena) (void)hipStreamSynchronize(st);  // as in make_program
     batch_free(b);
     throw;
   }
@@ -1953,6 +1995,7 @@ struct blz_server_s {
   std::unordered_map<int64_t, SrvQuery> qs;
   std::vector<SrvBatch *> inflight, held;
   size_t budget = 0, used = 0;
+  DeviceArena *arena = nullptr;  // outputs, workspaces and descriptors of admitted queries
   int max_batch = 64, n_streams = 4, r = 4, next_stream = 0, device = 0;
   std::vector<hipStream_t> streams;
   std::thread worker;
@@ -2034,7 +2077,7 @@ void srv_loop(blz_server s) {
         sb->live = (int)take.size();
         std::string err;
         try {
-          sb->b = batch_build(bq.data(), (int)bq.size(), s->r, 256 * s->r, false, st, false);
+          sb->b = batch_build(bq.data(), (int)bq.size(), s->r, 256 * s->r, false, st, s->arena);
           HIPCHECK(batch_enqueue(sb->b, st));
           HIPCHECK(hipEventCreateWithFlags(&sb->done, hipEventDisableTiming));
           HIPCHECK(hipEventRecord(sb->done, st));
@@ -2049,7 +2092,10 @@ void srv_loop(blz_server s) {
             s->qs[tk].t_done = now_s();
           }
           s->used -= std::min(s->used, add);
-          if (sb->b) batch_free(sb->b);
+          if (sb->b) {
+            (void)hipStreamSynchronize(st);  // its enqueued work may use the memory batch_free returns
+            batch_free(sb->b);
+          }
           if (sb->done) (void)hipEventDestroy(sb->done);
           delete sb;
           s->cv_done.notify_all();
@@ -2115,6 +2161,7 @@ int blz_server_create(blz_server *server, size_t hbm_budget_bytes, int max_batch
       HIPCHECK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
       s->streams.push_back(st);
     }
+    s->arena = new DeviceArena((size_t)4 << 30);
     s->worker = std::thread(srv_loop, s);
     {
       std::lock_guard<std::mutex> g(g_live_mu);
@@ -2182,9 +2229,9 @@ int blz_server_release(blz_server s, int64_t ticket) {
   if (sb) s->used -= std::min(s->used, it->second.out_bytes);
   if (sb && it->second.index >= 0) {
     blz_program p = sb->b->progs[it->second.index];
+    arena_or_free(p->arena, p->out_block);  // the query's outputs (its batch has completed)
+    p->out_block = nullptr;
     for (auto &c : p->outs) {
-      if (c.data) (void)hipFreeAsync(c.data, sb->st);
-      if (c.valid) (void)hipFreeAsync(c.valid, sb->st);
       c.data = nullptr;
       c.valid = nullptr;
     }
@@ -2369,6 +2416,7 @@ int blz_server_destroy(blz_server s) {
     delete sb;
   }
   for (auto st : s->streams) (void)hipStreamDestroy(st);
+  delete s->arena;  // every batch has returned its memory
   delete s;
   return BLZ_OK;
 }
