@@ -743,8 +743,7 @@ RedPart reduce_column(const gdf_column *c, int op, hipStream_t st) {
   const int load = column_load_kind(c->dtype);
   if (load == LD_NONE) throw BlzError(BLZ_ERR_UNSUPPORTED, "aggregation over this dtype is not supported");
   const int64_t n = c->size;
-  const int64_t bt = block_threads();
-  const int grid = (int)std::max<int64_t>(1, std::min<int64_t>(1024, (n + bt - 1) / bt));
+  const int grid = reduce_grid(load, n);
   RedPart *parts = nullptr;
   HIPCHECK(hipMallocAsync((void **)&parts, sizeof(RedPart) * (grid + 1), st));
   RedPart r{};

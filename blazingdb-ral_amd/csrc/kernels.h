@@ -86,11 +86,16 @@ hipError_t launch_null_count(const uint8_t *valid, int64_t n, unsigned long long
 // Reduction of a column's valid rows (op 0 SUM, 1 MIN, 2 MAX; load: its LoadKind):
 // parts[0..grid) the workgroups' partials, parts[grid] the result (value as the
 // interpreter's 64-bit pattern: int64, or binary64 for float columns; cnt = valid
-// rows). Deterministic for a given grid (1 <= grid <= 65536).
+// rows). Deterministic for a given grid; reduce_grid gives the grid for a row
+// count (every grid from 1 to kRedMaxGrid is valid). The data of a column of
+// width w is w-byte aligned.
 struct RedPart {
   uint64_t v;
   unsigned long long cnt;
 };
+constexpr int kRedLoads = 4;        // 16-byte loads in flight per lane
+constexpr int kRedMaxGrid = 2048;   // workgroups of the first pass
+int reduce_grid(int load, int64_t n);
 hipError_t launch_reduce(int op, const void *data, int load, const uint8_t *valid, int64_t n, RedPart *parts,
                          int grid, hipStream_t st);
 

@@ -32,6 +32,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <algorithm>
 #include <map>
 #include <mutex>
 #include <utility>
@@ -525,10 +526,16 @@ null_count_kernel(const uint8_t *valid, int64_t n, unsigned long long *nulls_out
 // aggregation without GROUP BY (GroupBy.cpp:230-290): op 0 SUM, 1 MIN, 2 MAX.
 // Values are widened as the interpreter widens them (integers to int64, floats to
 // binary64); SUM accumulates in int64 (wrapping) or binary64, MIN / MAX compare in
-// the widened type (a NaN never replaces the running value). Thread t of
-// workgroup b takes rows b*256 + t + k*grid*256 in that order, then a fixed tree;
-// the final pass combines the partials in a fixed order: a float sum is the same
-// on every run with the same grid.
+// the widened type (a NaN never replaces the running value).
+//
+// A step is 256 lanes x kRedLoads loads of 16 bytes (E = 16 / width rows per
+// load): in load u lane t holds rows step*S + u*256*E + t*E + [0, E), S = the
+// step's rows. Workgroup b takes steps b, b + grid, ... in that order and combines
+// its rows in the order above; the rows past the last whole step go to workgroup
+// (whole steps) % grid, lane t taking rows t, t + 256, ... of them; then a fixed
+// tree over the lanes and waves, and the final pass combines the partials in
+// workgroup order. A float sum is therefore the same on every run with the same
+// row count (the grid is a function of the row count and the dtype).
 
 template <int OP, bool FLT>
 DEV uint64_t red_identity() {
@@ -570,16 +577,90 @@ DEV void red_block(uint64_t acc, unsigned long long cnt, RedPart *dst) {
     dst->cnt = c;
   }
 }
-template <int OP, bool FLT>
+
+template <int LD>
+constexpr int red_width() {
+  return LD == LD_I64 || LD == LD_F64 ? 8 : LD == LD_I32 || LD == LD_F32 ? 4 : LD == LD_I16 ? 2 : 1;
+}
+// Element e (of E) of a 16-byte load, widened.
+template <int LD>
+DEV uint64_t red_elem(const u32x4 &v, int e) {
+  constexpr int W = red_width<LD>();
+  if constexpr (W == 8) return (uint64_t)v[2 * e] | ((uint64_t)v[2 * e + 1] << 32);
+  else if constexpr (W == 4) return widen32(LD, v[e]);
+  else if constexpr (W == 2) return widen32(LD, (v[e >> 1] >> (16 * (e & 1))) & 0xffffu);
+  else return widen32(LD, (v[e >> 2] >> (8 * (e & 3))) & 0xffu);
+}
+// The 16 bytes from row `row` (a multiple of E): one load when the column is
+// 16-byte aligned, else element loads packed the same way.
+template <int LD, bool VEC>
+DEV u32x4 red_load(const void *data, int64_t row) {
+  constexpr int W = red_width<LD>();
+  const GLOBAL uint8_t *p = (const GLOBAL uint8_t *)data + row * W;
+  if constexpr (VEC) {
+    return __builtin_nontemporal_load((const GLOBAL u32x4 *)p);
+  } else {
+    u32x4 v = {0u, 0u, 0u, 0u};
+    if constexpr (W == 8) {
+      for (int e = 0; e < 2; e++) {
+        const uint64_t x = ((const GLOBAL uint64_t *)p)[e];
+        v[2 * e] = (uint32_t)x;
+        v[2 * e + 1] = (uint32_t)(x >> 32);
+      }
+    } else if constexpr (W == 4) {
+      for (int e = 0; e < 4; e++) v[e] = ((const GLOBAL uint32_t *)p)[e];
+    } else if constexpr (W == 2) {
+      for (int e = 0; e < 8; e++) v[e >> 1] |= (uint32_t)((const GLOBAL uint16_t *)p)[e] << (16 * (e & 1));
+    } else {
+      for (int e = 0; e < 16; e++) v[e >> 2] |= (uint32_t)p[e] << (8 * (e & 3));
+    }
+    return v;
+  }
+}
+// Validity bits of rows row .. row + E - 1 (row a multiple of E).
+template <int E>
+DEV uint32_t red_valid(const uint8_t *valid, int64_t row) {
+  const GLOBAL uint8_t *v = (const GLOBAL uint8_t *)valid;
+  if constexpr (E == 16) return (uint32_t)v[row >> 3] | ((uint32_t)v[(row >> 3) + 1] << 8);
+  else if constexpr (E == 8) return v[row >> 3];
+  else return (v[row >> 3] >> (row & 7)) & ((1u << E) - 1);
+}
+
+template <int OP, bool FLT, int LD, bool VEC>
 __global__ void __launch_bounds__(kBlock)
-reduce_kernel(const void *data, int load, const uint8_t *valid, int64_t n, RedPart *parts) {
+reduce_kernel(const void *data, const uint8_t *valid, int64_t n, RedPart *parts) {
+  constexpr int W = red_width<LD>(), E = 16 / W;
+  constexpr int64_t S = (int64_t)kBlock * E * kRedLoads;
+  const int t = threadIdx.x;
+  const int64_t whole = n / S;
   uint64_t acc = red_identity<OP, FLT>();
   unsigned long long cnt = 0;
-  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
-    if (!valid_bit(valid, i)) continue;
-    const uint64_t v = is_wide(load) ? gload<uint64_t>(data, i) : widen32(load, load_narrow(load, data, i));
-    acc = red_combine<OP, FLT>(acc, v);
-    cnt++;
+  for (int64_t s = blockIdx.x; s < whole; s += gridDim.x) {
+    const int64_t r0 = s * S + (int64_t)t * E;
+    u32x4 v[kRedLoads];
+    uint32_t vb[kRedLoads];
+#pragma unroll
+    for (int u = 0; u < kRedLoads; u++) v[u] = red_load<LD, VEC>(data, r0 + (int64_t)u * kBlock * E);
+#pragma unroll
+    for (int u = 0; u < kRedLoads; u++)
+      vb[u] = valid ? red_valid<E>(valid, r0 + (int64_t)u * kBlock * E) : (1u << E) - 1;
+#pragma unroll
+    for (int u = 0; u < kRedLoads; u++) {
+      cnt += __builtin_popcount(vb[u]);
+#pragma unroll
+      for (int e = 0; e < E; e++) {
+        const uint64_t x = red_combine<OP, FLT>(acc, red_elem<LD>(v[u], e));
+        acc = (vb[u] >> e) & 1 ? x : acc;
+      }
+    }
+  }
+  if (blockIdx.x == whole % gridDim.x) {
+    for (int64_t i = whole * S + t; i < n; i += kBlock) {
+      if (!valid_bit(valid, i)) continue;
+      const uint64_t x = W == 8 ? gload<uint64_t>(data, i) : widen32(LD, load_narrow(LD, data, i));
+      acc = red_combine<OP, FLT>(acc, x);
+      cnt++;
+    }
   }
   red_block<OP, FLT>(acc, cnt, parts + blockIdx.x);
 }
@@ -704,23 +785,42 @@ hipError_t launch_null_count(const uint8_t *valid, int64_t n, unsigned long long
   return hipGetLastError();
 }
 
-template <int OP, bool FLT>
-static hipError_t launch_reduce_t(const void *data, int load, const uint8_t *valid, int64_t n, RedPart *parts,
-                                  int grid, hipStream_t st) {
-  hipLaunchKernelGGL((reduce_kernel<OP, FLT>), dim3(grid), dim3(kBlock), 0, st, data, load, valid, n, parts);
+template <int OP, int LD>
+static hipError_t launch_reduce_t(const void *data, const uint8_t *valid, int64_t n, RedPart *parts, int grid,
+                                  hipStream_t st) {
+  constexpr bool FLT = LD == LD_F32 || LD == LD_F64;
+  if (((uintptr_t)data & 15) == 0)
+    hipLaunchKernelGGL((reduce_kernel<OP, FLT, LD, true>), dim3(grid), dim3(kBlock), 0, st, data, valid, n, parts);
+  else
+    hipLaunchKernelGGL((reduce_kernel<OP, FLT, LD, false>), dim3(grid), dim3(kBlock), 0, st, data, valid, n, parts);
   hipLaunchKernelGGL((reduce_final_kernel<OP, FLT>), dim3(1), dim3(kBlock), 0, st, parts, grid, parts + grid);
   return hipGetLastError();
 }
+template <int OP>
+static hipError_t launch_reduce_op(const void *data, int load, const uint8_t *valid, int64_t n, RedPart *parts,
+                                   int grid, hipStream_t st) {
+  switch (load) {
+    case LD_I8: return launch_reduce_t<OP, LD_I8>(data, valid, n, parts, grid, st);
+    case LD_U8: return launch_reduce_t<OP, LD_U8>(data, valid, n, parts, grid, st);
+    case LD_I16: return launch_reduce_t<OP, LD_I16>(data, valid, n, parts, grid, st);
+    case LD_I32: return launch_reduce_t<OP, LD_I32>(data, valid, n, parts, grid, st);
+    case LD_I64: return launch_reduce_t<OP, LD_I64>(data, valid, n, parts, grid, st);
+    case LD_F32: return launch_reduce_t<OP, LD_F32>(data, valid, n, parts, grid, st);
+    case LD_F64: return launch_reduce_t<OP, LD_F64>(data, valid, n, parts, grid, st);
+    default: return hipErrorInvalidValue;
+  }
+}
+int reduce_grid(int load, int64_t n) {
+  const int w = load == LD_I64 || load == LD_F64 ? 8 : load == LD_I32 || load == LD_F32 ? 4 : load == LD_I16 ? 2 : 1;
+  const int64_t step = (int64_t)kBlock * (16 / w) * kRedLoads;
+  return (int)std::max<int64_t>(1, std::min<int64_t>(kRedMaxGrid, (n + step - 1) / step));
+}
 hipError_t launch_reduce(int op, const void *data, int load, const uint8_t *valid, int64_t n, RedPart *parts,
                          int grid, hipStream_t st) {
-  const bool flt = load == LD_F32 || load == LD_F64;
-  switch (op * 2 + flt) {
-    case 0: return launch_reduce_t<0, false>(data, load, valid, n, parts, grid, st);
-    case 1: return launch_reduce_t<0, true>(data, load, valid, n, parts, grid, st);
-    case 2: return launch_reduce_t<1, false>(data, load, valid, n, parts, grid, st);
-    case 3: return launch_reduce_t<1, true>(data, load, valid, n, parts, grid, st);
-    case 4: return launch_reduce_t<2, false>(data, load, valid, n, parts, grid, st);
-    case 5: return launch_reduce_t<2, true>(data, load, valid, n, parts, grid, st);
+  switch (op) {
+    case 0: return launch_reduce_op<0>(data, load, valid, n, parts, grid, st);
+    case 1: return launch_reduce_op<1>(data, load, valid, n, parts, grid, st);
+    case 2: return launch_reduce_op<2>(data, load, valid, n, parts, grid, st);
     default: return hipErrorInvalidValue;
   }
 }

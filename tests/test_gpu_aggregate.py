@@ -153,3 +153,36 @@ def test_count_avg_sum_over_a_nullable_expression(B):
     _check(got, ref, [None] * 8, "|".join(["x"] * 8))
     for c in got + dev:
         c.free()
+
+
+@pytest.mark.parametrize("dtype", [P.GDF_INT8, P.GDF_INT16, P.GDF_INT32, P.GDF_INT64, P.GDF_FLOAT32,
+                                   P.GDF_FLOAT64])
+@pytest.mark.parametrize("nulls", [0.0, 0.2])
+def test_aggregates_of_a_column_view_not_16_byte_aligned(B, dtype, nulls):
+    """A column whose data starts one element into an allocation (a view, as a
+    slice of another column would be) is read element by element, in the same
+    order as an aligned one: the same results, integer bit-exact."""
+    rng = np.random.default_rng(515 + dtype)
+    n = 300_001
+    full = gen_column(rng, dtype, n + 1, 0.0)
+    sub = O.HostColumn(full.data[1:].copy(), dtype, (rng.random(n) >= nulls) if nulls else None)
+    _case(test="aggregate_view", dtype=dtype, nulls=nulls, n=n)
+    ref = O.process_aggregate([sub], ALL6)
+    base = to_device(B, full)
+    bits = to_device(B, sub)              # owns the view's bitmap (and an aligned copy)
+    view = B.GdfColumn()
+    w = np.dtype(full.data.dtype).itemsize
+    view.data = base.col.data + w
+    view.valid = bits.col.valid
+    view.size = n
+    view.dtype = dtype
+    view.null_count = bits.col.null_count
+    vcol = B.DeviceColumn.adopt(view, owned=False)
+    got = B.process_aggregate([vcol], ALL6)
+    labels = "COUNT|SUM|AVG|MAX|MIN|COUNT"
+    _check(got, ref, [None] + [sub] * 4 + [None], labels)
+    aligned = B.process_aggregate([bits], ALL6)
+    for g, a in zip(got, aligned):
+        assert g.data().tobytes() == a.data().tobytes()
+    for c in got + aligned + [base, bits]:
+        c.free()
