@@ -119,11 +119,14 @@ def lib():
             "blz_program_destroy": [P],
             "blz_program_source": [colpp, I, C.c_char_p, C.c_char_p, I, C.c_char_p, SZ, C.POINTER(SZ), ERR, SZ],
             "blz_jit_compile": [C.c_char_p, ERR, SZ],
+            "blz_jit_compile_many": [C.POINTER(C.c_char_p), C.c_int, ERR, SZ],
+            "blz_jit_counters": [C.POINTER(C.c_longlong), C.POINTER(C.c_longlong), C.POINTER(C.c_double)],
             "blz_batch_create": [C.POINTER(P), C.POINTER(BlzQuery), I, I, I, I, ERR, SZ],
             "blz_batch_launch": [P, P, ERR, SZ],
             "blz_batch_result": [P, P, I, C.POINTER(C.c_int64), I32P, colp, I, C.POINTER(C.c_int), ERR, SZ],
             "blz_batch_time": [P, P, I, C.POINTER(C.c_double), ERR, SZ],
             "blz_batch_info": [P] + [C.POINTER(C.c_int)] * 5,
+            "blz_batch_query_kernel": [P, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)],
             "blz_batch_destroy": [P],
             "blz_server_create": [C.POINTER(P), SZ, I, I, I, ERR, SZ],
             "blz_server_submit": [P, C.POINTER(BlzQuery), C.POINTER(C.c_int64), ERR, SZ],
@@ -591,6 +594,19 @@ def jit_compile(src: str):
         raise BlzError(rc, err.value.decode(errors="replace"))
 
 
+def jit_compile_many(srcs):
+    """Compiles generated kernel sources not compiled yet as one program."""
+    arr = (C.c_char_p * len(srcs))(*[x.encode() for x in srcs])
+    _call(lib().blz_jit_compile_many, arr, len(srcs))
+
+
+def jit_counters():
+    """(compilations, cache hits, milliseconds compiling) of this process."""
+    c, h, ms = C.c_longlong(0), C.c_longlong(0), C.c_double(0)
+    lib().blz_jit_counters(C.byref(c), C.byref(h), C.byref(ms))
+    return c.value, h.value, ms.value
+
+
 def plan_expression(frame_dtypes: Sequence[int], expression: str, cap=256):
     """The planner alone (host): evaluate_expression's plan for a frame of dtypes."""
     I16 = C.c_int16 * cap
@@ -672,6 +688,15 @@ class Batch:
         info = [C.c_int(0) for _ in range(5)]
         lib().blz_batch_info(self.h, *[C.byref(x) for x in info])
         self.r, self.t, self.n_queries, self.graph, self.tuned = [x.value for x in info]
+        self.kernels = [self.query_kernel(q) for q in range(self.n_queries)]
+        self.n_compiled = sum(1 for k, _ in self.kernels if k == 5)
+
+    def query_kernel(self, q):
+        """(kernel kind running query q: 4 tile VM, 5 compiled tier, ...; True when
+        it runs inside a shared launch)."""
+        k, sh = C.c_int(0), C.c_int(0)
+        _check(lib().blz_batch_query_kernel(self.h, q, C.byref(k), C.byref(sh)), "blz_batch_query_kernel")
+        return k.value, bool(sh.value)
 
     def launch(self, stream=None):
         _call(lib().blz_batch_launch, self.h, stream)

@@ -1347,53 +1347,62 @@ int blz_program_predicate(blz_program prog, int *direct, int *terms, int *wmask)
 // The kernel source the compiled tier writes for a plan over a frame of this shape
 // (dtypes, bitmaps present, null counts, rows); nothing is allocated or launched,
 // column data is not read.
+namespace {
+// The compiled tier's source of a filter / projection over `frame` (the text
+// make_program compiles for kernel 5). Throws as the planner and generator do.
+std::string plan_jit_source(gdf_column *const *frame, int n_frame, const char *filter_qp, const char *proj_qp,
+                            int scalar_mode) {
+  check_frame(frame, n_frame);
+  const int64_t n = frame[0]->size;
+  const auto dt = frame_dtypes(frame, n_frame);
+  InputMap im(n_frame);
+  planner::ExprPlan fp;
+  const bool has_filter = filter_qp != nullptr;
+  if (has_filter) {
+    fp = planner::plan_expression(dt, condition_of(filter_qp));
+    if (fp.bare) im.use(frame, fp.bare_col);
+    else im.remap(frame, fp.spec);
+  }
+  planner::ProjectPlan pp;
+  std::vector<planner::ProjOut> outs;
+  if (proj_qp) {
+    pp = planner::parse_project(dt, proj_qp);
+    if (pp.has_eval) im.remap(frame, pp.spec);
+    outs = pp.outputs;
+  } else {
+    for (int i = 0; i < n_frame; i++) outs.push_back({1, i, dt[i], ""});
+  }
+  CompileRequest req;
+  req.n_rows = n;
+  req.scalar_mode = scalar_mode;
+  req.compact = has_filter;
+  req.kernel = 5;
+  // stand-in output buffers: the source depends only on which are present
+  static uint8_t stand_in[64];
+  for (const auto &o : outs) {
+    const bool with_valid = o.kind == 0 ? true : frame[o.index]->valid != nullptr;
+    if (o.kind == 0) req.outputs.push_back({stand_in, with_valid ? stand_in : nullptr, o.dtype, 0, o.index});
+    else req.outputs.push_back({stand_in, with_valid ? stand_in : nullptr, o.dtype, 1, im.use(frame, o.index)});
+  }
+  req.inputs = im.inputs;
+  if (has_filter) {
+    if (fp.bare) req.bare_stencil_input = im.frame_to_dev[fp.bare_col];
+    else req.pred = &fp.spec;
+  }
+  if (proj_qp && pp.has_eval) req.proj = &pp.spec;
+  if (req.inputs.empty()) invalid("nothing to evaluate");
+  return compile_program(req).jit_src;
+}
+}  // namespace
+
 int blz_program_source(gdf_column *const *frame, int n_frame, const char *filter_qp, const char *proj_qp,
                        int scalar_mode, char *src, size_t src_cap, size_t *src_len, char *err, size_t err_len) {
   return guarded(err, err_len, [&] {
-    check_frame(frame, n_frame);
-    const int64_t n = frame[0]->size;
-    const auto dt = frame_dtypes(frame, n_frame);
-    InputMap im(n_frame);
-    planner::ExprPlan fp;
-    const bool has_filter = filter_qp != nullptr;
-    if (has_filter) {
-      fp = planner::plan_expression(dt, condition_of(filter_qp));
-      if (fp.bare) im.use(frame, fp.bare_col);
-      else im.remap(frame, fp.spec);
-    }
-    planner::ProjectPlan pp;
-    std::vector<planner::ProjOut> outs;
-    if (proj_qp) {
-      pp = planner::parse_project(dt, proj_qp);
-      if (pp.has_eval) im.remap(frame, pp.spec);
-      outs = pp.outputs;
-    } else {
-      for (int i = 0; i < n_frame; i++) outs.push_back({1, i, dt[i], ""});
-    }
-    CompileRequest req;
-    req.n_rows = n;
-    req.scalar_mode = scalar_mode;
-    req.compact = has_filter;
-    req.kernel = 5;
-    // stand-in output buffers: the source depends only on which are present
-    static uint8_t stand_in[64];
-    for (const auto &o : outs) {
-      const bool with_valid = o.kind == 0 ? true : frame[o.index]->valid != nullptr;
-      if (o.kind == 0) req.outputs.push_back({stand_in, with_valid ? stand_in : nullptr, o.dtype, 0, o.index});
-      else req.outputs.push_back({stand_in, with_valid ? stand_in : nullptr, o.dtype, 1, im.use(frame, o.index)});
-    }
-    req.inputs = im.inputs;
-    if (has_filter) {
-      if (fp.bare) req.bare_stencil_input = im.frame_to_dev[fp.bare_col];
-      else req.pred = &fp.spec;
-    }
-    if (proj_qp && pp.has_eval) req.proj = &pp.spec;
-    if (req.inputs.empty()) invalid("nothing to evaluate");
-    const Compiled c = compile_program(req);
-    if (src_len) *src_len = c.jit_src.size();
+    const std::string js = plan_jit_source(frame, n_frame, filter_qp, proj_qp, scalar_mode);
+    if (src_len) *src_len = js.size();
     if (src && src_cap > 0) {
-      const size_t k = std::min(src_cap - 1, c.jit_src.size());
-      std::memcpy(src, c.jit_src.data(), k);
+      const size_t k = std::min(src_cap - 1, js.size());
+      std::memcpy(src, js.data(), k);
       src[k] = 0;
     }
   });
@@ -1406,6 +1415,23 @@ int blz_jit_compile(const char *src, char *err, size_t err_len) {
     if (!src) invalid("src is NULL");
     jit_compile_only(src);
   });
+}
+
+int blz_jit_compile_many(const char *const *srcs, int n, char *err, size_t err_len) {
+  return guarded(err, err_len, [&] {
+    if (!srcs || n < 0) invalid("bad arguments");
+    std::vector<std::string> v;
+    for (int i = 0; i < n; i++) {
+      if (!srcs[i]) invalid("a source is NULL");
+      v.emplace_back(srcs[i]);
+    }
+    jit_compile_many(v);
+  });
+}
+
+int blz_jit_counters(long long *compiles, long long *hits, double *compile_ms) {
+  jit_counters(compiles, hits, compile_ms);
+  return BLZ_OK;
 }
 
 int blz_program_destroy(blz_program prog) {
@@ -1599,6 +1625,7 @@ struct blz_batch_s {
   hipStream_t cap = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   int tuned = 0;
+  int n_compiled = 0;        // queries on the compiled tier (their own launches)
   std::vector<uint8_t> img;  // host image of the descriptors
   DeviceArena *arena = nullptr;  // device memory from the serving loop's arena (else the driver)
   hipStream_t mem_st = nullptr;
@@ -1640,9 +1667,11 @@ void batch_free(blz_batch b) {
   delete b;
 }
 
-// Builds one batch of fixed shape (r, t). Throws on error.
+// Builds one batch of fixed shape (r, t). Queries flagged in `compiled` run on the
+// compiled tier, each as its own launch (its source compiled beforehand, or now).
+// Throws on error.
 blz_batch batch_build(const blz_query *qs, int n, int r, int t, bool use_graph, hipStream_t st = nullptr,
-                      DeviceArena *arena = nullptr) {
+                      DeviceArena *arena = nullptr, const std::vector<char> *compiled = nullptr) {
   const bool sync = arena == nullptr;
   auto *b = new blz_batch_s();
   try {
@@ -1650,6 +1679,13 @@ blz_batch batch_build(const blz_query *qs, int n, int r, int t, bool use_graph, 
     b->t = t;
     for (int i = 0; i < n; i++) {
       const blz_query &q = qs[i];
+      if (compiled && (*compiled)[i]) {  // kernel 0: the compiled tier, else the interpreters
+        b->progs.push_back(make_program(q.frame, q.n_frame, q.filter_query_part, q.project_query_part,
+                                        q.scalar_mode, 0, 0, 0, 0, st, arena));
+        b->others.push_back(i);
+        b->n_compiled += b->progs.back()->c.jit ? 1 : 0;
+        continue;
+      }
       blz_program p = nullptr;
       try {
         p = make_program(q.frame, q.n_frame, q.filter_query_part, q.project_query_part, q.scalar_mode, 4, 0, r, t,
@@ -1829,6 +1865,89 @@ double batch_time_ms(blz_batch b, hipStream_t st, int iters) {
   return total / iters;
 }
 
+// The autotuner's per-query tier. The largest queries (rows x bytes per row of
+// their frame), each of at least kJitMinRows rows, at most kJitMaxPlans of them and
+// until they hold 90 % of the batch's bytes, are compiled for the compiled tier in
+// one program; each is then timed alone on the compiled tier and on the tile VM at
+// the batch's shape (r, t), and flagged in `compiled` when the compiled kernel is
+// faster (every one that compiles when `always`). Returns the number flagged.
+constexpr int kJitMaxPlans = 40;
+constexpr int64_t kJitMinRows = 1 << 20;
+
+int choose_compiled(const blz_query *qs, int n, int r, int t, bool always, std::vector<char> &compiled) {
+  compiled.assign(n, 0);
+  std::vector<std::pair<double, int>> by;
+  double total = 0;
+  for (int i = 0; i < n; i++) {
+    double bpr = 0;
+    for (int c = 0; c < qs[i].n_frame; c++)
+      bpr += blz_dtype_width(qs[i].frame[c]->dtype) + (qs[i].frame[c]->valid ? 0.125 : 0.0);
+    const double bytes = (double)qs[i].frame[0]->size * bpr;
+    total += bytes;
+    by.push_back({bytes, i});
+  }
+  std::sort(by.begin(), by.end(), [](const auto &x, const auto &y) { return x.first > y.first; });
+  std::vector<int> cand;
+  std::vector<std::string> srcs;
+  double covered = 0;
+  for (const auto &bi : by) {
+    const blz_query &q = qs[bi.second];
+    if ((int)cand.size() >= kJitMaxPlans || q.frame[0]->size < kJitMinRows || covered >= 0.9 * total) break;
+    covered += bi.first;
+    try {
+      srcs.push_back(plan_jit_source(q.frame, q.n_frame, q.filter_query_part, q.project_query_part, q.scalar_mode));
+      cand.push_back(bi.second);
+    } catch (const BlzError &) {
+      throw;
+    } catch (const std::exception &) {  // a plan the generator does not take stays on the tile VM
+    }
+  }
+  if (cand.empty()) return 0;
+  jit_compile_many(srcs);
+  auto timed = [](blz_program p) {
+    double ms = 0;
+    char e2[256];
+    int rc = blz_program_time(p, nullptr, 1, &ms, e2, sizeof(e2));  // warm-up
+    if (rc == BLZ_OK) rc = blz_program_time(p, nullptr, 3, &ms, e2, sizeof(e2));
+    if (rc != BLZ_OK) throw BlzError(rc, e2);
+    return ms;
+  };
+  int k = 0;
+  for (int i : cand) {
+    const blz_query &q = qs[i];
+    blz_program pc = nullptr, pv = nullptr;
+    try {
+      pc = make_program(q.frame, q.n_frame, q.filter_query_part, q.project_query_part, q.scalar_mode, 5, 0, 0, 0);
+      if (always) {
+        compiled[i] = 1;
+        k++;
+        blz_program_destroy(pc);
+        continue;
+      }
+      try {
+        pv = make_program(q.frame, q.n_frame, q.filter_query_part, q.project_query_part, q.scalar_mode, 4, 0, r, t);
+      } catch (const BlzError &) {
+        throw;
+      } catch (const std::exception &) {  // outside the shared shape: its own interpreter launch
+        pv = make_program(q.frame, q.n_frame, q.filter_query_part, q.project_query_part, q.scalar_mode,
+                          kInterpAuto, 0, 0, 0);
+      }
+      if (timed(pc) < timed(pv)) {
+        compiled[i] = 1;
+        k++;
+      }
+    } catch (const BlzError &) {
+      blz_program_destroy(pc);
+      blz_program_destroy(pv);
+      throw;
+    } catch (const std::exception &) {  // not compiled after all: stays on the tile VM
+    }
+    blz_program_destroy(pc);
+    blz_program_destroy(pv);
+  }
+  return k;
+}
+
 }  // namespace
 
 int blz_batch_create(blz_batch *batch, const blz_query *queries, int n_queries, int r, int t, int use_graph,
@@ -1866,9 +1985,26 @@ int blz_batch_create(blz_batch *batch, const blz_query *queries, int n_queries, 
     }
     if (!best) invalid("no tile VM shape fits every query of the batch");
     const int br = best->r, bt = best->t;
+    // Then the tier of each large query; the batch with the chosen ones compiled
+    // replaces the tile-VM batch only if it runs faster.
+    std::vector<char> compiled;
+    const bool always = t < 0;
+    if (choose_compiled(queries, n_queries, br, bt, always, compiled) > 0) {
+      blz_batch b = batch_build(queries, n_queries, br, bt, false, nullptr, nullptr, &compiled);
+      HIPCHECK(batch_enqueue(b, nullptr));
+      const double ms = batch_time_ms(b, nullptr, 3);
+      if (always || ms < best_ms) {
+        batch_free(best);
+        best = b;
+        best_ms = ms;
+      } else {
+        batch_free(b);
+        compiled.assign(n_queries, 0);
+      }
+    }
     if (use_graph) {
       batch_free(best);
-      best = batch_build(queries, n_queries, br, bt, true);
+      best = batch_build(queries, n_queries, br, bt, true, nullptr, nullptr, compiled.empty() ? nullptr : &compiled);
     }
     best->tuned = tried;
     *batch = best;
@@ -1920,6 +2056,14 @@ int blz_batch_info(blz_batch batch, int *r, int *t, int *n_queries, int *graph, 
   if (n_queries) *n_queries = (int)batch->progs.size();
   if (graph) *graph = batch->use_graph ? 1 : 0;
   if (tuned_candidates) *tuned_candidates = batch->tuned;
+  return BLZ_OK;
+}
+
+int blz_batch_query_kernel(blz_batch batch, int q, int *kernel, int *shared) {
+  if (!batch || q < 0 || q >= (int)batch->progs.size()) return BLZ_ERR_INVALID;
+  const bool own = std::find(batch->others.begin(), batch->others.end(), q) != batch->others.end();
+  if (shared) *shared = own ? 0 : 1;
+  if (kernel) return blz_program_tile(batch->progs[q], nullptr, nullptr, kernel);
   return BLZ_OK;
 }
 

@@ -3,7 +3,7 @@
  *
  * Semantics follow the interpreter kernels op for op: an input column is widened
  * into a 64-bit slot as interpreter_ops.cuh:386-452 widens it (and as the tile VM's
- * operand classes do), each op node is op_eval(code, conv(a), conv(b), raw validity
+ * operand classes do), each op node is op_t<code>(conv(a), conv(b), raw validity
  * bit) with the result conversion after it (the DevOp of program.cpp's build_plan),
  * outputs are narrowed as interpreter_ops.cuh:317-381 narrows them, bare-column
  * outputs are copied by width (materialize_column), and validity is the AND / OR
@@ -188,7 +188,7 @@ struct Gen {
       if (n.op.vcol != 0xFF && in.in_valid[n.op.vcol])
         vb = per_lane_bits ? "vb" + std::to_string(n.op.vcol) + "[u]"
                            : "((readlane64(mv" + std::to_string(n.op.vcol) + ", u) >> lane) & 1) != 0";
-      const std::string e = std::string("op_eval(") + op_name(n.op.code) + ", " + conv_expr(ca, operand(n.a)) +
+      const std::string e = std::string("op_t<") + op_name(n.op.code) + ">(" + conv_expr(ca, operand(n.a)) +
                             ", " + conv_expr(cb, operand(n.b)) + ", " + vb + ")";
       o << ind << "const uint64_t v" << id << " = " << conv_expr(co, e) << ";\n";
     }
@@ -246,6 +246,8 @@ struct Gen {
 };
 
 }  // namespace
+
+const char *codegen_prelude() { return kPrelude; }
 
 CodegenOutput generate_kernel_source(const CodegenInput &in) {
   Gen g(in);

@@ -60,5 +60,10 @@ struct CodegenOutput {
 // Throws std::runtime_error when the plan has a shape the generator does not take.
 CodegenOutput generate_kernel_source(const CodegenInput &in);
 
+// The text every generated source starts with (includes and shared helpers); the
+// rest of a source is its kernels, named blz_cg_project, or blz_cg_pred and
+// blz_cg_gather, and the RS / GR macros they use.
+const char *codegen_prelude();
+
 }  // namespace blz
 #endif

@@ -310,63 +310,120 @@ DEV void exec_op(int code, const uint64_t (&A)[R], const uint64_t (&B)[R], const
 #undef BLZ_CASE
 }
 
-// One fully typed op for one row, every code inline: the run-time compiled
-// kernels call it with a constant `code`, so it folds to that op's expression (the
-// same expressions as exec_op and slow_op).
-DEV uint64_t op_eval(int code, uint64_t a, uint64_t b, bool vb) {
+// One fully typed op for one row, its code a template argument: the run-time
+// compiled kernels instantiate the ops they use, each folded to that op's
+// expression (the same expressions as exec_op and slow_op); op_eval dispatches a
+// run-time code to them.
+template <int C>
+DEV uint64_t op_t(uint64_t a, uint64_t b, bool vb) {
   const double x = b2d(a), y = b2d(b);
+  (void)x; (void)y; (void)vb;
+  if constexpr (C == OP_ADD_I) return a + b;
+  else if constexpr (C == OP_ADD_F) return d2b(x + y);
+  else if constexpr (C == OP_SUB_I) return a - b;
+  else if constexpr (C == OP_SUB_F) return d2b(x - y);
+  else if constexpr (C == OP_MUL_I) return a * b;
+  else if constexpr (C == OP_MUL_F) return d2b(x * y);
+  else if constexpr (C == OP_DIV_I) return (uint64_t)idiv((int64_t)a, (int64_t)b);
+  else if constexpr (C == OP_DIV_F) return d2b(x / y);
+  else if constexpr (C == OP_MOD_I) return (uint64_t)imod((int64_t)a, (int64_t)b);
+  else if constexpr (C == OP_POW_I) return ipow(a, (int64_t)b);
+  else if constexpr (C == OP_POW_F) return d2b(pow(x, y));
+  else if constexpr (C == OP_EQ_I) return (uint64_t)(a == b);
+  else if constexpr (C == OP_EQ_F) return (uint64_t)(x == y);
+  else if constexpr (C == OP_NE_I) return (uint64_t)(a != b);
+  else if constexpr (C == OP_NE_F) return (uint64_t)(x != y);
+  else if constexpr (C == OP_LT_I) return (uint64_t)((int64_t)a < (int64_t)b);
+  else if constexpr (C == OP_LT_F) return (uint64_t)(x < y);
+  else if constexpr (C == OP_GT_I) return (uint64_t)((int64_t)a > (int64_t)b);
+  else if constexpr (C == OP_GT_F) return (uint64_t)(x > y);
+  else if constexpr (C == OP_LE_I) return (uint64_t)((int64_t)a <= (int64_t)b);
+  else if constexpr (C == OP_LE_F) return (uint64_t)(x <= y);
+  else if constexpr (C == OP_GE_I) return (uint64_t)((int64_t)a >= (int64_t)b);
+  else if constexpr (C == OP_GE_F) return (uint64_t)(x >= y);
+  else if constexpr (C == OP_COALESCE) return vb ? a : b;
+  else if constexpr (C == OP_AND_B) return a & b;
+  else if constexpr (C == OP_FLOOR) return d2b(floor(x));
+  else if constexpr (C == OP_CEIL) return d2b(ceil(x));
+  else if constexpr (C == OP_SIN) return d2b(sin(x));
+  else if constexpr (C == OP_COS) return d2b(cos(x));
+  else if constexpr (C == OP_ASIN) return d2b(asin(x));
+  else if constexpr (C == OP_ACOS) return d2b(acos(x));
+  else if constexpr (C == OP_TAN) return d2b(tan(x));
+  else if constexpr (C == OP_COTAN) return d2b(cos(x) / sin(x));
+  else if constexpr (C == OP_ATAN) return d2b(atan(x));
+  else if constexpr (C == OP_ABS) return d2b(fabs(x));
+  else if constexpr (C == OP_LN) return d2b(log(x));
+  else if constexpr (C == OP_LOG10) return d2b(log10(x));
+  else if constexpr (C == OP_NOT_I) return (uint64_t)(a == 0);
+  else if constexpr (C == OP_NOT_F) return (uint64_t)(!(x != 0.0));
+  else if constexpr (C == OP_YEAR32) return (uint64_t)year_d((int64_t)a);
+  else if constexpr (C == OP_MONTH32) return (uint64_t)month_d((int64_t)a);
+  else if constexpr (C == OP_DAY32) return (uint64_t)day_d((int64_t)a);
+  else if constexpr (C == OP_YEAR64) return (uint64_t)year_d(ms_days((int64_t)a));
+  else if constexpr (C == OP_MONTH64) return (uint64_t)month_d(ms_days((int64_t)a));
+  else if constexpr (C == OP_DAY64) return (uint64_t)day_d(ms_days((int64_t)a));
+  else if constexpr (C == OP_HOUR64) return (uint64_t)hour_ms((int64_t)a);
+  else if constexpr (C == OP_MINUTE64) return (uint64_t)minute_ms((int64_t)a);
+  else if constexpr (C == OP_SECOND64) return (uint64_t)second_ms((int64_t)a);
+  else if constexpr (C == OP_ZERO) return 0ull;
+  else if constexpr (C == OP_IS_NULL) return (uint64_t)(!vb);
+  else if constexpr (C == OP_IS_NOT_NULL) return (uint64_t)vb;
+  else return a;  // OP_IDENT
+}
+DEV uint64_t op_eval(int code, uint64_t a, uint64_t b, bool vb) {
   switch (code) {
-    case OP_ADD_I: return a + b;
-    case OP_ADD_F: return d2b(x + y);
-    case OP_SUB_I: return a - b;
-    case OP_SUB_F: return d2b(x - y);
-    case OP_MUL_I: return a * b;
-    case OP_MUL_F: return d2b(x * y);
-    case OP_DIV_I: return (uint64_t)idiv((int64_t)a, (int64_t)b);
-    case OP_DIV_F: return d2b(x / y);
-    case OP_MOD_I: return (uint64_t)imod((int64_t)a, (int64_t)b);
-    case OP_POW_I: return ipow(a, (int64_t)b);
-    case OP_POW_F: return d2b(pow(x, y));
-    case OP_EQ_I: return (uint64_t)(a == b);
-    case OP_EQ_F: return (uint64_t)(x == y);
-    case OP_NE_I: return (uint64_t)(a != b);
-    case OP_NE_F: return (uint64_t)(x != y);
-    case OP_LT_I: return (uint64_t)((int64_t)a < (int64_t)b);
-    case OP_LT_F: return (uint64_t)(x < y);
-    case OP_GT_I: return (uint64_t)((int64_t)a > (int64_t)b);
-    case OP_GT_F: return (uint64_t)(x > y);
-    case OP_LE_I: return (uint64_t)((int64_t)a <= (int64_t)b);
-    case OP_LE_F: return (uint64_t)(x <= y);
-    case OP_GE_I: return (uint64_t)((int64_t)a >= (int64_t)b);
-    case OP_GE_F: return (uint64_t)(x >= y);
-    case OP_COALESCE: return vb ? a : b;
-    case OP_AND_B: return a & b;
-    case OP_FLOOR: return d2b(floor(x));
-    case OP_CEIL: return d2b(ceil(x));
-    case OP_SIN: return d2b(sin(x));
-    case OP_COS: return d2b(cos(x));
-    case OP_ASIN: return d2b(asin(x));
-    case OP_ACOS: return d2b(acos(x));
-    case OP_TAN: return d2b(tan(x));
-    case OP_COTAN: return d2b(cos(x) / sin(x));
-    case OP_ATAN: return d2b(atan(x));
-    case OP_ABS: return d2b(fabs(x));
-    case OP_LN: return d2b(log(x));
-    case OP_LOG10: return d2b(log10(x));
-    case OP_NOT_I: return (uint64_t)(a == 0);
-    case OP_NOT_F: return (uint64_t)(!(x != 0.0));
-    case OP_YEAR32: return (uint64_t)year_d((int64_t)a);
-    case OP_MONTH32: return (uint64_t)month_d((int64_t)a);
-    case OP_DAY32: return (uint64_t)day_d((int64_t)a);
-    case OP_YEAR64: return (uint64_t)year_d(ms_days((int64_t)a));
-    case OP_MONTH64: return (uint64_t)month_d(ms_days((int64_t)a));
-    case OP_DAY64: return (uint64_t)day_d(ms_days((int64_t)a));
-    case OP_HOUR64: return (uint64_t)hour_ms((int64_t)a);
-    case OP_MINUTE64: return (uint64_t)minute_ms((int64_t)a);
-    case OP_SECOND64: return (uint64_t)second_ms((int64_t)a);
-    case OP_ZERO: return 0ull;
-    case OP_IS_NULL: return (uint64_t)(!vb);
-    case OP_IS_NOT_NULL: return (uint64_t)vb;
+    case OP_ADD_I: return op_t<OP_ADD_I>(a, b, vb);
+    case OP_ADD_F: return op_t<OP_ADD_F>(a, b, vb);
+    case OP_SUB_I: return op_t<OP_SUB_I>(a, b, vb);
+    case OP_SUB_F: return op_t<OP_SUB_F>(a, b, vb);
+    case OP_MUL_I: return op_t<OP_MUL_I>(a, b, vb);
+    case OP_MUL_F: return op_t<OP_MUL_F>(a, b, vb);
+    case OP_DIV_I: return op_t<OP_DIV_I>(a, b, vb);
+    case OP_DIV_F: return op_t<OP_DIV_F>(a, b, vb);
+    case OP_MOD_I: return op_t<OP_MOD_I>(a, b, vb);
+    case OP_POW_I: return op_t<OP_POW_I>(a, b, vb);
+    case OP_POW_F: return op_t<OP_POW_F>(a, b, vb);
+    case OP_EQ_I: return op_t<OP_EQ_I>(a, b, vb);
+    case OP_EQ_F: return op_t<OP_EQ_F>(a, b, vb);
+    case OP_NE_I: return op_t<OP_NE_I>(a, b, vb);
+    case OP_NE_F: return op_t<OP_NE_F>(a, b, vb);
+    case OP_LT_I: return op_t<OP_LT_I>(a, b, vb);
+    case OP_LT_F: return op_t<OP_LT_F>(a, b, vb);
+    case OP_GT_I: return op_t<OP_GT_I>(a, b, vb);
+    case OP_GT_F: return op_t<OP_GT_F>(a, b, vb);
+    case OP_LE_I: return op_t<OP_LE_I>(a, b, vb);
+    case OP_LE_F: return op_t<OP_LE_F>(a, b, vb);
+    case OP_GE_I: return op_t<OP_GE_I>(a, b, vb);
+    case OP_GE_F: return op_t<OP_GE_F>(a, b, vb);
+    case OP_COALESCE: return op_t<OP_COALESCE>(a, b, vb);
+    case OP_AND_B: return op_t<OP_AND_B>(a, b, vb);
+    case OP_FLOOR: return op_t<OP_FLOOR>(a, b, vb);
+    case OP_CEIL: return op_t<OP_CEIL>(a, b, vb);
+    case OP_SIN: return op_t<OP_SIN>(a, b, vb);
+    case OP_COS: return op_t<OP_COS>(a, b, vb);
+    case OP_ASIN: return op_t<OP_ASIN>(a, b, vb);
+    case OP_ACOS: return op_t<OP_ACOS>(a, b, vb);
+    case OP_TAN: return op_t<OP_TAN>(a, b, vb);
+    case OP_COTAN: return op_t<OP_COTAN>(a, b, vb);
+    case OP_ATAN: return op_t<OP_ATAN>(a, b, vb);
+    case OP_ABS: return op_t<OP_ABS>(a, b, vb);
+    case OP_LN: return op_t<OP_LN>(a, b, vb);
+    case OP_LOG10: return op_t<OP_LOG10>(a, b, vb);
+    case OP_NOT_I: return op_t<OP_NOT_I>(a, b, vb);
+    case OP_NOT_F: return op_t<OP_NOT_F>(a, b, vb);
+    case OP_YEAR32: return op_t<OP_YEAR32>(a, b, vb);
+    case OP_MONTH32: return op_t<OP_MONTH32>(a, b, vb);
+    case OP_DAY32: return op_t<OP_DAY32>(a, b, vb);
+    case OP_YEAR64: return op_t<OP_YEAR64>(a, b, vb);
+    case OP_MONTH64: return op_t<OP_MONTH64>(a, b, vb);
+    case OP_DAY64: return op_t<OP_DAY64>(a, b, vb);
+    case OP_HOUR64: return op_t<OP_HOUR64>(a, b, vb);
+    case OP_MINUTE64: return op_t<OP_MINUTE64>(a, b, vb);
+    case OP_SECOND64: return op_t<OP_SECOND64>(a, b, vb);
+    case OP_ZERO: return op_t<OP_ZERO>(a, b, vb);
+    case OP_IS_NULL: return op_t<OP_IS_NULL>(a, b, vb);
+    case OP_IS_NOT_NULL: return op_t<OP_IS_NOT_NULL>(a, b, vb);
     default: return a;  // OP_IDENT
   }
 }

@@ -8,6 +8,8 @@
  */
 #include "jit.h"
 
+#include "codegen.h"
+
 #include <hip/hiprtc.h>
 
 #include <chrono>
@@ -31,6 +33,10 @@ struct Entry {
   std::vector<char> code;                 // code object for gfx950
   std::map<int, hipModule_t> modules;     // per device
   std::map<std::pair<int, std::string>, hipFunction_t> functions;  // per device and name
+  // Compiled as part of a combined program: the entry holding its code object, and
+  // the suffix of its kernels' names there.
+  std::shared_ptr<Entry> shared;
+  std::string suffix;
 };
 
 std::mutex g_mu;
@@ -78,6 +84,19 @@ void compile(Entry &e, const std::string &src) {
   g_compile_ms += ms;
 }
 
+// The module of `e`'s code object on device `dev`, loaded on first use (e.mu held).
+hipModule_t module_on(Entry &e, int dev) {
+  auto m = e.modules.find(dev);
+  if (m == e.modules.end()) {
+    hipModule_t mod = nullptr;
+    const hipError_t err = hipModuleLoadData(&mod, e.code.data());
+    if (err != hipSuccess)
+      throw std::runtime_error(std::string("compiled tier: hipModuleLoadData: ") + hipGetErrorString(err));
+    m = e.modules.emplace(dev, mod).first;
+  }
+  return m->second;
+}
+
 }  // namespace
 
 hipFunction_t jit_function(const std::string &src, const std::string &name) {
@@ -98,20 +117,74 @@ hipFunction_t jit_function(const std::string &src, const std::string &name) {
   }
   auto it = e->functions.find({dev, name});
   if (it != e->functions.end()) return it->second;
-  auto m = e->modules.find(dev);
-  if (m == e->modules.end()) {
-    hipModule_t mod = nullptr;
-    const hipError_t err = hipModuleLoadData(&mod, e->code.data());
-    if (err != hipSuccess)
-      throw std::runtime_error(std::string("compiled tier: hipModuleLoadData: ") + hipGetErrorString(err));
-    m = e->modules.emplace(dev, mod).first;
-  }
+  const std::string sym = name + e->suffix;
   hipFunction_t f = nullptr;
-  const hipError_t err = hipModuleGetFunction(&f, m->second, name.c_str());
+  hipError_t err = hipSuccess;
+  if (e->shared) {
+    std::lock_guard<std::mutex> g2(e->shared->mu);
+    err = hipModuleGetFunction(&f, module_on(*e->shared, dev), sym.c_str());
+  } else {
+    err = hipModuleGetFunction(&f, module_on(*e, dev), sym.c_str());
+  }
   if (err != hipSuccess)
-    throw std::runtime_error(std::string("compiled tier: hipModuleGetFunction(") + name + "): " + hipGetErrorString(err));
+    throw std::runtime_error(std::string("compiled tier: hipModuleGetFunction(") + sym + "): " + hipGetErrorString(err));
   e->functions[{dev, name}] = f;
   return f;
+}
+
+void jit_compile_many(const std::vector<std::string> &srcs) {
+  const std::string pre = codegen_prelude();
+  std::vector<std::pair<const std::string *, std::shared_ptr<Entry>>> todo;
+  std::vector<const std::string *> alone;
+  {
+    std::lock_guard<std::mutex> g(g_mu);
+    for (const std::string &src : srcs) {
+      auto &slot = g_cache[src];
+      if (!slot) slot = std::make_shared<Entry>();
+      if (slot->built) continue;
+      if (src.compare(0, pre.size(), pre) == 0) todo.push_back({&src, slot});
+      else alone.push_back(&src);
+    }
+  }
+  auto each_alone = [](const std::string &src) {
+    try {
+      jit_compile_only(src);
+    } catch (const std::exception &) {  // reported by jit_function when it is used
+    }
+  };
+  for (const std::string *src : alone) each_alone(*src);
+  if (todo.size() < 2) {
+    for (auto &t : todo) each_alone(*t.first);
+    return;
+  }
+  std::string all = pre;
+  for (size_t k = 0; k < todo.size(); k++) {
+    std::string body = todo[k].first->substr(pre.size());
+    const std::string suffix = "_m" + std::to_string(k);
+    for (const char *kn : {"blz_cg_project", "blz_cg_pred", "blz_cg_gather"}) {
+      const std::string from = std::string("__launch_bounds__(256) ") + kn + "(";
+      const std::string to = std::string("__launch_bounds__(256) ") + kn + suffix + "(";
+      for (size_t at = body.find(from); at != std::string::npos; at = body.find(from, at + to.size()))
+        body.replace(at, from.size(), to);
+    }
+    all += "#undef RS\n#undef GR\n" + body;
+  }
+  auto shared = std::make_shared<Entry>();
+  try {
+    compile(*shared, all);
+    shared->built = true;
+  } catch (const std::exception &) {
+    for (auto &t : todo) each_alone(*t.first);
+    return;
+  }
+  for (size_t k = 0; k < todo.size(); k++) {
+    Entry &e = *todo[k].second;
+    std::lock_guard<std::mutex> g(e.mu);
+    if (e.built) continue;
+    e.shared = shared;
+    e.suffix = "_m" + std::to_string(k);
+    e.built = true;
+  }
 }
 
 void jit_compile_only(const std::string &src) {

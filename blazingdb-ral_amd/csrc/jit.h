@@ -14,6 +14,7 @@
 #include <hip/hip_runtime.h>
 
 #include <string>
+#include <vector>
 
 #include "program.h"
 
@@ -26,6 +27,15 @@ hipFunction_t jit_function(const std::string &src, const std::string &name);
 // Compiles `src` (cached as jit_function caches it) without loading it: needs no
 // device. Throws as jit_function does.
 void jit_compile_only(const std::string &src);
+
+// Compiles the generated sources of `srcs` that are not compiled yet as one
+// program: the prelude once, then each source's kernels under names of their own,
+// so the compiler's fixed cost (most of a small plan's compile time) is paid once.
+// jit_function then finds each source compiled (its kernels load from the shared
+// code object). A source that is not a generated one, or a combined compile that
+// fails, is compiled on its own; a source that does not compile is left for
+// jit_function to report. Needs no device.
+void jit_compile_many(const std::vector<std::string> &srcs);
 
 // Workgroups of 256 threads of `f` that stay resident on one CU.
 int jit_blocks_per_cu(hipFunction_t f);

@@ -227,6 +227,14 @@ int blz_program_source(gdf_column *const *frame, int n_frame, const char *filter
 /* Compiles a generated kernel source for gfx950 without a device (the compiled
  * tier's compiler, for inspection and tests). No reference counterpart. */
 int blz_jit_compile(const char *src, char *err, size_t err_len);
+/* Compiles n generated sources that are not compiled yet as one program (the
+ * compiler's fixed cost paid once; programs created later over these plans find
+ * their kernels compiled). Sources that do not compile are left to be reported
+ * when used. No device needed. No reference counterpart. */
+int blz_jit_compile_many(const char *const *srcs, int n, char *err, size_t err_len);
+/* Compilations done and cache hits since the process began, and milliseconds spent
+ * compiling. No reference counterpart. */
+int blz_jit_counters(long long *compiles, long long *hits, double *compile_ms);
 
 /* ---- batches (no reference counterpart: many independent queries, one launch) ----
  *
@@ -239,8 +247,14 @@ int blz_jit_compile(const char *src, char *err, size_t err_len);
  *
  * r / t: rows per lane (2, 4 or 8) and tile rows of every query; r = 0 runs the
  * autotuner, which times each candidate shape on the batch itself (a few
- * executions each) and keeps the fastest. use_graph != 0 captures the launch
- * sequence into a HIP graph once and replays it on every blz_batch_launch. */
+ * executions each) and keeps the fastest, then compiles the largest queries (at
+ * least 2^20 rows, at most 40, in one run-time compilation) for the compiled tier
+ * and runs each of them as a launch of its own where that times faster than the
+ * tile VM (the whole batch is timed again, and kept only if faster); r = 0 with
+ * t < 0 runs every such query that compiles on the compiled tier, untimed.
+ * use_graph != 0
+ * captures the launch sequence into a HIP graph once and replays it on every
+ * blz_batch_launch. */
 typedef struct blz_batch_s *blz_batch;
 typedef struct {
   gdf_column *const *frame;
@@ -264,6 +278,10 @@ int blz_batch_time(blz_batch batch, blz_stream_t stream, int iters, double *mean
 /* r, tile rows, number of queries, whether a graph is replayed, autotuner's
  * candidates timed. */
 int blz_batch_info(blz_batch batch, int *r, int *t, int *n_queries, int *graph, int *tuned_candidates);
+/* The kernel kind running query q (as blz_program_tile reports it: 4 tile VM,
+ * 5 compiled tier, ...) and whether it runs inside a shared launch (*shared = 1)
+ * or as a launch of its own. */
+int blz_batch_query_kernel(blz_batch batch, int q, int *kernel, int *shared);
 int blz_batch_destroy(blz_batch batch);
 
 /* ---- serving (no reference counterpart; the reference runs one detached thread

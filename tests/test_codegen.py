@@ -75,3 +75,38 @@ def test_bad_source_reports_the_compiler_log(B):
     with pytest.raises(B.BlzError) as e:
         B.jit_compile("this is not HIP")
     assert "hiprtc" in str(e.value)
+
+
+def test_many_sources_compile_as_one_program(B):
+    """Sources compiled together are compiled once (one run of the compiler for
+    all of them, filters and projections alike), and each is found compiled
+    afterwards; a source that does not compile leaves the others compiled."""
+    rng = np.random.default_rng(4242)
+    dts = [P.GDF_INT32, P.GDF_INT64, P.GDF_FLOAT32, P.GDF_FLOAT64]
+    exprs = [exprgen.rand_expr(rng, dts, 3, True, []) for _ in range(5)]
+
+    def sources(base, ks):
+        out = []
+        for k in ks:
+            proj = f"LogicalProject(E=[+({exprs[k]}, {base + k})], F=[$1])"  # distinct literals: not cached yet
+            filt = f"LogicalFilter(condition=[<($0, {base + 100 + k})])" if k % 2 else None
+            out.append(B.program_source(dts, 4096, filt, proj, has_valid=[True, False, True, False],
+                                        null_counts=[2, 0, 2, 0]))
+        return out
+    srcs = sources(7000, range(5))
+    broken = srcs[0].replace("extern \"C\"", "extern \"C\" this_does_not_parse", 1) + "\n// x"
+    c0, _, _ = B.jit_counters()
+    B.jit_compile_many(srcs)
+    c1, h1, _ = B.jit_counters()
+    assert c1 - c0 == 1
+    for s in srcs:
+        B.jit_compile(s)                      # found compiled: no new compilation
+    assert B.jit_counters()[0] == c1
+    more = sources(8000, range(3))
+    B.jit_compile_many(more + [broken])     # the combined compile fails: each compiled on its own
+    assert B.jit_counters()[0] == c1 + 3      # (a failed compilation is not counted)
+    for s in more:
+        B.jit_compile(s)
+    assert B.jit_counters()[0] == c1 + 3
+    with pytest.raises(B.BlzError):
+        B.jit_compile(broken)

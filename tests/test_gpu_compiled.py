@@ -149,3 +149,36 @@ def test_random_projections(B, seed):
     assert prog.kernel == 5
     _check(prog, ref, proj, launches=2)
     prog.close()
+
+
+def test_programs_over_sources_compiled_together(B):
+    """Plans whose sources were compiled as one program (one compilation for all of
+    them): each prepared program finds its kernels there and gives the oracle's
+    result, filters and projections alike."""
+    rng = np.random.default_rng(6400)
+    dts = [P.GDF_INT32, P.GDF_INT64, P.GDF_FLOAT32, P.GDF_FLOAT64]
+    n = 77_777
+    host = [gen_column(rng, dt, n, nr) for dt, nr in zip(dts, [0.1, 0.0, 0.05, 0.0])]
+    plans = []
+    for k in range(5):
+        e = exprgen.rand_expr(rng, dts, 3, False, [])
+        proj = f"LogicalProject(E=[+({e}, {6400 + k})], F=[$1], G=[*($3, {k + 2})])"
+        filt = f"LogicalFilter(condition=[<($0, {100 * k - 150})])" if k % 2 else None
+        plans.append((filt, proj))
+    srcs = [B.program_source(dts, n, f, p, has_valid=[h.valid is not None for h in host],
+                             null_counts=[h.null_count for h in host]) for f, p in plans]
+    c0 = B.jit_counters()[0]
+    B.jit_compile_many(srcs)
+    assert B.jit_counters()[0] == c0 + 1
+    dev = [to_device(B, h) for h in host]
+    for filt, proj in plans:
+        _print_case(test="compiled_together", filt=filt, proj=proj)
+        mid = O.process_filter(host, filt) if filt else host
+        ref = O.execute_project(mid, proj)
+        prog = B.Program(dev, filt, proj, kernel=5)
+        assert prog.kernel == 5
+        _check(prog, ref, f"{filt} {proj}", launches=2)
+        prog.close()
+    assert B.jit_counters()[0] == c0 + 1   # no compilation when the programs were made
+    for c in dev:
+        c.free()

@@ -95,6 +95,34 @@ def test_batch_of_workload_plans(B, shape):
     batch.close()
 
 
+@pytest.mark.parametrize("mode", ["compile_all", "timed"])
+def test_batch_with_large_plans_on_the_compiled_tier(B, mode):
+    """Plans of at least 2^20 rows are compiled (in one compilation) and run as
+    launches of their own beside the shared tile-VM launch: every one when asked
+    (t < 0), else where the autotuner times them faster. Every query's result
+    equals the oracle's over three launches of the graph."""
+    rng = np.random.default_rng(46_000 + (mode == "timed"))
+    big, ref_big = _batch_queries(B, rng, 3, 1.1e6, 1.6e6, 0.33)
+    small, ref_small = _batch_queries(B, rng, 7, 1e4, 1e5, 0.33)
+    qs, refs = big + small, ref_big + ref_small
+    print(f"CASE batch_compiled mode={mode} rows={[q[0][0].col.size for q in qs]}", flush=True)
+    batch = B.Batch(qs, r=0, t=-1 if mode == "compile_all" else 0, use_graph=True)
+    print(f"kernels {batch.kernels}", flush=True)
+    if mode == "compile_all":
+        assert batch.n_compiled == 3
+        assert all(k == (5, False) for k in batch.kernels[:3])
+    assert all(k[0] != 5 for k in batch.kernels[3:])
+    for rep in range(3):
+        batch.launch()
+        for q, ((dev, filt, proj), ref) in enumerate(zip(qs, refs)):
+            kept, cols, nulls = batch.result(q)
+            _check(kept, cols, nulls, ref, f"batch {mode} q={q} rep={rep}: {filt} {proj}")
+    batch.close()
+    for dev, _, _ in qs:
+        for c in dev:
+            c.free()
+
+
 def test_served_poisson_replay_every_result(B):
     """A Poisson replay (2000 queries/s) of filter + project queries of the served
     workload over four resident tables of the mix: every query's result, read
