@@ -214,7 +214,7 @@ def run_batch(B, n_plans, rows_lo, rows_hi, seed=404):
             "speedup_vs_prepared": float(done[-1] / batch_ms)}
 
 
-def run_serve(B, total_gb, rates, seconds, seed=505):
+def run_serve(B, total_gb, rates, seconds, seed=505, r=4):
     """Served query loop over device-resident tables (wide-expression column mix,
     5% nulls) totalling total_gb: Poisson arrivals at each rate, filter + project
     queries of 4-40 ops, admitted by a 200 GB HBM budget and continuously batched
@@ -241,7 +241,7 @@ def run_serve(B, total_gb, rates, seconds, seed=505):
         f, p = W.rand_query(rng, specs[i], True)
         warm.append((tables[i], f, p))
     for mb, ns in ((64, 4), (1, 1)):
-        srv = B.Server(200 << 30, max_batch=mb, n_streams=ns, r=4)
+        srv = B.Server(200 << 30, max_batch=mb, n_streams=ns, r=r)
         srv.replay(warm, np.zeros(len(warm)))
         srv.close()
     for qps in rates:
@@ -254,7 +254,7 @@ def run_serve(B, total_gb, rates, seconds, seed=505):
             qs.append((tables[i], f, p))
         row = {}
         for mode, mb, ns in (("served", 64, 4), ("per_query", 1, 1)):
-            srv = B.Server(200 << 30, max_batch=mb, n_streams=ns, r=4)
+            srv = B.Server(200 << 30, max_batch=mb, n_streams=ns, r=r)
             # untimed: the first queries of this rate's schedule (the pool settles to
             # this mix's allocation sizes)
             k = min(n, 256)

@@ -164,7 +164,7 @@ void layout(LaunchBuffers &lb, const Compiled &c, int n_out, bool compact, int64
   size_t off = lb.zero_off + lb.zero_bytes;
   const bool vmf = (c.vm || c.jit) && compact;  // predicate, slot scan, gather
   const int64_t slots = vmf ? n_tiles * 4 : 0;
-  lb.n_chunks = (slots + kScanChunk - 1) / kScanChunk;
+  lb.n_chunks = (slots + scan_chunk_slots(c.T) - 1) / scan_chunk_slots(c.T);
   lb.keep_off = off;
   off += round_up(vmf ? sizeof(uint64_t) * n_tiles * (c.T / 64) : 0, 256);
   lb.count_off = off;
@@ -265,11 +265,14 @@ hipError_t launch_program(const Compiled &c, bool compact, int grid, hipStream_t
       // count and 256 per CU — many kept rows run faster with more workgroups in
       // flight than stay resident (500M rows at s = 0.5: 8.3 -> 6.7 ms), few rows
       // slower (C1's 2.3%: 0.070 ms at the resident count, 0.129 at 256 per CU).
+      // Outputs with validity add one null-count atomic per workgroup each, and
+      // atomics on one address serialize: then 8 workgroups per CU at most.
       const int64_t chunks = (c.n_rows + 255) / 256;  // most gather chunks (every row kept)
       const int64_t resident = (int64_t)device_cus() * jit_blocks_per_cu(fg);
       int64_t gg = std::min<int64_t>((chunks + 3) / 4, resident);
       if (sel_hint >= 0)
         gg = std::max<int64_t>(gg, std::min<int64_t>((sel_hint + 1023) / 1024, (int64_t)device_cus() * 256));
+      if (c.jit_null_outputs > 0) gg = std::min<int64_t>(gg, (int64_t)device_cus() * 8);
       e = launch_jit(fg, (int)std::max<int64_t>(1, gg), st, blob, ws);
     }
     return e == hipSuccess ? mark(3) : e;
@@ -1753,7 +1756,7 @@ blz_batch batch_build(const blz_query *qs, int n, int r, int t, bool use_graph, 
         const int64_t tiles = (c.n_rows + c.T - 1) / c.T;
         tile0 += tiles;
         gchunks += (c.n_rows + 64 * b->r - 1) / (64 * b->r);
-        chunk0 += (tiles * 4 + kScanChunk - 1) / kScanChunk;
+        chunk0 += (tiles * 4 + scan_chunk_slots(c.T) - 1) / scan_chunk_slots(c.T);
         maxbuf = std::max(maxbuf, c.vm_bufb);
         maxtmp = std::max(maxtmp, c.lds_bytes - c.vm_tmp_off);
         maxvs = std::max(maxvs, c.n_vslots);

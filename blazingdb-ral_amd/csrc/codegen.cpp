@@ -205,7 +205,8 @@ struct Gen {
   }
   // Rows (g0 + u) * 64 + lane of `cols` into r<c>[u], from a uniform base plus the
   // lane offset and a constant per group; in a step reaching past the last row
-  // (CLAMP) the lanes past it load nothing (0).
+  // (CLAMP) the lanes past it hold 0 (they load the step's first row, in range, so
+  // that no load sits behind a branch of its own and waits alone).
   void emit_loads(const std::set<int> &cols, const std::string &ind) {
     for (int c : cols) o << ind << uint_of_width(in.in_width[c]) << " r" << c << "[RS];\n";
     if (cols.empty()) return;
@@ -215,7 +216,11 @@ struct Gen {
     o << ind << "  const uint32_t lo = (uint32_t)(u * 64 + lane);\n";
     o << ind << "  const bool in_range = !CLAMP || (g0 + u) * 64 + lane < n;\n";
     for (int c : cols)
-      o << ind << "  r" << c << "[u] = in_range ? __builtin_nontemporal_load(b" << c << " + lo) : 0;\n";
+      o << ind << "  r" << c << "[u] = __builtin_nontemporal_load(b" << c << " + (in_range ? lo : 0u));\n";
+    o << ind << "}\n";
+    o << ind << "#pragma unroll\n" << ind << "for (int u = 0; u < RS; u++) {\n";
+    o << ind << "  const bool in_range = !CLAMP || (g0 + u) * 64 + lane < n;\n";
+    for (int c : cols) o << ind << "  if (!in_range) r" << c << "[u] = 0;\n";
     o << ind << "}\n";
   }
   // Bitmap words of the groups g0 .. g0 + RS - 1: lane u < RS holds group u's word
@@ -223,8 +228,11 @@ struct Gen {
   void emit_bitmap_words(const std::set<int> &bm, const std::string &ind, const std::string &clamp) {
     for (int c : bm) {
       o << ind << "uint64_t mv" << c << " = 0;\n";
-      o << ind << "if (lane < RS) mv" << c << " = " << clamp << " ? load_vword(m" << c << ", g0 + lane, n) : "
-        << "((const GLOBAL uint64_t *)m" << c << ")[g0 + lane];\n";
+      // whole steps: every lane loads (lanes past RS reload word g0, then hold 0), so
+      // the load is not behind a branch of its own
+      o << ind << "if (" << clamp << ") { if (lane < RS) mv" << c << " = load_vword(m" << c << ", g0 + lane, n); }\n"
+        << ind << "else { const uint64_t t = ((const GLOBAL uint64_t *)m" << c << ")[g0 + (lane < RS ? lane : 0)];"
+        << " mv" << c << " = lane < RS ? t : 0ull; }\n";
     }
   }
   // Stores of the outputs at row index `pos` (row u); a bare-column output copies
@@ -477,26 +485,36 @@ CodegenOutput generate_kernel_source(const CodegenInput &in) {
          "    #pragma unroll\n    for (int u = 0; u < GR; u++) {\n"
          "      const int64_t j = j0 + 64 * u + lane;\n"
          "      live[u] = j < n_sel;\n"
-         "      row[u] = live[u] ? (int64_t)__builtin_nontemporal_load(ws.idx + j) : 0;\n"
+         "      // every lane loads (a lane past the last output reads the chunk's first\n"
+         "      // index): a load behind a condition is waited for on its own\n"
+         "      const uint32_t ri = __builtin_nontemporal_load(ws.idx + (live[u] ? j : j0));\n"
+         "      row[u] = live[u] ? (int64_t)ri : 0;\n"
          "    }\n";
     if (!side.empty()) {
       o << "    int64_t sp[GR];  // place of the row in its slot's staged values\n"
            "    #pragma unroll\n    for (int u = 0; u < GR; u++) {\n"
+           "      // lanes past the last output: row 0, slot 0, whose prefix is 0, so sp = 0\n"
            "      const int64_t sl = row[u] / (64 * RS);\n"
-           "      sp[u] = live[u] ? sl * 64 * RS + (j0 + 64 * u + lane - (int64_t)ws.wave_count[sl]) : 0;\n"
+           "      const int64_t pre = (int64_t)__builtin_nontemporal_load(ws.wave_count + sl);\n"
+           "      const int64_t jl = live[u] ? j0 + 64 * u + lane : 0;\n"
+           "      sp[u] = sl * 64 * RS + (jl - pre);\n"
            "    }\n";
     }
     // raw values: staged predicate columns, the rest at the source rows
     for (int c : side) o << "    " << uint_of_width(in.in_width[c]) << " r" << c << "[GR];\n";
     for (int c : own) o << "    " << uint_of_width(in.in_width[c]) << " r" << c << "[GR];\n";
     o << "    #pragma unroll\n    for (int u = 0; u < GR; u++) {\n";
-    for (int c : side) o << "      r" << c << "[u] = live[u] ? __builtin_nontemporal_load(s" << c << " + sp[u]) : 0;\n";
-    for (int c : own) o << "      r" << c << "[u] = live[u] ? __builtin_nontemporal_load(c" << c << " + row[u]) : 0;\n";
+    // unconditional loads (sp and row are 0, in range, in the lanes past the last
+    // output, whose values are never stored)
+    for (int c : side) o << "      r" << c << "[u] = __builtin_nontemporal_load(s" << c << " + sp[u]);\n";
+    for (int c : own) o << "      r" << c << "[u] = __builtin_nontemporal_load(c" << c << " + row[u]);\n";
     o << "    }\n";
     for (int c : bm_q) o << "    bool vb" << c << "[GR];\n";
     if (!bm_q.empty()) {
       o << "    #pragma unroll\n    for (int u = 0; u < GR; u++) {\n";
-      for (int c : bm_q) o << "      vb" << c << "[u] = valid_bit(m" << c << ", row[u]);\n";
+      // the bitmap exists (the source is written for a frame whose column has one)
+      for (int c : bm_q)
+        o << "      vb" << c << "[u] = ((((const GLOBAL uint8_t *)m" << c << ")[row[u] >> 3] >> (row[u] & 7)) & 1) != 0;\n";
       o << "    }\n";
     }
     o << "    #pragma unroll\n    for (int u = 0; u < GR; u++) {\n"
@@ -527,12 +545,25 @@ CodegenOutput generate_kernel_source(const CodegenInput &in) {
     }
     o << "  }\n";
   }
-  // null counts: one atomic per wave and output
-  for (int k = 0; k < n_out; k++) {
-    if (!in.out_valid[k]) continue;
-    o << "  { unsigned long long t = nn" << k << ";\n"
-      << "    for (int s = 32; s > 0; s >>= 1) t += __shfl_xor(t, s);\n"
-      << "    if (lane == 0 && t) atomicAdd(ws.null_counts + " << k << ", t); }\n";
+  // null counts: summed over the workgroup's four waves in LDS, then one atomic per
+  // workgroup and output (atomics on one address serialize, and one per wave over
+  // a large grid cost more than the rest of the kernel)
+  std::vector<int> nk;
+  for (int k = 0; k < n_out; k++)
+    if (in.out_valid[k]) nk.push_back(k);
+  res.null_outputs = (int)nk.size();
+  if (!nk.empty()) {
+    o << "  __shared__ unsigned long long s_nn[4][" << nk.size() << "];\n";
+    for (size_t i = 0; i < nk.size(); i++)
+      o << "  { unsigned long long t = nn" << nk[i] << ";\n"
+        << "    for (int s = 32; s > 0; s >>= 1) t += __shfl_xor(t, s);\n"
+        << "    if (lane == 0) s_nn[threadIdx.x >> 6][" << i << "] = t; }\n";
+    o << "  __syncthreads();\n";
+    for (size_t i = 0; i < nk.size(); i++)
+      o << "  if (threadIdx.x == " << i << ") {\n"
+        << "    const unsigned long long t = s_nn[0][" << i << "] + s_nn[1][" << i << "] + s_nn[2][" << i
+        << "] + s_nn[3][" << i << "];\n"
+        << "    if (t) atomicAdd(ws.null_counts + " << nk[i] << ", t);\n  }\n";
   }
   o << "}\n";
   res.src = o.str();

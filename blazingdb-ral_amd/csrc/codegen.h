@@ -55,6 +55,7 @@ struct CodegenOutput {
   int rs = 8;
   int tile_rows = 0;  // filter kernel: rows per tile (256 * rs); projection: 64 * rs per wave step
   int side_bytes = 0;  // filter: bytes per row of the staged predicate columns
+  int null_outputs = 0;  // outputs with validity (a null-count atomic per workgroup each)
 };
 
 // Throws std::runtime_error when the plan has a shape the generator does not take.

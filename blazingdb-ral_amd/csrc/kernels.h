@@ -63,10 +63,18 @@ hipError_t launch_pred_direct(const VmBatch *bd, int n_terms, int wmask, int gri
 // Exclusive prefix of every program's slot counts (ws.wave_count, written over the
 // counts; total to
 // ws.n_sel) and, from it and the keep words, the row of every kept row in output
-// order (ws.idx): chunks of kScanChunk slots, combined within a program by a single-pass
-// decoupled look-back over its ws.tile_state (zero before the scan).
+// order (ws.idx): chunks of scan_chunk_slots(tile rows) slots, combined within a
+// program by a single-pass decoupled look-back over its ws.tile_state (zero before
+// the scan).
 hipError_t launch_slot_scan(const VmBatch *bd, int64_t n_chunks, hipStream_t st);
-constexpr int64_t kScanChunk = 512;  // slots per scan chunk
+constexpr int64_t kScanChunk = 512;  // most slots per scan chunk
+// Slots per scan chunk of a program with `tile_rows`-row tiles (four slots per
+// tile, tile_rows / 256 keep words per slot): 2048 keep words (131072 rows) per
+// chunk, at most kScanChunk slots, so a program's scan has as many workgroups
+// whatever its slot size.
+constexpr int64_t scan_chunk_slots(int64_t tile_rows) {
+  return tile_rows / 256 <= 4 ? kScanChunk : 2048 / (tile_rows / 256);
+}
 // Seeded synthetic column in place (dist: 0 int uniform [a,b], 1 real uniform [a,b),
 // 2 normal(a,b), 3 lognormal(a,b), 4 k/100 for k in [a,b], 5 a + row).
 hipError_t launch_synth(void *data, uint8_t *valid, int64_t n, int dtype, uint64_t seed, int dist, double a,

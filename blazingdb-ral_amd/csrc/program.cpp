@@ -1558,6 +1558,7 @@ Compiled compile_program(const CompileRequest &req) {
     out.jit_src = g.src;
     out.jit_name = g.name;
     out.jit_side_bytes = g.side_bytes;
+    out.jit_null_outputs = g.null_outputs;
     out.R = g.rs;
     out.T = req.compact ? g.tile_rows : 0;
     if (req.compact) {  // the slot scan reads the slot shape from the header

@@ -79,6 +79,7 @@ struct Compiled {
   bool jit = false;
   std::string jit_src, jit_name;  // projection kernel, or a filter's predicate kernel
   int jit_side_bytes = 0;         // filter: bytes per row of staged predicate columns
+  int jit_null_outputs = 0;       // outputs with validity (null-count atomics per workgroup)
 };
 
 // Throws std::runtime_error with a message on an unsupported or malformed plan.

@@ -21,7 +21,7 @@ BLZ_VM_SHAPES(BLZ_VM_DECL)
 // Exclusive prefix of every program's per-wave-slot kept counts (VM_PRED) in one
 // pass (written over the counts), expanded into the row index of every kept row in
 // output order: each
-// workgroup scans a chunk of 512 counts of one program and combines its total
+// workgroup scans a chunk of counts of one program (scan_chunk_slots) and combines its total
 // with the chunks before it by a single-pass decoupled look-back over that
 // program's state words. Chunks are taken in the order workgroups start, so
 // every chunk a look-back waits for belongs to a workgroup that is running.
@@ -45,13 +45,15 @@ slot_scan_kernel(const VmBatch *__restrict__ bd_arg) {
   const CONSTANT ProgHeader &h = *(const CONSTANT ProgHeader *)plans[p].blob;
   const int64_t n_slots = ((h.n_rows + h.tile_rows - 1) / h.tile_rows) * kWaves;
   const int64_t b = gch - plans[p].chunk0;                 // chunk within the program
-  const int64_t n_blocks = (n_slots + kScanChunk - 1) / kScanChunk;
-  const int64_t base = b * kScanChunk + (int64_t)threadIdx.x * kScanItems;
+  const int64_t cs = scan_chunk_slots(h.tile_rows);        // slots per chunk
+  const int64_t n_blocks = (n_slots + cs - 1) / cs;
+  const int64_t base = b * cs + (int64_t)threadIdx.x * kScanItems;
+  const int64_t lim = n_slots < (b + 1) * cs ? n_slots : (b + 1) * cs;  // the chunk's end
   uint32_t v[kScanItems];
   uint64_t sum = 0;
 #pragma unroll
   for (int i = 0; i < kScanItems; i++) {
-    v[i] = base + i < n_slots ? ws.wave_count[base + i] : 0u;
+    v[i] = base + i < lim ? ws.wave_count[base + i] : 0u;
     sum += v[i];
   }
   uint64_t inc = sum;
@@ -87,14 +89,14 @@ slot_scan_kernel(const VmBatch *__restrict__ bd_arg) {
     s_pre[threadIdx.x * kScanItems + i] = (uint32_t)run;
     // each slot's count is replaced by its exclusive prefix (the compiled gather
     // finds a kept row's place in its slot's staged values by it)
-    if (base + i < n_slots) ws.wave_count[base + i] = (uint32_t)run;
+    if (base + i < lim) ws.wave_count[base + i] = (uint32_t)run;
     run += v[i];
   }
   __syncthreads();
   const int wps = h.tile_rows / 256;  // keep words per slot (a power of two <= 64)
-  const int64_t ns = n_slots - b * kScanChunk < kScanChunk ? n_slots - b * kScanChunk : kScanChunk;
+  const int64_t ns = lim - b * cs;          // slots of the chunk
   const int64_t nw = ns * wps;              // keep words of the chunk
-  const int64_t w0 = b * kScanChunk * wps;  // the chunk's first keep word in the program
+  const int64_t w0 = b * cs * wps;          // the chunk's first keep word in the program
   for (int64_t g0 = (int64_t)wave * 64; g0 < nw; g0 += kBlock) {
     const int64_t k = g0 + lane;
     const uint64_t word = k < nw ? ws.keep[w0 + k] : 0ull;
