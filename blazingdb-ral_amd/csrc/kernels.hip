@@ -132,6 +132,10 @@ interp_kernel(const uint8_t *__restrict__ blob_arg, Workspace ws) {
   for (int i = 0; i < S * R; i++) rf[i] = 0;
 #pragma unroll
   for (int i = 0; i < S; i++) vrf[i] = 0;
+  // lane o: nulls of output o (< 64) counted by this wave over its tiles, added once
+  // at the end (atomics on one address serialize: one per tile would cost more than
+  // the tile)
+  unsigned long long nacc = 0;
 
   int64_t tile = blockIdx.x;
   if (COMPACT) {
@@ -224,7 +228,8 @@ interp_kernel(const uint8_t *__restrict__ blob_arg, Workspace ws) {
             nulls = __builtin_popcountll(~word & low_mask(n - w * 64));
           }
           for (int off = 32; off > 0; off >>= 1) nulls += __shfl_xor(nulls, off);
-          if (lane == 0 && nulls) atomicAdd(ws.null_counts + o, (unsigned long long)nulls);
+          if (o < 64) nacc += lane == o ? nulls : 0;
+          else if (lane == 0 && nulls) atomicAdd(ws.null_counts + o, (unsigned long long)nulls);
         }
       }
       tile += gridDim.x;
@@ -362,7 +367,8 @@ interp_kernel(const uint8_t *__restrict__ blob_arg, Workspace ws) {
             else gstore<uint64_t>(gw, 0, s_vbits[i]);
           }
           for (int off = 32; off > 0; off >>= 1) nulls += __shfl_xor(nulls, off);
-          if (lane == 0 && nulls) atomicAdd(ws.null_counts + o, (unsigned long long)nulls);
+          if (o < 64) nacc += lane == o ? nulls : 0;
+          else if (lane == 0 && nulls) atomicAdd(ws.null_counts + o, (unsigned long long)nulls);
           __syncthreads();
         }
       }
@@ -372,6 +378,21 @@ interp_kernel(const uint8_t *__restrict__ blob_arg, Workspace ws) {
     if (threadIdx.x == 0) s_tile = (int64_t)atomicAdd(ws.tile_counter, 1u);
     __syncthreads();
     tile = s_tile;
+  }
+  if (nacc) atomicAdd(ws.null_counts + lane, nacc);
+}
+
+// Adds each wave's `v` (lane 0's) to *dst, summed over the workgroup first: one
+// atomic per workgroup (atomics on one address serialize). Every thread of the
+// workgroup calls it.
+DEV void block_add(unsigned long long *dst, unsigned long long v) {
+  __shared__ unsigned long long s_part[kWaves];
+  if ((threadIdx.x & 63) == 0) s_part[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long t = 0;
+    for (int w = 0; w < kWaves; w++) t += s_part[w];
+    if (t) atomicAdd(dst, t);
   }
 }
 
@@ -419,7 +440,7 @@ gather_kernel(const T *__restrict__ in, const uint8_t *__restrict__ in_valid, in
       }
     }
   }
-  if (out_valid != nullptr && lane == 0 && nulls) atomicAdd(null_count, (unsigned long long)nulls);
+  if (out_valid != nullptr) block_add(null_count, lane == 0 ? nulls : 0);  // out_valid: uniform
 }
 
 // idx[i] = init + i * step (gdf_sequence, Utils.cu:4-12)
@@ -518,7 +539,7 @@ null_count_kernel(const uint8_t *valid, int64_t n, unsigned long long *nulls_out
     nulls += __builtin_popcountll(~word & low_mask(n - w * 64));
   }
   for (int off = 32; off > 0; off >>= 1) nulls += __shfl_xor(nulls, off);
-  if ((threadIdx.x & 63) == 0 && nulls) atomicAdd(nulls_out, (unsigned long long)nulls);
+  block_add(nulls_out, nulls);
 }
 
 // ---------------------------------------------------------------------------
