@@ -214,12 +214,13 @@ def run_batch(B, n_plans, rows_lo, rows_hi, seed=404):
             "speedup_vs_prepared": float(done[-1] / batch_ms)}
 
 
-def run_serve(B, total_gb, rates, seconds, seed=505, r=4):
+def run_serve(B, total_gb, rates, seconds, seed=505, r=4, r_served=8):
     """Served query loop over device-resident tables (wide-expression column mix,
     5% nulls) totalling total_gb: Poisson arrivals at each rate, filter + project
     queries of 4-40 ops, admitted by a 200 GB HBM budget and continuously batched
-    over 4 streams; against per-query execution (one query per launch, one
-    stream). p50 / p99 latency from submission to completion."""
+    over 4 streams with 8 rows per lane; against per-query execution (one query
+    per launch, one stream, 4 rows per lane: each mode at the shape it runs best
+    at). p50 / p99 latency from submission to completion."""
     from blz_ral import workloads as W
     rng = np.random.default_rng(seed)
     mix = [B.GDF_INT32, B.GDF_INT32, B.GDF_INT64, B.GDF_INT64, B.GDF_FLOAT32, B.GDF_FLOAT32,
@@ -240,8 +241,8 @@ def run_serve(B, total_gb, rates, seconds, seed=505, r=4):
     for i in range(len(tables)):
         f, p = W.rand_query(rng, specs[i], True)
         warm.append((tables[i], f, p))
-    for mb, ns in ((64, 4), (1, 1)):
-        srv = B.Server(200 << 30, max_batch=mb, n_streams=ns, r=r)
+    for mb, ns, mr in ((64, 4, r_served), (1, 1, r)):
+        srv = B.Server(200 << 30, max_batch=mb, n_streams=ns, r=mr)
         srv.replay(warm, np.zeros(len(warm)))
         srv.close()
     for qps in rates:
@@ -253,8 +254,8 @@ def run_serve(B, total_gb, rates, seconds, seed=505, r=4):
             f, p = W.rand_query(rng, specs[i], True)
             qs.append((tables[i], f, p))
         row = {}
-        for mode, mb, ns in (("served", 64, 4), ("per_query", 1, 1)):
-            srv = B.Server(200 << 30, max_batch=mb, n_streams=ns, r=r)
+        for mode, mb, ns, mr in (("served", 64, 4, r_served), ("per_query", 1, 1, r)):
+            srv = B.Server(200 << 30, max_batch=mb, n_streams=ns, r=mr)
             # untimed: the first queries of this rate's schedule (the pool settles to
             # this mix's allocation sizes)
             k = min(n, 256)
