@@ -110,3 +110,25 @@ def test_many_sources_compile_as_one_program(B):
     assert B.jit_counters()[0] == c1 + 3
     with pytest.raises(B.BlzError):
         B.jit_compile(broken)
+
+
+def test_generated_loads_are_unconditional_and_null_counts_per_workgroup(B):
+    """The generated gather loads every lane's index, staged value, source row and
+    bitmap byte without a condition of its own (a conditional load compiles to a
+    branch that waits for it alone), and the kernels add their null counts once per
+    workgroup (one atomic per wave serializes on the count words)."""
+    import re
+    dts = [P.GDF_INT32, P.GDF_INT32, P.GDF_INT64, P.GDF_INT64, P.GDF_FLOAT32, P.GDF_FLOAT32, P.GDF_FLOAT64,
+           P.GDF_FLOAT64]
+    filt = "LogicalFilter(condition=[<($4, 39.26)])"
+    proj = "LogicalProject(E0=[*(CASE(IS NOT NULL($6), $6, $2), 48)], E1=[AND(NOT($1), $5)], E2=[+($4, 1)])"
+    src = B.program_source(dts, 1 << 20, filt, proj, has_valid=[True] * 8, null_counts=[100] * 8)
+    gather = src[src.index("blz_cg_gather"):]
+    assert not re.search(r"\?\s*\(?(int64_t\)\s*)?__builtin_nontemporal_load", gather)
+    assert "valid_bit(" not in gather
+    assert gather.count("atomicAdd(ws.null_counts") == 3            # one per nullable output ...
+    assert "s_nn[threadIdx.x >> 6]" in gather and "__syncthreads()" in gather   # ... per workgroup
+    proj_src = B.program_source(dts, 1 << 20, None, proj, has_valid=[True] * 8, null_counts=[100] * 8)
+    assert "s_nn[threadIdx.x >> 6]" in proj_src
+    B.jit_compile(src)
+    B.jit_compile(proj_src)
