@@ -238,9 +238,11 @@ int program_stages(const Compiled &c, bool compact) { return (c.vm || c.jit) && 
 // recorded before each kernel and after the last.
 // sel_hint: rows a compiled filter kept the last time it ran (-1 unknown) — it sizes
 // the gather's grid, never its result.
+// stages (a tile-VM filter only): bit 0 the predicate, bit 1 the slot scan, bit 2
+// the gather, so a caller can read the kept count between the scan and the gather.
 hipError_t launch_program(const Compiled &c, bool compact, int grid, hipStream_t st, const uint8_t *blob,
                           const Workspace &ws, const VmBatch *bd, int64_t n_chunks, hipEvent_t *evs = nullptr,
-                          int64_t sel_hint = -1) {
+                          int64_t sel_hint = -1, int stages = 7) {
   auto mark = [&](int i) { return evs ? hipEventRecord(evs[i], st) : hipSuccess; };
   hipError_t e = mark(0);
   if (e != hipSuccess) return e;
@@ -301,18 +303,20 @@ hipError_t launch_program(const Compiled &c, bool compact, int grid, hipStream_t
       e = launch_vm(c.R, 0, c.vm_slow, grid_for(0, c.lds_bytes, tiles), c.lds_bytes, st, bd);
       return e == hipSuccess ? mark(1) : e;
     }
-    if (c.vm_direct_pred)  // 32 workgroups per CU: more than stay resident, so CUs that
-                           // finish early take queued ones (100M rows of three columns,
-                           // non-temporal loads: 0.264 / 0.259 / 0.262 / 0.267 / 0.274 ms
-                           // at 16 / 32 / 64 / 8 / 4 per CU)
+    if (!(stages & 1)) {
+    } else if (c.vm_direct_pred) {  // 32 workgroups per CU: more than stay resident, so
+                                    // CUs that finish early take queued ones (100M rows of
+                                    // three columns, non-temporal loads: 0.264 / 0.259 /
+                                    // 0.262 / 0.267 / 0.274 ms at 16 / 32 / 64 / 8 / 4 per CU)
       e = launch_pred_direct(bd, c.vm_direct_terms, c.vm_direct_wmask,
                              (int)std::max<int64_t>(1, std::min<int64_t>(tiles, (int64_t)device_cus() * 32)), st);
-    else
+    } else {
       e = launch_vm(c.R, 1, c.vm_slow, grid_for(1, c.lds_bytes, tiles), c.lds_bytes, st, bd);
+    }
     if (e == hipSuccess) e = mark(1);
-    if (e == hipSuccess) e = launch_slot_scan(bd, n_chunks, st);
+    if (e == hipSuccess && (stages & 2)) e = launch_slot_scan(bd, n_chunks, st);
     if (e == hipSuccess) e = mark(2);
-    if (e == hipSuccess)
+    if (e == hipSuccess && (stages & 4))
       e = launch_vm(c.R, 2, c.vm_slow, grid_for(2, c.lds_gather, (chunks + 3) / 4), c.lds_gather, st, bd);
     return e == hipSuccess ? mark(3) : e;
   }
@@ -354,6 +358,57 @@ RunResult run_once(const Compiled &c, bool compact, int64_t n, int n_out, hipStr
   HIPCHECK(s);
   for (int o = 0; o < n_out; o++) r.nulls[o] = (int32_t)host[o];
   r.n_sel = compact ? (int64_t)host[n_out] : n;
+  return r;
+}
+
+// One-shot tile-VM filter whose outputs are allocated after its slot scan (blocking):
+// `c` was compiled from `req` with stand-in output pointers; allocate(rows) fills
+// req.outputs with columns of `rows` rows (at least one, so every bitmap exists).
+template <typename Alloc>
+RunResult run_filter_sized(const Compiled &c, CompileRequest &req, Alloc allocate, int64_t n, int n_out,
+                           hipStream_t st) {
+  RunResult r;
+  r.nulls.assign(n_out, 0);
+  LaunchBuffers lb;
+  layout(lb, c, n_out, true, n);
+  lb.base = (uint8_t *)pool_alloc(lb.bytes, st);
+  bind(lb, n_out);
+  std::vector<unsigned long long> host(n_out + 1, 0);
+  hipError_t e = hipSuccess;
+  try {
+    const std::vector<uint8_t> img = device_image(lb, c);
+    e = hipMemcpyAsync(lb.blob, img.data(), img.size(), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess)
+      e = launch_fill_zero(lb.base + lb.zero_off, (int64_t)lb.zero_bytes,
+                           (int)std::min<size_t>(1024, lb.zero_bytes / 2048 + 1), st);
+    if (e == hipSuccess)
+      e = launch_program(c, true, interp_grid(c, true, n), st, lb.blob, lb.ws, lb.bd, lb.n_chunks, nullptr, -1, 3);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(host.data() + n_out, lb.ws.n_sel, sizeof(unsigned long long), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    HIPCHECK(e);
+    r.n_sel = (int64_t)host[n_out];
+    allocate(std::max<int64_t>(r.n_sel, 1));
+    const Compiled c2 = compile_program(req);
+    if (c2.blob.size() != c.blob.size() || !c2.vm || c2.R != c.R || c2.T != c.T)
+      throw std::runtime_error("internal: the filter's program changed with its outputs");
+    const std::vector<uint8_t> img2 = device_image(lb, c2);
+    e = hipMemcpyAsync(lb.blob, img2.data(), img2.size(), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess)
+      e = launch_program(c2, true, interp_grid(c2, true, n), st, lb.blob, lb.ws, lb.bd, lb.n_chunks, nullptr, -1, 4);
+    if (e == hipSuccess && n_out > 0)
+      e = hipMemcpyAsync(host.data(), lb.ws.null_counts, sizeof(unsigned long long) * n_out,
+                         hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    HIPCHECK(e);
+  } catch (...) {
+    (void)hipStreamSynchronize(st);
+    (void)hipFreeAsync(lb.base, st);
+    throw;
+  }
+  hipError_t f = hipFreeAsync(lb.base, st);
+  HIPCHECK(f);
+  for (int o = 0; o < n_out; o++) r.nulls[o] = (int32_t)host[o];
   return r;
 }
 
@@ -636,14 +691,22 @@ void filter_project_impl(gdf_column *const *frame, int n_frame, const char *filt
   req.scalar_mode = scalar_mode;
   req.compact = has_filter;
   int64_t kept = n;
+  uint8_t *placeholder = nullptr;
   try {
-    for (size_t i = 0; i < outs.size(); i++) {
-      const auto &o = outs[i];
-      const bool with_valid = o.kind == 0 ? true : frame[o.index]->valid != nullptr;
-      res[i] = alloc_column(o.dtype, n, with_valid, st);
-      if (o.kind == 0) req.outputs.push_back({res[i].data, res[i].valid, o.dtype, 0, o.index});
-      else req.outputs.push_back({res[i].data, res[i].valid, o.dtype, 1, im.use(frame, o.index)});
-    }
+    // Output columns of `rows` rows, their pointers into req.outputs.
+    auto allocate_outputs = [&](int64_t rows) {
+      req.outputs.clear();
+      for (size_t i = 0; i < outs.size(); i++) {
+        const auto &o = outs[i];
+        const bool with_valid = o.kind == 0 ? true : frame[o.index]->valid != nullptr;
+        res[i] = alloc_column(o.dtype, rows, with_valid, st);
+        if (o.kind == 0) req.outputs.push_back({res[i].data, res[i].valid, o.dtype, 0, o.index});
+        else req.outputs.push_back({res[i].data, res[i].valid, o.dtype, 1, im.use(frame, o.index)});
+      }
+    };
+    // the columns bare outputs copy are inputs too (use() keeps one index per column)
+    for (const auto &o : outs)
+      if (o.kind != 0) (void)im.use(frame, o.index);
     req.inputs = im.inputs;
     if (has_filter) {
       if (fp.bare) req.bare_stencil_input = im.frame_to_dev[fp.bare_col];
@@ -652,13 +715,45 @@ void filter_project_impl(gdf_column *const *frame, int n_frame, const char *filt
     if (proj_qp && pp.has_eval) req.proj = &pp.spec;
     std::vector<int32_t> nulls(outs.size(), 0);
     if (n > 0 && !req.inputs.empty()) {
-      Compiled c = compile_program(req);
-      RunResult r = run_once(c, has_filter, n, (int)outs.size(), st);
-      kept = r.n_sel;
-      nulls = r.nulls;
+      // A filter the tile VM runs in three launches gets outputs of the rows it keeps,
+      // as the reference materializes the kept rows (CalciteInterpreter.cpp:543-560):
+      // the program is compiled with stand-in output pointers (the predicate and the
+      // slot scan never touch the outputs), the kept count is read after the scan,
+      // the outputs are allocated to it, and the program compiled again with them
+      // (the same plan: the same layout) runs the gather.
+      Compiled c;
+      bool sized = false;
+      if (has_filter) {
+        placeholder = (uint8_t *)pool_alloc(256, st);
+        for (size_t i = 0; i < outs.size(); i++) {
+          const auto &o = outs[i];
+          const bool with_valid = o.kind == 0 ? true : frame[o.index]->valid != nullptr;
+          uint8_t *v = with_valid ? placeholder : nullptr;
+          if (o.kind == 0) req.outputs.push_back({placeholder, v, o.dtype, 0, o.index});
+          else req.outputs.push_back({placeholder, v, o.dtype, 1, im.use(frame, o.index)});
+        }
+        c = compile_program(req);
+        sized = c.vm;
+      }
+      if (sized) {
+        RunResult r = run_filter_sized(c, req, allocate_outputs, n, (int)outs.size(), st);
+        kept = r.n_sel;
+        nulls = r.nulls;
+      } else {
+        allocate_outputs(n);
+        c = compile_program(req);
+        RunResult r = run_once(c, has_filter, n, (int)outs.size(), st);
+        kept = r.n_sel;
+        nulls = r.nulls;
+      }
     } else {
+      allocate_outputs(n);
       HIPCHECK(hipStreamSynchronize(st));
       if (n > 0 && req.inputs.empty()) invalid("nothing to evaluate");
+    }
+    if (placeholder) {
+      (void)hipFreeAsync(placeholder, st);
+      placeholder = nullptr;
     }
     for (size_t i = 0; i < outs.size(); i++) {
       res[i].size = (gdf_size_type)kept;
@@ -667,6 +762,8 @@ void filter_project_impl(gdf_column *const *frame, int n_frame, const char *filt
       if (is_alias) is_alias[i] = 0;
     }
   } catch (...) {
+    (void)hipStreamSynchronize(st);  // the stream may still use what is freed here
+    if (placeholder) (void)hipFree(placeholder);
     for (auto &c : res) { (void)hipFree(c.data); (void)hipFree(c.valid); }
     throw;
   }
