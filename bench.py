@@ -327,6 +327,25 @@ def cpu_baseline(sample_rows=10_000_000, min_seconds=10.0, max_threads=16):
             "single_thread_value": gbs(t1), "cpu": cpu_model(), "nproc": os.cpu_count()}
 
 
+def run_drop_in_call(B, frame, n, iters=20):
+    """The same filter + project through the drop-in one-shot call (blz_filter_project:
+    plan, compile, allocate the kept rows' outputs, run the tile VM, wait, as the
+    reference's process_filter + execute_project_plan return): wall milliseconds per
+    call, outputs freed after each."""
+    for _ in range(3):
+        outs, _ = B.filter_project(frame, FILTER, PROJECT)
+        for c in outs:
+            c.free()
+    t0 = time.perf_counter()
+    for _ in range(iters):
+        outs, _ = B.filter_project(frame, FILTER, PROJECT)
+        for c in outs:
+            c.free()
+    ms = (time.perf_counter() - t0) / iters * 1e3
+    return {"ms": ms, "GBps": algorithmic_bytes(n, int(outs[0].col.size)) / (ms * 1e-3) / 1e9,
+            "path": "blz_filter_project one-shot (plan, compile, allocate the kept rows, run the tile VM, wait)"}
+
+
 def run_aggregate(B, frame, prog, kept, n, iters=20):
     """Aggregation without GROUP BY through the drop-in call (process_aggregate: parse,
     reduce, one-row results, blocking): SUM over the 100M-row float64 column b (one
@@ -711,6 +730,7 @@ def main():
         print("[bench] calibration copy: 3 x 1 GiB", file=sys.stderr, flush=True)
     aggregate = run_aggregate(B, frame, prog, kept, n)
     prog.close()
+    drop_in = run_drop_in_call(B, frame, n)
     for c in frame:
         c.free()
     gather = run_gather(B)
@@ -744,6 +764,7 @@ def main():
                      "achieved": bytes_step / (total_ms * 1e-3) / 1e9,
                      "frac": bytes_step / (total_ms * 1e-3) / 1e9 / HBM_PEAK_GBS},
         "cpu_baseline": cpu,
+        "drop_in_call": drop_in,
         "aggregate": aggregate,
         "gather": gather,
         "batch": batch,
