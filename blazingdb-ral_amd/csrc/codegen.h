@@ -7,7 +7,7 @@
  * column read once from HBM into registers, every SSA value a local, every
  * validity word a scalar — and compiles it at run time for gfx950 (jit.cpp). The
  * generated kernel keeps the interpreter's semantics exactly: it evaluates the
- * same SSA graph, op by op, through op_eval (device_common.h), and the same
+ * same SSA graph, op by op, through op_t<code> (device_common.h), and the same
  * validity program over 64-row words.
  *
  * Two kernel shapes:

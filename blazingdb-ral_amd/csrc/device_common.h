@@ -312,8 +312,7 @@ DEV void exec_op(int code, const uint64_t (&A)[R], const uint64_t (&B)[R], const
 
 // One fully typed op for one row, its code a template argument: the run-time
 // compiled kernels instantiate the ops they use, each folded to that op's
-// expression (the same expressions as exec_op and slow_op); op_eval dispatches a
-// run-time code to them.
+// expression (the same expressions as exec_op and slow_op).
 template <int C>
 DEV uint64_t op_t(uint64_t a, uint64_t b, bool vb) {
   const double x = b2d(a), y = b2d(b);
@@ -370,62 +369,6 @@ DEV uint64_t op_t(uint64_t a, uint64_t b, bool vb) {
   else if constexpr (C == OP_IS_NULL) return (uint64_t)(!vb);
   else if constexpr (C == OP_IS_NOT_NULL) return (uint64_t)vb;
   else return a;  // OP_IDENT
-}
-DEV uint64_t op_eval(int code, uint64_t a, uint64_t b, bool vb) {
-  switch (code) {
-    case OP_ADD_I: return op_t<OP_ADD_I>(a, b, vb);
-    case OP_ADD_F: return op_t<OP_ADD_F>(a, b, vb);
-    case OP_SUB_I: return op_t<OP_SUB_I>(a, b, vb);
-    case OP_SUB_F: return op_t<OP_SUB_F>(a, b, vb);
-    case OP_MUL_I: return op_t<OP_MUL_I>(a, b, vb);
-    case OP_MUL_F: return op_t<OP_MUL_F>(a, b, vb);
-    case OP_DIV_I: return op_t<OP_DIV_I>(a, b, vb);
-    case OP_DIV_F: return op_t<OP_DIV_F>(a, b, vb);
-    case OP_MOD_I: return op_t<OP_MOD_I>(a, b, vb);
-    case OP_POW_I: return op_t<OP_POW_I>(a, b, vb);
-    case OP_POW_F: return op_t<OP_POW_F>(a, b, vb);
-    case OP_EQ_I: return op_t<OP_EQ_I>(a, b, vb);
-    case OP_EQ_F: return op_t<OP_EQ_F>(a, b, vb);
-    case OP_NE_I: return op_t<OP_NE_I>(a, b, vb);
-    case OP_NE_F: return op_t<OP_NE_F>(a, b, vb);
-    case OP_LT_I: return op_t<OP_LT_I>(a, b, vb);
-    case OP_LT_F: return op_t<OP_LT_F>(a, b, vb);
-    case OP_GT_I: return op_t<OP_GT_I>(a, b, vb);
-    case OP_GT_F: return op_t<OP_GT_F>(a, b, vb);
-    case OP_LE_I: return op_t<OP_LE_I>(a, b, vb);
-    case OP_LE_F: return op_t<OP_LE_F>(a, b, vb);
-    case OP_GE_I: return op_t<OP_GE_I>(a, b, vb);
-    case OP_GE_F: return op_t<OP_GE_F>(a, b, vb);
-    case OP_COALESCE: return op_t<OP_COALESCE>(a, b, vb);
-    case OP_AND_B: return op_t<OP_AND_B>(a, b, vb);
-    case OP_FLOOR: return op_t<OP_FLOOR>(a, b, vb);
-    case OP_CEIL: return op_t<OP_CEIL>(a, b, vb);
-    case OP_SIN: return op_t<OP_SIN>(a, b, vb);
-    case OP_COS: return op_t<OP_COS>(a, b, vb);
-    case OP_ASIN: return op_t<OP_ASIN>(a, b, vb);
-    case OP_ACOS: return op_t<OP_ACOS>(a, b, vb);
-    case OP_TAN: return op_t<OP_TAN>(a, b, vb);
-    case OP_COTAN: return op_t<OP_COTAN>(a, b, vb);
-    case OP_ATAN: return op_t<OP_ATAN>(a, b, vb);
-    case OP_ABS: return op_t<OP_ABS>(a, b, vb);
-    case OP_LN: return op_t<OP_LN>(a, b, vb);
-    case OP_LOG10: return op_t<OP_LOG10>(a, b, vb);
-    case OP_NOT_I: return op_t<OP_NOT_I>(a, b, vb);
-    case OP_NOT_F: return op_t<OP_NOT_F>(a, b, vb);
-    case OP_YEAR32: return op_t<OP_YEAR32>(a, b, vb);
-    case OP_MONTH32: return op_t<OP_MONTH32>(a, b, vb);
-    case OP_DAY32: return op_t<OP_DAY32>(a, b, vb);
-    case OP_YEAR64: return op_t<OP_YEAR64>(a, b, vb);
-    case OP_MONTH64: return op_t<OP_MONTH64>(a, b, vb);
-    case OP_DAY64: return op_t<OP_DAY64>(a, b, vb);
-    case OP_HOUR64: return op_t<OP_HOUR64>(a, b, vb);
-    case OP_MINUTE64: return op_t<OP_MINUTE64>(a, b, vb);
-    case OP_SECOND64: return op_t<OP_SECOND64>(a, b, vb);
-    case OP_ZERO: return op_t<OP_ZERO>(a, b, vb);
-    case OP_IS_NULL: return op_t<OP_IS_NULL>(a, b, vb);
-    case OP_IS_NOT_NULL: return op_t<OP_IS_NOT_NULL>(a, b, vb);
-    default: return a;  // OP_IDENT
-  }
 }
 
 // Validity program: one 64-row word per lane (lanes [0, R) of each wave).
