@@ -107,6 +107,75 @@ void *pool_alloc(size_t bytes, hipStream_t st) {
   return p;
 }
 
+// Launch buffers of the blocking one-shot calls, kept per (device, stream) and grown
+// on demand. The stream-ordered pool hands freed memory back at every
+// synchronisation, so a 100M-row filter mapped its ~400 MB of kept-row index space
+// afresh on every call (0.38 ms of a 1.04 ms call). A lease holds its slot for the
+// whole call (calls on one stream from several threads take turns); a buffer larger
+// than kLbCacheMax is allocated for the call alone.
+constexpr size_t kLbCacheMax = (size_t)4 << 30;
+struct LbSlot {
+  uint8_t *p = nullptr;
+  size_t bytes = 0;
+  std::mutex busy;
+};
+std::mutex g_lb_mu;
+std::map<std::pair<int, hipStream_t>, std::unique_ptr<LbSlot>> g_lb;
+
+class LbLease {
+ public:
+  LbLease(size_t bytes, hipStream_t st) : st_(st) {
+    if (bytes > kLbCacheMax) {
+      p_ = (uint8_t *)pool_alloc(bytes, st);
+      return;
+    }
+    int dev = 0;
+    HIPCHECK(hipGetDevice(&dev));
+    {
+      std::lock_guard<std::mutex> g(g_lb_mu);
+      auto &slot = g_lb[{dev, st}];
+      if (!slot) slot.reset(new LbSlot());
+      slot_ = slot.get();
+    }
+    lock_ = std::unique_lock<std::mutex>(slot_->busy);
+    if (slot_->bytes < bytes) {  // the slot's previous call has completed (calls block)
+      if (slot_->p) HIPCHECK(hipFree(slot_->p));
+      slot_->p = nullptr;
+      slot_->bytes = 0;
+      HIPCHECK(hipMalloc((void **)&slot_->p, bytes));
+      slot_->bytes = bytes;
+    }
+    p_ = slot_->p;
+  }
+  // The call's work on st_ has completed (or the stream was synchronised) by now.
+  ~LbLease() {
+    if (!slot_ && p_) (void)hipFreeAsync(p_, st_);
+  }
+  LbLease(const LbLease &) = delete;
+  LbLease &operator=(const LbLease &) = delete;
+  uint8_t *get() const { return p_; }
+
+ private:
+  hipStream_t st_;
+  LbSlot *slot_ = nullptr;
+  uint8_t *p_ = nullptr;
+  std::unique_lock<std::mutex> lock_;
+};
+
+// Frees the cached launch buffers of `st` (on every device) when the library's
+// stream is destroyed.
+void lb_cache_drop(hipStream_t st) {
+  std::lock_guard<std::mutex> g(g_lb_mu);
+  for (auto it = g_lb.begin(); it != g_lb.end();) {
+    if (it->first.second != st) { ++it; continue; }
+    {
+      std::lock_guard<std::mutex> b(it->second->busy);
+      if (it->second->p) (void)hipFree(it->second->p);
+    }
+    it = g_lb.erase(it);
+  }
+}
+
 int64_t rows_per_tile(int r) { return (int64_t)block_threads() * r; }
 
 int64_t tile_rows(const Compiled &c) { return c.T > 0 ? c.T : rows_per_tile(c.R); }
@@ -337,7 +406,8 @@ RunResult run_once(const Compiled &c, bool compact, int64_t n, int n_out, hipStr
   const int64_t tiles = (n + tile_rows(c) - 1) / tile_rows(c);
   LaunchBuffers lb;
   layout(lb, c, n_out, compact, n);
-  lb.base = (uint8_t *)pool_alloc(lb.bytes, st);
+  LbLease lease(lb.bytes, st);
+  lb.base = lease.get();
   bind(lb, n_out);
   std::vector<unsigned long long> host(n_out + 1, 0);
   const std::vector<uint8_t> img = device_image(lb, c);
@@ -351,10 +421,8 @@ RunResult run_once(const Compiled &c, bool compact, int64_t n, int n_out, hipStr
                        hipMemcpyDeviceToHost, st);
   if (e == hipSuccess && compact)
     e = hipMemcpyAsync(host.data() + n_out, lb.ws.n_sel, sizeof(unsigned long long), hipMemcpyDeviceToHost, st);
-  hipError_t f = hipFreeAsync(lb.base, st);
   hipError_t s = hipStreamSynchronize(st);
   HIPCHECK(e);
-  HIPCHECK(f);
   HIPCHECK(s);
   for (int o = 0; o < n_out; o++) r.nulls[o] = (int32_t)host[o];
   r.n_sel = compact ? (int64_t)host[n_out] : n;
@@ -371,7 +439,8 @@ RunResult run_filter_sized(const Compiled &c, CompileRequest &req, Alloc allocat
   r.nulls.assign(n_out, 0);
   LaunchBuffers lb;
   layout(lb, c, n_out, true, n);
-  lb.base = (uint8_t *)pool_alloc(lb.bytes, st);
+  LbLease lease(lb.bytes, st);
+  lb.base = lease.get();
   bind(lb, n_out);
   std::vector<unsigned long long> host(n_out + 1, 0);
   hipError_t e = hipSuccess;
@@ -402,12 +471,9 @@ RunResult run_filter_sized(const Compiled &c, CompileRequest &req, Alloc allocat
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     HIPCHECK(e);
   } catch (...) {
-    (void)hipStreamSynchronize(st);
-    (void)hipFreeAsync(lb.base, st);
+    (void)hipStreamSynchronize(st);  // before the lease lets the buffer go
     throw;
   }
-  hipError_t f = hipFreeAsync(lb.base, st);
-  HIPCHECK(f);
   for (int o = 0; o < n_out; o++) r.nulls[o] = (int32_t)host[o];
   return r;
 }
@@ -1595,6 +1661,7 @@ int blz_stream_synchronize(blz_stream_t stream) {
   return hipStreamSynchronize(S(stream)) == hipSuccess ? BLZ_OK : BLZ_ERR_HIP;
 }
 int blz_stream_destroy(blz_stream_t stream) {
+  lb_cache_drop(S(stream));
   return hipStreamDestroy(S(stream)) == hipSuccess ? BLZ_OK : BLZ_ERR_HIP;
 }
 int blz_free_column(gdf_column *column) {

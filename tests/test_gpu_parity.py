@@ -755,3 +755,30 @@ def test_register_load_predicate(B, case, t):
             assert g.data().tobytes() == r.data.tobytes()
         for c in outs + dev:
             c.free()
+
+
+def test_one_shot_calls_reuse_and_release_the_streams_launch_buffers(B):
+    """One-shot filters keep their launch buffers per stream, growing them for a
+    larger input and reusing them for smaller ones: calls of alternating sizes on the
+    default stream and on a stream of the library's, whose buffers go with it when
+    it is destroyed, all give the oracle's result."""
+    import ctypes as C
+    st = C.c_void_p(0)
+    assert B.lib().blz_stream_create(C.byref(st)) == B.BLZ_OK
+    try:
+        for i, n in enumerate([70_001, 1_000_003, 5_000, 1_000_003, 1]):
+            host = c1_frame(n, seed=900 + i)
+            ref = O.execute_project(O.process_filter(host, C1_FILTER), C1_PROJECT)
+            dev = [to_device(B, h) for h in host]
+            for stream in (None, st):
+                print(f"CASE one_shot_lease n={n} stream={'lib' if stream else 'default'}", flush=True)
+                outs, kept = B.filter_project(dev, C1_FILTER, C1_PROJECT, stream=stream)
+                assert kept == len(ref[0])
+                for g, r in zip(outs, ref):
+                    assert_column_equal(g, r, what=f"n={n}")
+                for c in outs:
+                    c.free()
+            for c in dev:
+                c.free()
+    finally:
+        assert B.lib().blz_stream_destroy(st) == B.BLZ_OK
