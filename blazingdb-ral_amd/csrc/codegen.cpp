@@ -237,17 +237,20 @@ struct Gen {
   }
   // Stores of the outputs at row index `pos` (row u); a bare-column output copies
   // the input's raw bits (<pre>r<c>[u] when the input is in `pre_cols`).
+  // Output data is written once and not read again by the kernel: non-temporal
+  // stores (the 500M-row gather at s = 0.9: 10.1 -> 9.0-9.8 ms).
   void emit_stores(const std::string &pos, const std::string &ind, const std::string &pre,
                    const std::set<int> *pre_cols = nullptr) {
     for (size_t k = 0; k < in.out_node.size(); k++) {
+      const std::string dst = std::string(out_base ? "ob" : "o") + std::to_string(k) + " + " + pos;
       if (in.out_kind[k] == 1) {
         const int c = in.out_input[k];
         const std::string p = pre_cols && pre_cols->count(c) ? pre : "";
-        o << ind << (out_base ? "ob" : "o") << k << "[" << pos << "] = " << p << "r" << c << "[u];\n";
+        o << ind << "__builtin_nontemporal_store(" << p << "r" << c << "[u], " << dst << ");\n";
       } else {
         if (!store_type(in.out_store[k])) continue;
-        o << ind << (out_base ? "ob" : "o") << k << "[" << pos << "] = "
-          << store_value(in.out_store[k], val(in.out_node[k])) << ";\n";
+        o << ind << "__builtin_nontemporal_store(" << store_value(in.out_store[k], val(in.out_node[k])) << ", "
+          << dst << ");\n";
       }
     }
   }
