@@ -430,7 +430,7 @@ gather_kernel(const T *__restrict__ in, const uint8_t *__restrict__ in_valid, in
       const int64_t w = w0 + u;
       if (w >= n_words) break;  // wave-uniform
       const int64_t i = w * 64 + lane;
-      if (ok[u]) out[i] = v[u];
+      if (ok[u]) __builtin_nontemporal_store(v[u], out + i);  // written once
       if (out_valid != nullptr) {
         const uint64_t word = __builtin_amdgcn_ballot_w64(ok[u] && valid_bit(in_valid, j[u]));
         if (lane == 0) {
