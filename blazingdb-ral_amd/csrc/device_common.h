@@ -105,6 +105,9 @@ template <typename T>
 DEV T gload(const void *base, int64_t i) { return ((const GLOBAL T *)base)[i]; }
 template <typename T>
 DEV void gstore(void *base, int64_t i, T v) { ((GLOBAL T *)base)[i] = v; }
+// A store of data written once and not read back by the kernel (output columns).
+template <typename T>
+DEV void gstore_nt(void *base, int64_t i, T v) { __builtin_nontemporal_store(v, (GLOBAL T *)base + i); }
 
 DEV bool valid_bit(const void *v, int64_t row) {
   return v == nullptr ? true : ((gload<uint8_t>(v, row >> 3) >> (row & 7)) & 1);

@@ -527,23 +527,23 @@ DEV void vm_emit(const CONSTANT OutCol *out, int arg, int kind, const uint64_t (
   switch (kind) {
     case ST_I64: case ST_F64:
 #pragma unroll
-      for (int u = 0; u < R; u++) if (live[u]) gstore<uint64_t>(base, obase + orel[u], acc[u]);
+      for (int u = 0; u < R; u++) if (live[u]) gstore_nt<uint64_t>(base, obase + orel[u], acc[u]);
       break;
     case ST_I32:
 #pragma unroll
-      for (int u = 0; u < R; u++) if (live[u]) gstore<int32_t>(base, obase + orel[u], (int32_t)acc[u]);
+      for (int u = 0; u < R; u++) if (live[u]) gstore_nt<int32_t>(base, obase + orel[u], (int32_t)acc[u]);
       break;
     case ST_F32:
 #pragma unroll
-      for (int u = 0; u < R; u++) if (live[u]) gstore<float>(base, obase + orel[u], (float)b2d(acc[u]));
+      for (int u = 0; u < R; u++) if (live[u]) gstore_nt<float>(base, obase + orel[u], (float)b2d(acc[u]));
       break;
     case ST_I16:
 #pragma unroll
-      for (int u = 0; u < R; u++) if (live[u]) gstore<int16_t>(base, obase + orel[u], (int16_t)acc[u]);
+      for (int u = 0; u < R; u++) if (live[u]) gstore_nt<int16_t>(base, obase + orel[u], (int16_t)acc[u]);
       break;
     case ST_I8:
 #pragma unroll
-      for (int u = 0; u < R; u++) if (live[u]) gstore<int8_t>(base, obase + orel[u], (int8_t)acc[u]);
+      for (int u = 0; u < R; u++) if (live[u]) gstore_nt<int8_t>(base, obase + orel[u], (int8_t)acc[u]);
       break;
     default: break;
   }
