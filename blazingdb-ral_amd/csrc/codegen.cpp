@@ -205,8 +205,9 @@ struct Gen {
   }
   // Rows (g0 + u) * 64 + lane of `cols` into r<c>[u], from a uniform base plus the
   // lane offset and a constant per group; in a step reaching past the last row
-  // (CLAMP) the lanes past it hold 0 (they load the step's first row, in range, so
-  // that no load sits behind a branch of its own and waits alone).
+  // (CLAMP) the lanes past it load the last row (in range: a filter's last tile can
+  // hold slots that start past the last row) and then hold 0, so that no load sits
+  // behind a branch of its own and waits alone.
   void emit_loads(const std::set<int> &cols, const std::string &ind) {
     for (int c : cols) o << ind << uint_of_width(in.in_width[c]) << " r" << c << "[RS];\n";
     if (cols.empty()) return;
@@ -216,7 +217,8 @@ struct Gen {
     o << ind << "  const uint32_t lo = (uint32_t)(u * 64 + lane);\n";
     o << ind << "  const bool in_range = !CLAMP || (g0 + u) * 64 + lane < n;\n";
     for (int c : cols)
-      o << ind << "  r" << c << "[u] = __builtin_nontemporal_load(b" << c << " + (in_range ? lo : 0u));\n";
+      o << ind << "  r" << c << "[u] = __builtin_nontemporal_load(in_range ? b" << c << " + lo : c" << c
+        << " + (n - 1));\n";
     o << ind << "}\n";
     o << ind << "#pragma unroll\n" << ind << "for (int u = 0; u < RS; u++) {\n";
     o << ind << "  const bool in_range = !CLAMP || (g0 + u) * 64 + lane < n;\n";

@@ -132,3 +132,18 @@ def test_generated_loads_are_unconditional_and_null_counts_per_workgroup(B):
     assert "s_nn[threadIdx.x >> 6]" in proj_src
     B.jit_compile(src)
     B.jit_compile(proj_src)
+
+
+def test_tail_step_loads_stay_inside_the_column(B):
+    """A filter's last tile holds slots that start past the last row (four slots per
+    tile): the generated predicate's tail step loads the column's last row in the
+    lanes past it, never row g0 * 64 of such a slot (past the column's end)."""
+    src = B.program_source([P.GDF_FLOAT64, P.GDF_INT16, P.GDF_FLOAT64], 70_001,
+                           "LogicalFilter(condition=[<(MOD(*($2, $2), CEIL($0)), 0)])",
+                           "LogicalProject(E0=[/(5, $1)], A=[$1])", has_valid=[False, False, True],
+                           null_counts=[0, 0, 7000])
+    pred = src[:src.index("blz_cg_gather")]
+    loads = [l for l in pred.splitlines() if "__builtin_nontemporal_load" in l]
+    assert loads
+    for l in loads:
+        assert "in_range ? b" in l and "+ (n - 1))" in l, l

@@ -146,7 +146,7 @@ def test_random_expressions_with_nulls(B, seed):
     assert nulls == ref.null_count
 
 
-@pytest.mark.parametrize("seed", range(12))
+@pytest.mark.parametrize("seed", range(24))
 def test_random_filter_project_with_nulls(B, seed):
     rng = np.random.default_rng(2000 + seed)
     k = int(rng.integers(2, 6))
