@@ -165,6 +165,7 @@ def lib():
             "blz_stream_create": [C.POINTER(P)],
             "blz_stream_synchronize": [P],
             "blz_stream_destroy": [P],
+            "blz_release_launch_buffers": [],
             "blz_free_column": [colp],
             "blz_device_count": [],
             "blz_plan_expression": [I32P, I, C.c_char_p, I, I16P, I16P, I16P, I32P, I32P, scp, scp,

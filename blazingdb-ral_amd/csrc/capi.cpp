@@ -1660,6 +1660,17 @@ int blz_stream_create(blz_stream_t *stream) {
 int blz_stream_synchronize(blz_stream_t stream) {
   return hipStreamSynchronize(S(stream)) == hipSuccess ? BLZ_OK : BLZ_ERR_HIP;
 }
+int blz_release_launch_buffers(void) {
+  std::lock_guard<std::mutex> g(g_lb_mu);
+  for (auto &e : g_lb) {
+    std::lock_guard<std::mutex> b(e.second->busy);  // after any call in flight
+    if (e.second->p) (void)hipFree(e.second->p);
+    e.second->p = nullptr;
+    e.second->bytes = 0;
+  }
+  return BLZ_OK;
+}
+
 int blz_stream_destroy(blz_stream_t stream) {
   lb_cache_drop(S(stream));
   return hipStreamDestroy(S(stream)) == hipSuccess ? BLZ_OK : BLZ_ERR_HIP;

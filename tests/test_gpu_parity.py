@@ -760,8 +760,9 @@ def test_register_load_predicate(B, case, t):
 def test_one_shot_calls_reuse_and_release_the_streams_launch_buffers(B):
     """One-shot filters keep their launch buffers per stream, growing them for a
     larger input and reusing them for smaller ones: calls of alternating sizes on the
-    default stream and on a stream of the library's, whose buffers go with it when
-    it is destroyed, all give the oracle's result."""
+    default stream and on a stream of the library's, with every buffer released
+    midway and the stream's going with it when it is destroyed, all give the
+    oracle's result."""
     import ctypes as C
     st = C.c_void_p(0)
     assert B.lib().blz_stream_create(C.byref(st)) == B.BLZ_OK
@@ -780,5 +781,7 @@ def test_one_shot_calls_reuse_and_release_the_streams_launch_buffers(B):
                     c.free()
             for c in dev:
                 c.free()
+            if i == 2:   # every kept buffer released: the next calls allocate again
+                assert B.lib().blz_release_launch_buffers() == B.BLZ_OK
     finally:
         assert B.lib().blz_stream_destroy(st) == B.BLZ_OK
