@@ -785,3 +785,47 @@ def test_one_shot_calls_reuse_and_release_the_streams_launch_buffers(B):
                 assert B.lib().blz_release_launch_buffers() == B.BLZ_OK
     finally:
         assert B.lib().blz_stream_destroy(st) == B.BLZ_OK
+
+
+def test_one_shot_calls_from_threads_on_shared_and_own_streams(B):
+    """Four threads issue one-shot filters at once, two on streams of their own and
+    two on the default stream (whose kept launch buffer they take in turns): every
+    result equals the oracle's."""
+    import ctypes as C
+    import threading
+    cases = []
+    for i, n in enumerate([300_001, 40_000, 1_000_003, 77_777]):
+        host = c1_frame(n, seed=950 + i)
+        ref = O.execute_project(O.process_filter(host, C1_FILTER), C1_PROJECT)
+        cases.append(([to_device(B, h) for h in host], ref))
+    streams = [C.c_void_p(0), C.c_void_p(0)]
+    for st in streams:
+        assert B.lib().blz_stream_create(C.byref(st)) == B.BLZ_OK
+    errors = []
+
+    def work(i):
+        dev, ref = cases[i]
+        stream = streams[i] if i < 2 else None
+        try:
+            for _ in range(5):
+                outs, kept = B.filter_project(dev, C1_FILTER, C1_PROJECT, stream=stream)
+                assert kept == len(ref[0])
+                for g, r in zip(outs, ref):
+                    assert_column_equal(g, r, what=f"thread {i}")
+                for c in outs:
+                    c.free()
+        except Exception as e:  # reported by the main thread
+            errors.append(f"thread {i}: {e!r}")
+
+    print("CASE one_shot_threads", flush=True)
+    threads = [threading.Thread(target=work, args=(i,)) for i in range(4)]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join()
+    for st in streams:
+        assert B.lib().blz_stream_destroy(st) == B.BLZ_OK
+    for dev, _ in cases:
+        for c in dev:
+            c.free()
+    assert not errors, errors
