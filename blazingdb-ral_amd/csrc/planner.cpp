@@ -27,6 +27,8 @@
 #include <algorithm>
 #include <cstdio>
 #include <map>
+#include <cctype>
+#include <cstring>
 #include <regex>
 #include <stdexcept>
 
@@ -107,9 +109,26 @@ const std::regex &count_re() {
   return re;
 }
 
+// The regular expressions below run on every query and token and std::regex is
+// slow (a C1 filter took 0.23 ms to plan); each is tried only when a text the
+// pattern needs is present, which leaves every result as the pattern gives it.
+bool contains_icase(const std::string &s, const char *lit) {
+  const size_t m = std::strlen(lit);
+  if (s.size() < m) return false;
+  for (size_t i = 0; i + m <= s.size(); i++) {
+    size_t k = 0;
+    while (k < m && std::tolower((unsigned char)s[i + k]) == std::tolower((unsigned char)lit[k])) k++;
+    if (k == m) return true;
+  }
+  return false;
+}
+std::string rewrite_coalesce(const std::string &s) {
+  return contains_icase(s, "CASE(IS NOT NULL(") ? std::regex_replace(s, coalesce_re(), "COALESCE($1, $2)") : s;
+}
+
 std::string common_rewrites(std::string s) {
-  s = std::regex_replace(s, coalesce_re(), "COALESCE($1, $2)");
-  s = std::regex_replace(s, count_re(), "COUNT_DISTINCT($1)");
+  s = rewrite_coalesce(s);
+  if (contains_icase(s, "COUNT(DISTINCT ")) s = std::regex_replace(s, count_re(), "COUNT_DISTINCT($1)");
   replace_all(s, " NOT NULL", "");
   replace_all(s, "):DOUBLE", "");
   replace_all(s, "CAST(", "");
@@ -236,7 +255,7 @@ std::string expand_if_logical_op(std::string expression) {
 }
 
 std::string clean_calcite_expression(std::string e) {
-  e = std::regex_replace(e, coalesce_re(), "COALESCE($1, $2)");
+  e = rewrite_coalesce(e);
   replace_all(e, "IS NOT NULL", "IS_NOT_NULL");
   replace_all(e, "IS NULL", "IS_NULL");
   replace_all(e, " NOT NULL", "");
@@ -261,14 +280,54 @@ std::string clean_calcite_expression(std::string e) {
 
 bool is_string(const std::string &t) { return !t.empty() && t[0] == '\'' && t.back() == '\''; }
 bool is_literal(const std::string &t) { return t.empty() || t[0] != '$'; }
-bool is_number(const std::string &t) {
-  static const std::regex re{R"(^[-+]?[0-9]*\.?[0-9]+([eE][-+]?[0-9]+)?$)"};
-  return std::regex_match(t, re);
+// Exact matchers of the literal grammars the reference's planner tests with regular
+// expressions (CalciteExpressionParsing.cpp:274, :324-325, :585): ^[-+]?[0-9]+$ (match_int),
+// ^[-+]?[0-9]*\.?[0-9]+([eE][-+]?[0-9]+)?$ (match_float) and
+// [12]\d{3}-(0[1-9]|1[0-2])-(0[1-9]|[12]\d|3[01]) (match_date), written out because
+// std::regex costs microseconds per token.
+bool is_digit(char c) { return c >= '0' && c <= '9'; }
+bool match_int(const std::string &t) {
+  size_t i = 0;
+  if (i < t.size() && (t[i] == '+' || t[i] == '-')) i++;
+  const size_t d = i;
+  while (i < t.size() && is_digit(t[i])) i++;
+  return i > d && i == t.size();
 }
-bool is_date(const std::string &t) {
-  static const std::regex re{R"([12]\d{3}-(0[1-9]|1[0-2])-(0[1-9]|[12]\d|3[01]))"};
-  return std::regex_match(t, re);
+bool match_float(const std::string &t) {
+  const size_t n = t.size();
+  size_t i = 0;
+  if (i < n && (t[i] == '+' || t[i] == '-')) i++;
+  const size_t a = i;
+  while (i < n && is_digit(t[i])) i++;
+  if (i < n && t[i] == '.') {  // [0-9]*\.[0-9]+
+    const size_t b = ++i;
+    while (i < n && is_digit(t[i])) i++;
+    if (i == b) return false;
+  } else if (i == a) {  // [0-9]+
+    return false;
+  }
+  if (i < n && (t[i] == 'e' || t[i] == 'E')) {
+    i++;
+    if (i < n && (t[i] == '+' || t[i] == '-')) i++;
+    const size_t c = i;
+    while (i < n && is_digit(t[i])) i++;
+    if (i == c) return false;
+  }
+  return i == n;
 }
+bool match_date(const std::string &t) {
+  if (t.size() != 10 || (t[0] != '1' && t[0] != '2') || !is_digit(t[1]) || !is_digit(t[2]) || !is_digit(t[3]) ||
+      t[4] != '-' || t[7] != '-')
+    return false;
+  const char m0 = t[5], m1 = t[6], d0 = t[8], d1 = t[9];
+  const bool month = (m0 == '0' && m1 >= '1' && m1 <= '9') || (m0 == '1' && m1 >= '0' && m1 <= '2');
+  const bool day = (d0 == '0' && d1 >= '1' && d1 <= '9') || ((d0 == '1' || d0 == '2') && is_digit(d1)) ||
+                   (d0 == '3' && (d1 == '0' || d1 == '1'));
+  return month && day;
+}
+
+bool is_number(const std::string &t) { return match_float(t); }
+bool is_date(const std::string &t) { return match_date(t); }
 bool is_operator_token(const std::string &t) {
   return !t.empty() && t[0] != '$' && !is_number(t) && !is_date(t) && !is_string(t);
 }
@@ -293,10 +352,8 @@ size_t get_index(const std::string &operand) {
 }
 
 int get_type_from_string(const std::string &s) {
-  static const std::regex re_int{R"(^[-+]?[0-9]+$)"};
-  static const std::regex re_float{R"(^[-+]?[0-9]*\.?[0-9]+([eE][-+]?[0-9]+)?$)"};
-  if (std::regex_match(s, re_int)) return GDF_INT64;
-  if (std::regex_match(s, re_float)) return GDF_FLOAT64;
+  if (match_int(s)) return GDF_INT64;
+  if (match_float(s)) return GDF_FLOAT64;
   return GDF_DATE64;
 }
 
