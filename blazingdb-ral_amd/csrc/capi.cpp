@@ -114,9 +114,13 @@ void *pool_alloc(size_t bytes, hipStream_t st) {
 // whole call (calls on one stream from several threads take turns); a buffer larger
 // than kLbCacheMax is allocated for the call alone.
 constexpr size_t kLbCacheMax = (size_t)4 << 30;
+// Pinned host words per slot for the calls' readbacks (kept count, null counts):
+// a device-to-host copy into pinned memory needs no staging copy on the host.
+constexpr int kLbHostWords = 128;
 struct LbSlot {
   uint8_t *p = nullptr;
   size_t bytes = 0;
+  unsigned long long *host = nullptr;
   std::mutex busy;
 };
 std::mutex g_lb_mu;
@@ -138,6 +142,10 @@ class LbLease {
       slot_ = slot.get();
     }
     lock_ = std::unique_lock<std::mutex>(slot_->busy);
+    if (!slot_->host &&
+        hipHostMalloc((void **)&slot_->host, sizeof(unsigned long long) * kLbHostWords, hipHostMallocDefault) !=
+            hipSuccess)
+      slot_->host = nullptr;  // readbacks go through pageable memory
     if (slot_->bytes < bytes) {  // the slot's previous call has completed (calls block)
       if (slot_->p) HIPCHECK(hipFree(slot_->p));
       slot_->p = nullptr;
@@ -154,6 +162,11 @@ class LbLease {
   LbLease(const LbLease &) = delete;
   LbLease &operator=(const LbLease &) = delete;
   uint8_t *get() const { return p_; }
+  // Pinned readback words for `words` values, or `fallback` (pageable) when the
+  // lease has none or too few.
+  unsigned long long *host(size_t words, unsigned long long *fallback) const {
+    return slot_ && slot_->host && words <= (size_t)kLbHostWords ? slot_->host : fallback;
+  }
 
  private:
   hipStream_t st_;
@@ -161,6 +174,20 @@ class LbLease {
   uint8_t *p_ = nullptr;
   std::unique_lock<std::mutex> lock_;
 };
+
+// A device address for the stand-in output pointers of a sized filter's first
+// compilation (the predicate and the slot scan never read or write outputs): 256
+// bytes per device, allocated once and kept for the process.
+uint8_t *stand_in_address() {
+  static std::mutex mu;
+  static std::map<int, uint8_t *> addr;
+  int dev = 0;
+  HIPCHECK(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> g(mu);
+  uint8_t *&p = addr[dev];
+  if (!p) HIPCHECK(hipMalloc((void **)&p, 256));
+  return p;
+}
 
 // Frees the cached launch buffers of `st` (on every device) when the library's
 // stream is destroyed.
@@ -171,6 +198,7 @@ void lb_cache_drop(hipStream_t st) {
     {
       std::lock_guard<std::mutex> b(it->second->busy);
       if (it->second->p) (void)hipFree(it->second->p);
+      if (it->second->host) (void)hipHostFree(it->second->host);
     }
     it = g_lb.erase(it);
   }
@@ -409,7 +437,8 @@ RunResult run_once(const Compiled &c, bool compact, int64_t n, int n_out, hipStr
   LbLease lease(lb.bytes, st);
   lb.base = lease.get();
   bind(lb, n_out);
-  std::vector<unsigned long long> host(n_out + 1, 0);
+  std::vector<unsigned long long> host_v(n_out + 1, 0);
+  unsigned long long *host = lease.host(n_out + 1, host_v.data());
   const std::vector<uint8_t> img = device_image(lb, c);
   hipError_t e = hipMemcpyAsync(lb.blob, img.data(), img.size(), hipMemcpyHostToDevice, st);
   if (e == hipSuccess) e = launch_fill_zero(lb.base + lb.zero_off, (int64_t)lb.zero_bytes,
@@ -417,10 +446,9 @@ RunResult run_once(const Compiled &c, bool compact, int64_t n, int n_out, hipStr
   if (e == hipSuccess)
     e = launch_program(c, compact, interp_grid(c, compact, n), st, lb.blob, lb.ws, lb.bd, lb.n_chunks);
   if (e == hipSuccess && n_out > 0)
-    e = hipMemcpyAsync(host.data(), lb.ws.null_counts, sizeof(unsigned long long) * n_out,
-                       hipMemcpyDeviceToHost, st);
+    e = hipMemcpyAsync(host, lb.ws.null_counts, sizeof(unsigned long long) * n_out, hipMemcpyDeviceToHost, st);
   if (e == hipSuccess && compact)
-    e = hipMemcpyAsync(host.data() + n_out, lb.ws.n_sel, sizeof(unsigned long long), hipMemcpyDeviceToHost, st);
+    e = hipMemcpyAsync(host + n_out, lb.ws.n_sel, sizeof(unsigned long long), hipMemcpyDeviceToHost, st);
   hipError_t s = hipStreamSynchronize(st);
   HIPCHECK(e);
   HIPCHECK(s);
@@ -442,7 +470,8 @@ RunResult run_filter_sized(const Compiled &c, CompileRequest &req, Alloc allocat
   LbLease lease(lb.bytes, st);
   lb.base = lease.get();
   bind(lb, n_out);
-  std::vector<unsigned long long> host(n_out + 1, 0);
+  std::vector<unsigned long long> host_v(n_out + 1, 0);
+  unsigned long long *host = lease.host(n_out + 1, host_v.data());
   hipError_t e = hipSuccess;
   try {
     const std::vector<uint8_t> img = device_image(lb, c);
@@ -453,7 +482,7 @@ RunResult run_filter_sized(const Compiled &c, CompileRequest &req, Alloc allocat
     if (e == hipSuccess)
       e = launch_program(c, true, interp_grid(c, true, n), st, lb.blob, lb.ws, lb.bd, lb.n_chunks, nullptr, -1, 3);
     if (e == hipSuccess)
-      e = hipMemcpyAsync(host.data() + n_out, lb.ws.n_sel, sizeof(unsigned long long), hipMemcpyDeviceToHost, st);
+      e = hipMemcpyAsync(host + n_out, lb.ws.n_sel, sizeof(unsigned long long), hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     HIPCHECK(e);
     r.n_sel = (int64_t)host[n_out];
@@ -466,7 +495,7 @@ RunResult run_filter_sized(const Compiled &c, CompileRequest &req, Alloc allocat
     if (e == hipSuccess)
       e = launch_program(c2, true, interp_grid(c2, true, n), st, lb.blob, lb.ws, lb.bd, lb.n_chunks, nullptr, -1, 4);
     if (e == hipSuccess && n_out > 0)
-      e = hipMemcpyAsync(host.data(), lb.ws.null_counts, sizeof(unsigned long long) * n_out,
+      e = hipMemcpyAsync(host, lb.ws.null_counts, sizeof(unsigned long long) * n_out,
                          hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     HIPCHECK(e);
@@ -790,7 +819,7 @@ void filter_project_impl(gdf_column *const *frame, int n_frame, const char *filt
       Compiled c;
       bool sized = false;
       if (has_filter) {
-        placeholder = (uint8_t *)pool_alloc(256, st);
+        placeholder = stand_in_address();
         for (size_t i = 0; i < outs.size(); i++) {
           const auto &o = outs[i];
           const bool with_valid = o.kind == 0 ? true : frame[o.index]->valid != nullptr;
@@ -817,10 +846,6 @@ void filter_project_impl(gdf_column *const *frame, int n_frame, const char *filt
       HIPCHECK(hipStreamSynchronize(st));
       if (n > 0 && req.inputs.empty()) invalid("nothing to evaluate");
     }
-    if (placeholder) {
-      (void)hipFreeAsync(placeholder, st);
-      placeholder = nullptr;
-    }
     for (size_t i = 0; i < outs.size(); i++) {
       res[i].size = (gdf_size_type)kept;
       res[i].null_count = (res[i].valid && kept > 0) ? nulls[i] : 0;
@@ -829,7 +854,6 @@ void filter_project_impl(gdf_column *const *frame, int n_frame, const char *filt
     }
   } catch (...) {
     (void)hipStreamSynchronize(st);  // the stream may still use what is freed here
-    if (placeholder) (void)hipFree(placeholder);
     for (auto &c : res) { (void)hipFree(c.data); (void)hipFree(c.valid); }
     throw;
   }
@@ -1665,8 +1689,10 @@ int blz_release_launch_buffers(void) {
   for (auto &e : g_lb) {
     std::lock_guard<std::mutex> b(e.second->busy);  // after any call in flight
     if (e.second->p) (void)hipFree(e.second->p);
+    if (e.second->host) (void)hipHostFree(e.second->host);
     e.second->p = nullptr;
     e.second->bytes = 0;
+    e.second->host = nullptr;
   }
   return BLZ_OK;
 }

@@ -427,9 +427,9 @@ int blz_memset_device(void *dst, int value, size_t bytes, blz_stream_t stream);
 int blz_device_copy(void *dst, const void *src, size_t bytes, blz_stream_t stream);
 int blz_stream_create(blz_stream_t *stream);
 int blz_stream_synchronize(blz_stream_t stream);
-/* Frees the launch buffers the one-shot calls keep per (device, stream) between
- * calls (a stream's next call allocates them again); waits for calls in flight.
- * No reference counterpart. */
+/* Frees the launch buffers (device memory and pinned readback words) the one-shot
+ * calls keep per (device, stream) between calls (a stream's next call allocates
+ * them again); waits for calls in flight. No reference counterpart. */
 int blz_release_launch_buffers(void);
 int blz_stream_destroy(blz_stream_t stream);
 /* Frees data and validity of a callee-allocated column and zeroes the struct. */
