@@ -943,27 +943,41 @@ RedPart reduce_column(const gdf_column *c, int op, hipStream_t st) {
   if (load == LD_NONE) throw BlzError(BLZ_ERR_UNSUPPORTED, "aggregation over this dtype is not supported");
   const int64_t n = c->size;
   const int grid = reduce_grid(load, n);
-  RedPart *parts = nullptr;
-  parts = (RedPart *)pool_alloc(sizeof(RedPart) * (grid + 1), st);
-  RedPart r{};
+  // the partials live in the stream's launch buffers, the result is read back
+  // through its pinned words
+  LbLease lease(sizeof(RedPart) * (grid + 1), st);
+  RedPart *parts = (RedPart *)lease.get();
+  RedPart r_v{};
+  RedPart *r = (RedPart *)lease.host(sizeof(RedPart) / sizeof(unsigned long long), (unsigned long long *)&r_v);
   const uint8_t *valid = c->null_count > 0 ? (const uint8_t *)c->valid : nullptr;
   hipError_t e = launch_reduce(op, c->data, load, valid, n, parts, grid, st);
-  if (e == hipSuccess) e = hipMemcpyAsync(&r, parts + grid, sizeof(r), hipMemcpyDeviceToHost, st);
-  hipError_t f = hipFreeAsync(parts, st);
+  if (e == hipSuccess) e = hipMemcpyAsync(r, parts + grid, sizeof(RedPart), hipMemcpyDeviceToHost, st);
   hipError_t s = hipStreamSynchronize(st);
   HIPCHECK(e);
-  HIPCHECK(f);
   HIPCHECK(s);
-  return r;
+  return *r;
 }
 
 // A one-row column holding `bits` (the first width bytes) and its validity.
 gdf_column scalar_column(int dtype, uint64_t bits, bool valid, hipStream_t st) {
-  gdf_column c = alloc_column(dtype, 1, true, st);
-  const uint8_t vbyte = valid ? 1 : 0;
-  HIPCHECK(hipMemcpyAsync(c.data, &bits, blz_dtype_width((gdf_dtype)dtype), hipMemcpyHostToDevice, st));
-  HIPCHECK(hipMemcpyAsync(c.valid, &vbyte, 1, hipMemcpyHostToDevice, st));
-  HIPCHECK(hipStreamSynchronize(st));
+  gdf_column c{};
+  c.dtype = (gdf_dtype)dtype;
+  c.size = 1;
+  const size_t vb = gdf_valid_allocation_size(1);
+  c.data = dalloc(64, st);
+  c.valid = (gdf_valid_type *)dalloc(vb, st);
+  // the whole padded bitmap is written (no separate clearing launch)
+  std::vector<uint8_t> vbytes(vb, 0);
+  vbytes[0] = valid ? 1 : 0;
+  hipError_t e = hipMemcpyAsync(c.data, &bits, blz_dtype_width((gdf_dtype)dtype), hipMemcpyHostToDevice, st);
+  if (e == hipSuccess) e = hipMemcpyAsync(c.valid, vbytes.data(), vb, hipMemcpyHostToDevice, st);
+  hipError_t s = hipStreamSynchronize(st);
+  if (e != hipSuccess || s != hipSuccess) {
+    (void)hipFree(c.data);
+    (void)hipFree(c.valid);
+    HIPCHECK(e);
+    HIPCHECK(s);
+  }
   c.null_count = valid ? 0 : 1;
   return c;
 }
