@@ -293,9 +293,9 @@ int blz_batch_destroy(blz_batch batch);
  * n_streams stream slots is free, batched (up to max_batch per launch) and
  * launched, so batches admitted while others run overlap on the device. Results
  * are identical to running each query alone. r: rows per lane of every batch.
- * The device's default stream-ordered memory pool keeps up to hbm_budget_bytes of
- * freed memory (its release threshold) instead of returning it at each
- * synchronisation. */
+ * Outputs and workspaces come from the server's own device arena (4 GiB chunks,
+ * reserved as admitted queries need them and held until blz_server_destroy);
+ * the device's stream-ordered memory pool is left as it is. */
 typedef struct blz_server_s *blz_server;
 int blz_server_create(blz_server *server, size_t hbm_budget_bytes, int max_batch, int n_streams, int r,
                       char *err, size_t err_len);
